@@ -1,0 +1,191 @@
+/*
+ * pdekit_rd.h -- C ABI of the MI355X residual-distribution cell loop.
+ *
+ * The reference's plug-in surface for this path is the C++ class template
+ *   PGRDMethodExplicit<MeshConfig, Physics, SchemeTag>
+ *     (src/solver/rdm/PGRDMethodExplicit.hpp:19-193, virtual API in
+ *      src/solver/rdm/RDMethod.hpp:33-139)
+ * with the element scheme selected by CellSchemeSelector<Physics, Tag>
+ *     (src/solver/rdm/cellsplitters/CellSchemeSelector.hpp:17; PGLDA.hpp:746,
+ *      PGN.hpp:673-676, PGB.hpp:1071).
+ * Every entry point below names the member it replaces. Plain pointers and
+ * sizes only; all arrays are row-major, nodal fields are AoS [N][m] doubles
+ * as in VectorMeshFunction (src/interpolation/mesh_function/
+ * VectorMeshFunction.hpp:145-249). Every call returns PDRD_OK (0) or a
+ * negative status; pdrd_last_error() describes the last failure of the
+ * calling thread. One handle serves one in-flight call at a time
+ * (stream-ordered); distinct handles may be used concurrently.
+ */
+#ifndef PDEKIT_RD_H
+#define PDEKIT_RD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------- */
+enum pdrd_status {
+  PDRD_OK = 0,
+  PDRD_EINVAL = -1,        /* bad argument / size / state */
+  PDRD_ENOMEM = -2,        /* device allocation failed */
+  PDRD_EHIP = -3,          /* HIP runtime error */
+  PDRD_ENOTSUP = -4,       /* combination not built */
+  PDRD_ENEGPRESSURE = -5,  /* non-positive pressure met in a cell (the reference
+                              prints and continues, Euler2DCons.hpp:188-201) */
+  PDRD_ENONFINITE = -6     /* NaN/Inf in the residual */
+};
+
+/* ---- enumerations ------------------------------------------------------- */
+enum pdrd_shape { PDRD_TRIAG = 0, PDRD_QUAD = 1, PDRD_TETRA = 2 };
+enum pdrd_point_set { PDRD_EQUIDIST = 0, PDRD_WARPBLEND = 1 };
+/* Physics template argument of PGRDMethodExplicit:
+ * ADVECTION2D = RotationAdvection2D with a constant velocity (params[0..1]),
+ * BURGERS2D = Burgers2D, EULER2D = Euler2DCons, EULER3D = Euler3DCons
+ * (params[0] = gamma, params[1] = R). */
+enum pdrd_physics { PDRD_ADVECTION2D = 0, PDRD_BURGERS2D = 1, PDRD_EULER2D = 2, PDRD_EULER3D = 3 };
+/* SchemeTag: PGLDA, PGN, PGB; PSI = N residuals limited with the x+ limiter
+ * of LFResidualLimiter.hpp:84-110. */
+enum pdrd_scheme { PDRD_LDA = 0, PDRD_N = 1, PDRD_B = 2, PDRD_PSI = 3 };
+/* Nodal fields held by a method (RDMethod::vec_function / scal_function). */
+enum pdrd_field {
+  PDRD_SOLUTION = 0,   /* [N][m] */
+  PDRD_RESIDUAL = 1,   /* [N][m] */
+  PDRD_TIME_STEP = 2,  /* [N]    */
+  PDRD_WAVE_SPEED = 3, /* [N]    sum of element wave speeds (RDTimeUpdate) */
+  PDRD_DUAL_VOLUME = 4 /* [N]    */
+};
+/* Options of a method. */
+enum pdrd_option {
+  PDRD_OPT_SCATTER = 0,   /* 0 = FP64 atomics, 1 = colour-by-colour (deterministic) */
+  PDRD_OPT_PRECISION = 1, /* 0 = plain FP64, 1 = mixed: FP32 interpolation/fluxes,
+                             FP64 inverse, accumulation, update and norms */
+  PDRD_OPT_BLOCK = 2      /* threads per block of the cell kernels (64..1024) */
+};
+
+/* ---- FE inputs (src/interpolation, src/mesh/point_set, shape_function) -- */
+/* Sizes of one element type: solution order sol_order on a geometry of
+ * order geo_order, P+1 flux element, Gauss quadrature of order quad_order. */
+typedef struct pdrd_fe_desc {
+  int shape, geo_order, sol_order, quad_order, point_set;
+  int dim, ng, nd, nf, nq;
+  size_t table_size; /* doubles in the packed table */
+} pdrd_fe_desc;
+
+/* FunctionSpaceCells::set_reference_fe_values + FluxSpaceMetric::
+ * allocate_buffer (FunctionSpaceCells.hpp:135-190, FluxSpaceMetric.hpp:237-456,
+ * FEValues.cpp:157-279). */
+int pdrd_fe_describe(int shape, int geo_order, int sol_order, int quad_order, int point_set,
+                     pdrd_fe_desc *out);
+/* Packed tables, concatenated in this order (row-major):
+ * qp[nq][dim], w[nq], Vg[nq][ng], dVg[dim][nq][ng], V[nq][nd], dV[dim][nq][nd],
+ * Vf[nq][nf], dVf[dim][nq][nf], Vgf[nf][ng], Vsf[nf][nd], sol_nodes[nd][dim],
+ * flux_nodes[nf][dim]. */
+int pdrd_fe_fill(const pdrd_fe_desc *desc, double *tables);
+/* Reference nodes and Gauss rules on their own. */
+int pdrd_reference_nodes(int shape, int order, int point_set, double *xi, int *n);
+int pdrd_quadrature(int shape, int order, double *xi, double *w, int *nq);
+
+/* ---- mesh inputs (src/mesh/io/MeshCreator, src/mesh/containers/DofMap) -- */
+typedef struct pdrd_meshgen *pdrd_meshgen_t;
+/* kind: 0 = MeshCreator::make_unit_quad(N = i0) on [-1,1]^2,
+ *       1 = rectangle nx=i0, ny=i1 on [r0,r1]x[r2,r3], split i2 (1,2,4),
+ *           quad_rows i3,
+ *       2 = box tetrahedra nx=i0, ny=i1, nz=i2 on [0,r0]x[0,r1]x[0,r2]. */
+int pdrd_mesh_generate(int kind, const int64_t *iparams, const double *rparams,
+                       pdrd_meshgen_t *out);
+int pdrd_mesh_free(pdrd_meshgen_t m);
+/* seeded jitter of non-boundary vertices: U(-f h_d, f h_d) per axis */
+int pdrd_mesh_jitter(pdrd_meshgen_t m, const double *h, double f, uint64_t seed);
+/* mode 0: 2D bump y -> yb + y (H - yb)/H, yb = amp exp(-k (x-xc)^2);
+ * mode 1: 3D bump in z with centre (xc, yc). p = {H, amp, k, xc, yc}. */
+int pdrd_mesh_bump(pdrd_meshgen_t m, int mode, const double *p);
+/* DofMap::clone_continuous to order P (DofMap.hpp:2225-2290, 2556-3073). */
+int pdrd_mesh_clone(pdrd_meshgen_t m, int order, int point_set);
+/* counts: dim, nb_vertices, nb_cells, nb_nodes (after clone), nb_boundaries */
+int pdrd_mesh_info(pdrd_meshgen_t m, int64_t *info5);
+int pdrd_mesh_coords(pdrd_meshgen_t m, double *coords);          /* [nv][dim] */
+int pdrd_mesh_node_coords(pdrd_meshgen_t m, double *coords);     /* [N][dim]  */
+int pdrd_mesh_cell_shapes(pdrd_meshgen_t m, int32_t *shapes);    /* [ncells]  */
+/* geometry / solution connectivity, packed cell after cell */
+int pdrd_mesh_geo_conn(pdrd_meshgen_t m, uint32_t *conn, int64_t *size);
+int pdrd_mesh_sol_conn(pdrd_meshgen_t m, uint32_t *conn, int64_t *size);
+int pdrd_mesh_boundary_name(pdrd_meshgen_t m, int idx, char *name, int cap);
+int pdrd_mesh_boundary_nodes(pdrd_meshgen_t m, const char *name, uint32_t *nodes, int64_t *n);
+
+/* ---- the method --------------------------------------------------------- */
+typedef struct pdrd_method *pdrd_method_t;
+
+/* PGRDMethodExplicit() + RDMethod::configure_mesh_data(mesh, geo, sol)
+ * (RDMethod.hpp:158-173): vertex coordinates [nv][dim] and the number of
+ * solution nodes. The method lives on the current HIP device. */
+int pdrd_create(int dim, uint32_t nb_vertices, const double *vertex_coords, uint32_t nb_nodes,
+                int physics, int scheme, const double *params, int nb_params, pdrd_method_t *out);
+/* One block of cells of one element type (the per-type DataMap entries of
+ * PGRDMExplicitCellWorker::configure_cell_spaces, PGRDMExplicitCellWorker.hpp:178-284).
+ * geo_conn [ne][ng], sol_conn [ne][nd]; tables as pdrd_fe_fill writes them. */
+int pdrd_add_cells(pdrd_method_t h, const pdrd_fe_desc *fe, const double *tables, uint32_t ne,
+                   const uint32_t *geo_conn, const uint32_t *sol_conn);
+/* initialize_work_data (PGRDMethodExplicit.hpp:231-299) + RDTimeUpdate::setup
+ * (RDTimeUpdate.hpp:71-194): uploads the mesh, builds the colouring and the
+ * dual volumes. */
+int pdrd_initialize(pdrd_method_t h);
+int pdrd_destroy(pdrd_method_t h);
+int pdrd_set_option(pdrd_method_t h, int option, int value);
+/* HIP stream (hipStream_t) the method enqueues on; NULL = the method's own. */
+int pdrd_set_stream(pdrd_method_t h, void *stream);
+int pdrd_synchronize(pdrd_method_t h);
+
+/* set_vec_function / vec_function: host <-> device copies of a field. */
+int pdrd_set_field(pdrd_method_t h, int field, const double *host);
+int pdrd_get_field(pdrd_method_t h, int field, double *host);
+/* Device address of a field for callers that keep data resident. */
+int pdrd_field_device_ptr(pdrd_method_t h, int field, double **dptr);
+
+/* add_boundary_condition("StrongDirichlet", ...) with precomputed values:
+ * u[nodes[k]] = values[k][0..m) after every cell loop
+ * (StrongDirichletBC::apply, StrongDirichletBC.hpp:202-233). */
+int pdrd_add_strong_dirichlet(pdrd_method_t h, uint32_t n, const uint32_t *nodes, const double *values);
+
+/* assemble_rhs (PGRDMethodExplicit.hpp:427-463): residual = sum Phi / V_dual,
+ * wave speeds, then the boundary conditions. */
+int pdrd_assemble_rhs(pdrd_method_t h);
+/* assemble_lhs_and_rhs(CFL) (PGRDMethodExplicit.hpp:416-423): assemble_rhs +
+ * RDTimeUpdate::compute_local_time_step (dt = CFL V / sum lambda). */
+int pdrd_assemble_lhs_and_rhs(pdrd_method_t h, double cfl);
+/* solve (PGRDMethodExplicit.hpp:467-477): u -= dt * res. */
+int pdrd_solve(pdrd_method_t h);
+/* compute_residual_norm (PGRDMethodExplicit.hpp:481-514): sqrt(sum (r dt)^2 / N). */
+int pdrd_compute_residual_norm(pdrd_method_t h, double *norm);
+/* norm_L2(residual) (MeshFunctionNorm.hpp:92-): sqrt(sum r^2 / n) per field;
+ * exclude_dirichlet != 0 leaves out the strong-Dirichlet rows (n counts the
+ * remaining nodes). */
+int pdrd_residual_norm_L2(pdrd_method_t h, int exclude_dirichlet, double *norm);
+/* n_iter times { assemble_lhs_and_rhs(cfl); solve() } on the device (the
+ * driver loop of examples/testcases-rdm/Advection-Continuous-RDS-2D-Explicit.cpp:160-163),
+ * replayed from one captured HIP graph; norm receives norm_L2 of the last
+ * residual (exclude_dirichlet as above). */
+int pdrd_iterate(pdrd_method_t h, double cfl, int n_iter, int exclude_dirichlet, double *norm);
+
+/* Status of the last cell loop: PDRD_OK or PDRD_ENEGPRESSURE. */
+int pdrd_check_status(pdrd_method_t h);
+
+/* Counts of the method: nb_nodes, neq, nb_blocks, total cells, nb_colours
+ * (max over blocks), device bytes held. */
+int pdrd_method_info(pdrd_method_t h, int64_t *info6);
+
+/* Per-kernel timing of the last pdrd_iterate/assemble call measured with HIP
+ * events on the method's stream: ms of the cell kernels and of the nodal
+ * passes. */
+int pdrd_last_timing(pdrd_method_t h, double *cell_ms, double *node_ms);
+
+const char *pdrd_last_error(void);
+const char *pdrd_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PDEKIT_RD_H */

@@ -1,0 +1,171 @@
+"""ctypes binding of the CPU oracle (oracle/liboracle_rd.so) -- TEST INFRASTRUCTURE.
+
+Only the tests, the smoke check and the CPU-baseline leg of bench.py use this
+module, and only as the checker of the GPU path.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "liboracle_rd.so")
+
+ADVECTION2D, BURGERS2D, EULER2D, EULER3D = 0, 1, 2, 3
+LDA, N, B, PSI = 0, 1, 2, 3
+
+
+class OrcBlock(C.Structure):
+    _fields_ = [
+        ("ng", C.c_int), ("nd", C.c_int), ("nf", C.c_int), ("nq", C.c_int), ("vol_nq", C.c_int),
+        ("ne", C.c_uint32),
+        ("w", C.POINTER(C.c_double)), ("dVg", C.POINTER(C.c_double)), ("V", C.POINTER(C.c_double)),
+        ("dV", C.POINTER(C.c_double)), ("dVf", C.POINTER(C.c_double)), ("Vsf", C.POINTER(C.c_double)),
+        ("Vgf", C.POINTER(C.c_double)), ("vol_w", C.POINTER(C.c_double)), ("vol_dVg", C.POINTER(C.c_double)),
+        ("geo_conn", C.POINTER(C.c_uint32)), ("sol_conn", C.POINTER(C.c_uint32)),
+    ]
+
+
+class OrcProblem(C.Structure):
+    _fields_ = [
+        ("dim", C.c_int), ("physics", C.c_int), ("scheme", C.c_int), ("nblocks", C.c_int),
+        ("nv", C.c_uint32), ("nn", C.c_uint32), ("coords", C.POINTER(C.c_double)),
+        ("params", C.c_double * 4), ("blocks", C.POINTER(OrcBlock)),
+    ]
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} missing: build it with __graft_entry__.build()")
+        L = C.CDLL(LIB_PATH)
+        PP = C.POINTER(OrcProblem)
+        PD, PU = C.POINTER(C.c_double), C.POINTER(C.c_uint32)
+        for name, args in {
+            "orc_neq": [C.c_int],
+            "orc_dual_volume": [PP, PD],
+            "orc_assemble_rhs": [PP, PD, PD, PD, PD],
+            "orc_cell_residuals": [PP, C.c_int, PD, PD, PD],
+            "orc_iterate": [PP, C.c_double, C.c_int, C.c_uint32, PU, PD, PD, PD, PD, PD],
+            "orc_invert": [C.c_int, PD, PD],
+            "orc_euler_eigen": [C.c_int, C.c_double, PD, PD, PD, PD, PD],
+            "orc_euler_kplus": [C.c_int, C.c_double, PD, PD, PD, PD],
+        }.items():
+            f = getattr(L, name)
+            f.argtypes = args
+            f.restype = C.c_int
+        L.orc_norm_L2.argtypes = [PD, C.c_uint32, C.c_int, PD]
+        L.orc_norm_L2.restype = None
+        _lib = L
+    return _lib
+
+
+def _p(a, t=C.c_double):
+    return a.ctypes.data_as(C.POINTER(t))
+
+
+class Problem:
+    """Mesh + FE tables + physics/scheme for the oracle. ``tables`` maps the
+    shape of each mesh block to its FE tables (pdekit_amd.fe.ElementTables);
+    ``vol_tables`` holds the tables of the dual-volume quadrature (order =
+    cell order)."""
+
+    def __init__(self, mesh, tables, vol_tables, physics, scheme, params=(0.0, 1.0)):
+        self._keep = []
+        blocks = (OrcBlock * len(mesh.blocks))()
+        for i, blk in enumerate(mesh.blocks):
+            t, tv = tables[blk.shape], vol_tables[blk.shape]
+            arrs = {k: np.ascontiguousarray(getattr(t, k), dtype=np.float64)
+                    for k in ("w", "dVg", "V", "dV", "dVf", "Vsf", "Vgf")}
+            vw = np.ascontiguousarray(tv.w, dtype=np.float64)
+            vdg = np.ascontiguousarray(tv.dVg, dtype=np.float64)
+            g = np.ascontiguousarray(blk.geo_conn, dtype=np.uint32)
+            s = np.ascontiguousarray(blk.sol_conn, dtype=np.uint32)
+            self._keep += list(arrs.values()) + [vw, vdg, g, s]
+            b = blocks[i]
+            b.ng, b.nd, b.nf, b.nq, b.vol_nq = t.ng, t.nd, t.nf, t.nq, tv.nq
+            b.ne = g.shape[0]
+            for k, a in arrs.items():
+                setattr(b, k, _p(a))
+            b.vol_w, b.vol_dVg = _p(vw), _p(vdg)
+            b.geo_conn, b.sol_conn = _p(g, C.c_uint32), _p(s, C.c_uint32)
+        coords = np.ascontiguousarray(mesh.coords, dtype=np.float64)
+        self._keep += [coords, blocks]
+        p = OrcProblem()
+        p.dim, p.physics, p.scheme, p.nblocks = mesh.dim, physics, scheme, len(mesh.blocks)
+        p.nv, p.nn = coords.shape[0], mesh.nb_nodes
+        p.coords = _p(coords)
+        for i, v in enumerate(params[:4]):
+            p.params[i] = v
+        p.blocks = blocks
+        self.p = p
+        self.nn = mesh.nb_nodes
+        self.m = lib().orc_neq(physics)
+
+    def dual_volume(self):
+        v = np.zeros(self.nn)
+        rc = lib().orc_dual_volume(C.byref(self.p), _p(v))
+        assert rc == 0
+        return v
+
+    def assemble_rhs(self, u):
+        u = np.ascontiguousarray(u, dtype=np.float64).reshape(self.nn, self.m)
+        vol = self.dual_volume()
+        res = np.zeros((self.nn, self.m))
+        ws = np.zeros(self.nn)
+        rc = lib().orc_assemble_rhs(C.byref(self.p), _p(u), _p(vol), _p(res), _p(ws))
+        return res, ws, rc
+
+    def cell_residuals(self, block, u):
+        u = np.ascontiguousarray(u, dtype=np.float64).reshape(self.nn, self.m)
+        b = self.p.blocks[block]
+        phi = np.zeros((b.ne, b.nd, self.m))
+        wave = np.zeros(b.ne)
+        rc = lib().orc_cell_residuals(C.byref(self.p), block, _p(u), _p(phi), _p(wave))
+        return phi, wave, rc
+
+    def iterate(self, u, cfl, n_iter, dir_nodes=None, dir_values=None):
+        u = np.array(u, dtype=np.float64).reshape(self.nn, self.m)
+        res = np.zeros((self.nn, self.m))
+        dt = np.zeros(self.nn)
+        norms = np.zeros((n_iter, self.m))
+        if dir_nodes is None:
+            dir_nodes = np.zeros(0, dtype=np.uint32)
+            dir_values = np.zeros((0, self.m))
+        dn = np.ascontiguousarray(dir_nodes, dtype=np.uint32)
+        dv = np.ascontiguousarray(np.broadcast_to(dir_values, (dn.size, self.m)), dtype=np.float64)
+        rc = lib().orc_iterate(C.byref(self.p), float(cfl), int(n_iter), dn.size, _p(dn, C.c_uint32), _p(dv), _p(u),
+                               _p(res), _p(dt), _p(norms))
+        return u, res, dt, norms, rc
+
+
+def invert(a):
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    out = np.zeros_like(a)
+    rc = lib().orc_invert(a.shape[0], _p(a), _p(out))
+    assert rc == 0
+    return out
+
+
+def euler_eigen(dim, gamma, u, g):
+    m = dim + 2
+    u = np.ascontiguousarray(u, dtype=np.float64)
+    g = np.ascontiguousarray(g, dtype=np.float64)
+    R, Lm, D = np.zeros((m, m)), np.zeros((m, m)), np.zeros(m)
+    rc = lib().orc_euler_eigen(dim, gamma, _p(u), _p(g), _p(R), _p(Lm), _p(D))
+    return R, Lm, D, rc
+
+
+def euler_kplus(dim, gamma, u, g):
+    m = dim + 2
+    u = np.ascontiguousarray(u, dtype=np.float64)
+    g = np.ascontiguousarray(g, dtype=np.float64)
+    K, e = np.zeros((m, m)), np.zeros(m)
+    rc = lib().orc_euler_kplus(dim, gamma, _p(u), _p(g), _p(K), _p(e))
+    return K, e, rc

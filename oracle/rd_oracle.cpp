@@ -1,0 +1,678 @@
+// CPU oracle of the residual-distribution cell loop -- TEST INFRASTRUCTURE.
+//
+// Plain FP64 restatement of pdekit's explicit continuous RD assembly, written
+// to follow the reference's own steps:
+//   cell loop / scatter   src/solver/rdm/assembly/PGRDMExplicitCellWorker.hpp:340-574
+//   geometry metric       src/interpolation/GeometryMetric.hpp:320-480 (3D 487-615)
+//   solution metric       src/interpolation/SolutionSpaceMetric.hpp:287-327
+//   flux metric (P+1)     src/interpolation/FluxSpaceMetric.hpp:494-746 (3D 753-975)
+//   PGLDA                 src/solver/rdm/cellsplitters/PGLDA.hpp:109-311
+//   PGN                   src/solver/rdm/cellsplitters/PGN.hpp:96-290
+//   PGB (cell-wise theta) src/solver/rdm/cellsplitters/PGB.hpp:314-498
+//   PSI limiter           src/solver/rdm/cellsplitters/LFResidualLimiter.hpp:84-110
+//   physics               src/physics/euler/Euler2DCons.hpp:159-536, Euler3DCons.hpp:159-450,
+//                         src/physics/scalar/Burgers2D.hpp:154-275,
+//                         src/physics/scalar/RotationAdvection2D.hpp:201-336
+//   small inverses        src/math/unary_ops/matrix_unary_ops/MatrixInverse.hpp:113-345
+//   dual volume / dt      src/solver/rdm/RDTimeUpdate.hpp:71-194, RDTimeUpdate.cpp:41-67
+//   explicit update/norm  src/solver/rdm/PGRDMethodExplicit.hpp:416-514,
+//                         src/interpolation/mesh_function/function_ops/MeshFunctionNorm.hpp:92-
+//   strong Dirichlet      src/solver/rdm/bc/StrongDirichletBC.hpp:202-233
+//
+// Pinning status of each part is recorded by the tests that check it; the
+// PSI scheme has no reference counterpart on this path (the limiter formula
+// is the reference's LF limiter applied to the N-scheme residuals) and is
+// pinned only by its invariants (conservation, positivity).
+#include "rd_oracle.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+namespace {
+
+constexpr int MAXM = 5;
+constexpr int MAXN = 20;  // nodes per element
+constexpr int MAXF = 35;  // flux nodes per element
+constexpr int MAXQ = 16;
+
+using Mat = double[MAXM][MAXM];
+
+struct Props
+{
+  double rho = 0, inv_rho = 0, V[3] = {0, 0, 0}, v2 = 0, P = 0, H = 0, a = 0;
+  double gamma = 1.4, gm1 = 0.4;
+  double u = 0;  // scalar state
+  bool bad = false;
+};
+
+int neq_of(int physics)
+{
+  switch (physics) {
+    case ORC_ADVECTION2D:
+    case ORC_BURGERS2D: return 1;
+    case ORC_EULER2D: return 4;
+    case ORC_EULER3D: return 5;
+  }
+  return -1;
+}
+
+// --- physics ---------------------------------------------------------------
+
+// compute_properties (Euler2DCons.hpp:159-218, Euler3DCons.hpp:159-222,
+// Burgers2D.hpp:154-173, RotationAdvection2D.hpp:201-224 with a constant
+// advection velocity)
+void properties(const orc_problem *p, const double *s, Props &pr)
+{
+  switch (p->physics) {
+    case ORC_ADVECTION2D:
+      pr.u = s[0];
+      pr.V[0] = p->params[0];
+      pr.V[1] = p->params[1];
+      return;
+    case ORC_BURGERS2D:
+      pr.u = s[0];
+      pr.V[0] = s[0];
+      pr.V[1] = 1.;
+      return;
+    default: break;
+  }
+  const int D = (p->physics == ORC_EULER2D) ? 2 : 3;
+  pr.gamma = p->params[0];
+  pr.gm1 = pr.gamma - 1.;
+  pr.rho = s[0];
+  pr.inv_rho = 1. / pr.rho;
+  pr.v2 = 0.0;
+  for (int d = 0; d < D; ++d) {
+    pr.V[d] = s[1 + d] * pr.inv_rho;
+  }
+  pr.v2 = (D == 2) ? pr.V[0] * pr.V[0] + pr.V[1] * pr.V[1]
+                   : pr.V[0] * pr.V[0] + pr.V[1] * pr.V[1] + pr.V[2] * pr.V[2];
+  const double rhoE = s[D + 1];
+  pr.P = pr.gm1 * (rhoE - 0.5 * pr.rho * pr.v2);
+  if (pr.P <= 0.) pr.bad = true;
+  const double RT = pr.P * pr.inv_rho;
+  const double E = rhoE * pr.inv_rho;
+  pr.H = E + RT;
+  pr.a = std::sqrt(pr.gamma * RT);
+}
+
+// flux(p, F): F[d][eq] (Euler2DCons.hpp:235-249, Euler3DCons.hpp:241-262,
+// Burgers2D.hpp:187-194, RotationAdvection2D.hpp:236-242)
+void flux(const orc_problem *p, const double *s, const Props &pr, double F[3][MAXM])
+{
+  switch (p->physics) {
+    case ORC_ADVECTION2D:
+      F[0][0] = pr.V[0] * s[0];
+      F[1][0] = pr.V[1] * s[0];
+      return;
+    case ORC_BURGERS2D:
+      F[0][0] = 0.5 * pr.V[0] * s[0];
+      F[1][0] = pr.V[1] * s[0];
+      return;
+    case ORC_EULER2D: {
+      const double rhou = s[1], rhov = s[2];
+      F[0][0] = rhou;
+      F[0][1] = rhou * pr.V[0] + pr.P;
+      F[0][2] = rhou * pr.V[1];
+      F[0][3] = rhou * pr.H;
+      F[1][0] = rhov;
+      F[1][1] = rhov * pr.V[0];
+      F[1][2] = rhov * pr.V[1] + pr.P;
+      F[1][3] = rhov * pr.H;
+      return;
+    }
+    case ORC_EULER3D: {
+      const double m[3] = {s[1], s[2], s[3]};
+      for (int d = 0; d < 3; ++d) {
+        F[d][0] = m[d];
+        for (int k = 0; k < 3; ++k) F[d][1 + k] = m[d] * pr.V[k] + (k == d ? pr.P : 0.0);
+        F[d][4] = m[d] * pr.H;
+      }
+      return;
+    }
+  }
+}
+
+// flux_jacobian_eigen_structure: R, L = R^-1, D (Euler2DCons.hpp:316-398,
+// Euler3DCons.hpp:331-447; scalars: R = L = 1, D = V.g)
+void eigen_structure(int physics, const Props &pr, const double *g, Mat &R, Mat &L, double *D)
+{
+  if (physics == ORC_ADVECTION2D || physics == ORC_BURGERS2D) {
+    R[0][0] = 1.;
+    L[0][0] = 1.;
+    D[0] = pr.V[0] * g[0] + pr.V[1] * g[1];
+    return;
+  }
+  if (physics == ORC_EULER2D) {
+    const double norm = std::sqrt(g[0] * g[0] + g[1] * g[1]);
+    const double nx = (norm < 1.e-9) ? g[0] : g[0] / norm;
+    const double ny = (norm < 1.e-9) ? g[1] : g[1] / norm;
+    const double u = pr.V[0], v = pr.V[1], a = pr.a, rho = pr.rho, ir = pr.inv_rho;
+    const double inv_a = 1. / a, inv_a2 = inv_a * inv_a;
+    const double um = u * nx + v * ny;
+    const double ra = 0.5 * rho * inv_a;
+    const double coeffM2 = (0.5 * (pr.gamma - 1.0) * pr.v2) * inv_a2;
+    const double uDivA = pr.gm1 * u * inv_a, vDivA = pr.gm1 * v * inv_a;
+    const double gm1_ov_rhoa = pr.gm1 / (rho * a);
+    const double Rv[4][4] = {{1., 0., ra, ra},
+                             {u, rho * ny, ra * (u + a * nx), ra * (u - a * nx)},
+                             {v, -rho * nx, ra * (v + a * ny), ra * (v - a * ny)},
+                             {0.5 * pr.v2, rho * (u * ny - v * nx), ra * (pr.H + a * um), ra * (pr.H - a * um)}};
+    const double Lv[4][4] = {{1. - coeffM2, uDivA * inv_a, vDivA * inv_a, -pr.gm1 * inv_a2},
+                             {ir * (v * nx - u * ny), ir * ny, -ir * nx, 0.0},
+                             {a * ir * (coeffM2 - um * inv_a), ir * (nx - uDivA), ir * (ny - vDivA), gm1_ov_rhoa},
+                             {a * ir * (coeffM2 + um * inv_a), -ir * (nx + uDivA), -ir * (ny + vDivA), gm1_ov_rhoa}};
+    for (int i = 0; i < 4; ++i)
+      for (int j = 0; j < 4; ++j) { R[i][j] = Rv[i][j]; L[i][j] = Lv[i][j]; }
+    D[0] = norm * um;
+    D[1] = norm * um;
+    D[2] = norm * (um + a);
+    D[3] = norm * (um - a);
+    return;
+  }
+  // 3D
+  const double norm = std::sqrt(g[0] * g[0] + g[1] * g[1] + g[2] * g[2]);
+  const double nx = (norm == 0) ? g[0] : g[0] / norm;
+  const double ny = (norm == 0) ? g[1] : g[1] / norm;
+  const double nz = (norm == 0) ? g[2] : g[2] / norm;
+  const double u = pr.V[0], v = pr.V[1], w = pr.V[2], a = pr.a, rho = pr.rho, ir = pr.inv_rho;
+  const double inv_a = 1. / a, inv_a2 = inv_a * inv_a;
+  const double um = u * nx + v * ny + w * nz;
+  const double ra = 0.5 * rho * inv_a;
+  const double gu_a = pr.gm1 * u * inv_a, gv_a = pr.gm1 * v * inv_a, gw_a = pr.gm1 * w * inv_a;
+  const double gm1_ov_rhoa = pr.gm1 / (rho * a);
+  const double ke = 0.5 * pr.v2;
+  const double gm2 = (0.5 * pr.gm1 * pr.v2) * inv_a2;
+  const double k2 = 1.0 - gm2;
+  const double k3 = -pr.gm1 * inv_a2;
+  const double Rv[5][5] = {
+      {nx, ny, nz, ra, ra},
+      {u * nx, u * ny - rho * nz, u * nz + rho * ny, ra * (u + a * nx), ra * (u - a * nx)},
+      {v * nx + rho * nz, v * ny, v * nz - rho * nx, ra * (v + a * ny), ra * (v - a * ny)},
+      {w * nx - rho * ny, w * ny + rho * nx, w * nz, ra * (w + a * nz), ra * (w - a * nz)},
+      {ke * nx + rho * (v * nz - w * ny), ke * ny + rho * (w * nx - u * nz), ke * nz + rho * (u * ny - v * nx),
+       ra * (pr.H + a * um), ra * (pr.H - a * um)}};
+  const double Lv[5][5] = {
+      {nx * k2 - ir * (v * nz - w * ny), gu_a * inv_a * nx, gv_a * inv_a * nx + nz * ir, gw_a * inv_a * nx - ny * ir, k3 * nx},
+      {ny * k2 - ir * (w * nx - u * nz), gu_a * inv_a * ny - nz * ir, gv_a * inv_a * ny, gw_a * inv_a * ny + nx * ir, k3 * ny},
+      {nz * k2 - ir * (u * ny - v * nx), gu_a * inv_a * nz + ny * ir, gv_a * inv_a * nz - nx * ir, gw_a * inv_a * nz, k3 * nz},
+      {a * ir * (gm2 - um / a), ir * (nx - gu_a), ir * (ny - gv_a), ir * (nz - gw_a), gm1_ov_rhoa},
+      {a * ir * (gm2 + um / a), ir * (-nx - gu_a), ir * (-ny - gv_a), ir * (-nz - gw_a), gm1_ov_rhoa}};
+  for (int i = 0; i < 5; ++i)
+    for (int j = 0; j < 5; ++j) { R[i][j] = Rv[i][j]; L[i][j] = Lv[i][j]; }
+  D[0] = norm * um;
+  D[1] = norm * um;
+  D[2] = norm * um;
+  D[3] = norm * (um + a);
+  D[4] = norm * (um - a);
+}
+
+// K = R diag(d) L, evaluated as (R d) L like the reference's Rv*Dvp*Lv and
+// build_K_mat (PGN.hpp:203-205, Euler2DCons.hpp:487-536).
+void product_RDL(int m, const Mat &R, const double *d, const Mat &L, Mat &K)
+{
+  for (int i = 0; i < m; ++i)
+    for (int j = 0; j < m; ++j) {
+      double s = 0.0;
+      for (int k = 0; k < m; ++k) s += R[i][k] * d[k] * L[k][j];
+      K[i][j] = s;
+    }
+}
+
+// --- small dense inverse -----------------------------------------------------
+
+// (sum K+)^-1: 1x1 reciprocal, 2x2/3x3 adjugate, 4x4 adjugate by cofactor
+// expansion (the reference uses a closed-form cofactor inverse up to 4x4,
+// MatrixInverse.hpp:113-300), 5x5 partial-pivot Gauss-Jordan (the reference
+// calls LAPACK dgetrf/dgetri, MatrixInverse.hpp:77-104).
+double det3(const double *a, int s0, int s1, int s2, int c0, int c1, int c2, int n)
+{
+  auto A = [&](int r, int c) { return a[r * n + c]; };
+  return A(s0, c0) * (A(s1, c1) * A(s2, c2) - A(s1, c2) * A(s2, c1)) -
+         A(s0, c1) * (A(s1, c0) * A(s2, c2) - A(s1, c2) * A(s2, c0)) +
+         A(s0, c2) * (A(s1, c0) * A(s2, c1) - A(s1, c1) * A(s2, c0));
+}
+
+int invert(int n, const double *a, double *inv)
+{
+  if (n == 1) {
+    inv[0] = 1.0 / a[0];
+    return 0;
+  }
+  if (n == 2) {
+    const double det = a[0] * a[3] - a[1] * a[2];
+    const double id = 1. / det;
+    inv[0] = a[3] * id;
+    inv[1] = -a[1] * id;
+    inv[2] = -a[2] * id;
+    inv[3] = a[0] * id;
+    return 0;
+  }
+  if (n == 3) {
+    const double det = det3(a, 0, 1, 2, 0, 1, 2, 3);
+    const double id = 1. / det;
+    inv[0] = (a[4] * a[8] - a[5] * a[7]) * id;
+    inv[1] = -(a[1] * a[8] - a[2] * a[7]) * id;
+    inv[2] = (a[1] * a[5] - a[4] * a[2]) * id;
+    inv[3] = -(a[3] * a[8] - a[5] * a[6]) * id;
+    inv[4] = (a[0] * a[8] - a[2] * a[6]) * id;
+    inv[5] = -(a[0] * a[5] - a[2] * a[3]) * id;
+    inv[6] = (a[3] * a[7] - a[4] * a[6]) * id;
+    inv[7] = -(a[0] * a[7] - a[1] * a[6]) * id;
+    inv[8] = (a[0] * a[4] - a[1] * a[3]) * id;
+    return 0;
+  }
+  if (n == 4) {
+    // cofactors C(i,j) = (-1)^(i+j) * minor(i,j); inv = C^T / det
+    double C[16];
+    for (int i = 0; i < 4; ++i)
+      for (int j = 0; j < 4; ++j) {
+        int r[3], c[3], kr = 0, kc = 0;
+        for (int t = 0; t < 4; ++t) {
+          if (t != i) r[kr++] = t;
+          if (t != j) c[kc++] = t;
+        }
+        const double mnr = det3(a, r[0], r[1], r[2], c[0], c[1], c[2], 4);
+        C[i * 4 + j] = ((i + j) % 2 == 0) ? mnr : -mnr;
+      }
+    const double det = a[0] * C[0] + a[1] * C[1] + a[2] * C[2] + a[3] * C[3];
+    const double id = 1. / det;
+    for (int i = 0; i < 4; ++i)
+      for (int j = 0; j < 4; ++j) inv[i * 4 + j] = C[j * 4 + i] * id;
+    return 0;
+  }
+  std::vector<double> w(a, a + n * n);
+  for (int i = 0; i < n * n; ++i) inv[i] = 0.0;
+  for (int i = 0; i < n; ++i) inv[i * n + i] = 1.0;
+  for (int c = 0; c < n; ++c) {
+    int piv = c;
+    for (int r = c + 1; r < n; ++r)
+      if (std::fabs(w[r * n + c]) > std::fabs(w[piv * n + c])) piv = r;
+    if (w[piv * n + c] == 0.0) return -1;
+    if (piv != c)
+      for (int k = 0; k < n; ++k) {
+        std::swap(w[c * n + k], w[piv * n + k]);
+        std::swap(inv[c * n + k], inv[piv * n + k]);
+      }
+    const double d = 1.0 / w[c * n + c];
+    for (int k = 0; k < n; ++k) { w[c * n + k] *= d; inv[c * n + k] *= d; }
+    for (int r = 0; r < n; ++r) {
+      if (r == c) continue;
+      const double f = w[r * n + c];
+      for (int k = 0; k < n; ++k) { w[r * n + k] -= f * w[c * n + k]; inv[r * n + k] -= f * inv[c * n + k]; }
+    }
+  }
+  return 0;
+}
+
+// --- the cell ------------------------------------------------------------------
+
+struct CellOut
+{
+  double phi[MAXN][MAXM];
+  double wave;  // element wave speed (volume * max lambda)
+  bool bad;
+};
+
+void cell(const orc_problem *p, const orc_block &b, uint32_t c, const double *u, CellOut &out)
+{
+  const int D = p->dim, m = neq_of(p->physics);
+  const int ng = b.ng, nd = b.nd, nf = b.nf, nq = b.nq;
+  double X[8][3];
+  double U[MAXN][MAXM];
+  for (int g = 0; g < ng; ++g) {
+    const uint32_t v = b.geo_conn[static_cast<size_t>(c) * ng + g];
+    for (int d = 0; d < D; ++d) X[g][d] = p->coords[static_cast<size_t>(v) * D + d];
+  }
+  for (int n = 0; n < nd; ++n) {
+    const uint32_t v = b.sol_conn[static_cast<size_t>(c) * nd + n];
+    for (int e = 0; e < m; ++e) U[n][e] = u[static_cast<size_t>(v) * m + e];
+  }
+  out.bad = false;
+
+  // Geometry metric: J(xi,x) = dVg_xi * X, det J, J^-1 (GeometryMetric.hpp:366-425)
+  double detJ[MAXQ], invJ[MAXQ][3][3];
+  for (int q = 0; q < nq; ++q) {
+    double J[9];
+    for (int xi = 0; xi < D; ++xi)
+      for (int x = 0; x < D; ++x) {
+        double s = 0.0;
+        for (int g = 0; g < ng; ++g) s += b.dVg[(static_cast<size_t>(xi) * nq + q) * ng + g] * X[g][x];
+        J[xi * D + x] = s;
+      }
+    if (D == 2) {
+      detJ[q] = J[0] * J[3] - J[1] * J[2];
+      const double id = 1.0 / detJ[q];
+      invJ[q][0][0] = id * J[3];
+      invJ[q][1][1] = id * J[0];
+      invJ[q][0][1] = -id * J[1];
+      invJ[q][1][0] = -id * J[2];
+    } else {
+      detJ[q] = det3(J, 0, 1, 2, 0, 1, 2, 3);
+      double inv[9];
+      invert(3, J, inv);
+      for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) invJ[q][i][j] = inv[i * 3 + j];
+    }
+  }
+
+  // Solution values at quadrature points: uq = V * U (SolutionSpaceMetric.hpp:316-321)
+  double uq[MAXQ][MAXM];
+  for (int q = 0; q < nq; ++q)
+    for (int e = 0; e < m; ++e) {
+      double s = 0.0;
+      for (int n = 0; n < nd; ++n) s += b.V[q * nd + n] * U[n][e];
+      uq[q][e] = s;
+    }
+
+  // Flux metric: solution at P+1 flux nodes, fluxes there, reference
+  // derivatives dVf * F, mapped with J^-1 at the quadrature point; keep
+  // dF_icomp/dx_icomp (FluxSpaceMetric.hpp:562-743)
+  double Fn[MAXF][3][MAXM];
+  for (int f = 0; f < nf; ++f) {
+    double sf[MAXM];
+    for (int e = 0; e < m; ++e) {
+      double s = 0.0;
+      for (int n = 0; n < nd; ++n) s += b.Vsf[f * nd + n] * U[n][e];
+      sf[e] = s;
+    }
+    Props pr;
+    properties(p, sf, pr);
+    if (pr.bad) out.bad = true;
+    double F[3][MAXM];
+    flux(p, sf, pr, F);
+    for (int d = 0; d < D; ++d)
+      for (int e = 0; e < m; ++e) Fn[f][d][e] = F[d][e];
+  }
+  double divF[MAXQ][MAXM];
+  for (int q = 0; q < nq; ++q)
+    for (int e = 0; e < m; ++e) divF[q][e] = 0.0;
+  for (int icomp = 0; icomp < D; ++icomp) {
+    for (int q = 0; q < nq; ++q)
+      for (int e = 0; e < m; ++e) {
+        double dref[3];
+        for (int xi = 0; xi < D; ++xi) {
+          double s = 0.0;
+          for (int f = 0; f < nf; ++f) s += b.dVf[(static_cast<size_t>(xi) * nq + q) * nf + f] * Fn[f][icomp][e];
+          dref[xi] = s;
+        }
+        double dphys = 0.0;
+        for (int xi = 0; xi < D; ++xi) dphys += invJ[q][icomp][xi] * dref[xi];
+        divF[q][e] += dphys;  // res_at_point = sum_dim grad_F[dim](q)
+      }
+  }
+
+  // Scheme (PGLDA.hpp:156-311, PGN.hpp:141-290, PGB.hpp:314-498)
+  for (int n = 0; n < nd; ++n)
+    for (int e = 0; e < m; ++e) out.phi[n][e] = 0.0;
+  double lda[MAXN][MAXM], ndis[MAXN][MAXM];
+  for (int n = 0; n < nd; ++n)
+    for (int e = 0; e < m; ++e) { lda[n][e] = 0.0; ndis[n][e] = 0.0; }
+  double ws[MAXN];
+  for (int n = 0; n < nd; ++n) ws[n] = 0.0;
+  double elem_volume = 0.0;
+  double flux_integral[MAXM] = {0, 0, 0, 0, 0};
+
+  static thread_local Mat Kp[MAXN];
+  for (int q = 0; q < nq; ++q) {
+    const double wj = b.w[q] * detJ[q];
+    elem_volume += wj;
+    Props pr;
+    properties(p, uq[q], pr);
+    if (pr.bad) out.bad = true;
+    Mat sumKp;
+    for (int i = 0; i < m; ++i)
+      for (int j = 0; j < m; ++j) sumKp[i][j] = 0.0;
+    for (int n = 0; n < nd; ++n) {
+      double gref[3], g[3];
+      for (int xi = 0; xi < D; ++xi) gref[xi] = b.dV[(static_cast<size_t>(xi) * nq + q) * nd + n];
+      for (int x = 0; x < D; ++x) {
+        double s = 0.0;
+        for (int xi = 0; xi < D; ++xi) s += invJ[q][x][xi] * gref[xi];
+        g[x] = s;
+      }
+      Mat R, L;
+      double Dv[MAXM], Dp[MAXM];
+      eigen_structure(p->physics, pr, g, R, L, Dv);
+      double lmax = 1.e-6;
+      for (int e = 0; e < m; ++e) {
+        lmax = std::max(lmax, std::fabs(Dv[e]));
+        Dp[e] = std::max(0.0, Dv[e]);
+      }
+      ws[n] = std::max(ws[n], lmax);
+      product_RDL(m, R, Dp, L, Kp[n]);
+      for (int i = 0; i < m; ++i)
+        for (int j = 0; j < m; ++j) sumKp[i][j] += Kp[n][i][j];
+    }
+    double a[MAXM * MAXM], inv[MAXM * MAXM];
+    for (int i = 0; i < m; ++i)
+      for (int j = 0; j < m; ++j) a[i * m + j] = sumKp[i][j];
+    invert(m, a, inv);
+    const double *r = divF[q];
+    for (int e = 0; e < m; ++e) flux_integral[e] += wj * r[e];
+    // K+_n (sum K+)^-1 r
+    double Ir[MAXM];
+    for (int i = 0; i < m; ++i) {
+      double s = 0.0;
+      for (int j = 0; j < m; ++j) s += inv[i * m + j] * r[j];
+      Ir[i] = s;
+    }
+    for (int n = 0; n < nd; ++n) {
+      // (Kp[n] * inv) * r as the expression template evaluates it
+      double KI[MAXM][MAXM];
+      for (int i = 0; i < m; ++i)
+        for (int j = 0; j < m; ++j) {
+          double s = 0.0;
+          for (int k = 0; k < m; ++k) s += Kp[n][i][k] * inv[k * m + j];
+          KI[i][j] = s;
+        }
+      for (int i = 0; i < m; ++i) {
+        double s = 0.0;
+        for (int j = 0; j < m; ++j) s += KI[i][j] * r[j];
+        lda[n][i] += wj * s;
+      }
+      if (p->scheme == ORC_LDA) continue;
+      // dissipation: sum_{k != n} Kp[n] inv Kp[k] (U_n - U_k)
+      for (int k = 0; k < nd; ++k) {
+        if (k == n) continue;
+        double du[MAXM], t[MAXM];
+        for (int e = 0; e < m; ++e) du[e] = U[n][e] - U[k][e];
+        for (int i = 0; i < m; ++i) {
+          double s = 0.0;
+          for (int j = 0; j < m; ++j) s += Kp[k][i][j] * du[j];
+          t[i] = s;
+        }
+        for (int i = 0; i < m; ++i) {
+          double s = 0.0;
+          for (int j = 0; j < m; ++j) s += KI[i][j] * t[j];
+          ndis[n][i] += wj * s;
+        }
+      }
+    }
+    (void)Ir;
+  }
+
+  if (p->scheme == ORC_LDA) {
+    for (int n = 0; n < nd; ++n)
+      for (int e = 0; e < m; ++e) out.phi[n][e] = lda[n][e];
+  } else if (p->scheme == ORC_N) {
+    for (int n = 0; n < nd; ++n)
+      for (int e = 0; e < m; ++e) out.phi[n][e] = lda[n][e] + ndis[n][e];
+  } else if (p->scheme == ORC_B) {
+    // cell-wise theta from equation 0 only, +1e-6 in the denominator
+    double theta[MAXM];
+    for (int e = 0; e < m; ++e) {
+      double abs_phiN = 0.0;
+      for (int n = 0; n < nd; ++n) abs_phiN += std::fabs(lda[n][e] + ndis[n][e]);
+      theta[e] = std::fabs(flux_integral[e]) / (abs_phiN + 1.e-6);
+    }
+    for (int n = 0; n < nd; ++n)
+      for (int e = 0; e < m; ++e) out.phi[n][e] = lda[n][e] + theta[0] * ndis[n][e];
+  } else {
+    // PSI: the limiter x_i = max(phi_i^N / phi, 0) + 1e-12 applied to the N
+    // residuals, per equation, when |phi| > 1e-14 (LFResidualLimiter.hpp:84-110)
+    for (int n = 0; n < nd; ++n)
+      for (int e = 0; e < m; ++e) out.phi[n][e] = lda[n][e] + ndis[n][e];
+    for (int e = 0; e < m; ++e) {
+      const double phi = flux_integral[e];
+      if (std::fabs(phi) > 1.e-14) {
+        double sum = 0.0;
+        for (int n = 0; n < nd; ++n) {
+          out.phi[n][e] = std::max(out.phi[n][e] / phi, 0.0) + 1.e-12;
+          sum += out.phi[n][e];
+        }
+        for (int n = 0; n < nd; ++n) out.phi[n][e] = out.phi[n][e] / sum * phi;
+      }
+    }
+  }
+  double wmax = ws[0];
+  for (int n = 1; n < nd; ++n) wmax = std::max(wmax, ws[n]);
+  out.wave = elem_volume * wmax;
+}
+
+}  // namespace
+
+extern "C" {
+
+int orc_neq(int physics) { return neq_of(physics); }
+
+int orc_invert(int n, const double *a, double *inv) { return invert(n, a, inv); }
+
+int orc_euler_eigen(int dim, double gamma, const double *u, const double *g, double *R, double *L, double *D)
+{
+  orc_problem p{};
+  p.dim = dim;
+  p.physics = (dim == 2) ? ORC_EULER2D : ORC_EULER3D;
+  p.params[0] = gamma;
+  Props pr;
+  properties(&p, u, pr);
+  Mat Rm, Lm;
+  eigen_structure(p.physics, pr, g, Rm, Lm, D);
+  const int m = dim + 2;
+  for (int i = 0; i < m; ++i)
+    for (int j = 0; j < m; ++j) { R[i * m + j] = Rm[i][j]; L[i * m + j] = Lm[i][j]; }
+  return pr.bad ? -5 : 0;
+}
+
+int orc_euler_kplus(int dim, double gamma, const double *u, const double *g, double *K, double *eig)
+{
+  const int m = dim + 2;
+  double R[25], L[25], D[5];
+  const int rc = orc_euler_eigen(dim, gamma, u, g, R, L, D);
+  Mat Rm, Lm, Km;
+  double Dp[5] = {0, 0, 0, 0, 0};
+  for (int i = 0; i < m; ++i) {
+    eig[i] = D[i];
+    Dp[i] = std::max(0.0, D[i]);
+    for (int j = 0; j < m; ++j) { Rm[i][j] = R[i * m + j]; Lm[i][j] = L[i * m + j]; }
+  }
+  product_RDL(m, Rm, Dp, Lm, Km);
+  for (int i = 0; i < m; ++i)
+    for (int j = 0; j < m; ++j) K[i * m + j] = Km[i][j];
+  return rc;
+}
+
+int orc_dual_volume(const orc_problem *p, double *vdual)
+{
+  const int D = p->dim;
+  for (uint32_t i = 0; i < p->nn; ++i) vdual[i] = 0.0;
+  for (int bi = 0; bi < p->nblocks; ++bi) {
+    const orc_block &b = p->blocks[bi];
+    for (uint32_t c = 0; c < b.ne; ++c) {
+      double X[8][3];
+      for (int g = 0; g < b.ng; ++g) {
+        const uint32_t v = b.geo_conn[static_cast<size_t>(c) * b.ng + g];
+        for (int d = 0; d < D; ++d) X[g][d] = p->coords[static_cast<size_t>(v) * D + d];
+      }
+      double vol = 0.0;
+      for (int q = 0; q < b.vol_nq; ++q) {
+        double J[9];
+        for (int xi = 0; xi < D; ++xi)
+          for (int x = 0; x < D; ++x) {
+            double s = 0.0;
+            for (int g = 0; g < b.ng; ++g) s += b.vol_dVg[(static_cast<size_t>(xi) * b.vol_nq + q) * b.ng + g] * X[g][x];
+            J[xi * D + x] = s;
+          }
+        const double det = (D == 2) ? J[0] * J[3] - J[1] * J[2] : det3(J, 0, 1, 2, 0, 1, 2, 3);
+        vol += b.vol_w[q] * det;
+      }
+      for (int n = 0; n < b.nd; ++n) vdual[b.sol_conn[static_cast<size_t>(c) * b.nd + n]] += vol;
+    }
+  }
+  return 0;
+}
+
+int orc_cell_residuals(const orc_problem *p, int block, const double *u, double *phi, double *wave)
+{
+  const int m = neq_of(p->physics);
+  const orc_block &b = p->blocks[block];
+  int rc = 0;
+  CellOut out;
+  for (uint32_t c = 0; c < b.ne; ++c) {
+    cell(p, b, c, u, out);
+    if (out.bad) rc = -5;
+    for (int n = 0; n < b.nd; ++n)
+      for (int e = 0; e < m; ++e) phi[(static_cast<size_t>(c) * b.nd + n) * m + e] = out.phi[n][e];
+    wave[c] = out.wave;
+  }
+  return rc;
+}
+
+int orc_assemble_rhs(const orc_problem *p, const double *u, const double *vdual, double *res, double *ws)
+{
+  const int m = neq_of(p->physics);
+  if (m < 0) return -1;
+  std::memset(res, 0, sizeof(double) * p->nn * m);
+  std::memset(ws, 0, sizeof(double) * p->nn);
+  int rc = 0;
+  CellOut out;
+  for (int bi = 0; bi < p->nblocks; ++bi) {
+    const orc_block &b = p->blocks[bi];
+    for (uint32_t c = 0; c < b.ne; ++c) {
+      cell(p, b, c, u, out);
+      if (out.bad) rc = -5;
+      for (int n = 0; n < b.nd; ++n) {
+        const uint32_t v = b.sol_conn[static_cast<size_t>(c) * b.nd + n];
+        const double inv_vol = 1. / vdual[v];
+        for (int e = 0; e < m; ++e) res[static_cast<size_t>(v) * m + e] += inv_vol * out.phi[n][e];
+        ws[v] += out.wave;
+      }
+    }
+  }
+  return rc;
+}
+
+void orc_norm_L2(const double *f, uint32_t n, int m, double *out)
+{
+  for (int e = 0; e < m; ++e) out[e] = 0.0;
+  for (uint32_t i = 0; i < n; ++i)
+    for (int e = 0; e < m; ++e) out[e] += f[static_cast<size_t>(i) * m + e] * f[static_cast<size_t>(i) * m + e];
+  for (int e = 0; e < m; ++e) out[e] = std::sqrt(out[e] / n);
+}
+
+int orc_iterate(const orc_problem *p, double cfl, int n_iter, uint32_t nb_dirichlet, const uint32_t *dnodes,
+                const double *dvalues, double *u, double *res, double *dt, double *norms)
+{
+  const int m = neq_of(p->physics);
+  std::vector<double> vdual(p->nn), ws(p->nn);
+  orc_dual_volume(p, vdual.data());
+  int rc = 0;
+  for (int it = 0; it < n_iter; ++it) {
+    // assemble_rhs: cell loop, then the boundary conditions
+    const int r = orc_assemble_rhs(p, u, vdual.data(), res, ws.data());
+    if (r) rc = r;
+    for (uint32_t k = 0; k < nb_dirichlet; ++k)
+      for (int e = 0; e < m; ++e) u[static_cast<size_t>(dnodes[k]) * m + e] = dvalues[static_cast<size_t>(k) * m + e];
+    // compute_local_time_step
+    for (uint32_t i = 0; i < p->nn; ++i) dt[i] = cfl * vdual[i] / ws[i];
+    // solve: u -= dt * res
+    for (uint32_t i = 0; i < p->nn; ++i)
+      for (int e = 0; e < m; ++e) u[static_cast<size_t>(i) * m + e] -= dt[i] * res[static_cast<size_t>(i) * m + e];
+    if (norms) orc_norm_L2(res, p->nn, m, norms + static_cast<size_t>(it) * m);
+  }
+  return rc;
+}
+
+}  // extern "C"

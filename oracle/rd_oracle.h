@@ -1,0 +1,85 @@
+/*
+ * CPU oracle for the residual-distribution cell loop -- TEST INFRASTRUCTURE.
+ *
+ * A plain FP64 restatement of pdekit's explicit continuous RD residual
+ * assembly (PGRDMethodExplicit + PGRDMExplicitCellWorker + the PGLDA / PGN /
+ * PGB element schemes + RDTimeUpdate + StrongDirichletBC), used only as the
+ * checker of the GPU path (tests, smoke check, CPU baseline of the bench).
+ * It shares no code with the GPU library; it receives the same mesh and FE
+ * tables as inputs.
+ */
+#ifndef PDRD_ORACLE_H
+#define PDRD_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum { ORC_ADVECTION2D = 0, ORC_BURGERS2D = 1, ORC_EULER2D = 2, ORC_EULER3D = 3 };
+enum { ORC_LDA = 0, ORC_N = 1, ORC_B = 2, ORC_PSI = 3 };
+
+/* One block of cells of a single element type. Tables row-major:
+ * w[nq], dVg[dim][nq][ng], V[nq][nd], dV[dim][nq][nd], dVf[dim][nq][nf],
+ * Vsf[nf][nd], Vgf[nf][ng]; connectivity geo[ne][ng], sol[ne][nd]. */
+typedef struct orc_block
+{
+  int ng, nd, nf, nq;
+  int vol_nq; /* dual-volume quadrature (order = cell order) */
+  uint32_t ne;
+  const double *w, *dVg, *V, *dV, *dVf, *Vsf, *Vgf;
+  const double *vol_w, *vol_dVg; /* [vol_nq], [dim][vol_nq][ng] */
+  const uint32_t *geo_conn, *sol_conn;
+} orc_block;
+
+typedef struct orc_problem
+{
+  int dim, physics, scheme, nblocks;
+  uint32_t nv, nn;
+  const double *coords; /* [nv][dim] */
+  double params[4];     /* advection: ax, ay; Euler: gamma, R */
+  const orc_block *blocks;
+} orc_problem;
+
+int orc_neq(int physics);
+
+/* RDTimeUpdate::setup: dual volume V_i = sum of the volumes of all cells that
+ * hold node i. */
+int orc_dual_volume(const orc_problem *p, double *vdual);
+
+/* One assemble_rhs sweep: res[N][m] = sum_cells Phi/V_dual, ws[N] = sum of
+ * element wave speeds. Returns 0, or -5 if a non-positive pressure was met. */
+int orc_assemble_rhs(const orc_problem *p, const double *u, const double *vdual, double *res,
+                     double *ws);
+
+/* Per-cell residuals and wave speed of one block (before the scatter):
+ * phi[ne][nd][m], wave[ne]. */
+int orc_cell_residuals(const orc_problem *p, int block, const double *u, double *phi,
+                       double *wave);
+
+/* n explicit iterations of assemble_lhs_and_rhs(CFL) + solve() with strong
+ * Dirichlet values g[nb][m] written into u after every cell loop
+ * (StrongDirichletBC::apply). norms[n][m] receives norm_L2(residual) over all
+ * nodes per iteration when non-null. On return u holds the new state and
+ * res / dt the last residual and time step. */
+int orc_iterate(const orc_problem *p, double cfl, int n_iter, uint32_t nb_dirichlet,
+                const uint32_t *dirichlet_nodes, const double *dirichlet_values, double *u,
+                double *res, double *dt, double *norms);
+
+/* norm_L2 per field: sqrt(sum_i r_i^2 / N) (MeshFunctionNorm.hpp). */
+void orc_norm_L2(const double *f, uint32_t n, int m, double *out);
+
+/* Small dense helpers exposed for the invariant tests. */
+int orc_invert(int n, const double *a, double *inv);
+/* Euler eigen-structure at state u, direction g: R, L, D (diagonal) */
+int orc_euler_eigen(int dim, double gamma, const double *u, const double *g, double *R, double *L,
+                    double *D);
+/* Closed-form K+ (PGLDA's build_K_mat with max(0, .)) */
+int orc_euler_kplus(int dim, double gamma, const double *u, const double *g, double *K, double *eig);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,15 @@
+// Cell-kernel instantiations for one physics (compiled as its own unit so the
+// instantiations build in parallel).
+#include "rd_launch.hpp"
+
+namespace pdrd {
+
+#define PHYS_ PH_ADVECTION2D
+CellLauncher find_cell_scalar_adv(const CellKey &k)
+{
+  PDRD_ELEMENT_TYPES(PDRD_CELL_CASE)
+  return nullptr;
+}
+#undef PHYS_
+
+}  // namespace pdrd

@@ -1,0 +1,863 @@
+// The explicit continuous RD method on one device: the C ABI entry points
+// declared in include/pdekit_rd.h that own device state, the nodal passes
+// (dual-volume scaling, time step, update, norms) and HIP-graph replay of the
+// pseudo-time iteration.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../../include/pdekit_rd.h"
+#include "../host/capi_common.hpp"
+#include "../host/fe_tables.hpp"
+#include "../host/mesh.hpp"
+#include "rd_launch.hpp"
+
+using pdrd::set_error;
+
+namespace {
+
+constexpr int kNormBlock = 256;
+constexpr int kNormMaxBlocks = 1024;
+
+struct Block
+{
+  pdrd_fe_desc fe;
+  uint32_t ne = 0;
+  std::vector<double> tables;
+  std::vector<uint32_t> geo, sol;      // host copies, [ne][ng] / [ne][nd] (caller order)
+  std::vector<uint32_t> perm;          // block position -> caller cell index
+  std::vector<uint32_t> colour_start;  // [ncolours + 1]
+  double *d_tab = nullptr;
+  uint32_t *d_gconn = nullptr;  // [ng][ne] in colour-sorted order
+  uint32_t *d_sconn = nullptr;  // [nd][ne]
+  pdrd::CellLauncher launch = nullptr;
+};
+
+}  // namespace
+
+struct pdrd_method
+{
+  int dim = 2, physics = 0, scheme = 0, m = 1;
+  uint32_t nv = 0, nn = 0;
+  double params[4] = {0, 0, 0, 0};
+  std::vector<double> coords;
+  std::vector<Block> blocks;
+  bool ready = false;
+  int device = 0;
+
+  // options
+  int scatter_mode = 0;  // 0 atomics, 1 colours
+  int precision = 0;
+  int block_threads = 256;
+
+  // device state
+  double *d_coords = nullptr, *d_u = nullptr, *d_res = nullptr, *d_dt = nullptr, *d_ws = nullptr;
+  double *d_vol = nullptr, *d_acc_res = nullptr, *d_acc_ws = nullptr, *d_partial = nullptr;
+  int *d_status = nullptr;
+  uint32_t *d_dir_nodes = nullptr;
+  double *d_dir_vals = nullptr;
+  unsigned char *d_is_dir = nullptr;
+  uint32_t n_dir = 0;
+  size_t bytes = 0;
+
+  hipStream_t stream = nullptr;
+  bool own_stream = false;
+  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+  double last_cell_ms = 0.0, last_node_ms = 0.0;
+
+  // cached graph of one iteration without outputs
+  hipGraphExec_t graph = nullptr;
+  double graph_cfl = -1.0;
+};
+
+namespace {
+
+// ---- nodal kernels -------------------------------------------------------------
+
+// residual = acc / V, wave speed = acc_ws; the accumulators are cleared for
+// the next cell loop.
+__global__ void finalize_kernel(uint32_t nn, int m, double *__restrict__ acc_res, double *__restrict__ acc_ws,
+                                const double *__restrict__ vol, double *__restrict__ res, double *__restrict__ ws)
+{
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nn) return;
+  const double iv = 1.0 / vol[i];
+  for (int k = 0; k < m; ++k) {
+    const size_t j = static_cast<size_t>(i) * m + k;
+    res[j] = acc_res[j] * iv;
+    acc_res[j] = 0.0;
+  }
+  ws[i] = acc_ws[i];
+  acc_ws[i] = 0.0;
+}
+
+__global__ void dirichlet_kernel(uint32_t n, int m, const uint32_t *__restrict__ nodes,
+                                 const double *__restrict__ vals, double *__restrict__ u)
+{
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const size_t b = static_cast<size_t>(nodes[i]) * m;
+  for (int k = 0; k < m; ++k) u[b + k] = vals[static_cast<size_t>(i) * m + k];
+}
+
+__global__ void dt_kernel(uint32_t nn, double cfl, const double *__restrict__ vol, const double *__restrict__ ws,
+                          double *__restrict__ dt)
+{
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nn) return;
+  dt[i] = cfl * vol[i] / ws[i];
+}
+
+__global__ void update_kernel(uint32_t nn, int m, const double *__restrict__ dt, const double *__restrict__ res,
+                              double *__restrict__ u)
+{
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nn) return;
+  for (int k = 0; k < m; ++k) {
+    const size_t j = static_cast<size_t>(i) * m + k;
+    u[j] -= dt[i] * res[j];
+  }
+}
+
+// Fused nodal pass of one pseudo-time iteration (after the cell loop and the
+// strong Dirichlet values): res = acc / V, dt = CFL V / sum lambda,
+// u -= dt res, accumulators cleared. When `outputs` is set, res / dt / ws are
+// stored and per-block partial sums of res^2 (field-wise, optionally without
+// Dirichlet rows) and the count of summed rows are written to `partial`.
+template <int M>
+__global__ __launch_bounds__(kNormBlock) void node_pass_kernel(uint32_t nn, double cfl, double *__restrict__ acc_res,
+                                                               double *__restrict__ acc_ws,
+                                                               const double *__restrict__ vol, double *__restrict__ u,
+                                                               double *__restrict__ res, double *__restrict__ dt,
+                                                               double *__restrict__ ws, int outputs, int excl,
+                                                               const unsigned char *__restrict__ is_dir,
+                                                               double *__restrict__ partial)
+{
+  double sq[M + 1];
+#pragma unroll
+  for (int k = 0; k <= M; ++k) sq[k] = 0.0;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nn; i += gridDim.x * blockDim.x) {
+    const double v = vol[i];
+    const double w = acc_ws[i];
+    const double iv = 1.0 / v;
+    const double d = cfl * v / w;
+    const bool count = outputs && !(excl && is_dir[i]);
+#pragma unroll
+    for (int k = 0; k < M; ++k) {
+      const size_t j = static_cast<size_t>(i) * M + k;
+      const double r = acc_res[j] * iv;
+      acc_res[j] = 0.0;
+      u[j] -= d * r;
+      if (outputs) res[j] = r;
+      if (count) sq[k] += r * r;
+    }
+    if (count) sq[M] += 1.0;
+    acc_ws[i] = 0.0;
+    if (outputs) {
+      dt[i] = d;
+      ws[i] = w;
+    }
+  }
+  if (!outputs) return;
+  __shared__ double red[kNormBlock][M + 1];
+#pragma unroll
+  for (int k = 0; k <= M; ++k) red[threadIdx.x][k] = sq[k];
+  __syncthreads();
+  for (int s = kNormBlock / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s) {
+#pragma unroll
+      for (int k = 0; k <= M; ++k) red[threadIdx.x][k] += red[threadIdx.x + s][k];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int k = 0; k <= M; ++k) partial[blockIdx.x * (M + 1) + k] = red[0][k];
+  }
+}
+
+// Per-block partial sums of (res * scale)^2 per field (scale = dt or 1) and
+// the number of rows summed.
+template <int M>
+__global__ __launch_bounds__(kNormBlock) void norm_partial_kernel(uint32_t nn, const double *__restrict__ res,
+                                                                  const double *__restrict__ scale, int excl,
+                                                                  const unsigned char *__restrict__ is_dir,
+                                                                  double *__restrict__ partial)
+{
+  double sq[M + 1];
+#pragma unroll
+  for (int k = 0; k <= M; ++k) sq[k] = 0.0;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nn; i += gridDim.x * blockDim.x) {
+    if (excl && is_dir[i]) continue;
+    const double s = scale ? scale[i] : 1.0;
+#pragma unroll
+    for (int k = 0; k < M; ++k) {
+      const double r = res[static_cast<size_t>(i) * M + k] * s;
+      sq[k] += r * r;
+    }
+    sq[M] += 1.0;
+  }
+  __shared__ double red[kNormBlock][M + 1];
+#pragma unroll
+  for (int k = 0; k <= M; ++k) red[threadIdx.x][k] = sq[k];
+  __syncthreads();
+  for (int s = kNormBlock / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s) {
+#pragma unroll
+      for (int k = 0; k <= M; ++k) red[threadIdx.x][k] += red[threadIdx.x + s][k];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int k = 0; k <= M; ++k) partial[blockIdx.x * (M + 1) + k] = red[0][k];
+  }
+}
+
+// ---- helpers ----------------------------------------------------------------------
+
+int hip_fail(hipError_t e, const char *what)
+{
+  return set_error(PDRD_EHIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+#define HIP_CHECK(call)                                       \
+  do {                                                        \
+    hipError_t _e = (call);                                   \
+    if (_e != hipSuccess) return hip_fail(_e, #call);         \
+  } while (0)
+
+template <typename T>
+int dev_alloc(pdrd_method *h, T **p, size_t count)
+{
+  if (count == 0) count = 1;
+  hipError_t e = hipMalloc(reinterpret_cast<void **>(p), count * sizeof(T));
+  if (e != hipSuccess) return set_error(PDRD_ENOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
+  h->bytes += count * sizeof(T);
+  return PDRD_OK;
+}
+
+unsigned grid_for(uint32_t n, int threads) { return static_cast<unsigned>((n + threads - 1) / threads); }
+
+int field_size(const pdrd_method *h, int field, size_t *count, double **dptr)
+{
+  switch (field) {
+    case PDRD_SOLUTION: *count = static_cast<size_t>(h->nn) * h->m; *dptr = h->d_u; return PDRD_OK;
+    case PDRD_RESIDUAL: *count = static_cast<size_t>(h->nn) * h->m; *dptr = h->d_res; return PDRD_OK;
+    case PDRD_TIME_STEP: *count = h->nn; *dptr = h->d_dt; return PDRD_OK;
+    case PDRD_WAVE_SPEED: *count = h->nn; *dptr = h->d_ws; return PDRD_OK;
+    case PDRD_DUAL_VOLUME: *count = h->nn; *dptr = h->d_vol; return PDRD_OK;
+  }
+  return set_error(PDRD_EINVAL, "unknown field");
+}
+
+// Cell loop over every block (and colour), accumulating into acc_res/acc_ws.
+int launch_cells(pdrd_method *h)
+{
+  pdrd::CellArgs a{};
+  a.coords = h->d_coords;
+  a.u = h->d_u;
+  a.acc_res = h->d_acc_res;
+  a.acc_ws = h->d_acc_ws;
+  a.status = h->d_status;
+  a.pp.p0 = h->params[0];
+  a.pp.p1 = h->params[1];
+  for (Block &b : h->blocks) {
+    a.tab = b.d_tab;
+    a.gconn = b.d_gconn;
+    a.sconn = b.d_sconn;
+    a.ne = b.ne;
+    if (h->scatter_mode == 0) {
+      a.atomic = 1;
+      a.first = 0;
+      a.count = b.ne;
+      HIP_CHECK(b.launch(a, grid_for(b.ne, h->block_threads), h->block_threads, h->stream));
+    } else {
+      a.atomic = 0;
+      for (size_t c = 0; c + 1 < b.colour_start.size(); ++c) {
+        a.first = b.colour_start[c];
+        a.count = b.colour_start[c + 1] - b.colour_start[c];
+        if (a.count == 0) continue;
+        HIP_CHECK(b.launch(a, grid_for(a.count, h->block_threads), h->block_threads, h->stream));
+      }
+    }
+  }
+  return PDRD_OK;
+}
+
+int launch_dirichlet(pdrd_method *h)
+{
+  if (h->n_dir == 0) return PDRD_OK;
+  hipLaunchKernelGGL(dirichlet_kernel, dim3(grid_for(h->n_dir, 256)), dim3(256), 0, h->stream, h->n_dir, h->m,
+                     h->d_dir_nodes, h->d_dir_vals, h->d_u);
+  HIP_CHECK(hipGetLastError());
+  return PDRD_OK;
+}
+
+unsigned norm_blocks(uint32_t nn)
+{
+  return std::min<unsigned>(kNormMaxBlocks, std::max<unsigned>(1, grid_for(nn, kNormBlock)));
+}
+
+int launch_node_pass(pdrd_method *h, double cfl, int outputs, int excl)
+{
+  const unsigned nb = norm_blocks(h->nn);
+#define NODE_CASE(MM)                                                                                        \
+  case MM:                                                                                                   \
+    hipLaunchKernelGGL(node_pass_kernel<MM>, dim3(nb), dim3(kNormBlock), 0, h->stream, h->nn, cfl,             \
+                       h->d_acc_res, h->d_acc_ws, h->d_vol, h->d_u, h->d_res, h->d_dt, h->d_ws, outputs, excl, \
+                       h->d_is_dir, h->d_partial);                                                           \
+    break;
+  switch (h->m) {
+    NODE_CASE(1)
+    NODE_CASE(4)
+    NODE_CASE(5)
+    default: return set_error(PDRD_ENOTSUP, "unsupported number of equations");
+  }
+#undef NODE_CASE
+  HIP_CHECK(hipGetLastError());
+  return PDRD_OK;
+}
+
+int reduce_partials(pdrd_method *h, unsigned nb, double *norm)
+{
+  std::vector<double> part(static_cast<size_t>(nb) * (h->m + 1));
+  HIP_CHECK(hipMemcpyAsync(part.data(), h->d_partial, part.size() * sizeof(double), hipMemcpyDeviceToHost,
+                           h->stream));
+  HIP_CHECK(hipStreamSynchronize(h->stream));
+  std::vector<double> s(h->m + 1, 0.0);
+  for (unsigned b = 0; b < nb; ++b)
+    for (int k = 0; k <= h->m; ++k) s[k] += part[static_cast<size_t>(b) * (h->m + 1) + k];
+  const double cnt = s[h->m] > 0 ? s[h->m] : 1.0;
+  for (int k = 0; k < h->m; ++k) norm[k] = std::sqrt(s[k] / cnt);
+  return PDRD_OK;
+}
+
+int norm_impl(pdrd_method *h, const double *scale, int excl, double *norm)
+{
+  const unsigned nb = norm_blocks(h->nn);
+#define NORM_CASE(MM)                                                                                           \
+  case MM:                                                                                                      \
+    hipLaunchKernelGGL(norm_partial_kernel<MM>, dim3(nb), dim3(kNormBlock), 0, h->stream, h->nn, h->d_res, scale, \
+                       excl, h->d_is_dir, h->d_partial);                                                        \
+    break;
+  switch (h->m) {
+    NORM_CASE(1)
+    NORM_CASE(4)
+    NORM_CASE(5)
+    default: return set_error(PDRD_ENOTSUP, "unsupported number of equations");
+  }
+#undef NORM_CASE
+  HIP_CHECK(hipGetLastError());
+  return reduce_partials(h, nb, norm);
+}
+
+int check_ready(pdrd_method *h)
+{
+  if (!h) return set_error(PDRD_EINVAL, "null method");
+  if (!h->ready) return set_error(PDRD_EINVAL, "method not initialized");
+  return PDRD_OK;
+}
+
+void drop_graph(pdrd_method *h)
+{
+  if (h->graph) {
+    (void)hipGraphExecDestroy(h->graph);
+    h->graph = nullptr;
+  }
+  h->graph_cfl = -1.0;
+}
+
+// Dual volume of every node: the whole volume of each cell holding it, the
+// volume integrated with the Gauss rule of the cell's order
+// (RDTimeUpdate.hpp:71-194).
+std::vector<double> dual_volumes(const pdrd_method *h)
+{
+  std::vector<double> vol(h->nn, 0.0);
+  const int D = h->dim;
+  for (const Block &b : h->blocks) {
+    const pdrd::Shape shape = static_cast<pdrd::Shape>(b.fe.shape);
+    std::vector<double> qp, w, dVg;
+    pdrd::quadrature(shape, b.fe.sol_order, qp, w);
+    pdrd::LagrangeBasis geo(shape, b.fe.geo_order, pdrd::PointSet::Equidist);
+    geo.derivatives(qp, dVg);
+    const int nq = static_cast<int>(w.size()), ng = b.fe.ng, nd = b.fe.nd;
+    for (uint32_t c = 0; c < b.ne; ++c) {
+      double cv = 0.0;
+      for (int q = 0; q < nq; ++q) {
+        double J[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+        for (int xi = 0; xi < D; ++xi)
+          for (int x = 0; x < D; ++x) {
+            double s = 0.0;
+            for (int g = 0; g < ng; ++g)
+              s += dVg[(static_cast<size_t>(xi) * nq + q) * ng + g] *
+                   h->coords[static_cast<size_t>(b.geo[static_cast<size_t>(c) * ng + g]) * D + x];
+            J[xi][x] = s;
+          }
+        const double det = (D == 2) ? J[0][0] * J[1][1] - J[0][1] * J[1][0]
+                                    : J[0][0] * (J[1][1] * J[2][2] - J[1][2] * J[2][1]) -
+                                          J[0][1] * (J[1][0] * J[2][2] - J[1][2] * J[2][0]) +
+                                          J[0][2] * (J[1][0] * J[2][1] - J[1][1] * J[2][0]);
+        cv += w[q] * det;
+      }
+      for (int n = 0; n < nd; ++n) vol[b.sol[static_cast<size_t>(c) * nd + n]] += cv;
+    }
+  }
+  return vol;
+}
+
+void free_device(pdrd_method *h)
+{
+  drop_graph(h);
+  for (Block &b : h->blocks) {
+    (void)hipFree(b.d_tab);
+    (void)hipFree(b.d_gconn);
+    (void)hipFree(b.d_sconn);
+    b.d_tab = nullptr;
+    b.d_gconn = nullptr;
+    b.d_sconn = nullptr;
+  }
+  double **dp[] = {&h->d_coords, &h->d_u, &h->d_res, &h->d_dt, &h->d_ws, &h->d_vol, &h->d_acc_res, &h->d_acc_ws,
+                   &h->d_partial, &h->d_dir_vals};
+  for (double **p : dp) {
+    (void)hipFree(*p);
+    *p = nullptr;
+  }
+  (void)hipFree(h->d_status);
+  (void)hipFree(h->d_dir_nodes);
+  (void)hipFree(h->d_is_dir);
+  h->d_status = nullptr;
+  h->d_dir_nodes = nullptr;
+  h->d_is_dir = nullptr;
+  for (hipEvent_t &e : h->ev) {
+    if (e) (void)hipEventDestroy(e);
+    e = nullptr;
+  }
+  if (h->own_stream && h->stream) (void)hipStreamDestroy(h->stream);
+  h->stream = nullptr;
+  h->own_stream = false;
+  h->bytes = 0;
+}
+
+// One iteration without outputs: cells, Dirichlet values, fused nodal pass.
+int enqueue_iteration(pdrd_method *h, double cfl, int outputs, int excl)
+{
+  int rc = launch_cells(h);
+  if (rc) return rc;
+  rc = launch_dirichlet(h);
+  if (rc) return rc;
+  return launch_node_pass(h, cfl, outputs, excl);
+}
+
+}  // namespace
+
+#define PDRD_TRY try {
+#define PDRD_CATCH                                   \
+  }                                                  \
+  catch (const std::exception &ex)                   \
+  {                                                  \
+    return set_error(PDRD_EINVAL, ex.what());        \
+  }
+
+extern "C" {
+
+int pdrd_create(int dim, uint32_t nb_vertices, const double *vertex_coords, uint32_t nb_nodes, int physics,
+                int scheme, const double *params, int nb_params, pdrd_method_t *out)
+{
+  PDRD_TRY
+  if (!out || !vertex_coords) return set_error(PDRD_EINVAL, "null argument");
+  if (dim != 2 && dim != 3) return set_error(PDRD_EINVAL, "dim must be 2 or 3");
+  if (physics < 0 || physics > 3) return set_error(PDRD_EINVAL, "unknown physics");
+  if (scheme < 0 || scheme > 3) return set_error(PDRD_EINVAL, "unknown scheme");
+  const int pdim = (physics == PDRD_EULER3D) ? 3 : 2;
+  if (pdim != dim) return set_error(PDRD_EINVAL, "physics dimension does not match the mesh");
+  if (nb_nodes == 0 || nb_vertices == 0) return set_error(PDRD_EINVAL, "empty mesh");
+  auto *h = new pdrd_method();
+  h->dim = dim;
+  h->physics = physics;
+  h->scheme = scheme;
+  h->m = (physics == PDRD_EULER2D) ? 4 : (physics == PDRD_EULER3D) ? 5 : 1;
+  h->nv = nb_vertices;
+  h->nn = nb_nodes;
+  if (physics == PDRD_EULER2D || physics == PDRD_EULER3D) {
+    h->params[0] = 1.4;
+    h->params[1] = 287.05;
+  } else if (physics == PDRD_ADVECTION2D) {
+    h->params[0] = 0.0;
+    h->params[1] = 1.0;
+  }
+  for (int i = 0; i < nb_params && i < 4; ++i) h->params[i] = params[i];
+  h->coords.assign(vertex_coords, vertex_coords + static_cast<size_t>(nb_vertices) * dim);
+  *out = h;
+  return PDRD_OK;
+  PDRD_CATCH
+}
+
+int pdrd_add_cells(pdrd_method_t h, const pdrd_fe_desc *fe, const double *tables, uint32_t ne,
+                   const uint32_t *geo_conn, const uint32_t *sol_conn)
+{
+  PDRD_TRY
+  if (!h || !fe || !tables || (ne && (!geo_conn || !sol_conn))) return set_error(PDRD_EINVAL, "null argument");
+  if (h->ready) return set_error(PDRD_EINVAL, "cells must be added before pdrd_initialize");
+  if (fe->dim != h->dim) return set_error(PDRD_EINVAL, "element dimension does not match the mesh");
+  // the descriptor must be one this library builds
+  pdrd_fe_desc chk;
+  int rc = pdrd_fe_describe(fe->shape, fe->geo_order, fe->sol_order, fe->quad_order, fe->point_set, &chk);
+  if (rc) return rc;
+  if (chk.ng != fe->ng || chk.nd != fe->nd || chk.nf != fe->nf || chk.nq != fe->nq ||
+      chk.table_size != fe->table_size)
+    return set_error(PDRD_EINVAL, "inconsistent element descriptor");
+  if (fe->geo_order != 1) return set_error(PDRD_ENOTSUP, "only P1/Q1 geometry is built");
+  pdrd::CellKey key{fe->shape, fe->sol_order, fe->nq, h->physics, h->scheme};
+  pdrd::CellLauncher launch = pdrd::find_cell(key);
+  if (!launch) return set_error(PDRD_ENOTSUP, "no cell kernel for this element type / physics / scheme");
+  for (size_t i = 0; i < static_cast<size_t>(ne) * fe->ng; ++i)
+    if (geo_conn[i] >= h->nv) return set_error(PDRD_EINVAL, "geometry connectivity out of range");
+  for (size_t i = 0; i < static_cast<size_t>(ne) * fe->nd; ++i)
+    if (sol_conn[i] >= h->nn) return set_error(PDRD_EINVAL, "solution connectivity out of range");
+  Block b;
+  b.fe = *fe;
+  b.ne = ne;
+  b.tables.assign(tables, tables + fe->table_size);
+  b.geo.assign(geo_conn, geo_conn + static_cast<size_t>(ne) * fe->ng);
+  b.sol.assign(sol_conn, sol_conn + static_cast<size_t>(ne) * fe->nd);
+  b.launch = launch;
+  h->blocks.push_back(std::move(b));
+  return PDRD_OK;
+  PDRD_CATCH
+}
+
+int pdrd_initialize(pdrd_method_t h)
+{
+  PDRD_TRY
+  if (!h) return set_error(PDRD_EINVAL, "null method");
+  if (h->ready) return set_error(PDRD_EINVAL, "already initialized");
+  if (h->blocks.empty()) return set_error(PDRD_EINVAL, "no cells");
+  // every node must belong to a cell (a node without cells has no dual volume)
+  std::vector<char> used(h->nn, 0);
+  for (const Block &b : h->blocks)
+    for (uint32_t v : b.sol) used[v] = 1;
+  for (uint32_t i = 0; i < h->nn; ++i)
+    if (!used[i]) return set_error(PDRD_EINVAL, "solution node not held by any cell");
+  HIP_CHECK(hipGetDevice(&h->device));
+  int rc;
+  const size_t nm = static_cast<size_t>(h->nn) * h->m;
+  if ((rc = dev_alloc(h, &h->d_coords, h->coords.size()))) { free_device(h); return rc; }
+  if ((rc = dev_alloc(h, &h->d_u, nm))) { free_device(h); return rc; }
+  if ((rc = dev_alloc(h, &h->d_res, nm))) { free_device(h); return rc; }
+  if ((rc = dev_alloc(h, &h->d_acc_res, nm))) { free_device(h); return rc; }
+  if ((rc = dev_alloc(h, &h->d_dt, h->nn))) { free_device(h); return rc; }
+  if ((rc = dev_alloc(h, &h->d_ws, h->nn))) { free_device(h); return rc; }
+  if ((rc = dev_alloc(h, &h->d_acc_ws, h->nn))) { free_device(h); return rc; }
+  if ((rc = dev_alloc(h, &h->d_vol, h->nn))) { free_device(h); return rc; }
+  if ((rc = dev_alloc(h, &h->d_partial, static_cast<size_t>(kNormMaxBlocks) * (h->m + 1)))) { free_device(h); return rc; }
+  if ((rc = dev_alloc(h, &h->d_status, 1))) { free_device(h); return rc; }
+  if ((rc = dev_alloc(h, &h->d_is_dir, h->nn))) { free_device(h); return rc; }
+
+  // colouring: greedy in caller order per block, then a stable sort by colour
+  for (Block &b : h->blocks) {
+    pdrd::SolMesh s;
+    s.nb_nodes = h->nn;
+    s.cell_offset.resize(b.ne + 1);
+    for (uint32_t c = 0; c <= b.ne; ++c) s.cell_offset[c] = c * b.fe.nd;
+    s.cell_nodes = b.sol;
+    std::vector<uint32_t> cells(b.ne);
+    for (uint32_t c = 0; c < b.ne; ++c) cells[c] = c;
+    uint32_t ncol = 0;
+    const std::vector<uint32_t> col = pdrd::greedy_colour(cells, s, &ncol);
+    b.colour_start.assign(ncol + 1, 0);
+    for (uint32_t c = 0; c < b.ne; ++c) b.colour_start[col[c] + 1]++;
+    for (uint32_t k = 0; k < ncol; ++k) b.colour_start[k + 1] += b.colour_start[k];
+    std::vector<uint32_t> fill(b.colour_start.begin(), b.colour_start.end() - 1);
+    b.perm.assign(b.ne, 0);
+    for (uint32_t c = 0; c < b.ne; ++c) b.perm[fill[col[c]]++] = c;
+    const int ng = b.fe.ng, nd = b.fe.nd;
+    std::vector<uint32_t> g(static_cast<size_t>(ng) * b.ne), so(static_cast<size_t>(nd) * b.ne);
+    for (uint32_t p = 0; p < b.ne; ++p) {
+      const uint32_t c = b.perm[p];
+      for (int k = 0; k < ng; ++k) g[static_cast<size_t>(k) * b.ne + p] = b.geo[static_cast<size_t>(c) * ng + k];
+      for (int k = 0; k < nd; ++k) so[static_cast<size_t>(k) * b.ne + p] = b.sol[static_cast<size_t>(c) * nd + k];
+    }
+    if ((rc = dev_alloc(h, &b.d_tab, b.tables.size()))) { free_device(h); return rc; }
+    if ((rc = dev_alloc(h, &b.d_gconn, g.size()))) { free_device(h); return rc; }
+    if ((rc = dev_alloc(h, &b.d_sconn, so.size()))) { free_device(h); return rc; }
+    HIP_CHECK(hipMemcpy(b.d_tab, b.tables.data(), b.tables.size() * sizeof(double), hipMemcpyHostToDevice));
+    HIP_CHECK(hipMemcpy(b.d_gconn, g.data(), g.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+    HIP_CHECK(hipMemcpy(b.d_sconn, so.data(), so.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+  }
+  const std::vector<double> vol = dual_volumes(h);
+  HIP_CHECK(hipMemcpy(h->d_coords, h->coords.data(), h->coords.size() * sizeof(double), hipMemcpyHostToDevice));
+  HIP_CHECK(hipMemcpy(h->d_vol, vol.data(), vol.size() * sizeof(double), hipMemcpyHostToDevice));
+  HIP_CHECK(hipMemset(h->d_u, 0, nm * sizeof(double)));
+  HIP_CHECK(hipMemset(h->d_res, 0, nm * sizeof(double)));
+  HIP_CHECK(hipMemset(h->d_acc_res, 0, nm * sizeof(double)));
+  HIP_CHECK(hipMemset(h->d_dt, 0, h->nn * sizeof(double)));
+  HIP_CHECK(hipMemset(h->d_ws, 0, h->nn * sizeof(double)));
+  HIP_CHECK(hipMemset(h->d_acc_ws, 0, h->nn * sizeof(double)));
+  HIP_CHECK(hipMemset(h->d_status, 0, sizeof(int)));
+  HIP_CHECK(hipMemset(h->d_is_dir, 0, h->nn));
+  if (!h->stream) {
+    HIP_CHECK(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
+    h->own_stream = true;
+  }
+  for (hipEvent_t &e : h->ev) HIP_CHECK(hipEventCreate(&e));
+  HIP_CHECK(hipDeviceSynchronize());
+  h->ready = true;
+  return PDRD_OK;
+  PDRD_CATCH
+}
+
+int pdrd_destroy(pdrd_method_t h)
+{
+  if (!h) return PDRD_OK;
+  if (h->stream) (void)hipStreamSynchronize(h->stream);
+  free_device(h);
+  delete h;
+  return PDRD_OK;
+}
+
+int pdrd_set_option(pdrd_method_t h, int option, int value)
+{
+  if (!h) return set_error(PDRD_EINVAL, "null method");
+  switch (option) {
+    case PDRD_OPT_SCATTER:
+      if (value != 0 && value != 1) return set_error(PDRD_EINVAL, "scatter mode is 0 or 1");
+      h->scatter_mode = value;
+      break;
+    case PDRD_OPT_PRECISION:
+      if (value != 0) return set_error(PDRD_ENOTSUP, "only the plain FP64 path is built");
+      h->precision = value;
+      break;
+    case PDRD_OPT_BLOCK:
+      if (value < 64 || value > 256 || (value % 64) != 0)
+        return set_error(PDRD_EINVAL, "block size must be a multiple of 64 in [64, 256]");
+      h->block_threads = value;
+      break;
+    default: return set_error(PDRD_EINVAL, "unknown option");
+  }
+  drop_graph(h);
+  return PDRD_OK;
+}
+
+int pdrd_set_stream(pdrd_method_t h, void *stream)
+{
+  if (!h) return set_error(PDRD_EINVAL, "null method");
+  if (h->stream) HIP_CHECK(hipStreamSynchronize(h->stream));
+  if (h->own_stream && h->stream) (void)hipStreamDestroy(h->stream);
+  h->own_stream = false;
+  h->stream = static_cast<hipStream_t>(stream);
+  if (!h->stream && h->ready) {
+    HIP_CHECK(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
+    h->own_stream = true;
+  }
+  drop_graph(h);
+  return PDRD_OK;
+}
+
+int pdrd_synchronize(pdrd_method_t h)
+{
+  if (int rc = check_ready(h)) return rc;
+  HIP_CHECK(hipStreamSynchronize(h->stream));
+  return PDRD_OK;
+}
+
+int pdrd_set_field(pdrd_method_t h, int field, const double *host)
+{
+  if (int rc = check_ready(h)) return rc;
+  if (!host) return set_error(PDRD_EINVAL, "null buffer");
+  size_t n;
+  double *d;
+  if (int rc = field_size(h, field, &n, &d)) return rc;
+  if (field == PDRD_DUAL_VOLUME) return set_error(PDRD_EINVAL, "the dual volume is computed by the method");
+  HIP_CHECK(hipMemcpyAsync(d, host, n * sizeof(double), hipMemcpyHostToDevice, h->stream));
+  HIP_CHECK(hipStreamSynchronize(h->stream));
+  return PDRD_OK;
+}
+
+int pdrd_get_field(pdrd_method_t h, int field, double *host)
+{
+  if (int rc = check_ready(h)) return rc;
+  if (!host) return set_error(PDRD_EINVAL, "null buffer");
+  size_t n;
+  double *d;
+  if (int rc = field_size(h, field, &n, &d)) return rc;
+  HIP_CHECK(hipMemcpyAsync(host, d, n * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+  HIP_CHECK(hipStreamSynchronize(h->stream));
+  return PDRD_OK;
+}
+
+int pdrd_field_device_ptr(pdrd_method_t h, int field, double **dptr)
+{
+  if (int rc = check_ready(h)) return rc;
+  if (!dptr) return set_error(PDRD_EINVAL, "null output");
+  size_t n;
+  return field_size(h, field, &n, dptr);
+}
+
+int pdrd_add_strong_dirichlet(pdrd_method_t h, uint32_t n, const uint32_t *nodes, const double *values)
+{
+  if (int rc = check_ready(h)) return rc;
+  if (n == 0) return PDRD_OK;
+  if (!nodes || !values) return set_error(PDRD_EINVAL, "null argument");
+  for (uint32_t i = 0; i < n; ++i)
+    if (nodes[i] >= h->nn) return set_error(PDRD_EINVAL, "Dirichlet node out of range");
+  // append to the existing list
+  std::vector<uint32_t> all_nodes(h->n_dir + n);
+  std::vector<double> all_vals((static_cast<size_t>(h->n_dir) + n) * h->m);
+  if (h->n_dir) {
+    HIP_CHECK(hipMemcpy(all_nodes.data(), h->d_dir_nodes, h->n_dir * sizeof(uint32_t), hipMemcpyDeviceToHost));
+    HIP_CHECK(hipMemcpy(all_vals.data(), h->d_dir_vals, static_cast<size_t>(h->n_dir) * h->m * sizeof(double),
+                        hipMemcpyDeviceToHost));
+  }
+  std::memcpy(all_nodes.data() + h->n_dir, nodes, n * sizeof(uint32_t));
+  std::memcpy(all_vals.data() + static_cast<size_t>(h->n_dir) * h->m, values,
+              static_cast<size_t>(n) * h->m * sizeof(double));
+  (void)hipFree(h->d_dir_nodes);
+  (void)hipFree(h->d_dir_vals);
+  h->d_dir_nodes = nullptr;
+  h->d_dir_vals = nullptr;
+  int rc;
+  if ((rc = dev_alloc(h, &h->d_dir_nodes, all_nodes.size()))) return rc;
+  if ((rc = dev_alloc(h, &h->d_dir_vals, all_vals.size()))) return rc;
+  HIP_CHECK(hipMemcpy(h->d_dir_nodes, all_nodes.data(), all_nodes.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+  HIP_CHECK(hipMemcpy(h->d_dir_vals, all_vals.data(), all_vals.size() * sizeof(double), hipMemcpyHostToDevice));
+  h->n_dir += n;
+  std::vector<unsigned char> flag(h->nn, 0);
+  for (uint32_t v : all_nodes) flag[v] = 1;
+  HIP_CHECK(hipMemcpy(h->d_is_dir, flag.data(), h->nn, hipMemcpyHostToDevice));
+  drop_graph(h);
+  return PDRD_OK;
+}
+
+int pdrd_assemble_rhs(pdrd_method_t h)
+{
+  if (int rc = check_ready(h)) return rc;
+  HIP_CHECK(hipEventRecord(h->ev[0], h->stream));
+  if (int rc = launch_cells(h)) return rc;
+  HIP_CHECK(hipEventRecord(h->ev[1], h->stream));
+  hipLaunchKernelGGL(finalize_kernel, dim3(grid_for(h->nn, 256)), dim3(256), 0, h->stream, h->nn, h->m, h->d_acc_res,
+                     h->d_acc_ws, h->d_vol, h->d_res, h->d_ws);
+  HIP_CHECK(hipGetLastError());
+  if (int rc = launch_dirichlet(h)) return rc;
+  HIP_CHECK(hipEventRecord(h->ev[2], h->stream));
+  HIP_CHECK(hipEventSynchronize(h->ev[2]));
+  float a = 0.f, b = 0.f;
+  HIP_CHECK(hipEventElapsedTime(&a, h->ev[0], h->ev[1]));
+  HIP_CHECK(hipEventElapsedTime(&b, h->ev[1], h->ev[2]));
+  h->last_cell_ms = a;
+  h->last_node_ms = b;
+  return PDRD_OK;
+}
+
+int pdrd_assemble_lhs_and_rhs(pdrd_method_t h, double cfl)
+{
+  if (int rc = pdrd_assemble_rhs(h)) return rc;
+  hipLaunchKernelGGL(dt_kernel, dim3(grid_for(h->nn, 256)), dim3(256), 0, h->stream, h->nn, cfl, h->d_vol, h->d_ws,
+                     h->d_dt);
+  HIP_CHECK(hipGetLastError());
+  HIP_CHECK(hipStreamSynchronize(h->stream));
+  return PDRD_OK;
+}
+
+int pdrd_solve(pdrd_method_t h)
+{
+  if (int rc = check_ready(h)) return rc;
+  hipLaunchKernelGGL(update_kernel, dim3(grid_for(h->nn, 256)), dim3(256), 0, h->stream, h->nn, h->m, h->d_dt,
+                     h->d_res, h->d_u);
+  HIP_CHECK(hipGetLastError());
+  HIP_CHECK(hipStreamSynchronize(h->stream));
+  return PDRD_OK;
+}
+
+int pdrd_compute_residual_norm(pdrd_method_t h, double *norm)
+{
+  if (int rc = check_ready(h)) return rc;
+  if (!norm) return set_error(PDRD_EINVAL, "null output");
+  return norm_impl(h, h->d_dt, 0, norm);
+}
+
+int pdrd_residual_norm_L2(pdrd_method_t h, int exclude_dirichlet, double *norm)
+{
+  if (int rc = check_ready(h)) return rc;
+  if (!norm) return set_error(PDRD_EINVAL, "null output");
+  return norm_impl(h, nullptr, exclude_dirichlet ? 1 : 0, norm);
+}
+
+int pdrd_iterate(pdrd_method_t h, double cfl, int n_iter, int exclude_dirichlet, double *norm)
+{
+  if (int rc = check_ready(h)) return rc;
+  if (n_iter < 1) return set_error(PDRD_EINVAL, "n_iter must be >= 1");
+  const int excl = exclude_dirichlet ? 1 : 0;
+  HIP_CHECK(hipEventRecord(h->ev[0], h->stream));
+  if (n_iter > 1) {
+    if (!h->graph || h->graph_cfl != cfl) {
+      drop_graph(h);
+      hipGraph_t g = nullptr;
+      HIP_CHECK(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+      const int rc = enqueue_iteration(h, cfl, 0, excl);
+      hipError_t e = hipStreamEndCapture(h->stream, &g);
+      if (rc) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+      }
+      if (e != hipSuccess) return hip_fail(e, "hipStreamEndCapture");
+      e = hipGraphInstantiate(&h->graph, g, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(g);
+      if (e != hipSuccess) return hip_fail(e, "hipGraphInstantiate");
+      h->graph_cfl = cfl;
+    }
+    for (int it = 0; it < n_iter - 1; ++it) HIP_CHECK(hipGraphLaunch(h->graph, h->stream));
+  }
+  if (int rc = enqueue_iteration(h, cfl, 1, excl)) return rc;
+  HIP_CHECK(hipEventRecord(h->ev[1], h->stream));
+  HIP_CHECK(hipEventSynchronize(h->ev[1]));
+  float ms = 0.f;
+  HIP_CHECK(hipEventElapsedTime(&ms, h->ev[0], h->ev[1]));
+  h->last_cell_ms = ms;
+  h->last_node_ms = 0.0;
+  if (norm) return reduce_partials(h, norm_blocks(h->nn), norm);
+  return PDRD_OK;
+}
+
+int pdrd_check_status(pdrd_method_t h)
+{
+  if (int rc = check_ready(h)) return rc;
+  int s = 0;
+  HIP_CHECK(hipStreamSynchronize(h->stream));
+  HIP_CHECK(hipMemcpy(&s, h->d_status, sizeof(int), hipMemcpyDeviceToHost));
+  if (s & 1) return set_error(PDRD_ENEGPRESSURE, "non-positive pressure met in a cell");
+  return PDRD_OK;
+}
+
+int pdrd_method_info(pdrd_method_t h, int64_t *info)
+{
+  if (!h || !info) return set_error(PDRD_EINVAL, "null argument");
+  int64_t cells = 0, ncol = 0;
+  for (const Block &b : h->blocks) {
+    cells += b.ne;
+    ncol = std::max<int64_t>(ncol, b.colour_start.empty() ? 0 : static_cast<int64_t>(b.colour_start.size()) - 1);
+  }
+  info[0] = h->nn;
+  info[1] = h->m;
+  info[2] = static_cast<int64_t>(h->blocks.size());
+  info[3] = cells;
+  info[4] = ncol;
+  info[5] = static_cast<int64_t>(h->bytes);
+  return PDRD_OK;
+}
+
+int pdrd_last_timing(pdrd_method_t h, double *cell_ms, double *node_ms)
+{
+  if (!h) return set_error(PDRD_EINVAL, "null method");
+  if (cell_ms) *cell_ms = h->last_cell_ms;
+  if (node_ms) *node_ms = h->last_node_ms;
+  return PDRD_OK;
+}
+
+}  // extern "C"

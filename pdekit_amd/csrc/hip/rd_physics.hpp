@@ -1,0 +1,142 @@
+// Point-wise physics of the cell loop, device side.
+//
+// Restates compute_properties / flux / the upwind matrix K+ = R max(L,0) L^-1
+// of src/physics/scalar/RotationAdvection2D.hpp:201-336 (constant velocity),
+// src/physics/scalar/Burgers2D.hpp:154-275, src/physics/euler/
+// Euler2DCons.hpp:159-536 and Euler3DCons.hpp:159-450.
+//
+// For the Euler systems K+ is applied in the form
+//   K+ = l0+ I + (lp+ - l0+) r_p l_p^T + (lm+ - l0+) r_m l_m^T,
+// which follows from R L = I and the multiplicity of the convective
+// eigenvalue u.n: only the two acoustic eigenvector pairs (r_p, l_p),
+// (r_m, l_m) enter. Entries of r and l are those of the reference's R and
+// L columns/rows for the eigenvalues |g| (u.n + a) and |g| (u.n - a).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace pdrd {
+
+enum { PH_ADVECTION2D = 0, PH_BURGERS2D = 1, PH_EULER2D = 2, PH_EULER3D = 3 };
+enum { SC_LDA = 0, SC_N = 1, SC_B = 2, SC_PSI = 3 };
+
+template <int PH> struct PhysTraits;
+template <> struct PhysTraits<PH_ADVECTION2D> { static constexpr int M = 1, DIM = 2; };
+template <> struct PhysTraits<PH_BURGERS2D> { static constexpr int M = 1, DIM = 2; };
+template <> struct PhysTraits<PH_EULER2D> { static constexpr int M = 4, DIM = 2; };
+template <> struct PhysTraits<PH_EULER3D> { static constexpr int M = 5, DIM = 3; };
+
+struct PhysParams
+{
+  double p0, p1;  // advection velocity (ax, ay) or (gamma, R)
+};
+
+// Thermodynamic state at a point (Euler). T is the arithmetic type of the
+// interpolated state.
+template <typename T, int D>
+struct EulerState
+{
+  T rho, inv_rho, V[D], v2, P, H, a;
+  bool bad;
+};
+
+template <typename T, int D>
+__device__ __forceinline__ EulerState<T, D> euler_state(const T *s, T gamma)
+{
+  EulerState<T, D> st;
+  const T gm1 = gamma - T(1);
+  st.rho = s[0];
+  st.inv_rho = T(1) / st.rho;
+  T v2 = T(0);
+#pragma unroll
+  for (int d = 0; d < D; ++d) {
+    st.V[d] = s[1 + d] * st.inv_rho;
+    v2 += st.V[d] * st.V[d];
+  }
+  st.v2 = v2;
+  const T rhoE = s[D + 1];
+  st.P = gm1 * (rhoE - T(0.5) * st.rho * st.v2);
+  st.bad = !(st.P > T(0));
+  const T RT = st.P * st.inv_rho;
+  st.H = rhoE * st.inv_rho + RT;
+  st.a = sqrt(gamma * RT);
+  return st;
+}
+
+// Physical flux F[d][k] at a state.
+template <int PH, typename T>
+__device__ __forceinline__ void point_flux(const T *s, const PhysParams &pp, T F[][PhysTraits<PH>::M], bool &bad)
+{
+  if constexpr (PH == PH_ADVECTION2D) {
+    F[0][0] = T(pp.p0) * s[0];
+    F[1][0] = T(pp.p1) * s[0];
+  } else if constexpr (PH == PH_BURGERS2D) {
+    F[0][0] = T(0.5) * s[0] * s[0];
+    F[1][0] = s[0];
+  } else {
+    constexpr int D = PhysTraits<PH>::DIM;
+    const EulerState<T, D> st = euler_state<T, D>(s, T(pp.p0));
+    bad = bad || st.bad;
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+      const T md = s[1 + d];
+      F[d][0] = md;
+#pragma unroll
+      for (int k = 0; k < D; ++k) F[d][1 + k] = md * st.V[k] + ((k == d) ? st.P : T(0));
+      F[d][D + 1] = md * st.H;
+    }
+  }
+}
+
+// Compact K+ of one node at one quadrature point:
+//   scalar: kp = max(u_adv . g, 0)
+//   Euler:  l0 = max(|g| u.n, 0), dp = max(|g|(u.n+a),0) - l0,
+//           dm = max(|g|(u.n-a),0) - l0, unit normal n.
+template <int D>
+struct KCompact
+{
+  double l0, dp, dm, n[D], norm;
+};
+
+// Acoustic eigenvector pairs at a state for the unit direction n.
+template <int D>
+struct Acoustic
+{
+  double rp[D + 2], rm[D + 2], lp[D + 2], lm[D + 2];
+};
+
+template <int D>
+__device__ __forceinline__ void acoustic_vectors(const EulerState<double, D> &st, double gm1, const double *n,
+                                                 Acoustic<D> &ac)
+{
+  constexpr int M = D + 2;
+  const double a = st.a, ir = st.inv_rho;
+  const double inv_a = 1. / a, inv_a2 = inv_a * inv_a;
+  double um = 0.0;
+#pragma unroll
+  for (int d = 0; d < D; ++d) um += st.V[d] * n[d];
+  const double ra = 0.5 * st.rho * inv_a;
+  const double cM2 = (0.5 * gm1 * st.v2) * inv_a2;
+  const double gm1_ov_rhoa = gm1 / (st.rho * a);
+  ac.rp[0] = ra;
+  ac.rm[0] = ra;
+#pragma unroll
+  for (int d = 0; d < D; ++d) {
+    ac.rp[1 + d] = ra * (st.V[d] + a * n[d]);
+    ac.rm[1 + d] = ra * (st.V[d] - a * n[d]);
+  }
+  ac.rp[M - 1] = ra * (st.H + a * um);
+  ac.rm[M - 1] = ra * (st.H - a * um);
+  ac.lp[0] = a * ir * (cM2 - um * inv_a);
+  ac.lm[0] = a * ir * (cM2 + um * inv_a);
+#pragma unroll
+  for (int d = 0; d < D; ++d) {
+    const double gva = gm1 * st.V[d] * inv_a;
+    ac.lp[1 + d] = ir * (n[d] - gva);
+    ac.lm[1 + d] = -ir * (n[d] + gva);
+  }
+  ac.lp[M - 1] = gm1_ov_rhoa;
+  ac.lm[M - 1] = gm1_ov_rhoa;
+}
+
+}  // namespace pdrd

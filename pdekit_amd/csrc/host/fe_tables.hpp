@@ -1,0 +1,101 @@
+// Host-side finite-element inputs of the residual-distribution cell loop:
+// quadrature rules, reference-element nodes (equidistant / warp-blend) and
+// Lagrange shape-function tables at arbitrary reference points.
+//
+// These are the tables that pdekit's FEValues / FunctionSpaceCells /
+// FluxSpaceMetric build once at set-up time
+// (src/interpolation/FEValues.cpp:157-279,
+//  src/interpolation/FluxSpaceMetric.hpp:237-456,
+//  src/mesh/shape_function/ShapeFunction.cpp:24-240).
+#pragma once
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+namespace pdrd {
+
+enum class Shape : int { Triag = 0, Quad = 1, Tetra = 2 };
+enum class PointSet : int { Equidist = 0, Warpblend = 1 };
+
+int shape_dim(Shape s);
+// Number of Lagrange nodes of a reference element of order p.
+int shape_nb_nodes(Shape s, int p);
+// Number of P1 (vertex) nodes of the shape.
+int shape_nb_vertices(Shape s);
+
+// Reference-element node coordinates, row-major [n][dim].
+// Reference triangle (-1,-1),(1,-1),(-1,1); quad [-1,1]^2 counter-clockwise;
+// tetrahedron (-1,-1,-1),(1,-1,-1),(-1,1,-1),(-1,-1,1).
+// Node order: vertices, then edge nodes edge by edge, then interior nodes
+// (src/mesh/std_region/WarpblendStdRegionTriag.cpp:87-395,
+//  EquidistStdRegionQuad.cpp:264-, EquidistStdRegionTetra.cpp:522-).
+std::vector<double> reference_nodes(Shape s, int p, PointSet ps);
+
+// Gauss quadrature of the given polynomial order on the reference element.
+// The point/weight values are the literal tables of
+// src/mesh/point_set/QdPointSet{Triag,Quad,Tetra}Gauss.cpp.
+void quadrature(Shape s, int order, std::vector<double> &xi, std::vector<double> &w);
+
+// Lagrange basis of order p on the nodes of a reference element.
+class LagrangeBasis
+{
+  public:
+  LagrangeBasis(Shape s, int p, PointSet ps);
+  LagrangeBasis(Shape s, int p, const std::vector<double> &nodes);
+
+  int nb_nodes() const { return m_n; }
+  int dim() const { return m_dim; }
+  const std::vector<double> &nodes() const { return m_nodes; }
+
+  // values [npts][n]; near-zero (<1e-14) entries are clipped and every row
+  // is rescaled to sum to one, as ShapeFunction.cpp:149-210 does.
+  void values(const std::vector<double> &pts, std::vector<double> &V) const;
+  // derivatives [dim][npts][n] with respect to the reference coordinates
+  // (no clipping, ShapeFunction.cpp:212-240).
+  void derivatives(const std::vector<double> &pts, std::vector<double> &dV) const;
+
+  private:
+  void build();
+  void monomials(const double *x, long double *val, long double *grad) const;
+
+  Shape m_shape;
+  int m_p;
+  int m_dim;
+  int m_n;
+  std::vector<double> m_nodes;
+  std::vector<int> m_exp;           // [n][dim] monomial exponents
+  std::vector<long double> m_invV;  // [n modes][n nodes]
+};
+
+// One element type of the cell loop: solution order p on a geometry of order
+// geo_p, P+1 flux element, quadrature of order qorder. All tables row-major.
+struct ElementTables
+{
+  Shape shape;
+  int geo_order, sol_order, quad_order;
+  PointSet point_set;
+  int dim, ng, nd, nf, nq;
+  std::vector<double> qp;    // [nq][dim]
+  std::vector<double> w;     // [nq]
+  std::vector<double> Vg;    // [nq][ng]
+  std::vector<double> dVg;   // [dim][nq][ng]
+  std::vector<double> V;     // [nq][nd]
+  std::vector<double> dV;    // [dim][nq][nd]
+  std::vector<double> Vf;    // [nq][nf]
+  std::vector<double> dVf;   // [dim][nq][nf]
+  std::vector<double> Vgf;   // [nf][ng]
+  std::vector<double> Vsf;   // [nf][nd]
+  std::vector<double> sol_nodes;   // [nd][dim]
+  std::vector<double> flux_nodes;  // [nf][dim]
+};
+
+ElementTables make_element_tables(Shape s, int geo_order, int sol_order, int quad_order,
+                                  PointSet ps);
+
+// Packed layout used by the C ABI: the arrays above concatenated in this
+// order: qp, w, Vg, dVg, V, dV, Vf, dVf, Vgf, Vsf, sol_nodes, flux_nodes.
+size_t packed_size(const ElementTables &t);
+void pack(const ElementTables &t, double *out);
+
+}  // namespace pdrd

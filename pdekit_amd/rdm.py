@@ -1,0 +1,162 @@
+"""Host-side mirror of pdekit's explicit continuous RD method.
+
+``PGRDMethodExplicit`` keeps the member names of the reference class
+(src/solver/rdm/PGRDMethodExplicit.hpp:19-193 over the RDMethod API of
+src/solver/rdm/RDMethod.hpp:33-139) and runs every step on the GPU through
+the C ABI of include/pdekit_rd.h. Nodal fields are numpy arrays in the
+reference's AoS layout [N][m].
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib as L
+from .fe import element_tables
+from .mesh import Mesh
+
+PHYSICS = {"Advection2D": L.ADVECTION2D, "Burgers2D": L.BURGERS2D, "Euler2DCons": L.EULER2D,
+           "Euler3DCons": L.EULER3D}
+SCHEMES = {"PGLDA": L.LDA, "PGN": L.N, "PGB": L.B, "PSI": L.PSI}
+NEQ = {L.ADVECTION2D: 1, L.BURGERS2D: 1, L.EULER2D: 4, L.EULER3D: 5}
+FIELDS = {"solution": L.SOLUTION, "residuals": L.RESIDUAL, "time_step": L.TIME_STEP,
+          "wave_speed": L.WAVE_SPEED, "dual_volume": L.DUAL_VOLUME}
+
+
+class PGRDMethodExplicit:
+    """One explicit continuous RD method on the current GPU.
+
+    physics: 'Advection2D' (constant velocity params=(ax, ay)), 'Burgers2D',
+    'Euler2DCons', 'Euler3DCons' (params=(gamma, R)); scheme: 'PGLDA', 'PGN',
+    'PGB' or 'PSI'.
+    """
+
+    def __init__(self, physics: str, scheme: str, params=None):
+        self.physics = PHYSICS[physics] if isinstance(physics, str) else int(physics)
+        self.scheme = SCHEMES[scheme] if isinstance(scheme, str) else int(scheme)
+        self.params = np.zeros(4)
+        if params is not None:
+            self.params[:len(params)] = params
+            self.nb_params = len(params)
+        else:
+            self.nb_params = 0
+        self.neq = NEQ[self.physics]
+        self._h = None
+        self.mesh: Mesh | None = None
+        self.dirichlet_nodes = np.zeros(0, dtype=np.uint32)
+
+    # -- RDMethod::configure_mesh_data / initialize_work_data ---------------------------
+    def configure_mesh_data(self, mesh: Mesh):
+        self.mesh = mesh
+        return self
+
+    def initialize_work_data(self, quad_order=4, point_set=L.WARPBLEND, scatter: int = 0):
+        """Builds the FE tables of every element type (one quadrature order, or a
+        dict shape -> order) and uploads mesh and tables."""
+        if self.mesh is None:
+            raise RuntimeError("configure_mesh_data first")
+        m = self.mesh
+        lib = L.lib()
+        h = C.c_void_p()
+        coords = np.ascontiguousarray(m.coords, dtype=np.float64)
+        L.check(lib.pdrd_create(m.dim, coords.shape[0], L.ptr(coords), m.nb_nodes, self.physics, self.scheme,
+                                L.ptr(self.params), self.nb_params, C.byref(h)), "pdrd_create")
+        self._h = h
+        self.tables = {}
+        for blk in m.blocks:
+            q = quad_order[blk.shape] if isinstance(quad_order, dict) else quad_order
+            t = element_tables(blk.shape, m.order, q, 1, point_set)
+            self.tables[blk.shape] = t
+            L.check(lib.pdrd_add_cells(h, C.byref(t.desc), L.ptr(t.packed), blk.geo_conn.shape[0],
+                                       L.ptr(blk.geo_conn, C.c_uint32), L.ptr(blk.sol_conn, C.c_uint32)),
+                    "pdrd_add_cells")
+        L.check(lib.pdrd_set_option(h, L.OPT_SCATTER, int(scatter)), "pdrd_set_option")
+        L.check(lib.pdrd_initialize(h), "pdrd_initialize")
+        return self
+
+    def set_option(self, option: int, value: int):
+        L.check(L.lib().pdrd_set_option(self._h, option, int(value)), "pdrd_set_option")
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            L.lib().pdrd_destroy(h)
+            self._h = None
+
+    def destroy(self):
+        self.__del__()
+
+    # -- vec functions ------------------------------------------------------------------
+    def _shape(self, which: int):
+        n = self.mesh.nb_nodes
+        return (n, self.neq) if which in (L.SOLUTION, L.RESIDUAL) else (n,)
+
+    def set_vec_function(self, which, values):
+        w = FIELDS[which] if isinstance(which, str) else which
+        a = np.ascontiguousarray(np.broadcast_to(np.asarray(values, dtype=np.float64), self._shape(w)))
+        L.check(L.lib().pdrd_set_field(self._h, w, L.ptr(a)), "pdrd_set_field")
+
+    def vec_function(self, which) -> np.ndarray:
+        w = FIELDS[which] if isinstance(which, str) else which
+        a = np.zeros(self._shape(w))
+        L.check(L.lib().pdrd_get_field(self._h, w, L.ptr(a)), "pdrd_get_field")
+        return a
+
+    # -- boundary conditions --------------------------------------------------------------
+    def add_boundary_condition(self, bc_type: str, name: str, nodes, values):
+        """'StrongDirichlet': values [n][m] or a callable f(node_coords[n][dim]) -> [n][m]."""
+        if bc_type != "StrongDirichlet":
+            raise NotImplementedError(f"boundary condition {bc_type} is not built on the GPU path")
+        nodes = np.ascontiguousarray(nodes, dtype=np.uint32)
+        if callable(values):
+            values = values(self.mesh.node_coords[nodes])
+        vals = np.ascontiguousarray(np.broadcast_to(np.asarray(values, dtype=np.float64),
+                                                    (nodes.size, self.neq)))
+        L.check(L.lib().pdrd_add_strong_dirichlet(self._h, nodes.size, L.ptr(nodes, C.c_uint32), L.ptr(vals)),
+                "pdrd_add_strong_dirichlet")
+        self.dirichlet_nodes = np.union1d(self.dirichlet_nodes, nodes).astype(np.uint32)
+
+    # -- the iteration -----------------------------------------------------------------------
+    def assemble_rhs(self):
+        L.check(L.lib().pdrd_assemble_rhs(self._h), "pdrd_assemble_rhs")
+
+    def assemble_lhs_and_rhs(self, cfl: float):
+        L.check(L.lib().pdrd_assemble_lhs_and_rhs(self._h, float(cfl)), "pdrd_assemble_lhs_and_rhs")
+
+    def solve(self, solver_options=None):
+        L.check(L.lib().pdrd_solve(self._h), "pdrd_solve")
+
+    def compute_residual_norm(self) -> np.ndarray:
+        out = np.zeros(self.neq)
+        L.check(L.lib().pdrd_compute_residual_norm(self._h, L.ptr(out)), "pdrd_compute_residual_norm")
+        return out
+
+    def residual_norm_L2(self, exclude_dirichlet: bool = False) -> np.ndarray:
+        out = np.zeros(self.neq)
+        L.check(L.lib().pdrd_residual_norm_L2(self._h, int(exclude_dirichlet), L.ptr(out)), "pdrd_residual_norm_L2")
+        return out
+
+    def iterate(self, cfl: float, n_iter: int, exclude_dirichlet: bool = False) -> np.ndarray:
+        out = np.zeros(self.neq)
+        L.check(L.lib().pdrd_iterate(self._h, float(cfl), int(n_iter), int(exclude_dirichlet), L.ptr(out)),
+                "pdrd_iterate")
+        return out
+
+    def check_status(self):
+        L.check(L.lib().pdrd_check_status(self._h), "pdrd_check_status")
+
+    def synchronize(self):
+        L.check(L.lib().pdrd_synchronize(self._h), "pdrd_synchronize")
+
+    def info(self) -> dict:
+        a = np.zeros(6, dtype=np.int64)
+        L.check(L.lib().pdrd_method_info(self._h, L.ptr(a, C.c_int64)), "pdrd_method_info")
+        return dict(nb_nodes=int(a[0]), neq=int(a[1]), nb_blocks=int(a[2]), nb_cells=int(a[3]),
+                    nb_colours=int(a[4]), device_bytes=int(a[5]))
+
+    def last_timing(self):
+        c = C.c_double(0)
+        n = C.c_double(0)
+        L.check(L.lib().pdrd_last_timing(self._h, C.byref(c), C.byref(n)), "pdrd_last_timing")
+        return c.value, n.value
