@@ -1,0 +1,194 @@
+#!/usr/bin/env python3
+"""Benchmark of the MI355X residual-distribution cell loop.
+
+One step = one explicit pseudo-time iteration of PGRDMethodExplicit over the
+whole mesh on the GPU: the fused cell kernel (gather, metrics, P+1 flux,
+K+, B-scheme distribution, scatter), the strong Dirichlet values and the
+nodal pass (residual / V, dt = CFL V / sum lambda, u -= dt res). The default
+workload is 2D Burgers on 1 048 576 jittered P2 triangles with the B scheme
+(pdekit_amd.workloads.c2). Inputs are resident in HBM before the timed region.
+
+Prints one JSON line: throughput in element residual evaluations per second,
+the roofline of the cell kernel (HIP events around every cell-loop launch in
+the timed region) and the CPU oracle timed on a bounded sample.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0      # MI355X HBM3E, GB/s
+SCATTER_NAMES = {0: "atomic", 1: "colour", 2: "patch"}
+FP64_PEAK_TFLOPS = 78.6    # MI355X FP64 vector, TFLOP/s
+
+
+def algorithmic_bytes(wl, scatter: int):
+    """Compulsory HBM bytes of the dominant kernel and of one whole iteration.
+
+    Every iteration must read the solution connectivity (u32 per cell node),
+    the vertex coordinates, u and the dual volumes, and write u. With the
+    patch assembly (scatter 2) the cell loop and the nodal step are one
+    kernel, so both figures are the iteration's. With a global scatter the
+    cell loop alone reads connectivity, coordinates and u and writes the
+    residual and wave-speed accumulators."""
+    m = wl.mesh
+    neq = wl.state.shape[1]
+    conn = sum(4 * b.sol_conn.size for b in m.blocks)
+    coords = 8 * m.coords.size
+    N = m.nb_nodes
+    sweep = conn + coords + 8 * neq * N + 8 * N + 8 * neq * N
+    if scatter == 2:
+        return sweep, sweep
+    cell = conn + coords + 8 * neq * N + 8 * neq * N + 8 * N
+    return cell, sweep
+
+
+def traffic_from_profile(workload: str):
+    """HBM bytes per cell-loop launch measured with rocprofv3 --pmc
+    (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel factor), if a summary
+    for this workload is committed under profiles/."""
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    if not os.path.exists(path):
+        return None
+    try:
+        with open(path) as f:
+            data = json.load(f)
+        return data.get(workload, {}).get("bytes_per_launch")
+    except (OSError, ValueError):
+        return None
+
+
+def cpu_baseline(wl, seconds: float):
+    """The CPU oracle (single-threaded FP64 restatement of the reference cell
+    loop) on the same mesh and state, for a bounded number of iterations."""
+    from oracle.pyoracle import Problem
+    from pdekit_amd.fe import element_tables
+
+    m = wl.mesh
+    tables = {b.shape: element_tables(b.shape, m.order, wl.quad_order[b.shape]) for b in m.blocks}
+    vol = {b.shape: element_tables(b.shape, m.order, m.order) for b in m.blocks}
+    P = Problem(m, tables, vol, wl.physics, wl.scheme, wl.params)
+    t0 = time.perf_counter()
+    P.iterate(wl.state, wl.cfl, 1, wl.dirichlet_nodes, wl.dirichlet_values)
+    t1 = time.perf_counter() - t0
+    n = max(1, int(seconds / max(t1, 1e-3)))
+    t0 = time.perf_counter()
+    P.iterate(wl.state, wl.cfl, n, wl.dirichlet_nodes, wl.dirichlet_values)
+    t = time.perf_counter() - t0
+    return {"value": wl.nb_elements * n / t, "unit": "element residual evals/s", "cores": 1, "kind": "port",
+            "sample": f"{n} full iterations of the same mesh and state ({wl.nb_elements} elements), "
+                      f"oracle/rd_oracle.cpp, 1 thread, {t:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--workload", default="c2", choices=["c1", "c2", "c3", "c4"])
+    ap.add_argument("--scatter", default="auto", choices=["auto", "atomic", "colour", "patch"])
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: this build has no multi-GPU path (--gpus {args.gpus} requested). "
+              "Run with --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+
+    from pdekit_amd import _lib as L
+    from pdekit_amd.rdm import PGRDMethodExplicit
+    from pdekit_amd.workloads import BUILDERS
+
+    t_setup = time.perf_counter()
+    wl = BUILDERS[args.workload]()
+    E = wl.nb_elements
+    meth = PGRDMethodExplicit(wl.physics, wl.scheme, params=wl.params if wl.params else None)
+    meth.configure_mesh_data(wl.mesh).initialize_work_data(quad_order=wl.quad_order)
+    meth.set_vec_function("solution", wl.state)
+    if wl.dirichlet_nodes.size:
+        meth.add_boundary_condition("StrongDirichlet", "inflow", wl.dirichlet_nodes, wl.dirichlet_values)
+    setup_s = time.perf_counter() - t_setup
+    print(f"[bench] {wl.name}: {E} elements, {wl.mesh.nb_nodes} nodes, setup {setup_s:.1f} s", file=sys.stderr,
+          flush=True)
+
+    # scatter: measured choice between FP64 atomics and colour-by-colour launches
+    modes = {"atomic": [0], "colour": [1], "patch": [2], "auto": [2, 1, 0]}[args.scatter]
+    best = None
+    for mode in modes:
+        meth.set_option(L.OPT_SCATTER, mode)
+        meth.set_option(L.OPT_TIMING, 0)
+        meth.set_vec_function("solution", wl.state)
+        meth.iterate(wl.cfl, max(2, args.warmup))  # first launches load the code object
+        meth.set_option(L.OPT_TIMING, 1)
+        meth.iterate(wl.cfl, max(2, args.warmup))
+        cell_ms, node_ms = meth.last_timing()
+        print(f"[bench] scatter={SCATTER_NAMES[mode]}: cell loop {cell_ms * 1e3:.1f} us, "
+              f"nodal pass {node_ms * 1e3:.1f} us", file=sys.stderr, flush=True)
+        if best is None or cell_ms < best[1]:
+            best = (mode, cell_ms)
+    meth.set_option(L.OPT_SCATTER, best[0])
+
+    # warm-up (graph capture), then the timed region
+    meth.set_option(L.OPT_TIMING, 0)
+    meth.set_vec_function("solution", wl.state)
+    meth.iterate(wl.cfl, max(1, args.warmup))
+    meth.synchronize()
+    t0 = time.perf_counter()
+    meth.iterate(wl.cfl, args.steps)
+    meth.synchronize()
+    t = time.perf_counter() - t0
+    meth.check_status()
+    value = E * args.steps / t
+
+    # per-launch duration of the cell loop in a second timed pass of the same
+    # length, HIP events on the method's stream around every launch
+    meth.set_option(L.OPT_TIMING, 1)
+    meth.iterate(wl.cfl, args.steps)
+    cell_ms, node_ms = meth.last_timing()
+    info = meth.info()
+
+    cell_bytes, sweep_bytes = algorithmic_bytes(wl, best[0])
+    achieved = cell_bytes / (cell_ms * 1e-3) / 1e9
+    traffic = traffic_from_profile(wl.name)
+    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                "kernel": "rd_patch_kernel (cell loop + nodal step)" if best[0] == 2 else "rd_cell_kernel (cell loop)",
+                "algorithmic_bytes": int(cell_bytes),
+                "avg_launch_us": round(cell_ms * 1e3, 2)}
+    out = {
+        "metric": "element residual evals/s",
+        "value": round(value, 1),
+        "unit": "elements/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(t / args.steps * 1e3, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": wl.name, "elements": E, "nodes": wl.mesh.nb_nodes,
+                   "order": wl.mesh.order, "scheme": ["PGLDA", "PGN", "PGB", "PSI"][wl.scheme],
+                   "scatter": SCATTER_NAMES[best[0]], "colours": info["nb_colours"],
+                   "parallelism": "single GPU"},
+        "roofline": roofline,
+        "sweep_hbm_frac": round(sweep_bytes / (t / args.steps) / 1e9 / HBM_PEAK_GBS, 4),
+        "nodal_pass_us": round(node_ms * 1e3, 2),  # shared-node fix-up with the patch assembly
+    }
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(wl, args.cpu_seconds)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

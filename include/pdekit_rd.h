@@ -59,10 +59,16 @@ enum pdrd_field {
 };
 /* Options of a method. */
 enum pdrd_option {
-  PDRD_OPT_SCATTER = 0,   /* 0 = FP64 atomics, 1 = colour-by-colour (deterministic) */
+  PDRD_OPT_SCATTER = 0,   /* 0 = FP64 atomics, 1 = colour-by-colour launches,
+                             2 = patch assembly (default): Morton-ordered patches of
+                             cells summed in LDS, shared nodes summed by a fix-up pass;
+                             1 and 2 are deterministic */
   PDRD_OPT_PRECISION = 1, /* 0 = plain FP64, 1 = mixed: FP32 interpolation/fluxes,
                              FP64 inverse, accumulation, update and norms */
-  PDRD_OPT_BLOCK = 2      /* threads per block of the cell kernels (64..1024) */
+  PDRD_OPT_BLOCK = 2,     /* threads per block of the cell kernels (64, 128, 192, 256) */
+  PDRD_OPT_TIMING = 3     /* 1: pdrd_iterate runs eagerly and times the cell loop and the
+                             nodal pass of every iteration with HIP events on the method's
+                             stream; pdrd_last_timing then returns per-iteration means */
 };
 
 /* ---- FE inputs (src/interpolation, src/mesh/point_set, shape_function) -- */
@@ -176,9 +182,10 @@ int pdrd_check_status(pdrd_method_t h);
  * (max over blocks), device bytes held. */
 int pdrd_method_info(pdrd_method_t h, int64_t *info6);
 
-/* Per-kernel timing of the last pdrd_iterate/assemble call measured with HIP
- * events on the method's stream: ms of the cell kernels and of the nodal
- * passes. */
+/* Timing of the last pdrd_assemble_rhs (cell loop / nodal pass) or, with
+ * PDRD_OPT_TIMING, the per-iteration means of the last pdrd_iterate, measured
+ * with HIP events on the method's stream (ms). Without PDRD_OPT_TIMING,
+ * pdrd_iterate reports its whole duration as cell_ms and 0 as node_ms. */
 int pdrd_last_timing(pdrd_method_t h, double *cell_ms, double *node_ms);
 
 const char *pdrd_last_error(void);

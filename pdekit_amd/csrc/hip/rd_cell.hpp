@@ -174,36 +174,30 @@ __device__ __forceinline__ void apply_kplus(const Acoustic<D> &ac, double l0, do
   for (int k = 0; k < M; ++k) y[k] += coef * (l0 * x[k] + ac.rp[k] * sp + ac.rm[k] * sm);
 }
 
+// Quadrature loops are fully unrolled for small scalar elements; systems and
+// large elements keep a rolled loop with the flux divergence staged in LDS so
+// that the live state fits the register file.
+template <int NQ, int ND, int M>
+struct RollQ
+{
+  static constexpr bool value = (M > 1) || (NQ * ND > 60);
+};
+
+// Residuals phi[ND][M] and wave speed of one element from its vertex
+// coordinates X and nodal state U. lds_div: this thread's slice of
+// NQ*M doubles (stride `stride` between entries) when the loop is rolled.
 template <int D, int NG, int ND, int NF, int NQ, bool AFFINE, int PH, int SC>
-__global__ __launch_bounds__(256) void rd_cell_kernel(CellArgs a)
+__device__ __forceinline__ void cell_residual(const double *__restrict__ tab, const double (&X)[NG][D],
+                                              const double (&U)[ND][PhysTraits<PH>::M], const PhysParams &pp,
+                                              double (&phi)[ND][PhysTraits<PH>::M], double &wave, bool &bad,
+                                              double *lds_div, int stride)
 {
   constexpr int M = PhysTraits<PH>::M;
   static_assert(PhysTraits<PH>::DIM == D, "physics / element dimension mismatch");
   using O = TabOffsets<D, NG, ND, NF, NQ>;
-  const uint32_t local = xcd_block_index() * blockDim.x + threadIdx.x;
-  if (local >= a.count) return;
-  const uint32_t e = a.first + local;
-  const double *__restrict__ tab = a.tab;
-  bool bad = false;
+  constexpr bool ROLL = RollQ<NQ, ND, M>::value;
 
-  // ---- gather ----------------------------------------------------------
-  double X[NG][D];
-#pragma unroll
-  for (int g = 0; g < NG; ++g) {
-    const size_t v = a.gconn[static_cast<size_t>(g) * a.ne + e];
-#pragma unroll
-    for (int d = 0; d < D; ++d) X[g][d] = a.coords[v * D + d];
-  }
-  uint32_t nid[ND];
-  double U[ND][M];
-#pragma unroll
-  for (int n = 0; n < ND; ++n) {
-    nid[n] = a.sconn[static_cast<size_t>(n) * a.ne + e];
-#pragma unroll
-    for (int k = 0; k < M; ++k) U[n][k] = a.u[static_cast<size_t>(nid[n]) * M + k];
-  }
-
-  // ---- geometry --------------------------------------------------------
+  // ---- geometry ----------------------------------------------------------
   constexpr int NJ = AFFINE ? 1 : NQ;
   double Jinv[NJ][D][D];
   double detJ[NJ];
@@ -222,12 +216,16 @@ __global__ __launch_bounds__(256) void rd_cell_kernel(CellArgs a)
     detJ[q] = det_inv<D>(J, Jinv[q]);
   }
 
-  // ---- P+1 flux element: divergence of the interpolated flux at q -------
-  double div[NQ][M];
+  // ---- P+1 flux element: divergence of the interpolated flux at q ---------
+  double divr[ROLL ? 1 : NQ][M];
 #pragma unroll
-  for (int q = 0; q < NQ; ++q)
+  for (int q = 0; q < (ROLL ? 1 : NQ); ++q)
 #pragma unroll
-    for (int k = 0; k < M; ++k) div[q][k] = 0.0;
+    for (int k = 0; k < M; ++k) divr[q][k] = 0.0;
+  if constexpr (ROLL) {
+#pragma unroll
+    for (int i = 0; i < NQ * M; ++i) lds_div[i * stride] = 0.0;
+  }
 #pragma unroll
   for (int f = 0; f < NF; ++f) {
     double s[M];
@@ -239,9 +237,9 @@ __global__ __launch_bounds__(256) void rd_cell_kernel(CellArgs a)
       s[k] = t;
     }
     double F[D][M];
-    point_flux<PH, double>(s, a.pp, F, bad);
+    point_flux<PH, double>(s, pp, F, bad);
+    double G[AFFINE ? D : 1][M];
     if constexpr (AFFINE) {
-      double G[D][M];
 #pragma unroll
       for (int xi = 0; xi < D; ++xi)
 #pragma unroll
@@ -251,18 +249,19 @@ __global__ __launch_bounds__(256) void rd_cell_kernel(CellArgs a)
           for (int d = 0; d < D; ++d) t += Jinv[0][d][xi] * F[d][k];
           G[xi][k] = t;
         }
+    }
 #pragma unroll
-      for (int q = 0; q < NQ; ++q)
+    for (int q = 0; q < NQ; ++q) {
+      double add[M];
+      if constexpr (AFFINE) {
 #pragma unroll
         for (int k = 0; k < M; ++k) {
-          double t = div[q][k];
+          double t = 0.0;
 #pragma unroll
           for (int xi = 0; xi < D; ++xi) t += tab[O::dVf + (xi * NQ + q) * NF + f] * G[xi][k];
-          div[q][k] = t;
+          add[k] = t;
         }
-    } else {
-#pragma unroll
-      for (int q = 0; q < NQ; ++q) {
+      } else {
         double b[D];
 #pragma unroll
         for (int d = 0; d < D; ++d) {
@@ -273,38 +272,47 @@ __global__ __launch_bounds__(256) void rd_cell_kernel(CellArgs a)
         }
 #pragma unroll
         for (int k = 0; k < M; ++k) {
-          double t = div[q][k];
+          double t = 0.0;
 #pragma unroll
           for (int d = 0; d < D; ++d) t += b[d] * F[d][k];
-          div[q][k] = t;
+          add[k] = t;
         }
+      }
+#pragma unroll
+      for (int k = 0; k < M; ++k) {
+        if constexpr (ROLL)
+          lds_div[(q * M + k) * stride] += add[k];
+        else
+          divr[q][k] += add[k];
       }
     }
   }
 
-  // ---- element scheme ----------------------------------------------------
+  // ---- element scheme ------------------------------------------------------
   constexpr bool NDIS = (SC != SC_LDA);
   double lda[ND][M];
-  double dis[ND][M];
+  double dis[NDIS ? ND : 1][M];
   double lam[ND];
   double fint[M];
 #pragma unroll
   for (int n = 0; n < ND; ++n) {
     lam[n] = 0.0;
 #pragma unroll
-    for (int k = 0; k < M; ++k) { lda[n][k] = 0.0; dis[n][k] = 0.0; }
+    for (int k = 0; k < M; ++k) lda[n][k] = 0.0;
   }
+#pragma unroll
+  for (int n = 0; n < (NDIS ? ND : 1); ++n)
+#pragma unroll
+    for (int k = 0; k < M; ++k) dis[n][k] = 0.0;
 #pragma unroll
   for (int k = 0; k < M; ++k) fint[k] = 0.0;
   double vol = 0.0;
 
-#pragma unroll
-  for (int q = 0; q < NQ; ++q) {
-    const int jq = AFFINE ? 0 : q;
-    const double wj = tab[O::w + q] * detJ[jq];
+  auto quad_point = [&](const int q, const double (&r)[M], const double (&Jq)[D][D], const double detq) {
+    const double wj = tab[O::w + q] * detq;
     vol += wj;
 #pragma unroll
-    for (int k = 0; k < M; ++k) fint[k] += wj * div[q][k];
+    for (int k = 0; k < M; ++k) fint[k] += wj * r[k];
     // physical gradients of the solution basis at q
     double gphys[ND][D];
 #pragma unroll
@@ -313,15 +321,15 @@ __global__ __launch_bounds__(256) void rd_cell_kernel(CellArgs a)
       for (int x = 0; x < D; ++x) {
         double t = 0.0;
 #pragma unroll
-        for (int xi = 0; xi < D; ++xi) t += Jinv[jq][x][xi] * tab[O::dV + (xi * NQ + q) * ND + n];
+        for (int xi = 0; xi < D; ++xi) t += Jq[x][xi] * tab[O::dV + (xi * NQ + q) * ND + n];
         gphys[n][x] = t;
       }
 
     if constexpr (M == 1) {
       double vel0, vel1;
       if constexpr (PH == PH_ADVECTION2D) {
-        vel0 = a.pp.p0;
-        vel1 = a.pp.p1;
+        vel0 = pp.p0;
+        vel1 = pp.p1;
       } else {
         double uq = 0.0;
 #pragma unroll
@@ -334,17 +342,18 @@ __global__ __launch_bounds__(256) void rd_cell_kernel(CellArgs a)
 #pragma unroll
       for (int n = 0; n < ND; ++n) {
         const double kn = vel0 * gphys[n][0] + vel1 * gphys[n][1];
-        kp[n] = fmax(0.0, kn);
-        lam[n] = fmax(lam[n], fmax(1.e-6, fabs(kn)));
+        const double ak = fabs(kn);
+        kp[n] = kn > 0.0 ? kn : 0.0;
+        const double l = ak > 1.e-6 ? ak : 1.e-6;
+        lam[n] = l > lam[n] ? l : lam[n];
         S += kp[n];
         SU += kp[n] * U[n][0];
       }
       const double inv = 1.0 / S;
-      const double r = div[q][0];
       const double ub = SU * inv;
 #pragma unroll
       for (int n = 0; n < ND; ++n) {
-        lda[n][0] += wj * ((kp[n] * inv) * r);
+        lda[n][0] += wj * ((kp[n] * inv) * r[0]);
         if constexpr (NDIS) dis[n][0] += wj * (kp[n] * (U[n][0] - ub));
       }
     } else {
@@ -356,9 +365,9 @@ __global__ __launch_bounds__(256) void rd_cell_kernel(CellArgs a)
         for (int n = 0; n < ND; ++n) t += tab[O::V + q * ND + n] * U[n][k];
         uq[k] = t;
       }
-      const EulerState<double, D> st = euler_state<double, D>(uq, a.pp.p0);
+      const EulerState<double, D> st = euler_state<double, D>(uq, pp.p0);
       bad = bad || st.bad;
-      const double gm1 = a.pp.p0 - 1.0;
+      const double gm1 = pp.p0 - 1.0;
       // compact K+ of every node
       double l0[ND], dp[ND], dm[ND], nh[ND][D];
 #pragma unroll
@@ -367,8 +376,7 @@ __global__ __launch_bounds__(256) void rd_cell_kernel(CellArgs a)
 #pragma unroll
         for (int x = 0; x < D; ++x) nn += gphys[n][x] * gphys[n][x];
         const double norm = sqrt(nn);
-        const double small = (D == 2) ? 1.e-9 : 0.0;
-        const bool tiny = (D == 2) ? (norm < small) : (norm == 0.0);
+        const bool tiny = (D == 2) ? (norm < 1.e-9) : (norm == 0.0);
         const double inorm = tiny ? 0.0 : 1.0 / norm;
         double um = 0.0;
 #pragma unroll
@@ -393,7 +401,7 @@ __global__ __launch_bounds__(256) void rd_cell_kernel(CellArgs a)
       for (int i = 0; i < M; ++i) {
 #pragma unroll
         for (int j = 0; j < M; ++j) A[i][j] = (i == j) ? sl0 : 0.0;
-        B[i][0] = div[q][i];
+        B[i][0] = r[i];
         B[i][1] = 0.0;
       }
 #pragma unroll
@@ -430,10 +438,43 @@ __global__ __launch_bounds__(256) void rd_cell_kernel(CellArgs a)
         }
       }
     }
+  };
+
+  if constexpr (ROLL) {
+#pragma unroll 1
+    for (int q = 0; q < NQ; ++q) {
+      double r[M];
+#pragma unroll
+      for (int k = 0; k < M; ++k) r[k] = lds_div[(q * M + k) * stride];
+      if constexpr (AFFINE) {
+        quad_point(q, r, Jinv[0], detJ[0]);
+      } else {
+        // recompute the Jacobian at q instead of indexing a per-q array
+        double J[D][D], Ji[D][D];
+#pragma unroll
+        for (int xi = 0; xi < D; ++xi)
+#pragma unroll
+          for (int x = 0; x < D; ++x) {
+            double t = 0.0;
+#pragma unroll
+            for (int g = 0; g < NG; ++g) t += tab[O::dVg + (xi * NQ + q) * NG + g] * X[g][x];
+            J[xi][x] = t;
+          }
+        const double dq = det_inv<D>(J, Ji);
+        quad_point(q, r, Ji, dq);
+      }
+    }
+  } else {
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+      double r[M];
+#pragma unroll
+      for (int k = 0; k < M; ++k) r[k] = divr[ROLL ? 0 : q][k];
+      quad_point(q, r, Jinv[AFFINE ? 0 : q], detJ[AFFINE ? 0 : q]);
+    }
   }
 
-  // ---- blend / limit -----------------------------------------------------
-  double phi[ND][M];
+  // ---- blend / limit -------------------------------------------------------
   if constexpr (SC == SC_LDA) {
 #pragma unroll
     for (int n = 0; n < ND; ++n)
@@ -474,9 +515,45 @@ __global__ __launch_bounds__(256) void rd_cell_kernel(CellArgs a)
   double lmax = lam[0];
 #pragma unroll
   for (int n = 1; n < ND; ++n) lmax = fmax(lmax, lam[n]);
-  const double wave = vol * lmax;
+  wave = vol * lmax;
+}
 
-  // ---- scatter -------------------------------------------------------------
+template <int D, int NG, int NQ, int M>
+__device__ __forceinline__ void gather_coords(const CellArgs &a, uint32_t e, double (&X)[NG][D])
+{
+#pragma unroll
+  for (int g = 0; g < NG; ++g) {
+    const size_t v = a.gconn[static_cast<size_t>(g) * a.ne + e];
+#pragma unroll
+    for (int d = 0; d < D; ++d) X[g][d] = a.coords[v * D + d];
+  }
+}
+
+// Element-per-thread cell loop with a global scatter: FP64 atomics or, one
+// colour per launch, plain read-modify-write.
+template <int D, int NG, int ND, int NF, int NQ, bool AFFINE, int PH, int SC>
+__global__ __launch_bounds__(256) void rd_cell_kernel(CellArgs a)
+{
+  constexpr int M = PhysTraits<PH>::M;
+  extern __shared__ double lds[];
+  const uint32_t local = xcd_block_index() * blockDim.x + threadIdx.x;
+  if (local >= a.count) return;
+  const uint32_t e = a.first + local;
+  bool bad = false;
+  double X[NG][D];
+  gather_coords<D, NG, NQ, M>(a, e, X);
+  uint32_t nid[ND];
+  double U[ND][M];
+#pragma unroll
+  for (int n = 0; n < ND; ++n) {
+    nid[n] = a.sconn[static_cast<size_t>(n) * a.ne + e];
+#pragma unroll
+    for (int k = 0; k < M; ++k) U[n][k] = a.u[static_cast<size_t>(nid[n]) * M + k];
+  }
+  double phi[ND][M];
+  double wave;
+  cell_residual<D, NG, ND, NF, NQ, AFFINE, PH, SC>(a.tab, X, U, a.pp, phi, wave, bad, lds + threadIdx.x,
+                                                   static_cast<int>(blockDim.x));
 #pragma unroll
   for (int n = 0; n < ND; ++n) {
     const size_t b = static_cast<size_t>(nid[n]);
@@ -485,6 +562,247 @@ __global__ __launch_bounds__(256) void rd_cell_kernel(CellArgs a)
     scatter_add(a.acc_ws + b, wave, a.atomic);
   }
   if (bad) atomicOr(a.status, 1);
+}
+
+// ---------------------------------------------------------------------------
+// Patch assembly: one workgroup owns one patch of up to PATCH cells (a
+// spatially compact run of the Morton-ordered cells). The states of the
+// patch's nodes are gathered once into LDS, the cells' residuals are summed
+// into LDS in colour rounds (cells of one colour share no node, so the sum
+// order is fixed), then
+//   * nodes held by this patch only are finished here: residual / V, wave
+//     speed, and -- in iteration mode -- dt = CFL V / sum lambda,
+//     u -= dt res (after the strong Dirichlet value),
+//   * nodes shared with other patches write their partial sums to their slot;
+//     patch_fixup_kernel sums the slots of every shared node in a fixed order.
+// ---------------------------------------------------------------------------
+struct PatchArgs
+{
+  const double *__restrict__ tab;
+  const uint32_t *__restrict__ gconn;       // [NG][ne] global vertex ids
+  const uint16_t *__restrict__ lconn;       // [ND][ne] patch-local node indices
+  const uint32_t *__restrict__ pnode_off;   // [np + 1] first slot of each patch
+  const uint32_t *__restrict__ pnodes;      // [slots] global node id of each slot
+  const uint32_t *__restrict__ pn_inner;    // [np] number of patch-only nodes (listed first)
+  const uint16_t *__restrict__ pcol_off;    // [np][PCOL_STRIDE] colour offsets inside the patch
+  const double *__restrict__ coords;
+  double *__restrict__ u;                   // [N][M] (updated in place in iteration mode)
+  const double *__restrict__ vol;           // [N]
+  const int *__restrict__ dir_index;        // [N] row in dir_vals or -1
+  const double *__restrict__ dir_vals;      // [ndir][M]
+  double *__restrict__ res;                 // [N][M]
+  double *__restrict__ dt;                  // [N]
+  double *__restrict__ ws;                  // [N]
+  double *__restrict__ slot_acc;            // [slots][M + 1] partial sums of shared nodes
+  double *__restrict__ norm_partial;        // [np][M + 1]
+  int *__restrict__ status;
+  uint32_t ne, np, first_patch;             // first_patch: offset of this block's patches
+  int maxl;                                 // LDS rows of node state
+  int update;                               // 0: assemble (res, ws), 1: iterate
+  int outputs;                              // iterate: also store res, dt, ws and norm partials
+  int excl;                                 // norm without Dirichlet rows
+  double cfl;
+  PhysParams pp;
+};
+
+// Colour offsets of a patch: PCOL_STRIDE u16 entries, offsets [0..ncol] and
+// ncol in the last entry.
+constexpr int MAXCOL = 47;
+constexpr int PCOL_STRIDE = MAXCOL + 2;
+
+// Cells per patch (= threads per workgroup): 256 for scalar physics, 128 for
+// the Euler systems whose per-thread state and LDS slices are larger.
+template <int M>
+struct PatchSize
+{
+  static constexpr int value = (M == 1) ? 256 : 128;
+};
+
+// Dynamic LDS of one patch workgroup, in doubles: node state, node sums and
+// a work region shared by the flux-divergence slices and the final reduction.
+template <int NQ, int ND, int M>
+__host__ __device__ constexpr size_t patch_lds_doubles(int maxl)
+{
+  constexpr int P = PatchSize<M>::value;
+  constexpr size_t div = RollQ<NQ, ND, M>::value ? static_cast<size_t>(NQ) * M * P : 0;
+  constexpr size_t red = static_cast<size_t>(P) * (M + 1);
+  return static_cast<size_t>(maxl) * (2 * M + 1) + (div > red ? div : red);
+}
+
+template <int M>
+__device__ __forceinline__ void finish_node(const PatchArgs &a, uint32_t g, const double *acc, double (&sq)[M + 1])
+{
+  const double v = a.vol[g];
+  const double iv = 1.0 / v;
+  const double w = acc[M];
+  if (a.update) {
+    const double d = a.cfl * v / w;
+    const int di = a.dir_index[g];
+    const bool count = a.outputs && !(a.excl && di >= 0);
+#pragma unroll
+    for (int k = 0; k < M; ++k) {
+      const size_t j = static_cast<size_t>(g) * M + k;
+      const double r = acc[k] * iv;
+      const double u0 = (di >= 0) ? a.dir_vals[static_cast<size_t>(di) * M + k] : a.u[j];
+      a.u[j] = u0 - d * r;
+      if (a.outputs) a.res[j] = r;
+      if (count) sq[k] += r * r;
+    }
+    if (count) sq[M] += 1.0;
+    if (a.outputs) {
+      a.dt[g] = d;
+      a.ws[g] = w;
+    }
+  } else {
+#pragma unroll
+    for (int k = 0; k < M; ++k) a.res[static_cast<size_t>(g) * M + k] = acc[k] * iv;
+    a.ws[g] = w;
+  }
+}
+
+template <int M>
+__device__ __forceinline__ void block_sum_store(double (&sq)[M + 1], double *red, double *out)
+{
+  // red: blockDim * (M + 1) doubles of LDS
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k <= M; ++k) red[k * blockDim.x + threadIdx.x] = sq[k];
+  __syncthreads();
+  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s) {
+#pragma unroll
+      for (int k = 0; k <= M; ++k) red[k * blockDim.x + threadIdx.x] += red[k * blockDim.x + threadIdx.x + s];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int k = 0; k <= M; ++k) out[k] = red[k * blockDim.x];
+  }
+}
+
+template <int D, int NG, int ND, int NF, int NQ, bool AFFINE, int PH, int SC>
+__global__ __launch_bounds__(PatchSize<PhysTraits<PH>::M>::value) void rd_patch_kernel(PatchArgs a)
+{
+  constexpr int M = PhysTraits<PH>::M;
+  constexpr int PATCH = PatchSize<M>::value;
+  constexpr bool ROLL = RollQ<NQ, ND, M>::value;
+  extern __shared__ double lds[];
+  // LDS layout: node state [maxl][M], node sums [maxl][M+1], then a work
+  // region: the per-thread flux-divergence slices [NQ*M][PATCH] while the
+  // cells are computed, the norm reduction at the end
+  double *ul = lds;
+  double *acc = lds + static_cast<size_t>(a.maxl) * M;
+  double *work = acc + static_cast<size_t>(a.maxl) * (M + 1);
+  double *div_lds = work;
+
+  const uint32_t pl = xcd_block_index();
+  if (pl >= a.np) return;
+  const uint32_t p = a.first_patch + pl;
+  const uint32_t s0 = a.pnode_off[p];
+  const int nl = static_cast<int>(a.pnode_off[p + 1] - s0);
+  const int tid = threadIdx.x;
+  const uint32_t e0 = pl * PATCH;
+  const int ecount = static_cast<int>(min(static_cast<uint32_t>(PATCH), a.ne - e0));
+
+  for (int l = tid; l < nl; l += PATCH) {
+    const size_t g = a.pnodes[s0 + l];
+#pragma unroll
+    for (int k = 0; k < M; ++k) ul[l * M + k] = a.u[g * M + k];
+#pragma unroll
+    for (int k = 0; k <= M; ++k) acc[l * (M + 1) + k] = 0.0;
+  }
+  __syncthreads();
+
+  bool bad = false;
+  double phi[ND][M];
+  double wave = 0.0;
+  uint16_t loc[ND];
+  const bool active = tid < ecount;
+  if (active) {
+    const uint32_t e = e0 + tid;
+    double X[NG][D];
+#pragma unroll
+    for (int g = 0; g < NG; ++g) {
+      const size_t v = a.gconn[static_cast<size_t>(g) * a.ne + e];
+#pragma unroll
+      for (int d = 0; d < D; ++d) X[g][d] = a.coords[v * D + d];
+    }
+    double U[ND][M];
+#pragma unroll
+    for (int n = 0; n < ND; ++n) {
+      loc[n] = a.lconn[static_cast<size_t>(n) * a.ne + e];
+#pragma unroll
+      for (int k = 0; k < M; ++k) U[n][k] = ul[loc[n] * M + k];
+    }
+    cell_residual<D, NG, ND, NF, NQ, AFFINE, PH, SC>(a.tab, X, U, a.pp, phi, wave, bad,
+                                                     ROLL ? div_lds + tid : nullptr, PATCH);
+  }
+  // colour rounds: cells [pcol[c], pcol[c+1]) of the patch share no node
+  const uint16_t *pcol = a.pcol_off + static_cast<size_t>(p) * PCOL_STRIDE;
+  const int ncol = pcol[PCOL_STRIDE - 1];
+  for (int c = 0; c < ncol; ++c) {
+    __syncthreads();
+    if (active && tid >= pcol[c] && tid < pcol[c + 1]) {
+#pragma unroll
+      for (int n = 0; n < ND; ++n) {
+        double *row = acc + loc[n] * (M + 1);
+#pragma unroll
+        for (int k = 0; k < M; ++k) row[k] += phi[n][k];
+        row[M] += wave;
+      }
+    }
+  }
+  __syncthreads();
+
+  const int ninner = static_cast<int>(a.pn_inner[p]);
+  double sq[M + 1];
+#pragma unroll
+  for (int k = 0; k <= M; ++k) sq[k] = 0.0;
+  for (int l = tid; l < nl; l += PATCH) {
+    const double *row = acc + l * (M + 1);
+    if (l < ninner) {
+      finish_node<M>(a, a.pnodes[s0 + l], row, sq);
+    } else {
+      double *slot = a.slot_acc + static_cast<size_t>(s0 + l) * (M + 1);
+#pragma unroll
+      for (int k = 0; k <= M; ++k) slot[k] = row[k];
+    }
+  }
+  if (bad) atomicOr(a.status, 1);
+  if (a.update && a.outputs) block_sum_store<M>(sq, work, a.norm_partial + static_cast<size_t>(p) * (M + 1));
+}
+
+struct FixupArgs
+{
+  const uint32_t *__restrict__ nodes;     // [nb] shared nodes
+  const uint32_t *__restrict__ slot_off;  // [nb + 1]
+  const uint32_t *__restrict__ slots;     // slot ids, ascending per node
+  const double *__restrict__ slot_acc;
+  PatchArgs pa;                           // node outputs as in the patch kernel
+  uint32_t nb;
+  double *__restrict__ norm_partial;      // [gridDim.x][M + 1]
+};
+
+template <int M>
+__global__ __launch_bounds__(256) void patch_fixup_kernel(FixupArgs f)
+{
+  __shared__ double red[256 * (M + 1)];
+  double sq[M + 1];
+#pragma unroll
+  for (int k = 0; k <= M; ++k) sq[k] = 0.0;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < f.nb; i += gridDim.x * blockDim.x) {
+    double acc[M + 1];
+#pragma unroll
+    for (int k = 0; k <= M; ++k) acc[k] = 0.0;
+    for (uint32_t s = f.slot_off[i]; s < f.slot_off[i + 1]; ++s) {
+      const double *row = f.slot_acc + static_cast<size_t>(f.slots[s]) * (M + 1);
+#pragma unroll
+      for (int k = 0; k <= M; ++k) acc[k] += row[k];
+    }
+    finish_node<M>(f.pa, f.nodes[i], acc, sq);
+  }
+  if (f.pa.update && f.pa.outputs) block_sum_store<M>(sq, red, f.norm_partial + static_cast<size_t>(blockIdx.x) * (M + 1));
 }
 
 }  // namespace pdrd

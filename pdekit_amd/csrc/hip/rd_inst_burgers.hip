@@ -5,10 +5,10 @@
 namespace pdrd {
 
 #define PHYS_ PH_BURGERS2D
-CellLauncher find_cell_scalar_burgers(const CellKey &k)
+KernelSet find_cell_scalar_burgers(const CellKey &k)
 {
   PDRD_ELEMENT_TYPES(PDRD_CELL_CASE)
-  return nullptr;
+  return KernelSet{};
 }
 #undef PHYS_
 

@@ -8,13 +8,24 @@
 namespace pdrd {
 
 using CellLauncher = hipError_t (*)(const CellArgs &, unsigned grid, unsigned block, hipStream_t);
+using PatchLauncher = hipError_t (*)(const PatchArgs &, unsigned grid, hipStream_t);
 
 struct CellKey
 {
   int shape, sol_order, nq, physics, scheme;
 };
 
-// Element types built: (shape, P, quadrature points) with their sizes.
+// Everything the host needs to launch one instantiation.
+struct KernelSet
+{
+  CellLauncher cell = nullptr;    // element per thread, global scatter
+  PatchLauncher patch = nullptr;  // patch assembly
+  int patch_size = 0;             // cells per patch workgroup
+  size_t (*patch_lds_bytes)(int maxl) = nullptr;
+  explicit operator bool() const { return cell != nullptr; }
+};
+
+// Element types built: (shape, P, dim, ng, nd, nf, quadrature points, affine)
 //   triangle P1 / P4 rule (6 points), P2 / P4 rule, P3 / P6 rule (12 points),
 //   quadrilateral Q1 / P4 rule (8 points), tetrahedron P1 / P2 rule (4 points)
 #define PDRD_ELEMENT_TYPES(X)                      \
@@ -27,17 +38,46 @@ struct CellKey
 template <int D, int NG, int ND, int NF, int NQ, bool AFF, int PH, int SC>
 hipError_t launch_cell(const CellArgs &a, unsigned grid, unsigned block, hipStream_t s)
 {
-  hipLaunchKernelGGL((rd_cell_kernel<D, NG, ND, NF, NQ, AFF, PH, SC>), dim3(grid), dim3(block), 0, s, a);
+  constexpr int M = PhysTraits<PH>::M;
+  const size_t lds = RollQ<NQ, ND, M>::value ? static_cast<size_t>(NQ) * M * block * sizeof(double) : 0;
+  hipLaunchKernelGGL((rd_cell_kernel<D, NG, ND, NF, NQ, AFF, PH, SC>), dim3(grid), dim3(block), lds, s, a);
   return hipGetLastError();
 }
 
-// Lookup functions, one per compilation unit of instantiations.
-CellLauncher find_cell_scalar_adv(const CellKey &k);
-CellLauncher find_cell_scalar_burgers(const CellKey &k);
-CellLauncher find_cell_euler2d(const CellKey &k);
-CellLauncher find_cell_euler3d(const CellKey &k);
+template <int D, int NG, int ND, int NF, int NQ, bool AFF, int PH, int SC>
+hipError_t launch_patch(const PatchArgs &a, unsigned grid, hipStream_t s)
+{
+  constexpr int M = PhysTraits<PH>::M;
+  const size_t lds = patch_lds_doubles<NQ, ND, M>(a.maxl) * sizeof(double);
+  hipLaunchKernelGGL((rd_patch_kernel<D, NG, ND, NF, NQ, AFF, PH, SC>), dim3(grid), dim3(PatchSize<M>::value), lds, s,
+                     a);
+  return hipGetLastError();
+}
 
-inline CellLauncher find_cell(const CellKey &k)
+template <int NQ, int ND, int M>
+size_t patch_lds_bytes(int maxl)
+{
+  return patch_lds_doubles<NQ, ND, M>(maxl) * sizeof(double);
+}
+
+template <int D, int NG, int ND, int NF, int NQ, bool AFF, int PH, int SC>
+KernelSet kernel_set()
+{
+  KernelSet k;
+  k.cell = &launch_cell<D, NG, ND, NF, NQ, AFF, PH, SC>;
+  k.patch = &launch_patch<D, NG, ND, NF, NQ, AFF, PH, SC>;
+  k.patch_size = PatchSize<PhysTraits<PH>::M>::value;
+  k.patch_lds_bytes = &patch_lds_bytes<NQ, ND, PhysTraits<PH>::M>;
+  return k;
+}
+
+// Lookup functions, one per compilation unit of instantiations.
+KernelSet find_cell_scalar_adv(const CellKey &k);
+KernelSet find_cell_scalar_burgers(const CellKey &k);
+KernelSet find_cell_euler2d(const CellKey &k);
+KernelSet find_cell_euler3d(const CellKey &k);
+
+inline KernelSet find_cell(const CellKey &k)
 {
   switch (k.physics) {
     case PH_ADVECTION2D: return find_cell_scalar_adv(k);
@@ -45,29 +85,25 @@ inline CellLauncher find_cell(const CellKey &k)
     case PH_EULER2D: return find_cell_euler2d(k);
     case PH_EULER3D: return find_cell_euler3d(k);
   }
-  return nullptr;
+  return KernelSet{};
 }
 
-}  // namespace pdrd
-
-namespace pdrd {
-
 template <int SH, int P, int D, int NG, int ND, int NF, int NQ, bool AFF, int PH>
-CellLauncher pick_cell(const CellKey &k)
+KernelSet pick_cell(const CellKey &k)
 {
   if constexpr (D != PhysTraits<PH>::DIM) {
     (void)k;
-    return nullptr;
+    return KernelSet{};
   } else {
     if (k.shape == SH && k.sol_order == P && k.nq == NQ) {
       switch (k.scheme) {
-        case SC_LDA: return &launch_cell<D, NG, ND, NF, NQ, AFF, PH, SC_LDA>;
-        case SC_N: return &launch_cell<D, NG, ND, NF, NQ, AFF, PH, SC_N>;
-        case SC_B: return &launch_cell<D, NG, ND, NF, NQ, AFF, PH, SC_B>;
-        case SC_PSI: return &launch_cell<D, NG, ND, NF, NQ, AFF, PH, SC_PSI>;
+        case SC_LDA: return kernel_set<D, NG, ND, NF, NQ, AFF, PH, SC_LDA>();
+        case SC_N: return kernel_set<D, NG, ND, NF, NQ, AFF, PH, SC_N>();
+        case SC_B: return kernel_set<D, NG, ND, NF, NQ, AFF, PH, SC_B>();
+        case SC_PSI: return kernel_set<D, NG, ND, NF, NQ, AFF, PH, SC_PSI>();
       }
     }
-    return nullptr;
+    return KernelSet{};
   }
 }
 
@@ -76,4 +112,4 @@ CellLauncher pick_cell(const CellKey &k)
 // Body of a lookup function for one physics (PHYS_): every element type of
 // matching dimension and every scheme.
 #define PDRD_CELL_CASE(SH, P, D, NG, ND, NF, NQ, AFF) \
-  if (CellLauncher f = pick_cell<SH, P, D, NG, ND, NF, NQ, AFF, PHYS_>(k)) return f;
+  if (KernelSet f = pick_cell<SH, P, D, NG, ND, NF, NQ, AFF, PHYS_>(k)) return f;

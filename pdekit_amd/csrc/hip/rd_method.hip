@@ -16,6 +16,7 @@
 #include "../host/capi_common.hpp"
 #include "../host/fe_tables.hpp"
 #include "../host/mesh.hpp"
+#include "../host/patches.hpp"
 #include "rd_launch.hpp"
 
 using pdrd::set_error;
@@ -36,7 +37,12 @@ struct Block
   double *d_tab = nullptr;
   uint32_t *d_gconn = nullptr;  // [ng][ne] in colour-sorted order
   uint32_t *d_sconn = nullptr;  // [nd][ne]
-  pdrd::CellLauncher launch = nullptr;
+  pdrd::KernelSet ks;
+  // patch assembly (Morton-ordered patches)
+  uint32_t *d_pgconn = nullptr;  // [ng][ne] in patch order
+  uint16_t *d_lconn = nullptr;   // [nd][ne] patch-local node indices
+  uint32_t first_patch = 0, np = 0;
+  int maxl = 0;
 };
 
 }  // namespace
@@ -52,9 +58,12 @@ struct pdrd_method
   int device = 0;
 
   // options
-  int scatter_mode = 0;  // 0 atomics, 1 colours
+  int scatter_mode = 2;  // 0 atomics, 1 colours, 2 patches
+  bool patch_ok = false;
+  std::string patch_reason;
   int precision = 0;
   int block_threads = 256;
+  int timing = 0;
 
   // device state
   double *d_coords = nullptr, *d_u = nullptr, *d_res = nullptr, *d_dt = nullptr, *d_ws = nullptr;
@@ -63,7 +72,16 @@ struct pdrd_method
   uint32_t *d_dir_nodes = nullptr;
   double *d_dir_vals = nullptr;
   unsigned char *d_is_dir = nullptr;
+  int *d_dir_index = nullptr;  // [N] row of the node's Dirichlet values or -1
   uint32_t n_dir = 0;
+  std::vector<uint32_t> dir_nodes_host;
+  std::vector<double> dir_vals_host;
+  // patch plan
+  uint32_t np_total = 0, n_shared = 0;
+  uint32_t *d_pnode_off = nullptr, *d_pnodes = nullptr, *d_pn_inner = nullptr;
+  uint16_t *d_pcol = nullptr;
+  uint32_t *d_shared_nodes = nullptr, *d_shared_off = nullptr, *d_shared_slots = nullptr;
+  double *d_slot_acc = nullptr, *d_patch_partial = nullptr;
   size_t bytes = 0;
 
   hipStream_t stream = nullptr;
@@ -257,7 +275,8 @@ int field_size(const pdrd_method *h, int field, size_t *count, double **dptr)
   return set_error(PDRD_EINVAL, "unknown field");
 }
 
-// Cell loop over every block (and colour), accumulating into acc_res/acc_ws.
+// Cell loop of the atomic / colour modes over every block (and colour),
+// accumulating into acc_res / acc_ws.
 int launch_cells(pdrd_method *h)
 {
   pdrd::CellArgs a{};
@@ -277,17 +296,93 @@ int launch_cells(pdrd_method *h)
       a.atomic = 1;
       a.first = 0;
       a.count = b.ne;
-      HIP_CHECK(b.launch(a, grid_for(b.ne, h->block_threads), h->block_threads, h->stream));
+      HIP_CHECK(b.ks.cell(a, grid_for(b.ne, h->block_threads), h->block_threads, h->stream));
     } else {
       a.atomic = 0;
       for (size_t c = 0; c + 1 < b.colour_start.size(); ++c) {
         a.first = b.colour_start[c];
         a.count = b.colour_start[c + 1] - b.colour_start[c];
         if (a.count == 0) continue;
-        HIP_CHECK(b.launch(a, grid_for(a.count, h->block_threads), h->block_threads, h->stream));
+        HIP_CHECK(b.ks.cell(a, grid_for(a.count, h->block_threads), h->block_threads, h->stream));
       }
     }
   }
+  return PDRD_OK;
+}
+
+unsigned fixup_blocks(const pdrd_method *h)
+{
+  return std::min<unsigned>(kNormMaxBlocks, std::max<unsigned>(1, grid_for(h->n_shared, 256)));
+}
+
+pdrd::PatchArgs patch_args(const pdrd_method *h, const Block *b, int update, int outputs, int excl, double cfl)
+{
+  pdrd::PatchArgs a{};
+  if (b) {
+    a.tab = b->d_tab;
+    a.gconn = b->d_pgconn;
+    a.lconn = b->d_lconn;
+    a.ne = b->ne;
+    a.np = b->np;
+    a.first_patch = b->first_patch;
+    a.maxl = b->maxl;
+  }
+  a.pnode_off = h->d_pnode_off;
+  a.pnodes = h->d_pnodes;
+  a.pn_inner = h->d_pn_inner;
+  a.pcol_off = h->d_pcol;
+  a.coords = h->d_coords;
+  a.u = h->d_u;
+  a.vol = h->d_vol;
+  a.dir_index = h->d_dir_index;
+  a.dir_vals = h->d_dir_vals;
+  a.res = h->d_res;
+  a.dt = h->d_dt;
+  a.ws = h->d_ws;
+  a.slot_acc = h->d_slot_acc;
+  a.norm_partial = h->d_patch_partial;
+  a.status = h->d_status;
+  a.update = update;
+  a.outputs = outputs;
+  a.excl = excl;
+  a.cfl = cfl;
+  a.pp.p0 = h->params[0];
+  a.pp.p1 = h->params[1];
+  return a;
+}
+
+// Patch assembly of every block followed by the sums of the shared nodes.
+// update = 0: residual and wave speed of every node; update = 1: one
+// pseudo-time iteration (the strong Dirichlet values are applied in the
+// nodal step), with res / dt / ws / norm partials when outputs is set.
+int launch_patches(pdrd_method *h, int update, int outputs, int excl, double cfl)
+{
+  for (Block &b : h->blocks) {
+    const pdrd::PatchArgs a = patch_args(h, &b, update, outputs, excl, cfl);
+    HIP_CHECK(b.ks.patch(a, b.np, h->stream));
+  }
+  if (h->n_shared == 0) return PDRD_OK;
+  pdrd::FixupArgs f{};
+  f.nodes = h->d_shared_nodes;
+  f.slot_off = h->d_shared_off;
+  f.slots = h->d_shared_slots;
+  f.slot_acc = h->d_slot_acc;
+  f.pa = patch_args(h, nullptr, update, outputs, excl, cfl);
+  f.nb = h->n_shared;
+  f.norm_partial = h->d_patch_partial + static_cast<size_t>(h->np_total) * (h->m + 1);
+  const unsigned nb = fixup_blocks(h);
+#define FIX_CASE(MM)                                                                                    \
+  case MM:                                                                                              \
+    hipLaunchKernelGGL(pdrd::patch_fixup_kernel<MM>, dim3(nb), dim3(256), 0, h->stream, f);             \
+    break;
+  switch (h->m) {
+    FIX_CASE(1)
+    FIX_CASE(4)
+    FIX_CASE(5)
+    default: return set_error(PDRD_ENOTSUP, "unsupported number of equations");
+  }
+#undef FIX_CASE
+  HIP_CHECK(hipGetLastError());
   return PDRD_OK;
 }
 
@@ -325,11 +420,11 @@ int launch_node_pass(pdrd_method *h, double cfl, int outputs, int excl)
   return PDRD_OK;
 }
 
-int reduce_partials(pdrd_method *h, unsigned nb, double *norm)
+int reduce_partials(pdrd_method *h, unsigned nb, double *norm, const double *buffer = nullptr)
 {
   std::vector<double> part(static_cast<size_t>(nb) * (h->m + 1));
-  HIP_CHECK(hipMemcpyAsync(part.data(), h->d_partial, part.size() * sizeof(double), hipMemcpyDeviceToHost,
-                           h->stream));
+  HIP_CHECK(hipMemcpyAsync(part.data(), buffer ? buffer : h->d_partial, part.size() * sizeof(double),
+                           hipMemcpyDeviceToHost, h->stream));
   HIP_CHECK(hipStreamSynchronize(h->stream));
   std::vector<double> s(h->m + 1, 0.0);
   for (unsigned b = 0; b < nb; ++b)
@@ -419,10 +514,28 @@ void free_device(pdrd_method *h)
     (void)hipFree(b.d_tab);
     (void)hipFree(b.d_gconn);
     (void)hipFree(b.d_sconn);
+    (void)hipFree(b.d_pgconn);
+    (void)hipFree(b.d_lconn);
     b.d_tab = nullptr;
     b.d_gconn = nullptr;
     b.d_sconn = nullptr;
+    b.d_pgconn = nullptr;
+    b.d_lconn = nullptr;
   }
+  uint32_t **u32s[] = {&h->d_pnode_off, &h->d_pnodes, &h->d_pn_inner, &h->d_shared_nodes, &h->d_shared_off,
+                       &h->d_shared_slots};
+  for (uint32_t **p : u32s) {
+    (void)hipFree(*p);
+    *p = nullptr;
+  }
+  (void)hipFree(h->d_pcol);
+  (void)hipFree(h->d_slot_acc);
+  (void)hipFree(h->d_patch_partial);
+  (void)hipFree(h->d_dir_index);
+  h->d_pcol = nullptr;
+  h->d_slot_acc = nullptr;
+  h->d_patch_partial = nullptr;
+  h->d_dir_index = nullptr;
   double **dp[] = {&h->d_coords, &h->d_u, &h->d_res, &h->d_dt, &h->d_ws, &h->d_vol, &h->d_acc_res, &h->d_acc_ws,
                    &h->d_partial, &h->d_dir_vals};
   for (double **p : dp) {
@@ -448,11 +561,19 @@ void free_device(pdrd_method *h)
 // One iteration without outputs: cells, Dirichlet values, fused nodal pass.
 int enqueue_iteration(pdrd_method *h, double cfl, int outputs, int excl)
 {
+  if (h->scatter_mode == 2) return launch_patches(h, 1, outputs, excl, cfl);
   int rc = launch_cells(h);
   if (rc) return rc;
   rc = launch_dirichlet(h);
   if (rc) return rc;
   return launch_node_pass(h, cfl, outputs, excl);
+}
+
+int iterate_norm(pdrd_method *h, double *norm)
+{
+  if (h->scatter_mode == 2)
+    return reduce_partials(h, h->np_total + (h->n_shared ? fixup_blocks(h) : 0), norm, h->d_patch_partial);
+  return reduce_partials(h, norm_blocks(h->nn), norm);
 }
 
 }  // namespace
@@ -515,8 +636,8 @@ int pdrd_add_cells(pdrd_method_t h, const pdrd_fe_desc *fe, const double *tables
     return set_error(PDRD_EINVAL, "inconsistent element descriptor");
   if (fe->geo_order != 1) return set_error(PDRD_ENOTSUP, "only P1/Q1 geometry is built");
   pdrd::CellKey key{fe->shape, fe->sol_order, fe->nq, h->physics, h->scheme};
-  pdrd::CellLauncher launch = pdrd::find_cell(key);
-  if (!launch) return set_error(PDRD_ENOTSUP, "no cell kernel for this element type / physics / scheme");
+  const pdrd::KernelSet ks = pdrd::find_cell(key);
+  if (!ks) return set_error(PDRD_ENOTSUP, "no cell kernel for this element type / physics / scheme");
   for (size_t i = 0; i < static_cast<size_t>(ne) * fe->ng; ++i)
     if (geo_conn[i] >= h->nv) return set_error(PDRD_EINVAL, "geometry connectivity out of range");
   for (size_t i = 0; i < static_cast<size_t>(ne) * fe->nd; ++i)
@@ -527,7 +648,7 @@ int pdrd_add_cells(pdrd_method_t h, const pdrd_fe_desc *fe, const double *tables
   b.tables.assign(tables, tables + fe->table_size);
   b.geo.assign(geo_conn, geo_conn + static_cast<size_t>(ne) * fe->ng);
   b.sol.assign(sol_conn, sol_conn + static_cast<size_t>(ne) * fe->nd);
-  b.launch = launch;
+  b.ks = ks;
   h->blocks.push_back(std::move(b));
   return PDRD_OK;
   PDRD_CATCH
@@ -591,6 +712,81 @@ int pdrd_initialize(pdrd_method_t h)
     HIP_CHECK(hipMemcpy(b.d_gconn, g.data(), g.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
     HIP_CHECK(hipMemcpy(b.d_sconn, so.data(), so.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
   }
+  // patch plan (the default assembly); any block it does not fit sends the
+  // whole method to the colour-by-colour scatter
+  {
+    std::vector<pdrd::PatchBlockIn> pin;
+    for (const Block &b : h->blocks) {
+      pdrd::PatchBlockIn x;
+      x.ne = b.ne;
+      x.ng = b.fe.ng;
+      x.nd = b.fe.nd;
+      x.geo = b.geo.data();
+      x.sol = b.sol.data();
+      x.patch_size = b.ks.patch_size;
+      pin.push_back(x);
+    }
+    pdrd::PatchPlan plan;
+    h->patch_ok = true;
+    try {
+      plan = pdrd::build_patch_plan(h->dim, h->nv, h->coords.data(), h->nn, pin, pdrd::MAXCOL, pdrd::PCOL_STRIDE);
+    } catch (const std::exception &ex) {
+      h->patch_ok = false;
+      h->patch_reason = ex.what();
+    }
+    if (h->patch_ok) {
+      for (size_t i = 0; i < h->blocks.size(); ++i)
+        if (h->blocks[i].ks.patch_lds_bytes(plan.blocks[i].maxl) > 65536) {
+          h->patch_ok = false;
+          h->patch_reason = "patch LDS above 64 KiB";
+        }
+    }
+    if (h->patch_ok) {
+      for (size_t i = 0; i < h->blocks.size(); ++i) {
+        Block &b = h->blocks[i];
+        const pdrd::PatchBlockOut &o = plan.blocks[i];
+        b.first_patch = o.first_patch;
+        b.np = o.np;
+        b.maxl = o.maxl;
+        if ((rc = dev_alloc(h, &b.d_pgconn, o.gconn.size()))) { free_device(h); return rc; }
+        if ((rc = dev_alloc(h, &b.d_lconn, o.lconn.size()))) { free_device(h); return rc; }
+        HIP_CHECK(hipMemcpy(b.d_pgconn, o.gconn.data(), o.gconn.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+        HIP_CHECK(hipMemcpy(b.d_lconn, o.lconn.data(), o.lconn.size() * sizeof(uint16_t), hipMemcpyHostToDevice));
+      }
+      h->np_total = static_cast<uint32_t>(plan.pn_inner.size());
+      h->n_shared = static_cast<uint32_t>(plan.shared_nodes.size());
+      auto up32 = [&](uint32_t **d, const std::vector<uint32_t> &v) -> int {
+        int r = dev_alloc(h, d, v.size());
+        if (r) return r;
+        if (!v.empty() && hipMemcpy(*d, v.data(), v.size() * sizeof(uint32_t), hipMemcpyHostToDevice) != hipSuccess)
+          return set_error(PDRD_EHIP, "hipMemcpy of the patch plan");
+        return PDRD_OK;
+      };
+      if ((rc = up32(&h->d_pnode_off, plan.pnode_off)) || (rc = up32(&h->d_pnodes, plan.pnodes)) ||
+          (rc = up32(&h->d_pn_inner, plan.pn_inner)) || (rc = up32(&h->d_shared_nodes, plan.shared_nodes)) ||
+          (rc = up32(&h->d_shared_off, plan.shared_off)) || (rc = up32(&h->d_shared_slots, plan.shared_slots))) {
+        free_device(h);
+        return rc;
+      }
+      if ((rc = dev_alloc(h, &h->d_pcol, plan.pcol.size()))) { free_device(h); return rc; }
+      HIP_CHECK(hipMemcpy(h->d_pcol, plan.pcol.data(), plan.pcol.size() * sizeof(uint16_t), hipMemcpyHostToDevice));
+      if ((rc = dev_alloc(h, &h->d_slot_acc, static_cast<size_t>(plan.pnodes.size()) * (h->m + 1)))) {
+        free_device(h);
+        return rc;
+      }
+      if ((rc = dev_alloc(h, &h->d_patch_partial, static_cast<size_t>(h->np_total + kNormMaxBlocks) * (h->m + 1)))) {
+        free_device(h);
+        return rc;
+      }
+    } else {
+      h->scatter_mode = 1;
+    }
+  }
+  if ((rc = dev_alloc(h, &h->d_dir_index, h->nn))) { free_device(h); return rc; }
+  {
+    std::vector<int> di(h->nn, -1);
+    HIP_CHECK(hipMemcpy(h->d_dir_index, di.data(), di.size() * sizeof(int), hipMemcpyHostToDevice));
+  }
   const std::vector<double> vol = dual_volumes(h);
   HIP_CHECK(hipMemcpy(h->d_coords, h->coords.data(), h->coords.size() * sizeof(double), hipMemcpyHostToDevice));
   HIP_CHECK(hipMemcpy(h->d_vol, vol.data(), vol.size() * sizeof(double), hipMemcpyHostToDevice));
@@ -627,7 +823,9 @@ int pdrd_set_option(pdrd_method_t h, int option, int value)
   if (!h) return set_error(PDRD_EINVAL, "null method");
   switch (option) {
     case PDRD_OPT_SCATTER:
-      if (value != 0 && value != 1) return set_error(PDRD_EINVAL, "scatter mode is 0 or 1");
+      if (value < 0 || value > 2) return set_error(PDRD_EINVAL, "scatter mode is 0, 1 or 2");
+      if (value == 2 && h->ready && !h->patch_ok)
+        return set_error(PDRD_ENOTSUP, "patch assembly unavailable: " + h->patch_reason);
       h->scatter_mode = value;
       break;
     case PDRD_OPT_PRECISION:
@@ -638,6 +836,10 @@ int pdrd_set_option(pdrd_method_t h, int option, int value)
       if (value < 64 || value > 256 || (value % 64) != 0)
         return set_error(PDRD_EINVAL, "block size must be a multiple of 64 in [64, 256]");
       h->block_threads = value;
+      break;
+    case PDRD_OPT_TIMING:
+      if (value != 0 && value != 1) return set_error(PDRD_EINVAL, "timing is 0 or 1");
+      h->timing = value;
       break;
     default: return set_error(PDRD_EINVAL, "unknown option");
   }
@@ -707,30 +909,37 @@ int pdrd_add_strong_dirichlet(pdrd_method_t h, uint32_t n, const uint32_t *nodes
   if (!nodes || !values) return set_error(PDRD_EINVAL, "null argument");
   for (uint32_t i = 0; i < n; ++i)
     if (nodes[i] >= h->nn) return set_error(PDRD_EINVAL, "Dirichlet node out of range");
-  // append to the existing list
-  std::vector<uint32_t> all_nodes(h->n_dir + n);
-  std::vector<double> all_vals((static_cast<size_t>(h->n_dir) + n) * h->m);
-  if (h->n_dir) {
-    HIP_CHECK(hipMemcpy(all_nodes.data(), h->d_dir_nodes, h->n_dir * sizeof(uint32_t), hipMemcpyDeviceToHost));
-    HIP_CHECK(hipMemcpy(all_vals.data(), h->d_dir_vals, static_cast<size_t>(h->n_dir) * h->m * sizeof(double),
-                        hipMemcpyDeviceToHost));
+  // a node listed again takes its latest values (boundary conditions are
+  // applied in the order they were added)
+  std::vector<int> row(h->nn, -1);
+  for (size_t i = 0; i < h->dir_nodes_host.size(); ++i) row[h->dir_nodes_host[i]] = static_cast<int>(i);
+  for (uint32_t i = 0; i < n; ++i) {
+    int r = row[nodes[i]];
+    if (r < 0) {
+      r = static_cast<int>(h->dir_nodes_host.size());
+      row[nodes[i]] = r;
+      h->dir_nodes_host.push_back(nodes[i]);
+      h->dir_vals_host.resize(h->dir_vals_host.size() + h->m);
+    }
+    for (int k = 0; k < h->m; ++k)
+      h->dir_vals_host[static_cast<size_t>(r) * h->m + k] = values[static_cast<size_t>(i) * h->m + k];
   }
-  std::memcpy(all_nodes.data() + h->n_dir, nodes, n * sizeof(uint32_t));
-  std::memcpy(all_vals.data() + static_cast<size_t>(h->n_dir) * h->m, values,
-              static_cast<size_t>(n) * h->m * sizeof(double));
   (void)hipFree(h->d_dir_nodes);
   (void)hipFree(h->d_dir_vals);
   h->d_dir_nodes = nullptr;
   h->d_dir_vals = nullptr;
   int rc;
-  if ((rc = dev_alloc(h, &h->d_dir_nodes, all_nodes.size()))) return rc;
-  if ((rc = dev_alloc(h, &h->d_dir_vals, all_vals.size()))) return rc;
-  HIP_CHECK(hipMemcpy(h->d_dir_nodes, all_nodes.data(), all_nodes.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
-  HIP_CHECK(hipMemcpy(h->d_dir_vals, all_vals.data(), all_vals.size() * sizeof(double), hipMemcpyHostToDevice));
-  h->n_dir += n;
+  if ((rc = dev_alloc(h, &h->d_dir_nodes, h->dir_nodes_host.size()))) return rc;
+  if ((rc = dev_alloc(h, &h->d_dir_vals, h->dir_vals_host.size()))) return rc;
+  HIP_CHECK(hipMemcpy(h->d_dir_nodes, h->dir_nodes_host.data(), h->dir_nodes_host.size() * sizeof(uint32_t),
+                      hipMemcpyHostToDevice));
+  HIP_CHECK(hipMemcpy(h->d_dir_vals, h->dir_vals_host.data(), h->dir_vals_host.size() * sizeof(double),
+                      hipMemcpyHostToDevice));
+  h->n_dir = static_cast<uint32_t>(h->dir_nodes_host.size());
   std::vector<unsigned char> flag(h->nn, 0);
-  for (uint32_t v : all_nodes) flag[v] = 1;
+  for (uint32_t v : h->dir_nodes_host) flag[v] = 1;
   HIP_CHECK(hipMemcpy(h->d_is_dir, flag.data(), h->nn, hipMemcpyHostToDevice));
+  HIP_CHECK(hipMemcpy(h->d_dir_index, row.data(), h->nn * sizeof(int), hipMemcpyHostToDevice));
   drop_graph(h);
   return PDRD_OK;
 }
@@ -739,11 +948,16 @@ int pdrd_assemble_rhs(pdrd_method_t h)
 {
   if (int rc = check_ready(h)) return rc;
   HIP_CHECK(hipEventRecord(h->ev[0], h->stream));
-  if (int rc = launch_cells(h)) return rc;
-  HIP_CHECK(hipEventRecord(h->ev[1], h->stream));
-  hipLaunchKernelGGL(finalize_kernel, dim3(grid_for(h->nn, 256)), dim3(256), 0, h->stream, h->nn, h->m, h->d_acc_res,
-                     h->d_acc_ws, h->d_vol, h->d_res, h->d_ws);
-  HIP_CHECK(hipGetLastError());
+  if (h->scatter_mode == 2) {
+    if (int rc = launch_patches(h, 0, 0, 0, 0.0)) return rc;
+    HIP_CHECK(hipEventRecord(h->ev[1], h->stream));
+  } else {
+    if (int rc = launch_cells(h)) return rc;
+    HIP_CHECK(hipEventRecord(h->ev[1], h->stream));
+    hipLaunchKernelGGL(finalize_kernel, dim3(grid_for(h->nn, 256)), dim3(256), 0, h->stream, h->nn, h->m,
+                       h->d_acc_res, h->d_acc_ws, h->d_vol, h->d_res, h->d_ws);
+    HIP_CHECK(hipGetLastError());
+  }
   if (int rc = launch_dirichlet(h)) return rc;
   HIP_CHECK(hipEventRecord(h->ev[2], h->stream));
   HIP_CHECK(hipEventSynchronize(h->ev[2]));
@@ -794,6 +1008,58 @@ int pdrd_iterate(pdrd_method_t h, double cfl, int n_iter, int exclude_dirichlet,
   if (int rc = check_ready(h)) return rc;
   if (n_iter < 1) return set_error(PDRD_EINVAL, "n_iter must be >= 1");
   const int excl = exclude_dirichlet ? 1 : 0;
+  if (h->timing) {
+    // eager iterations with events around the cell loop and the nodal pass
+    double cell = 0.0, node = 0.0;
+    for (int it = 0; it < n_iter; ++it) {
+      const int out = (it == n_iter - 1) ? 1 : 0;
+      HIP_CHECK(hipEventRecord(h->ev[0], h->stream));
+      if (h->scatter_mode == 2) {
+        // the cell loop and the nodal step of patch-only nodes are one kernel;
+        // the second interval is the shared-node fix-up
+        for (Block &b : h->blocks) {
+          const pdrd::PatchArgs a = patch_args(h, &b, 1, out, excl, cfl);
+          HIP_CHECK(b.ks.patch(a, b.np, h->stream));
+        }
+        HIP_CHECK(hipEventRecord(h->ev[1], h->stream));
+        if (h->n_shared) {
+          // fix-up only: re-enqueue through launch_patches would repeat the cells
+          pdrd::FixupArgs f{};
+          f.nodes = h->d_shared_nodes;
+          f.slot_off = h->d_shared_off;
+          f.slots = h->d_shared_slots;
+          f.slot_acc = h->d_slot_acc;
+          f.pa = patch_args(h, nullptr, 1, out, excl, cfl);
+          f.nb = h->n_shared;
+          f.norm_partial = h->d_patch_partial + static_cast<size_t>(h->np_total) * (h->m + 1);
+          const unsigned nbk = fixup_blocks(h);
+          if (h->m == 1)
+            hipLaunchKernelGGL(pdrd::patch_fixup_kernel<1>, dim3(nbk), dim3(256), 0, h->stream, f);
+          else if (h->m == 4)
+            hipLaunchKernelGGL(pdrd::patch_fixup_kernel<4>, dim3(nbk), dim3(256), 0, h->stream, f);
+          else
+            hipLaunchKernelGGL(pdrd::patch_fixup_kernel<5>, dim3(nbk), dim3(256), 0, h->stream, f);
+          HIP_CHECK(hipGetLastError());
+        }
+      } else {
+        if (int rc = launch_cells(h)) return rc;
+        HIP_CHECK(hipEventRecord(h->ev[1], h->stream));
+        if (int rc = launch_dirichlet(h)) return rc;
+        if (int rc = launch_node_pass(h, cfl, out, excl)) return rc;
+      }
+      HIP_CHECK(hipEventRecord(h->ev[2], h->stream));
+      HIP_CHECK(hipEventSynchronize(h->ev[2]));
+      float a = 0.f, b = 0.f;
+      HIP_CHECK(hipEventElapsedTime(&a, h->ev[0], h->ev[1]));
+      HIP_CHECK(hipEventElapsedTime(&b, h->ev[1], h->ev[2]));
+      cell += a;
+      node += b;
+    }
+    h->last_cell_ms = cell / n_iter;
+    h->last_node_ms = node / n_iter;
+    if (norm) return iterate_norm(h, norm);
+    return PDRD_OK;
+  }
   HIP_CHECK(hipEventRecord(h->ev[0], h->stream));
   if (n_iter > 1) {
     if (!h->graph || h->graph_cfl != cfl) {
@@ -821,7 +1087,7 @@ int pdrd_iterate(pdrd_method_t h, double cfl, int n_iter, int exclude_dirichlet,
   HIP_CHECK(hipEventElapsedTime(&ms, h->ev[0], h->ev[1]));
   h->last_cell_ms = ms;
   h->last_node_ms = 0.0;
-  if (norm) return reduce_partials(h, norm_blocks(h->nn), norm);
+  if (norm) return iterate_norm(h, norm);
   return PDRD_OK;
 }
 

@@ -51,9 +51,10 @@ class PGRDMethodExplicit:
         self.mesh = mesh
         return self
 
-    def initialize_work_data(self, quad_order=4, point_set=L.WARPBLEND, scatter: int = 0):
+    def initialize_work_data(self, quad_order=4, point_set=L.WARPBLEND, scatter=None):
         """Builds the FE tables of every element type (one quadrature order, or a
-        dict shape -> order) and uploads mesh and tables."""
+        dict shape -> order) and uploads mesh and tables. scatter: None (the
+        library's default, patch assembly), 0 FP64 atomics, 1 colours, 2 patches."""
         if self.mesh is None:
             raise RuntimeError("configure_mesh_data first")
         m = self.mesh
@@ -71,7 +72,8 @@ class PGRDMethodExplicit:
             L.check(lib.pdrd_add_cells(h, C.byref(t.desc), L.ptr(t.packed), blk.geo_conn.shape[0],
                                        L.ptr(blk.geo_conn, C.c_uint32), L.ptr(blk.sol_conn, C.c_uint32)),
                     "pdrd_add_cells")
-        L.check(lib.pdrd_set_option(h, L.OPT_SCATTER, int(scatter)), "pdrd_set_option")
+        if scatter is not None:
+            L.check(lib.pdrd_set_option(h, L.OPT_SCATTER, int(scatter)), "pdrd_set_option")
         L.check(lib.pdrd_initialize(h), "pdrd_initialize")
         return self
 

@@ -1,0 +1,211 @@
+"""FE inputs: Gauss rules, reference nodes and Lagrange tables.
+
+Pins (no import of the reference): weight sums as in
+test/utest-quadrature/utest-quadrature-point-set.cpp:227-309, exactness on
+monomials (closed-form integrals), the closed-form P1/P2 triangle shape
+functions of test/utest-mesh/shape_function/fixture/ReferenceShapeFunctionsTriag.cpp
+(written here from the barycentric formulas) at 1e-13 as in
+utest-LagrangeSF.cpp:651-765, Kronecker/partition-of-unity invariants, the
+warp-blend edge nodes against Gauss-Lobatto points, and an independent numpy
+Lagrange construction for P3/P4.
+"""
+from math import comb, factorial
+
+import numpy as np
+import pytest
+
+from pdekit_amd import _lib as L
+from pdekit_amd.fe import element_tables, quadrature, reference_nodes
+
+
+def _tri_monomial_integral(a, b):
+    # xi = 2s-1, eta = 2t-1 over s,t >= 0, s+t <= 1 (Jacobian 4);
+    # int s^i t^j = i! j! / (i+j+2)!
+    tot = 0.0
+    for i in range(a + 1):
+        for j in range(b + 1):
+            c = comb(a, i) * comb(b, j) * 2 ** (i + j) * (-1) ** (a - i + b - j)
+            tot += c * factorial(i) * factorial(j) / factorial(i + j + 2)
+    return 4.0 * tot
+
+
+def _tet_monomial_integral(a, b, c):
+    tot = 0.0
+    for i in range(a + 1):
+        for j in range(b + 1):
+            for k in range(c + 1):
+                cc = comb(a, i) * comb(b, j) * comb(c, k) * 2 ** (i + j + k) * (-1) ** (a - i + b - j + c - k)
+                tot += cc * factorial(i) * factorial(j) * factorial(k) / factorial(i + j + k + 3)
+    return 8.0 * tot
+
+
+def _line_monomial(a):
+    return 0.0 if a % 2 else 2.0 / (a + 1)
+
+
+@pytest.mark.parametrize("order", [1, 2, 3, 4, 5, 6])
+def test_triangle_rules(order):
+    xi, w = quadrature(L.TRIAG, order)
+    assert abs(w.sum() - 2.0) < 3e-8
+    for a in range(order + 1):
+        for b in range(order + 1 - a):
+            exact = _tri_monomial_integral(a, b)
+            got = np.sum(w * xi[:, 0] ** a * xi[:, 1] ** b)
+            assert abs(got - exact) < 1e-13, (a, b)
+
+
+@pytest.mark.parametrize("order", [1, 2, 3, 4])
+def test_quad_rules(order):
+    xi, w = quadrature(L.QUAD, order)
+    assert abs(w.sum() - 4.0) < 3e-8
+    for a in range(order + 1):
+        for b in range(order + 1):
+            if a + b > order:
+                continue
+            exact = _line_monomial(a) * _line_monomial(b)
+            assert abs(np.sum(w * xi[:, 0] ** a * xi[:, 1] ** b) - exact) < 1e-13
+
+
+@pytest.mark.parametrize("order", [1, 2, 3])
+def test_tetra_rules(order):
+    xi, w = quadrature(L.TETRA, order)
+    assert abs(w.sum() - 4.0 / 3.0) < 3e-8
+    # the tabulated P2 points carry 10 significant digits
+    tol = 1e-9 if order == 2 else 1e-13
+    for a in range(order + 1):
+        for b in range(order + 1 - a):
+            for c in range(order + 1 - a - b):
+                exact = _tet_monomial_integral(a, b, c)
+                got = np.sum(w * xi[:, 0] ** a * xi[:, 1] ** b * xi[:, 2] ** c)
+                assert abs(got - exact) < tol, (a, b, c)
+
+
+def _bary(x):
+    xi, eta = x[:, 0], x[:, 1]
+    return np.stack([-(xi + eta) / 2, (1 + xi) / 2, (1 + eta) / 2], axis=1)
+
+
+@pytest.mark.parametrize("qorder", [1, 2, 3, 4, 5, 6])
+def test_p1_p2_closed_forms(qorder):
+    """Library tables at the Gauss points of every triangle rule against the
+    closed-form barycentric P1/P2 Lagrange functions and derivatives."""
+    dL = np.array([[-0.5, 0.5, 0.0], [-0.5, 0.0, 0.5]])  # dL_i/dxi, dL_i/deta
+    for p in (1, 2):
+        t = element_tables(L.TRIAG, p, qorder)
+        Lb = _bary(t.qp)
+        L0, L1, L2 = Lb.T
+        if p == 1:
+            ref = Lb
+            ref_d = [np.broadcast_to(dL[d], Lb.shape) for d in range(2)]
+        else:
+            ref = np.stack([L0 * (2 * L0 - 1), L1 * (2 * L1 - 1), L2 * (2 * L2 - 1), 4 * L0 * L1, 4 * L1 * L2,
+                            4 * L2 * L0], 1)
+            ref_d = []
+            for d in range(2):
+                g = dL[d]
+                ref_d.append(np.stack([g[0] * (4 * L0 - 1), g[1] * (4 * L1 - 1), g[2] * (4 * L2 - 1),
+                                       4 * (g[0] * L1 + g[1] * L0), 4 * (g[1] * L2 + g[2] * L1),
+                                       4 * (g[2] * L0 + g[0] * L2)], 1))
+        assert np.abs(t.V - ref).max() < 1e-13
+        for d in range(2):
+            assert np.abs(t.dV[d] - ref_d[d]).max() < 1e-13
+
+
+def _monomials(shape, p, x):
+    dim = x.shape[1]
+    exps = []
+    if shape == L.TRIAG:
+        exps = [(i, j) for i in range(p + 1) for j in range(p + 1 - i)]
+    elif shape == L.QUAD:
+        exps = [(i, j) for i in range(p + 1) for j in range(p + 1)]
+    else:
+        exps = [(i, j, k) for i in range(p + 1) for j in range(p + 1 - i) for k in range(p + 1 - i - j)]
+    xs = (x + 1.0) / 2.0
+    V = np.stack([np.prod([xs[:, d] ** e[d] for d in range(dim)], axis=0) for e in exps], 1)
+    dV = []
+    for d in range(dim):
+        cols = []
+        for e in exps:
+            if e[d] == 0:
+                cols.append(np.zeros(x.shape[0]))
+                continue
+            t = e[d] * xs[:, d] ** (e[d] - 1) * 0.5
+            for o in range(dim):
+                if o != d:
+                    t = t * xs[:, o] ** e[o]
+            cols.append(t)
+        dV.append(np.stack(cols, 1))
+    return V, dV
+
+
+def _numpy_lagrange(shape, p, nodes, pts):
+    Vn, _ = _monomials(shape, p, nodes)
+    inv = np.linalg.inv(Vn)
+    Vq, dVq = _monomials(shape, p, pts)
+    return Vq @ inv, [d @ inv for d in dVq]
+
+
+CASES = [(L.TRIAG, 1, 4), (L.TRIAG, 2, 4), (L.TRIAG, 3, 6), (L.QUAD, 1, 4), (L.TETRA, 1, 2)]
+
+
+@pytest.mark.parametrize("shape,p,q", CASES)
+def test_tables_against_numpy_lagrange(shape, p, q):
+    t = element_tables(shape, p, q)
+    Vs, dVs = _numpy_lagrange(shape, p, t.sol_nodes, t.qp)
+    Vf, dVf = _numpy_lagrange(shape, p + 1, t.flux_nodes, t.qp)
+    Vsf, _ = _numpy_lagrange(shape, p, t.sol_nodes, t.flux_nodes)
+    assert np.abs(t.V - Vs).max() < 1e-12
+    assert np.abs(t.Vf - Vf).max() < 1e-12
+    assert np.abs(t.Vsf - Vsf).max() < 1e-12
+    for d in range(t.dim):
+        assert np.abs(t.dV[d] - dVs[d]).max() < 1e-11
+        assert np.abs(t.dVf[d] - dVf[d]).max() < 1e-11
+
+
+@pytest.mark.parametrize("shape,p,q", CASES + [(L.TRIAG, 2, 6)])
+def test_table_invariants(shape, p, q):
+    t = element_tables(shape, p, q)
+    # partition of unity, derivative rows sum to zero
+    assert np.abs(t.V.sum(1) - 1).max() < 1e-14
+    assert np.abs(t.Vf.sum(1) - 1).max() < 1e-14
+    for d in range(t.dim):
+        assert np.abs(t.dV[d].sum(1)).max() < 1e-12
+        assert np.abs(t.dVf[d].sum(1)).max() < 1e-12
+    # the solution basis at its own nodes is the identity (after clipping)
+    Vn, _ = _numpy_lagrange(shape, p, t.sol_nodes, t.sol_nodes)
+    assert np.abs(Vn - np.eye(t.nd)).max() < 1e-12
+    # the flux element holds the solution nodes among its nodes for P1
+    # (vertices), and Vsf interpolates the P-order solution exactly there
+    assert t.Vsf.shape == (t.nf, t.nd)
+
+
+def test_warpblend_edge_nodes_are_gauss_lobatto():
+    # P3: edge nodes at +-1/sqrt(5); P4: 0, +-sqrt(3/7)
+    x3 = reference_nodes(L.TRIAG, 3)
+    bottom = sorted(x3[3:5, 0])
+    assert np.allclose(bottom, [-1 / np.sqrt(5), 1 / np.sqrt(5)], atol=1e-14)
+    assert np.allclose(x3[9], [-1 / 3, -1 / 3], atol=1e-14)
+    x4 = reference_nodes(L.TRIAG, 4)
+    assert np.allclose(sorted(x4[3:6, 0]), [-np.sqrt(3 / 7), 0.0, np.sqrt(3 / 7)], atol=1e-13)
+    # symmetry: the node set is invariant under the vertex rotation
+    # (xi, eta) -> (eta, -1 - xi - eta)
+    for x in (x3, x4):
+        rot = np.stack([x[:, 1], -1 - x[:, 0] - x[:, 1]], 1)
+        d = np.abs(rot[:, None, :] - x[None, :, :]).sum(2).min(1)
+        assert d.max() < 1e-12
+
+
+def test_equidistant_p2_matches_warpblend():
+    assert np.allclose(reference_nodes(L.TRIAG, 2, L.EQUIDIST), reference_nodes(L.TRIAG, 2, L.WARPBLEND))
+
+
+def test_node_orders():
+    x = reference_nodes(L.TRIAG, 2)
+    assert np.allclose(x, [[-1, -1], [1, -1], [-1, 1], [0, -1], [0, 0], [-1, 0]])
+    q = reference_nodes(L.QUAD, 2)
+    assert np.allclose(q, [[-1, -1], [1, -1], [1, 1], [-1, 1], [0, -1], [1, 0], [0, 1], [-1, 0], [0, 0]])
+    t = reference_nodes(L.TETRA, 2)
+    v = np.array([[-1, -1, -1], [1, -1, -1], [-1, 1, -1], [-1, -1, 1]], dtype=float)
+    edges = [(0, 1), (1, 2), (2, 0), (3, 0), (3, 2), (3, 1)]
+    assert np.allclose(t[:4], v)
+    assert np.allclose(t[4:], [(v[a] + v[b]) / 2 for a, b in edges])

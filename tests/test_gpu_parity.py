@@ -17,7 +17,10 @@ TOL = 1e-12
 SCHEMES = [L.LDA, L.N, L.B, L.PSI]
 
 
-def _method(mesh, physics, scheme, params, scatter=0):
+SCATTER = {"patch": 2, "atomic": 0, "colour": 1}
+
+
+def _method(mesh, physics, scheme, params, scatter=None):
     m = PGRDMethodExplicit(physics, scheme, params=params if params else None)
     m.configure_mesh_data(mesh).initialize_work_data(quad_order=quad_orders(mesh), scatter=scatter)
     return m
@@ -34,12 +37,13 @@ def _case(name):
     return _mesh_cache[name]
 
 
+@pytest.mark.parametrize("scatter", sorted(SCATTER))
 @pytest.mark.parametrize("name", sorted(CASES))
 @pytest.mark.parametrize("scheme", SCHEMES)
-def test_assemble_rhs_matches_oracle(name, scheme):
+def test_assemble_rhs_matches_oracle(name, scheme, scatter):
     mesh, physics, params, u = _case(name)
-    print(f"case {name} scheme {scheme} cells {mesh.nb_cells} nodes {mesh.nb_nodes}", flush=True)
-    m = _method(mesh, physics, scheme, params)
+    print(f"case {name} scheme {scheme} scatter {scatter} cells {mesh.nb_cells} nodes {mesh.nb_nodes}", flush=True)
+    m = _method(mesh, physics, scheme, params, scatter=SCATTER[scatter])
     m.set_vec_function("solution", u)
     m.assemble_lhs_and_rhs(0.5)
     m.check_status()
@@ -56,10 +60,10 @@ def test_assemble_rhs_matches_oracle(name, scheme):
 
 
 @pytest.mark.parametrize("name", ["adv_p1", "burgers_p2", "euler2d_mixed", "euler3d_p1"])
-def test_coloured_scatter_matches_atomic(name):
+def test_scatter_modes_agree(name):
     mesh, physics, params, u = _case(name)
     out = []
-    for scatter in (0, 1):
+    for scatter in (0, 1, 2):
         m = _method(mesh, physics, L.N, params, scatter=scatter)
         m.set_vec_function("solution", u)
         m.assemble_rhs()
@@ -68,11 +72,13 @@ def test_coloured_scatter_matches_atomic(name):
         m.destroy()
     assert info["nb_colours"] >= 3
     assert rel_err(out[1], out[0]) <= TOL
+    assert rel_err(out[2], out[0]) <= TOL
 
 
-def test_coloured_scatter_is_deterministic():
+@pytest.mark.parametrize("scatter", [1, 2])
+def test_deterministic_scatter_modes(scatter):
     mesh, physics, params, u = _case("burgers_p2")
-    m = _method(mesh, physics, L.B, params, scatter=1)
+    m = _method(mesh, physics, L.B, params, scatter=scatter)
     m.set_vec_function("solution", u)
     m.assemble_rhs()
     a = m.vec_function("residuals")
@@ -82,8 +88,9 @@ def test_coloured_scatter_is_deterministic():
     m.destroy()
 
 
+@pytest.mark.parametrize("scatter", [0, 2])
 @pytest.mark.parametrize("scheme", [L.LDA, L.N])
-def test_iterations_match_oracle(scheme):
+def test_iterations_match_oracle(scheme, scatter):
     """20 explicit iterations with strong Dirichlet inflow values (C1 set-up on a
     33 x 33 make_unit_quad mesh): GPU state vs oracle state."""
     from pdekit_amd.mesh import unit_quad_triag
@@ -91,7 +98,7 @@ def test_iterations_match_oracle(scheme):
     mesh = unit_quad_triag(33)
     bn = np.union1d(mesh.boundaries["Edge0"], mesh.boundaries["Edge3"]).astype(np.uint32)
     g = np.where(mesh.node_coords[bn, 1] < 0.5, 1.0, 0.0)[:, None]
-    m = _method(mesh, L.ADVECTION2D, scheme, (0.8, 0.6))
+    m = _method(mesh, L.ADVECTION2D, scheme, (0.8, 0.6), scatter=scatter)
     m.set_vec_function("solution", np.zeros((mesh.nb_nodes, 1)))
     m.add_boundary_condition("StrongDirichlet", "inflow", bn, g)
     norm = m.iterate(0.9, 20)
@@ -102,7 +109,7 @@ def test_iterations_match_oracle(scheme):
     assert rel_err(u, uo) <= 1e-12
     assert abs(norm[0] - norms[-1, 0]) <= 1e-12 * max(norms[-1, 0], 1e-300)
     # the step-by-step API gives the same state
-    m2 = _method(mesh, L.ADVECTION2D, scheme, (0.8, 0.6))
+    m2 = _method(mesh, L.ADVECTION2D, scheme, (0.8, 0.6), scatter=scatter)
     m2.set_vec_function("solution", np.zeros((mesh.nb_nodes, 1)))
     m2.add_boundary_condition("StrongDirichlet", "inflow", bn, g)
     for _ in range(20):
