@@ -97,7 +97,22 @@ def main():
     ap.add_argument("--scatter", default="auto", choices=["auto", "atomic", "colour", "patch"])
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-calibration-mb", type=int, default=0,
+                    help="only run 5 device-to-device copies of this many MiB (reference kernel for the "
+                         "FETCH_SIZE / WRITE_SIZE scale of rocprofv3 --pmc) and exit")
     args = ap.parse_args()
+
+    if args.copy_calibration_mb:
+        import torch
+
+        n = args.copy_calibration_mb * (1 << 20) // 8
+        a = torch.ones(n, dtype=torch.float64, device="cuda:0")
+        b = torch.empty_like(a)
+        for _ in range(5):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": 8 * n, "copy_bytes_written": 8 * n, "copies": 5}))
+        return
 
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path (--gpus {args.gpus} requested). "

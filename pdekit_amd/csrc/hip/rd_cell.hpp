@@ -60,6 +60,12 @@ struct TabOffsets
   static constexpr int sol_nodes = Vsf + NF * ND;
   static constexpr int flux_nodes = sol_nodes + ND * D;
   static constexpr int size = flux_nodes + NF * D;
+  // Device-only copies appended by the host, laid out so that the values one
+  // iteration of a rolled loop needs are contiguous (one scalar load each):
+  static constexpr int dVfT = size;                  // [NF][D][NQ]
+  static constexpr int dVT = dVfT + NF * D * NQ;     // [NQ][D][ND]
+  static constexpr int dVgT = dVT + NQ * D * ND;     // [NQ][D][NG]
+  static constexpr int dev_size = dVgT + NQ * D * NG;
 };
 
 // Workgroups are dealt round-robin to the 8 XCDs; give each XCD one
@@ -180,7 +186,7 @@ __device__ __forceinline__ void apply_kplus(const Acoustic<D> &ac, double l0, do
 template <int NQ, int ND, int M>
 struct RollQ
 {
-  static constexpr bool value = (M > 1) || (NQ * ND > 60);
+  static constexpr bool value = true;
 };
 
 // Residuals phi[ND][M] and wave speed of one element from its vertex
@@ -210,30 +216,31 @@ __device__ __forceinline__ void cell_residual(const double *__restrict__ tab, co
       for (int x = 0; x < D; ++x) {
         double s = 0.0;
 #pragma unroll
-        for (int g = 0; g < NG; ++g) s += tab[O::dVg + (xi * NQ + q) * NG + g] * X[g][x];
+        for (int g = 0; g < NG; ++g) s += tab[O::dVgT + (q * D + xi) * NG + g] * X[g][x];
         J[xi][x] = s;
       }
     detJ[q] = det_inv<D>(J, Jinv[q]);
   }
 
   // ---- P+1 flux element: divergence of the interpolated flux at q ---------
-  double divr[ROLL ? 1 : NQ][M];
+  // The flux-node loop stays rolled: each trip reads one row of Vsf and one
+  // [D][NQ] slab of dVf with scalar loads, so the table never has to be held
+  // in (and spilled from) scalar registers as a whole.
+  double divr[NQ][M];
 #pragma unroll
-  for (int q = 0; q < (ROLL ? 1 : NQ); ++q)
+  for (int q = 0; q < NQ; ++q)
 #pragma unroll
     for (int k = 0; k < M; ++k) divr[q][k] = 0.0;
-  if constexpr (ROLL) {
-#pragma unroll
-    for (int i = 0; i < NQ * M; ++i) lds_div[i * stride] = 0.0;
-  }
-#pragma unroll
+#pragma unroll 1
   for (int f = 0; f < NF; ++f) {
+    const double *vs = tab + O::Vsf + f * ND;
+    const double *dvf = tab + O::dVfT + f * D * NQ;
     double s[M];
 #pragma unroll
     for (int k = 0; k < M; ++k) {
       double t = 0.0;
 #pragma unroll
-      for (int n = 0; n < ND; ++n) t += tab[O::Vsf + f * ND + n] * U[n][k];
+      for (int n = 0; n < ND; ++n) t += vs[n] * U[n][k];
       s[k] = t;
     }
     double F[D][M];
@@ -252,14 +259,13 @@ __device__ __forceinline__ void cell_residual(const double *__restrict__ tab, co
     }
 #pragma unroll
     for (int q = 0; q < NQ; ++q) {
-      double add[M];
       if constexpr (AFFINE) {
 #pragma unroll
         for (int k = 0; k < M; ++k) {
-          double t = 0.0;
+          double t = divr[q][k];
 #pragma unroll
-          for (int xi = 0; xi < D; ++xi) t += tab[O::dVf + (xi * NQ + q) * NF + f] * G[xi][k];
-          add[k] = t;
+          for (int xi = 0; xi < D; ++xi) t += dvf[xi * NQ + q] * G[xi][k];
+          divr[q][k] = t;
         }
       } else {
         double b[D];
@@ -267,25 +273,24 @@ __device__ __forceinline__ void cell_residual(const double *__restrict__ tab, co
         for (int d = 0; d < D; ++d) {
           double t = 0.0;
 #pragma unroll
-          for (int xi = 0; xi < D; ++xi) t += Jinv[q][d][xi] * tab[O::dVf + (xi * NQ + q) * NF + f];
+          for (int xi = 0; xi < D; ++xi) t += Jinv[q][d][xi] * dvf[xi * NQ + q];
           b[d] = t;
         }
 #pragma unroll
         for (int k = 0; k < M; ++k) {
-          double t = 0.0;
+          double t = divr[q][k];
 #pragma unroll
           for (int d = 0; d < D; ++d) t += b[d] * F[d][k];
-          add[k] = t;
+          divr[q][k] = t;
         }
       }
-#pragma unroll
-      for (int k = 0; k < M; ++k) {
-        if constexpr (ROLL)
-          lds_div[(q * M + k) * stride] += add[k];
-        else
-          divr[q][k] += add[k];
-      }
     }
+  }
+  if constexpr (ROLL) {
+#pragma unroll
+    for (int q = 0; q < NQ; ++q)
+#pragma unroll
+      for (int k = 0; k < M; ++k) lds_div[(q * M + k) * stride] = divr[q][k];
   }
 
   // ---- element scheme ------------------------------------------------------
@@ -321,7 +326,7 @@ __device__ __forceinline__ void cell_residual(const double *__restrict__ tab, co
       for (int x = 0; x < D; ++x) {
         double t = 0.0;
 #pragma unroll
-        for (int xi = 0; xi < D; ++xi) t += Jq[x][xi] * tab[O::dV + (xi * NQ + q) * ND + n];
+        for (int xi = 0; xi < D; ++xi) t += Jq[x][xi] * tab[O::dVT + (q * D + xi) * ND + n];
         gphys[n][x] = t;
       }
 
@@ -457,7 +462,7 @@ __device__ __forceinline__ void cell_residual(const double *__restrict__ tab, co
           for (int x = 0; x < D; ++x) {
             double t = 0.0;
 #pragma unroll
-            for (int g = 0; g < NG; ++g) t += tab[O::dVg + (xi * NQ + q) * NG + g] * X[g][x];
+            for (int g = 0; g < NG; ++g) t += tab[O::dVgT + (q * D + xi) * NG + g] * X[g][x];
             J[xi][x] = t;
           }
         const double dq = det_inv<D>(J, Ji);
@@ -469,7 +474,7 @@ __device__ __forceinline__ void cell_residual(const double *__restrict__ tab, co
     for (int q = 0; q < NQ; ++q) {
       double r[M];
 #pragma unroll
-      for (int k = 0; k < M; ++k) r[k] = divr[ROLL ? 0 : q][k];
+      for (int k = 0; k < M; ++k) r[k] = divr[q][k];
       quad_point(q, r, Jinv[AFFINE ? 0 : q], detJ[AFFINE ? 0 : q]);
     }
   }

@@ -654,6 +654,30 @@ int pdrd_add_cells(pdrd_method_t h, const pdrd_fe_desc *fe, const double *tables
   PDRD_CATCH
 }
 
+// The packed FE tables of a block followed by transposed copies of the
+// derivative tables (dVf as [nf][dim][nq], dV as [nq][dim][nd], dVg as
+// [nq][dim][ng]); the offsets match TabOffsets in rd_cell.hpp.
+static std::vector<double> device_tables(const pdrd_fe_desc &fe, const std::vector<double> &t)
+{
+  const int D = fe.dim, NG = fe.ng, ND = fe.nd, NF = fe.nf, NQ = fe.nq;
+  const size_t o_dVg = static_cast<size_t>(NQ) * D + NQ + static_cast<size_t>(NQ) * NG;
+  const size_t o_V = o_dVg + static_cast<size_t>(D) * NQ * NG;
+  const size_t o_dV = o_V + static_cast<size_t>(NQ) * ND;
+  const size_t o_Vf = o_dV + static_cast<size_t>(D) * NQ * ND;
+  const size_t o_dVf = o_Vf + static_cast<size_t>(NQ) * NF;
+  std::vector<double> out(t);
+  for (int f = 0; f < NF; ++f)
+    for (int xi = 0; xi < D; ++xi)
+      for (int q = 0; q < NQ; ++q) out.push_back(t[o_dVf + (static_cast<size_t>(xi) * NQ + q) * NF + f]);
+  for (int q = 0; q < NQ; ++q)
+    for (int xi = 0; xi < D; ++xi)
+      for (int n = 0; n < ND; ++n) out.push_back(t[o_dV + (static_cast<size_t>(xi) * NQ + q) * ND + n]);
+  for (int q = 0; q < NQ; ++q)
+    for (int xi = 0; xi < D; ++xi)
+      for (int g = 0; g < NG; ++g) out.push_back(t[o_dVg + (static_cast<size_t>(xi) * NQ + q) * NG + g]);
+  return out;
+}
+
 int pdrd_initialize(pdrd_method_t h)
 {
   PDRD_TRY
@@ -705,10 +729,11 @@ int pdrd_initialize(pdrd_method_t h)
       for (int k = 0; k < ng; ++k) g[static_cast<size_t>(k) * b.ne + p] = b.geo[static_cast<size_t>(c) * ng + k];
       for (int k = 0; k < nd; ++k) so[static_cast<size_t>(k) * b.ne + p] = b.sol[static_cast<size_t>(c) * nd + k];
     }
-    if ((rc = dev_alloc(h, &b.d_tab, b.tables.size()))) { free_device(h); return rc; }
+    const std::vector<double> dtab = device_tables(b.fe, b.tables);
+    if ((rc = dev_alloc(h, &b.d_tab, dtab.size()))) { free_device(h); return rc; }
     if ((rc = dev_alloc(h, &b.d_gconn, g.size()))) { free_device(h); return rc; }
     if ((rc = dev_alloc(h, &b.d_sconn, so.size()))) { free_device(h); return rc; }
-    HIP_CHECK(hipMemcpy(b.d_tab, b.tables.data(), b.tables.size() * sizeof(double), hipMemcpyHostToDevice));
+    HIP_CHECK(hipMemcpy(b.d_tab, dtab.data(), dtab.size() * sizeof(double), hipMemcpyHostToDevice));
     HIP_CHECK(hipMemcpy(b.d_gconn, g.data(), g.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
     HIP_CHECK(hipMemcpy(b.d_sconn, so.data(), so.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
   }
