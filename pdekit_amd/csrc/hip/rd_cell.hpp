@@ -180,18 +180,10 @@ __device__ __forceinline__ void apply_kplus(const Acoustic<D> &ac, double l0, do
   for (int k = 0; k < M; ++k) y[k] += coef * (l0 * x[k] + ac.rp[k] * sp + ac.rm[k] * sm);
 }
 
-// Quadrature loops are fully unrolled for small scalar elements; systems and
-// large elements keep a rolled loop with the flux divergence staged in LDS so
-// that the live state fits the register file.
-template <int NQ, int ND, int M>
-struct RollQ
-{
-  static constexpr bool value = true;
-};
-
 // Residuals phi[ND][M] and wave speed of one element from its vertex
-// coordinates X and nodal state U. lds_div: this thread's slice of
-// NQ*M doubles (stride `stride` between entries) when the loop is rolled.
+// coordinates X and nodal state U. lds_div: this thread's slice of NQ*M
+// doubles of LDS (stride `stride` between entries) that carries the flux
+// divergence from the flux-node loop to the quadrature loop.
 template <int D, int NG, int ND, int NF, int NQ, bool AFFINE, int PH, int SC>
 __device__ __forceinline__ void cell_residual(const double *__restrict__ tab, const double (&X)[NG][D],
                                               const double (&U)[ND][PhysTraits<PH>::M], const PhysParams &pp,
@@ -201,7 +193,6 @@ __device__ __forceinline__ void cell_residual(const double *__restrict__ tab, co
   constexpr int M = PhysTraits<PH>::M;
   static_assert(PhysTraits<PH>::DIM == D, "physics / element dimension mismatch");
   using O = TabOffsets<D, NG, ND, NF, NQ>;
-  constexpr bool ROLL = RollQ<NQ, ND, M>::value;
 
   // ---- geometry ----------------------------------------------------------
   constexpr int NJ = AFFINE ? 1 : NQ;
@@ -286,12 +277,10 @@ __device__ __forceinline__ void cell_residual(const double *__restrict__ tab, co
       }
     }
   }
-  if constexpr (ROLL) {
 #pragma unroll
-    for (int q = 0; q < NQ; ++q)
+  for (int q = 0; q < NQ; ++q)
 #pragma unroll
-      for (int k = 0; k < M; ++k) lds_div[(q * M + k) * stride] = divr[q][k];
-  }
+    for (int k = 0; k < M; ++k) lds_div[(q * M + k) * stride] = divr[q][k];
 
   // ---- element scheme ------------------------------------------------------
   constexpr bool NDIS = (SC != SC_LDA);
@@ -445,37 +434,27 @@ __device__ __forceinline__ void cell_residual(const double *__restrict__ tab, co
     }
   };
 
-  if constexpr (ROLL) {
 #pragma unroll 1
-    for (int q = 0; q < NQ; ++q) {
-      double r[M];
+  for (int q = 0; q < NQ; ++q) {
+    double r[M];
 #pragma unroll
-      for (int k = 0; k < M; ++k) r[k] = lds_div[(q * M + k) * stride];
-      if constexpr (AFFINE) {
-        quad_point(q, r, Jinv[0], detJ[0]);
-      } else {
-        // recompute the Jacobian at q instead of indexing a per-q array
-        double J[D][D], Ji[D][D];
+    for (int k = 0; k < M; ++k) r[k] = lds_div[(q * M + k) * stride];
+    if constexpr (AFFINE) {
+      quad_point(q, r, Jinv[0], detJ[0]);
+    } else {
+      // recompute the Jacobian at q instead of indexing a per-q array
+      double J[D][D], Ji[D][D];
 #pragma unroll
-        for (int xi = 0; xi < D; ++xi)
+      for (int xi = 0; xi < D; ++xi)
 #pragma unroll
-          for (int x = 0; x < D; ++x) {
-            double t = 0.0;
+        for (int x = 0; x < D; ++x) {
+          double t = 0.0;
 #pragma unroll
-            for (int g = 0; g < NG; ++g) t += tab[O::dVgT + (q * D + xi) * NG + g] * X[g][x];
-            J[xi][x] = t;
-          }
-        const double dq = det_inv<D>(J, Ji);
-        quad_point(q, r, Ji, dq);
-      }
-    }
-  } else {
-#pragma unroll
-    for (int q = 0; q < NQ; ++q) {
-      double r[M];
-#pragma unroll
-      for (int k = 0; k < M; ++k) r[k] = divr[q][k];
-      quad_point(q, r, Jinv[AFFINE ? 0 : q], detJ[AFFINE ? 0 : q]);
+          for (int g = 0; g < NG; ++g) t += tab[O::dVgT + (q * D + xi) * NG + g] * X[g][x];
+          J[xi][x] = t;
+        }
+      const double dq = det_inv<D>(J, Ji);
+      quad_point(q, r, Ji, dq);
     }
   }
 
@@ -572,24 +551,29 @@ __global__ __launch_bounds__(256) void rd_cell_kernel(CellArgs a)
 // ---------------------------------------------------------------------------
 // Patch assembly: one workgroup owns one patch of up to PATCH cells (a
 // spatially compact run of the Morton-ordered cells). The states of the
-// patch's nodes are gathered once into LDS, the cells' residuals are summed
-// into LDS in colour rounds (cells of one colour share no node, so the sum
-// order is fixed), then
+// patch's nodes are gathered once into LDS; every cell writes its residuals
+// and wave speed to the LDS slot of each of its (cell, node) pairs, the pairs
+// being grouped by node (the rank of a pair within its node comes with the
+// connectivity), so each node then sums a contiguous run of slots in a fixed
+// order -- no atomics and the same result on every run. Then
 //   * nodes held by this patch only are finished here: residual / V, wave
 //     speed, and -- in iteration mode -- dt = CFL V / sum lambda,
 //     u -= dt res (after the strong Dirichlet value),
 //   * nodes shared with other patches write their partial sums to their slot;
 //     patch_fixup_kernel sums the slots of every shared node in a fixed order.
 // ---------------------------------------------------------------------------
+constexpr int LC_BITS = 10;  // local node index bits of a patch connectivity entry
+constexpr int LC_MASK = (1 << LC_BITS) - 1;
+
 struct PatchArgs
 {
   const double *__restrict__ tab;
   const uint32_t *__restrict__ gconn;       // [NG][ne] global vertex ids
-  const uint16_t *__restrict__ lconn;       // [ND][ne] patch-local node indices
+  const uint16_t *__restrict__ lconn;       // [ND][ne] local node | rank << LC_BITS
   const uint32_t *__restrict__ pnode_off;   // [np + 1] first slot of each patch
   const uint32_t *__restrict__ pnodes;      // [slots] global node id of each slot
   const uint32_t *__restrict__ pn_inner;    // [np] number of patch-only nodes (listed first)
-  const uint16_t *__restrict__ pcol_off;    // [np][PCOL_STRIDE] colour offsets inside the patch
+  const uint16_t *__restrict__ pn_coff;     // [slots] first (cell, node) pair of the slot's node
   const double *__restrict__ coords;
   double *__restrict__ u;                   // [N][M] (updated in place in iteration mode)
   const double *__restrict__ vol;           // [N]
@@ -610,32 +594,43 @@ struct PatchArgs
   PhysParams pp;
 };
 
-// Colour offsets of a patch: PCOL_STRIDE u16 entries, offsets [0..ncol] and
-// ncol in the last entry.
-constexpr int MAXCOL = 47;
-constexpr int PCOL_STRIDE = MAXCOL + 2;
-
-// Cells per patch (= threads per workgroup): 256 for scalar physics, 128 for
-// the Euler systems whose per-thread state and LDS slices are larger.
-template <int M>
+// Cells per patch (= threads per workgroup): 256 for scalar physics on
+// elements of up to 6 nodes, 128 for the Euler systems (larger per-thread
+// state) and for P3 elements (more nodes per patch).
+template <int M, int ND>
 struct PatchSize
 {
-  static constexpr int value = (M == 1) ? 256 : 128;
+  static constexpr int value = (M == 1 && ND <= 6) ? 256 : 128;
 };
 
-// Dynamic LDS of one patch workgroup, in doubles: node state, node sums and
-// a work region shared by the flux-divergence slices and the final reduction.
+// Dynamic LDS of one patch workgroup, in doubles: node state [maxl][M], the
+// pair offsets of the nodes [maxl] (u16), then a work region shared in turn
+// by the flux-divergence slices, the (cell, node) pair sums and the norm
+// reduction.
+template <int NQ, int ND, int M>
+__host__ __device__ constexpr size_t patch_work_doubles()
+{
+  constexpr int P = PatchSize<M, ND>::value;
+  constexpr size_t div = static_cast<size_t>(NQ) * M * P;
+  constexpr size_t pairs = static_cast<size_t>(P) * ND * (M + 1);
+  constexpr size_t red = static_cast<size_t>(P) * (M + 1);
+  constexpr size_t w = div > pairs ? div : pairs;
+  return w > red ? w : red;
+}
+
+__host__ __device__ constexpr size_t patch_coff_doubles(int maxl) { return (static_cast<size_t>(maxl) + 3) / 4; }
+
 template <int NQ, int ND, int M>
 __host__ __device__ constexpr size_t patch_lds_doubles(int maxl)
 {
-  constexpr int P = PatchSize<M>::value;
-  constexpr size_t div = RollQ<NQ, ND, M>::value ? static_cast<size_t>(NQ) * M * P : 0;
-  constexpr size_t red = static_cast<size_t>(P) * (M + 1);
-  return static_cast<size_t>(maxl) * (2 * M + 1) + (div > red ? div : red);
+  return static_cast<size_t>(maxl) * M + patch_coff_doubles(maxl) + patch_work_doubles<NQ, ND, M>();
 }
 
+// Finishes node g from its sums acc[M + 1] (residual sum, wave-speed sum);
+// u_old is the node's state at the start of the sweep.
 template <int M>
-__device__ __forceinline__ void finish_node(const PatchArgs &a, uint32_t g, const double *acc, double (&sq)[M + 1])
+__device__ __forceinline__ void finish_node(const PatchArgs &a, uint32_t g, const double *acc, const double *u_old,
+                                            double (&sq)[M + 1])
 {
   const double v = a.vol[g];
   const double iv = 1.0 / v;
@@ -648,7 +643,7 @@ __device__ __forceinline__ void finish_node(const PatchArgs &a, uint32_t g, cons
     for (int k = 0; k < M; ++k) {
       const size_t j = static_cast<size_t>(g) * M + k;
       const double r = acc[k] * iv;
-      const double u0 = (di >= 0) ? a.dir_vals[static_cast<size_t>(di) * M + k] : a.u[j];
+      const double u0 = (di >= 0) ? a.dir_vals[static_cast<size_t>(di) * M + k] : u_old[k];
       a.u[j] = u0 - d * r;
       if (a.outputs) a.res[j] = r;
       if (count) sq[k] += r * r;
@@ -687,19 +682,15 @@ __device__ __forceinline__ void block_sum_store(double (&sq)[M + 1], double *red
 }
 
 template <int D, int NG, int ND, int NF, int NQ, bool AFFINE, int PH, int SC>
-__global__ __launch_bounds__(PatchSize<PhysTraits<PH>::M>::value) void rd_patch_kernel(PatchArgs a)
+__global__ __launch_bounds__((PatchSize<PhysTraits<PH>::M, ND>::value)) void rd_patch_kernel(PatchArgs a)
 {
   constexpr int M = PhysTraits<PH>::M;
-  constexpr int PATCH = PatchSize<M>::value;
-  constexpr bool ROLL = RollQ<NQ, ND, M>::value;
+  constexpr int PATCH = PatchSize<M, ND>::value;
+  constexpr int NPAIR = PATCH * ND;  // stride of one component in the pair sums
   extern __shared__ double lds[];
-  // LDS layout: node state [maxl][M], node sums [maxl][M+1], then a work
-  // region: the per-thread flux-divergence slices [NQ*M][PATCH] while the
-  // cells are computed, the norm reduction at the end
   double *ul = lds;
-  double *acc = lds + static_cast<size_t>(a.maxl) * M;
-  double *work = acc + static_cast<size_t>(a.maxl) * (M + 1);
-  double *div_lds = work;
+  uint16_t *coff = reinterpret_cast<uint16_t *>(lds + static_cast<size_t>(a.maxl) * M);
+  double *work = lds + static_cast<size_t>(a.maxl) * M + patch_coff_doubles(a.maxl);
 
   const uint32_t pl = xcd_block_index();
   if (pl >= a.np) return;
@@ -709,69 +700,74 @@ __global__ __launch_bounds__(PatchSize<PhysTraits<PH>::M>::value) void rd_patch_
   const int tid = threadIdx.x;
   const uint32_t e0 = pl * PATCH;
   const int ecount = static_cast<int>(min(static_cast<uint32_t>(PATCH), a.ne - e0));
-
-  for (int l = tid; l < nl; l += PATCH) {
-    const size_t g = a.pnodes[s0 + l];
-#pragma unroll
-    for (int k = 0; k < M; ++k) ul[l * M + k] = a.u[g * M + k];
-#pragma unroll
-    for (int k = 0; k <= M; ++k) acc[l * (M + 1) + k] = 0.0;
-  }
-  __syncthreads();
-
-  bool bad = false;
-  double phi[ND][M];
-  double wave = 0.0;
-  uint16_t loc[ND];
   const bool active = tid < ecount;
+
+  // the cell's vertices and local nodes do not depend on LDS: issue their
+  // loads before the gather
+  double X[NG][D];
+  uint16_t lc[ND];
   if (active) {
     const uint32_t e = e0 + tid;
-    double X[NG][D];
 #pragma unroll
     for (int g = 0; g < NG; ++g) {
       const size_t v = a.gconn[static_cast<size_t>(g) * a.ne + e];
 #pragma unroll
       for (int d = 0; d < D; ++d) X[g][d] = a.coords[v * D + d];
     }
+#pragma unroll
+    for (int n = 0; n < ND; ++n) lc[n] = a.lconn[static_cast<size_t>(n) * a.ne + e];
+  }
+  for (int l = tid; l < nl; l += PATCH) {
+    const size_t g = a.pnodes[s0 + l];
+#pragma unroll
+    for (int k = 0; k < M; ++k) ul[l * M + k] = a.u[g * M + k];
+    coff[l] = a.pn_coff[s0 + l];
+  }
+  __syncthreads();
+
+  bool bad = false;
+  double phi[ND][M];
+  double wave = 0.0;
+  if (active) {
     double U[ND][M];
 #pragma unroll
-    for (int n = 0; n < ND; ++n) {
-      loc[n] = a.lconn[static_cast<size_t>(n) * a.ne + e];
+    for (int n = 0; n < ND; ++n)
 #pragma unroll
-      for (int k = 0; k < M; ++k) U[n][k] = ul[loc[n] * M + k];
-    }
-    cell_residual<D, NG, ND, NF, NQ, AFFINE, PH, SC>(a.tab, X, U, a.pp, phi, wave, bad,
-                                                     ROLL ? div_lds + tid : nullptr, PATCH);
+      for (int k = 0; k < M; ++k) U[n][k] = ul[(lc[n] & LC_MASK) * M + k];
+    cell_residual<D, NG, ND, NF, NQ, AFFINE, PH, SC>(a.tab, X, U, a.pp, phi, wave, bad, work + tid, PATCH);
   }
-  // colour rounds: cells [pcol[c], pcol[c+1]) of the patch share no node
-  const uint16_t *pcol = a.pcol_off + static_cast<size_t>(p) * PCOL_STRIDE;
-  const int ncol = pcol[PCOL_STRIDE - 1];
-  for (int c = 0; c < ncol; ++c) {
-    __syncthreads();
-    if (active && tid >= pcol[c] && tid < pcol[c + 1]) {
+  __syncthreads();  // the work region changes from divergence slices to pair sums
+  if (active) {
 #pragma unroll
-      for (int n = 0; n < ND; ++n) {
-        double *row = acc + loc[n] * (M + 1);
+    for (int n = 0; n < ND; ++n) {
+      const int slot = coff[lc[n] & LC_MASK] + (lc[n] >> LC_BITS);
 #pragma unroll
-        for (int k = 0; k < M; ++k) row[k] += phi[n][k];
-        row[M] += wave;
-      }
+      for (int k = 0; k < M; ++k) work[k * NPAIR + slot] = phi[n][k];
+      work[M * NPAIR + slot] = wave;
     }
   }
   __syncthreads();
 
   const int ninner = static_cast<int>(a.pn_inner[p]);
+  const int npair = ecount * ND;
   double sq[M + 1];
 #pragma unroll
   for (int k = 0; k <= M; ++k) sq[k] = 0.0;
   for (int l = tid; l < nl; l += PATCH) {
-    const double *row = acc + l * (M + 1);
+    const int b = coff[l];
+    const int e = (l + 1 < nl) ? static_cast<int>(coff[l + 1]) : npair;
+    double acc[M + 1];
+#pragma unroll
+    for (int k = 0; k <= M; ++k) acc[k] = 0.0;
+    for (int i = b; i < e; ++i)
+#pragma unroll
+      for (int k = 0; k <= M; ++k) acc[k] += work[k * NPAIR + i];
     if (l < ninner) {
-      finish_node<M>(a, a.pnodes[s0 + l], row, sq);
+      finish_node<M>(a, a.pnodes[s0 + l], acc, ul + l * M, sq);
     } else {
       double *slot = a.slot_acc + static_cast<size_t>(s0 + l) * (M + 1);
 #pragma unroll
-      for (int k = 0; k <= M; ++k) slot[k] = row[k];
+      for (int k = 0; k <= M; ++k) slot[k] = acc[k];
     }
   }
   if (bad) atomicOr(a.status, 1);
@@ -805,7 +801,11 @@ __global__ __launch_bounds__(256) void patch_fixup_kernel(FixupArgs f)
 #pragma unroll
       for (int k = 0; k <= M; ++k) acc[k] += row[k];
     }
-    finish_node<M>(f.pa, f.nodes[i], acc, sq);
+    const uint32_t g = f.nodes[i];
+    double u_old[M];
+#pragma unroll
+    for (int k = 0; k < M; ++k) u_old[k] = f.pa.u[static_cast<size_t>(g) * M + k];
+    finish_node<M>(f.pa, g, acc, u_old, sq);
   }
   if (f.pa.update && f.pa.outputs) block_sum_store<M>(sq, red, f.norm_partial + static_cast<size_t>(blockIdx.x) * (M + 1));
 }

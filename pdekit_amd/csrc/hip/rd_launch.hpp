@@ -39,7 +39,7 @@ template <int D, int NG, int ND, int NF, int NQ, bool AFF, int PH, int SC>
 hipError_t launch_cell(const CellArgs &a, unsigned grid, unsigned block, hipStream_t s)
 {
   constexpr int M = PhysTraits<PH>::M;
-  const size_t lds = RollQ<NQ, ND, M>::value ? static_cast<size_t>(NQ) * M * block * sizeof(double) : 0;
+  const size_t lds = static_cast<size_t>(NQ) * M * block * sizeof(double);
   hipLaunchKernelGGL((rd_cell_kernel<D, NG, ND, NF, NQ, AFF, PH, SC>), dim3(grid), dim3(block), lds, s, a);
   return hipGetLastError();
 }
@@ -49,7 +49,7 @@ hipError_t launch_patch(const PatchArgs &a, unsigned grid, hipStream_t s)
 {
   constexpr int M = PhysTraits<PH>::M;
   const size_t lds = patch_lds_doubles<NQ, ND, M>(a.maxl) * sizeof(double);
-  hipLaunchKernelGGL((rd_patch_kernel<D, NG, ND, NF, NQ, AFF, PH, SC>), dim3(grid), dim3(PatchSize<M>::value), lds, s,
+  hipLaunchKernelGGL((rd_patch_kernel<D, NG, ND, NF, NQ, AFF, PH, SC>), dim3(grid), dim3(PatchSize<M, ND>::value), lds, s,
                      a);
   return hipGetLastError();
 }
@@ -66,7 +66,7 @@ KernelSet kernel_set()
   KernelSet k;
   k.cell = &launch_cell<D, NG, ND, NF, NQ, AFF, PH, SC>;
   k.patch = &launch_patch<D, NG, ND, NF, NQ, AFF, PH, SC>;
-  k.patch_size = PatchSize<PhysTraits<PH>::M>::value;
+  k.patch_size = PatchSize<PhysTraits<PH>::M, ND>::value;
   k.patch_lds_bytes = &patch_lds_bytes<NQ, ND, PhysTraits<PH>::M>;
   return k;
 }

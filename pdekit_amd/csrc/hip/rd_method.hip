@@ -79,7 +79,7 @@ struct pdrd_method
   // patch plan
   uint32_t np_total = 0, n_shared = 0;
   uint32_t *d_pnode_off = nullptr, *d_pnodes = nullptr, *d_pn_inner = nullptr;
-  uint16_t *d_pcol = nullptr;
+  uint16_t *d_pn_coff = nullptr;
   uint32_t *d_shared_nodes = nullptr, *d_shared_off = nullptr, *d_shared_slots = nullptr;
   double *d_slot_acc = nullptr, *d_patch_partial = nullptr;
   size_t bytes = 0;
@@ -330,7 +330,7 @@ pdrd::PatchArgs patch_args(const pdrd_method *h, const Block *b, int update, int
   a.pnode_off = h->d_pnode_off;
   a.pnodes = h->d_pnodes;
   a.pn_inner = h->d_pn_inner;
-  a.pcol_off = h->d_pcol;
+  a.pn_coff = h->d_pn_coff;
   a.coords = h->d_coords;
   a.u = h->d_u;
   a.vol = h->d_vol;
@@ -528,11 +528,11 @@ void free_device(pdrd_method *h)
     (void)hipFree(*p);
     *p = nullptr;
   }
-  (void)hipFree(h->d_pcol);
+  (void)hipFree(h->d_pn_coff);
   (void)hipFree(h->d_slot_acc);
   (void)hipFree(h->d_patch_partial);
   (void)hipFree(h->d_dir_index);
-  h->d_pcol = nullptr;
+  h->d_pn_coff = nullptr;
   h->d_slot_acc = nullptr;
   h->d_patch_partial = nullptr;
   h->d_dir_index = nullptr;
@@ -654,6 +654,8 @@ int pdrd_add_cells(pdrd_method_t h, const pdrd_fe_desc *fe, const double *tables
   PDRD_CATCH
 }
 
+static_assert(pdrd::LC_BITS == pdrd::LCONN_BITS, "patch connectivity layout differs between host and device");
+
 // The packed FE tables of a block followed by transposed copies of the
 // derivative tables (dVf as [nf][dim][nq], dV as [nq][dim][nd], dVg as
 // [nq][dim][ng]); the offsets match TabOffsets in rd_cell.hpp.
@@ -754,7 +756,7 @@ int pdrd_initialize(pdrd_method_t h)
     pdrd::PatchPlan plan;
     h->patch_ok = true;
     try {
-      plan = pdrd::build_patch_plan(h->dim, h->nv, h->coords.data(), h->nn, pin, pdrd::MAXCOL, pdrd::PCOL_STRIDE);
+      plan = pdrd::build_patch_plan(h->dim, h->nv, h->coords.data(), h->nn, pin);
     } catch (const std::exception &ex) {
       h->patch_ok = false;
       h->patch_reason = ex.what();
@@ -793,8 +795,8 @@ int pdrd_initialize(pdrd_method_t h)
         free_device(h);
         return rc;
       }
-      if ((rc = dev_alloc(h, &h->d_pcol, plan.pcol.size()))) { free_device(h); return rc; }
-      HIP_CHECK(hipMemcpy(h->d_pcol, plan.pcol.data(), plan.pcol.size() * sizeof(uint16_t), hipMemcpyHostToDevice));
+      if ((rc = dev_alloc(h, &h->d_pn_coff, plan.pn_coff.size()))) { free_device(h); return rc; }
+      HIP_CHECK(hipMemcpy(h->d_pn_coff, plan.pn_coff.data(), plan.pn_coff.size() * sizeof(uint16_t), hipMemcpyHostToDevice));
       if ((rc = dev_alloc(h, &h->d_slot_acc, static_cast<size_t>(plan.pnodes.size()) * (h->m + 1)))) {
         free_device(h);
         return rc;

@@ -36,10 +36,8 @@ uint64_t spread2(uint64_t x)
 }  // namespace
 
 PatchPlan build_patch_plan(int dim, uint32_t nv, const double *coords, uint32_t nn,
-                           const std::vector<PatchBlockIn> &blocks, int max_colours, int pcol_stride)
+                           const std::vector<PatchBlockIn> &blocks)
 {
-  if (max_colours + 2 > pcol_stride) throw std::invalid_argument("colour table too small");
-  if (max_colours > 64) throw std::invalid_argument("at most 64 colours per patch");
   PatchPlan plan;
   double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
   for (uint32_t v = 0; v < nv; ++v)
@@ -92,14 +90,15 @@ PatchPlan build_patch_plan(int dim, uint32_t nv, const double *coords, uint32_t 
       for (int k = 0; k < in.nd; ++k) L.push_back(in.sol[static_cast<size_t>(c) * in.nd + k]);
     std::sort(L.begin(), L.end());
     L.erase(std::unique(L.begin(), L.end()), L.end());
-    if (L.size() > 65535) throw std::invalid_argument("patch with more than 65535 nodes");
+    if (L.size() > static_cast<size_t>(PATCH_MAX_NODES)) throw std::invalid_argument("patch with too many nodes");
+    if (patch_cells[p].size() * in.nd > 65535) throw std::invalid_argument("patch with too many (cell, node) pairs");
     for (uint32_t v : L) cnt[v]++;
   }
 
-  // 3. inner nodes first, then shared; slots; colours; local connectivity
+  // 3. inner nodes first, then shared; slots; local connectivity with ranks;
+  //    node-grouped pair offsets
   plan.pnode_off.assign(np + 1, 0);
   plan.pn_inner.assign(np, 0);
-  plan.pcol.assign(np * pcol_stride, 0);
   std::vector<int32_t> pos(nn, -1);
   for (size_t p = 0; p < np; ++p) {
     std::vector<uint32_t> inner, shared;
@@ -111,6 +110,7 @@ PatchPlan build_patch_plan(int dim, uint32_t nv, const double *coords, uint32_t 
   }
   plan.pnodes.reserve(plan.pnode_off[np]);
   for (size_t p = 0; p < np; ++p) plan.pnodes.insert(plan.pnodes.end(), pn[p].begin(), pn[p].end());
+  plan.pn_coff.assign(plan.pnode_off[np], 0);
 
   for (size_t b = 0; b < blocks.size(); ++b) {
     const PatchBlockIn &in = blocks[b];
@@ -124,37 +124,23 @@ PatchPlan build_patch_plan(int dim, uint32_t nv, const double *coords, uint32_t 
       const std::vector<uint32_t> &L = pn[p];
       out.maxl = std::max(out.maxl, static_cast<int>(L.size()));
       for (size_t l = 0; l < L.size(); ++l) pos[L[l]] = static_cast<int32_t>(l);
-      // greedy colouring of the patch's cells in Morton order
-      const std::vector<uint32_t> &cells = patch_cells[p];
-      std::vector<uint64_t> used(L.size(), 0);
-      std::vector<int> col(cells.size());
-      int ncol = 0;
-      for (size_t i = 0; i < cells.size(); ++i) {
-        uint64_t mask = 0;
-        for (int k = 0; k < in.nd; ++k) mask |= used[pos[in.sol[static_cast<size_t>(cells[i]) * in.nd + k]]];
-        int c = 0;
-        while (c < 64 && (mask >> c) & 1ULL) ++c;
-        if (c >= max_colours) throw std::invalid_argument("patch needs too many colours");
-        col[i] = c;
-        ncol = std::max(ncol, c + 1);
-        for (int k = 0; k < in.nd; ++k) used[pos[in.sol[static_cast<size_t>(cells[i]) * in.nd + k]]] |= (1ULL << c);
-      }
-      plan.max_colours = std::max(plan.max_colours, ncol);
-      uint16_t *pc = &plan.pcol[p * pcol_stride];
-      std::vector<int> count(ncol + 1, 0);
-      for (int c : col) count[c + 1]++;
-      for (int c = 0; c < ncol; ++c) count[c + 1] += count[c];
-      for (int c = 0; c <= ncol; ++c) pc[c] = static_cast<uint16_t>(count[c]);
-      pc[pcol_stride - 1] = static_cast<uint16_t>(ncol);
-      std::vector<uint32_t> sorted(cells.size());
-      std::vector<int> fill(count.begin(), count.end() - 1);
-      for (size_t i = 0; i < cells.size(); ++i) sorted[fill[col[i]]++] = cells[i];
-      for (uint32_t c : sorted) {
+      std::vector<int> deg(L.size(), 0);
+      for (uint32_t c : patch_cells[p]) {
         const size_t e = out.order.size();
         out.order.push_back(c);
         for (int g = 0; g < in.ng; ++g) out.gconn[static_cast<size_t>(g) * in.ne + e] = in.geo[static_cast<size_t>(c) * in.ng + g];
-        for (int k = 0; k < in.nd; ++k)
-          out.lconn[static_cast<size_t>(k) * in.ne + e] = static_cast<uint16_t>(pos[in.sol[static_cast<size_t>(c) * in.nd + k]]);
+        for (int k = 0; k < in.nd; ++k) {
+          const int l = pos[in.sol[static_cast<size_t>(c) * in.nd + k]];
+          const int r = deg[l]++;
+          if (r >= PATCH_MAX_RANK) throw std::invalid_argument("node held by too many cells of one patch");
+          plan.max_rank = std::max(plan.max_rank, r + 1);
+          out.lconn[static_cast<size_t>(k) * in.ne + e] = static_cast<uint16_t>(l | (r << LCONN_BITS));
+        }
+      }
+      uint32_t off = 0;
+      for (size_t l = 0; l < L.size(); ++l) {
+        plan.pn_coff[plan.pnode_off[p] + l] = static_cast<uint16_t>(off);
+        off += static_cast<uint32_t>(deg[l]);
       }
       for (uint32_t v : L) pos[v] = -1;
     }

@@ -22,8 +22,15 @@ struct PatchBlockOut
   uint32_t first_patch = 0, np = 0;
   int maxl = 0;                 // most nodes of one patch
   std::vector<uint32_t> gconn;  // [ng][ne] SoA, block order
-  std::vector<uint16_t> lconn;  // [nd][ne] SoA, patch-local node indices
+  // [nd][ne] SoA: patch-local node index l (low LCONN_BITS bits) and the rank
+  // of this (cell, node) pair among the pairs of node l (high bits); pairs of
+  // one node are ranked by cell position in the patch.
+  std::vector<uint16_t> lconn;
 };
+
+constexpr int LCONN_BITS = 10;
+constexpr int PATCH_MAX_NODES = 1 << LCONN_BITS;
+constexpr int PATCH_MAX_RANK = 1 << (16 - LCONN_BITS);
 
 struct PatchPlan
 {
@@ -31,20 +38,23 @@ struct PatchPlan
   std::vector<uint32_t> pnode_off;     // [np + 1]
   std::vector<uint32_t> pnodes;        // [slots] node id of each slot
   std::vector<uint32_t> pn_inner;      // [np] nodes held by this patch only (listed first)
-  std::vector<uint16_t> pcol;          // [np][pcol_stride] colour offsets, ncol last
+  // [slots] first position of the slot's node in the patch's list of
+  // (cell, node) pairs grouped by node: pairs of local node l occupy
+  // [pn_coff[l], pn_coff[l + 1]) (the last node ends at cells * nd)
+  std::vector<uint16_t> pn_coff;
   std::vector<uint32_t> shared_nodes;  // nodes held by several patches, ascending
   std::vector<uint32_t> shared_off;    // [nshared + 1]
   std::vector<uint32_t> shared_slots;  // slots of each shared node, ascending
-  int max_colours = 0;
+  int max_rank = 0;                    // most cells of one patch sharing one node
 };
 
 // Builds the plan for all blocks of a mesh: the Morton key of every cell
 // centroid (over the bounding box of all vertices), runs of patch_size cells,
-// greedy colouring of the cells inside each patch (cells are then sorted by
-// colour inside the patch), and the inner / shared split of every node.
-// Throws std::invalid_argument when a patch would need more than max_colours
-// colours or more than 65535 local nodes.
+// the inner / shared split of every node and the node-grouped order of the
+// (cell, node) pairs of each patch. Throws std::invalid_argument when a patch
+// has more than PATCH_MAX_NODES nodes, a node is held by more than
+// PATCH_MAX_RANK cells of one patch, or a patch has 65536 pairs or more.
 PatchPlan build_patch_plan(int dim, uint32_t nv, const double *coords, uint32_t nn,
-                           const std::vector<PatchBlockIn> &blocks, int max_colours, int pcol_stride);
+                           const std::vector<PatchBlockIn> &blocks);
 
 }  // namespace pdrd
