@@ -307,17 +307,7 @@ __device__ __forceinline__ void cell_residual(const double *__restrict__ tab, co
     vol += wj;
 #pragma unroll
     for (int k = 0; k < M; ++k) fint[k] += wj * r[k];
-    // physical gradients of the solution basis at q
-    double gphys[ND][D];
-#pragma unroll
-    for (int n = 0; n < ND; ++n)
-#pragma unroll
-      for (int x = 0; x < D; ++x) {
-        double t = 0.0;
-#pragma unroll
-        for (int xi = 0; xi < D; ++xi) t += Jq[x][xi] * tab[O::dVT + (q * D + xi) * ND + n];
-        gphys[n][x] = t;
-      }
+    const double *dvq = tab + O::dVT + q * D * ND;  // reference gradients [D][ND] at q
 
     if constexpr (M == 1) {
       double vel0, vel1;
@@ -331,11 +321,17 @@ __device__ __forceinline__ void cell_residual(const double *__restrict__ tab, co
         vel0 = uq;
         vel1 = 1.0;
       }
+      // k_n = a . grad(phi_n) = sum_xi c_xi dphi_n/dxi with c_xi = sum_x a_x dxi/dx
+      double c[D];
+#pragma unroll
+      for (int xi = 0; xi < D; ++xi) c[xi] = vel0 * Jq[0][xi] + vel1 * Jq[1][xi];
       double kp[ND];
       double S = 0.0, SU = 0.0;
 #pragma unroll
       for (int n = 0; n < ND; ++n) {
-        const double kn = vel0 * gphys[n][0] + vel1 * gphys[n][1];
+        double kn = 0.0;
+#pragma unroll
+        for (int xi = 0; xi < D; ++xi) kn += c[xi] * dvq[xi * ND + n];
         const double ak = fabs(kn);
         kp[n] = kn > 0.0 ? kn : 0.0;
         const double l = ak > 1.e-6 ? ak : 1.e-6;
@@ -345,9 +341,10 @@ __device__ __forceinline__ void cell_residual(const double *__restrict__ tab, co
       }
       const double inv = 1.0 / S;
       const double ub = SU * inv;
+      const double wr = wj * inv * r[0];
 #pragma unroll
       for (int n = 0; n < ND; ++n) {
-        lda[n][0] += wj * ((kp[n] * inv) * r[0]);
+        lda[n][0] += kp[n] * wr;
         if constexpr (NDIS) dis[n][0] += wj * (kp[n] * (U[n][0] - ub));
       }
     } else {
@@ -362,6 +359,17 @@ __device__ __forceinline__ void cell_residual(const double *__restrict__ tab, co
       const EulerState<double, D> st = euler_state<double, D>(uq, pp.p0);
       bad = bad || st.bad;
       const double gm1 = pp.p0 - 1.0;
+      // physical gradients of the solution basis at q
+      double gphys[ND][D];
+#pragma unroll
+      for (int n = 0; n < ND; ++n)
+#pragma unroll
+        for (int x = 0; x < D; ++x) {
+          double t = 0.0;
+#pragma unroll
+          for (int xi = 0; xi < D; ++xi) t += Jq[x][xi] * dvq[xi * ND + n];
+          gphys[n][x] = t;
+        }
       // compact K+ of every node
       double l0[ND], dp[ND], dm[ND], nh[ND][D];
 #pragma unroll
@@ -603,6 +611,16 @@ struct PatchSize
   static constexpr int value = (M == 1 && ND <= 6) ? 256 : 128;
 };
 
+// Waves per SIMD the register allocation of a patch kernel must leave room
+// for: 5 for the scalar affine P1 / P2 cells (at most 96 VGPRs, measured
+// faster than 4 waves); the other cells keep the compiler's choice, since
+// capping them spills.
+template <int M, int ND, bool AFFINE>
+struct PatchMinWaves
+{
+  static constexpr int value = (M == 1 && ND <= 6 && AFFINE) ? 5 : 1;
+};
+
 // Dynamic LDS of one patch workgroup, in doubles: node state [maxl][M], the
 // pair offsets of the nodes [maxl] (u16), then a work region shared in turn
 // by the flux-divergence slices, the (cell, node) pair sums and the norm
@@ -682,7 +700,8 @@ __device__ __forceinline__ void block_sum_store(double (&sq)[M + 1], double *red
 }
 
 template <int D, int NG, int ND, int NF, int NQ, bool AFFINE, int PH, int SC>
-__global__ __launch_bounds__((PatchSize<PhysTraits<PH>::M, ND>::value)) void rd_patch_kernel(PatchArgs a)
+__global__ __launch_bounds__((PatchSize<PhysTraits<PH>::M, ND>::value),
+                             (PatchMinWaves<PhysTraits<PH>::M, ND, AFFINE>::value)) void rd_patch_kernel(PatchArgs a)
 {
   constexpr int M = PhysTraits<PH>::M;
   constexpr int PATCH = PatchSize<M, ND>::value;
