@@ -121,6 +121,7 @@ __device__ __forceinline__ double det_inv(const double J[D][D], double Ji[D][D])
 template <int M, int R>
 __device__ __forceinline__ void solve_pp(double A[M][M], double B[M][R])
 {
+  double ipv[M];
 #pragma unroll
   for (int c = 0; c < M; ++c) {
 #pragma unroll
@@ -140,6 +141,7 @@ __device__ __forceinline__ void solve_pp(double A[M][M], double B[M][R])
       }
     }
     const double ip = 1.0 / A[c][c];
+    ipv[c] = ip;
 #pragma unroll
     for (int r = c + 1; r < M; ++r) {
       const double f = A[r][c] * ip;
@@ -151,7 +153,7 @@ __device__ __forceinline__ void solve_pp(double A[M][M], double B[M][R])
   }
 #pragma unroll
   for (int c = M - 1; c >= 0; --c) {
-    const double ip = 1.0 / A[c][c];
+    const double ip = ipv[c];
 #pragma unroll
     for (int j = 0; j < R; ++j) {
       double s = B[c][j];
@@ -162,33 +164,33 @@ __device__ __forceinline__ void solve_pp(double A[M][M], double B[M][R])
   }
 }
 
-// y += coef * K+ x for the compact Euler K+ of one node.
-template <int D>
-__device__ __forceinline__ void apply_kplus(const Acoustic<D> &ac, double l0, double dp, double dm, const double *x,
-                                            double coef, double *y)
+// Nodal states as the quadrature loop reads them: from the thread's
+// registers, or from the patch's LDS copy (which frees the registers of the
+// state after the flux-node loop).
+template <int ND, int M>
+struct URegs
 {
-  constexpr int M = D + 2;
-  double sp = 0.0, sm = 0.0;
-#pragma unroll
-  for (int k = 0; k < M; ++k) {
-    sp += ac.lp[k] * x[k];
-    sm += ac.lm[k] * x[k];
-  }
-  sp *= dp;
-  sm *= dm;
-#pragma unroll
-  for (int k = 0; k < M; ++k) y[k] += coef * (l0 * x[k] + ac.rp[k] * sp + ac.rm[k] * sm);
-}
+  const double (&U)[ND][M];
+  __device__ __forceinline__ double operator()(int n, int k) const { return U[n][k]; }
+};
+
+template <int ND, int M>
+struct ULds
+{
+  const double *ul;   // [local node][M]
+  const int (&row)[ND];  // local node of each cell node
+  __device__ __forceinline__ double operator()(int n, int k) const { return ul[row[n] * M + k]; }
+};
 
 // Residuals phi[ND][M] and wave speed of one element from its vertex
 // coordinates X and nodal state U. lds_div: this thread's slice of NQ*M
 // doubles of LDS (stride `stride` between entries) that carries the flux
 // divergence from the flux-node loop to the quadrature loop.
-template <int D, int NG, int ND, int NF, int NQ, bool AFFINE, int PH, int SC>
+template <int D, int NG, int ND, int NF, int NQ, bool AFFINE, int PH, int SC, class UA>
 __device__ __forceinline__ void cell_residual(const double *__restrict__ tab, const double (&X)[NG][D],
-                                              const double (&U)[ND][PhysTraits<PH>::M], const PhysParams &pp,
-                                              double (&phi)[ND][PhysTraits<PH>::M], double &wave, bool &bad,
-                                              double *lds_div, int stride)
+                                              const double (&U)[ND][PhysTraits<PH>::M], const UA &Uq,
+                                              const PhysParams &pp, double (&phi)[ND][PhysTraits<PH>::M],
+                                              double &wave, bool &bad, double *lds_div, int stride)
 {
   constexpr int M = PhysTraits<PH>::M;
   static_assert(PhysTraits<PH>::DIM == D, "physics / element dimension mismatch");
@@ -317,7 +319,7 @@ __device__ __forceinline__ void cell_residual(const double *__restrict__ tab, co
       } else {
         double uq = 0.0;
 #pragma unroll
-        for (int n = 0; n < ND; ++n) uq += tab[O::V + q * ND + n] * U[n][0];
+        for (int n = 0; n < ND; ++n) uq += tab[O::V + q * ND + n] * Uq(n, 0);
         vel0 = uq;
         vel1 = 1.0;
       }
@@ -337,7 +339,7 @@ __device__ __forceinline__ void cell_residual(const double *__restrict__ tab, co
         const double l = ak > 1.e-6 ? ak : 1.e-6;
         lam[n] = l > lam[n] ? l : lam[n];
         S += kp[n];
-        SU += kp[n] * U[n][0];
+        SU += kp[n] * Uq(n, 0);
       }
       const double inv = 1.0 / S;
       const double ub = SU * inv;
@@ -345,7 +347,7 @@ __device__ __forceinline__ void cell_residual(const double *__restrict__ tab, co
 #pragma unroll
       for (int n = 0; n < ND; ++n) {
         lda[n][0] += kp[n] * wr;
-        if constexpr (NDIS) dis[n][0] += wj * (kp[n] * (U[n][0] - ub));
+        if constexpr (NDIS) dis[n][0] += wj * (kp[n] * (Uq(n, 0) - ub));
       }
     } else {
       double uq[M];
@@ -353,7 +355,7 @@ __device__ __forceinline__ void cell_residual(const double *__restrict__ tab, co
       for (int k = 0; k < M; ++k) {
         double t = 0.0;
 #pragma unroll
-        for (int n = 0; n < ND; ++n) t += tab[O::V + q * ND + n] * U[n][k];
+        for (int n = 0; n < ND; ++n) t += tab[O::V + q * ND + n] * Uq(n, k);
         uq[k] = t;
       }
       const EulerState<double, D> st = euler_state<double, D>(uq, pp.p0);
@@ -370,8 +372,24 @@ __device__ __forceinline__ void cell_residual(const double *__restrict__ tab, co
           for (int xi = 0; xi < D; ++xi) t += Jq[x][xi] * dvq[xi * ND + n];
           gphys[n][x] = t;
         }
-      // compact K+ of every node
-      double l0[ND], dp[ND], dm[ND], nh[ND][D];
+      // Compact K+ of every node. With the acoustic pairs written as
+      //   r+- = (rho / 2a) (r0 +- a e_n),  l+- = (l0v +- a f_n) / (rho a),
+      //   r0 = (1, V, H), e_n = (0, n, V.n), l0v = (gm1 |V|^2 / 2, -gm1 V, gm1),
+      //   f_n = (-V.n, n, 0),
+      // K+_n = l0_n I + dp_n r+ l+^T + dm_n r- l-^T becomes
+      //   K+_n x = l0_n x + alpha r0 + beta e_n,
+      //   alpha = (sg_n P + a dl_n Q) / 2a^2, beta = sg_n Q / 2 + dl_n P / 2a,
+      //   P = l0v . x, Q = f_n . x, sg = dp + dm, dl = dp - dm,
+      // and S = sum_n K+_n needs only sum l0, sum sg, w = sum dl n and
+      // T = sum sg n n^T.
+      double l0[ND], sg[ND], dl[ND], um[ND], nh[ND][D];
+      double sl0 = 0.0, ssg = 0.0, w[D], T[D][D];
+#pragma unroll
+      for (int x = 0; x < D; ++x) {
+        w[x] = 0.0;
+#pragma unroll
+        for (int y = 0; y < D; ++y) T[x][y] = 0.0;
+      }
 #pragma unroll
       for (int n = 0; n < ND; ++n) {
         double nn = 0.0;
@@ -380,63 +398,129 @@ __device__ __forceinline__ void cell_residual(const double *__restrict__ tab, co
         const double norm = sqrt(nn);
         const bool tiny = (D == 2) ? (norm < 1.e-9) : (norm == 0.0);
         const double inorm = tiny ? 0.0 : 1.0 / norm;
-        double um = 0.0;
+        double u_n = 0.0;
 #pragma unroll
         for (int x = 0; x < D; ++x) {
           nh[n][x] = gphys[n][x] * inorm;
-          um += st.V[x] * nh[n][x];
+          u_n += st.V[x] * nh[n][x];
         }
-        const double e0 = norm * um, ep = norm * (um + st.a), em = norm * (um - st.a);
+        um[n] = u_n;
+        const double e0 = norm * u_n, ep = norm * (u_n + st.a), em = norm * (u_n - st.a);
         lam[n] = fmax(lam[n], fmax(1.e-6, fmax(fabs(e0), fmax(fabs(ep), fabs(em)))));
         const double p0 = fmax(0.0, e0);
+        const double dpn = tiny ? 0.0 : fmax(0.0, ep) - p0;
+        const double dmn = tiny ? 0.0 : fmax(0.0, em) - p0;
         l0[n] = tiny ? 0.0 : p0;
-        dp[n] = tiny ? 0.0 : fmax(0.0, ep) - p0;
-        dm[n] = tiny ? 0.0 : fmax(0.0, em) - p0;
+        sg[n] = dpn + dmn;
+        dl[n] = dpn - dmn;
+        sl0 += l0[n];
+        ssg += sg[n];
+#pragma unroll
+        for (int x = 0; x < D; ++x) {
+          w[x] += dl[n] * nh[n][x];
+#pragma unroll
+          for (int y = 0; y < D; ++y) T[x][y] += sg[n] * nh[n][x] * nh[n][y];
+        }
       }
-      // S = sum K+ ; Z = sum K+ U
+      const double a = st.a;
+      const double i2a = 0.5 / a, i2a2 = i2a / a;
+      double l0v[M];
+      l0v[0] = 0.5 * gm1 * st.v2;
+#pragma unroll
+      for (int x = 0; x < D; ++x) l0v[1 + x] = -gm1 * st.V[x];
+      l0v[M - 1] = gm1;
+      auto kplus = [&](const int n, const double (&v)[M], double (&out)[M]) {
+        double P = 0.0;
+#pragma unroll
+        for (int k = 0; k < M; ++k) P += l0v[k] * v[k];
+        double Q = -um[n] * v[0];
+#pragma unroll
+        for (int x = 0; x < D; ++x) Q += nh[n][x] * v[1 + x];
+        const double al = (sg[n] * P + a * dl[n] * Q) * i2a2;
+        const double be = 0.5 * sg[n] * Q + dl[n] * P * i2a;
+        out[0] = l0[n] * v[0] + al;
+#pragma unroll
+        for (int x = 0; x < D; ++x) out[1 + x] = l0[n] * v[1 + x] + al * st.V[x] + be * nh[n][x];
+        out[M - 1] = l0[n] * v[M - 1] + al * st.H + be * um[n];
+      };
+      // S = sl0 I + ssg/2a^2 r0 l0v^T + 1/2 sum sg e f^T + 1/2a (r0 (F w)^T + (E w) l0v^T)
+      double r0[M], ew[M], fw[M];
+      r0[0] = 1.0;
+#pragma unroll
+      for (int x = 0; x < D; ++x) r0[1 + x] = st.V[x];
+      r0[M - 1] = st.H;
+      double vw = 0.0;
+#pragma unroll
+      for (int x = 0; x < D; ++x) vw += st.V[x] * w[x];
+      ew[0] = 0.0;
+      fw[0] = -vw;
+#pragma unroll
+      for (int x = 0; x < D; ++x) {
+        ew[1 + x] = w[x];
+        fw[1 + x] = w[x];
+      }
+      ew[M - 1] = vw;
+      fw[M - 1] = 0.0;
+      double TV[D], VTV = 0.0;
+#pragma unroll
+      for (int x = 0; x < D; ++x) {
+        double t = 0.0;
+#pragma unroll
+        for (int y = 0; y < D; ++y) t += T[x][y] * st.V[y];
+        TV[x] = t;
+        VTV += st.V[x] * t;
+      }
+      const double c1 = ssg * i2a2;
+      constexpr int R = NDIS ? 2 : 1;
       double A[M][M];
-      double B[M][2];
-      double sl0 = 0.0;
+      double B[M][R];
 #pragma unroll
-      for (int n = 0; n < ND; ++n) sl0 += l0[n];
+      for (int i = 0; i < M; ++i)
 #pragma unroll
-      for (int i = 0; i < M; ++i) {
+        for (int j = 0; j < M; ++j) {
+          // 1/2 sum_n sg_n e_n f_n^T
+          double et = 0.0;
+          if (i >= 1 && i <= D) {
+            if (j == 0) et = -TV[i - 1];
+            else if (j <= D) et = T[i - 1][j - 1];
+          } else if (i == M - 1) {
+            if (j == 0) et = -VTV;
+            else if (j <= D) et = TV[j - 1];
+          }
+          A[i][j] = ((i == j) ? sl0 : 0.0) + c1 * r0[i] * l0v[j] + 0.5 * et + i2a * (r0[i] * fw[j] + ew[i] * l0v[j]);
+        }
 #pragma unroll
-        for (int j = 0; j < M; ++j) A[i][j] = (i == j) ? sl0 : 0.0;
-        B[i][0] = r[i];
-        B[i][1] = 0.0;
-      }
+      for (int i = 0; i < M; ++i) B[i][0] = r[i];
+      if constexpr (NDIS) {
 #pragma unroll
-      for (int n = 0; n < ND; ++n) {
-        Acoustic<D> ac;
-        acoustic_vectors<D>(st, gm1, nh[n], ac);
+        for (int i = 0; i < M; ++i) B[i][1] = 0.0;
 #pragma unroll
-        for (int i = 0; i < M; ++i)
+        for (int n = 0; n < ND; ++n) {
+          double un[M], z[M];
 #pragma unroll
-          for (int j = 0; j < M; ++j) A[i][j] += dp[n] * ac.rp[i] * ac.lp[j] + dm[n] * ac.rm[i] * ac.lm[j];
-        if constexpr (NDIS) {
-          double z[M];
-#pragma unroll
-          for (int k = 0; k < M; ++k) z[k] = 0.0;
-          apply_kplus<D>(ac, l0[n], dp[n], dm[n], U[n], 1.0, z);
+          for (int k = 0; k < M; ++k) un[k] = Uq(n, k);
+          kplus(n, un, z);
 #pragma unroll
           for (int k = 0; k < M; ++k) B[k][1] += z[k];
         }
       }
-      solve_pp<M, 2>(A, B);
-      double x[M], y[M];
+      solve_pp<M, R>(A, B);
+      double x[M];
 #pragma unroll
-      for (int k = 0; k < M; ++k) { x[k] = B[k][0]; y[k] = B[k][1]; }
+      for (int k = 0; k < M; ++k) x[k] = B[k][0] * wj;
 #pragma unroll
       for (int n = 0; n < ND; ++n) {
-        Acoustic<D> ac;
-        acoustic_vectors<D>(st, gm1, nh[n], ac);
-        apply_kplus<D>(ac, l0[n], dp[n], dm[n], x, wj, lda[n]);
+        double t[M];
+        kplus(n, x, t);
+#pragma unroll
+        for (int k = 0; k < M; ++k) lda[n][k] += t[k];
         if constexpr (NDIS) {
           double du[M];
 #pragma unroll
-          for (int k = 0; k < M; ++k) du[k] = U[n][k] - y[k];
-          apply_kplus<D>(ac, l0[n], dp[n], dm[n], du, wj, dis[n]);
+          for (int k = 0; k < M; ++k) du[k] = wj * (Uq(n, k) - B[k][R - 1]);
+          kplus(n, du, t);
+#pragma unroll
+          for (int k = 0; k < M; ++k) dis[n][k] += t[k];
         }
       }
     }
@@ -544,7 +628,8 @@ __global__ __launch_bounds__(256) void rd_cell_kernel(CellArgs a)
   }
   double phi[ND][M];
   double wave;
-  cell_residual<D, NG, ND, NF, NQ, AFFINE, PH, SC>(a.tab, X, U, a.pp, phi, wave, bad, lds + threadIdx.x,
+  cell_residual<D, NG, ND, NF, NQ, AFFINE, PH, SC>(a.tab, X, U, URegs<ND, M>{U}, a.pp, phi, wave, bad,
+                                                   lds + threadIdx.x,
                                                    static_cast<int>(blockDim.x));
 #pragma unroll
   for (int n = 0; n < ND; ++n) {
@@ -618,7 +703,7 @@ struct PatchSize
 template <int M, int ND, bool AFFINE>
 struct PatchMinWaves
 {
-  static constexpr int value = (M == 1 && ND <= 6 && AFFINE) ? 5 : 1;
+  static constexpr int value = (M == 1 && ND <= 6 && AFFINE) ? 5 : (M > 1 ? 2 : 1);
 };
 
 // Dynamic LDS of one patch workgroup, in doubles: node state [maxl][M], the
@@ -749,11 +834,19 @@ __global__ __launch_bounds__((PatchSize<PhysTraits<PH>::M, ND>::value),
   double wave = 0.0;
   if (active) {
     double U[ND][M];
+    int row[ND];
 #pragma unroll
-    for (int n = 0; n < ND; ++n)
+    for (int n = 0; n < ND; ++n) {
+      row[n] = lc[n] & LC_MASK;
 #pragma unroll
-      for (int k = 0; k < M; ++k) U[n][k] = ul[(lc[n] & LC_MASK) * M + k];
-    cell_residual<D, NG, ND, NF, NQ, AFFINE, PH, SC>(a.tab, X, U, a.pp, phi, wave, bad, work + tid, PATCH);
+      for (int k = 0; k < M; ++k) U[n][k] = ul[row[n] * M + k];
+    }
+    if constexpr (M == 1)
+      cell_residual<D, NG, ND, NF, NQ, AFFINE, PH, SC>(a.tab, X, U, URegs<ND, M>{U}, a.pp, phi, wave, bad, work + tid,
+                                                       PATCH);
+    else
+      cell_residual<D, NG, ND, NF, NQ, AFFINE, PH, SC>(a.tab, X, U, ULds<ND, M>{ul, row}, a.pp, phi, wave, bad,
+                                                       work + tid, PATCH);
   }
   __syncthreads();  // the work region changes from divergence slices to pair sums
   if (active) {

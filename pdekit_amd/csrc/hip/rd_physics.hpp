@@ -88,55 +88,4 @@ __device__ __forceinline__ void point_flux(const T *s, const PhysParams &pp, T F
   }
 }
 
-// Compact K+ of one node at one quadrature point:
-//   scalar: kp = max(u_adv . g, 0)
-//   Euler:  l0 = max(|g| u.n, 0), dp = max(|g|(u.n+a),0) - l0,
-//           dm = max(|g|(u.n-a),0) - l0, unit normal n.
-template <int D>
-struct KCompact
-{
-  double l0, dp, dm, n[D], norm;
-};
-
-// Acoustic eigenvector pairs at a state for the unit direction n.
-template <int D>
-struct Acoustic
-{
-  double rp[D + 2], rm[D + 2], lp[D + 2], lm[D + 2];
-};
-
-template <int D>
-__device__ __forceinline__ void acoustic_vectors(const EulerState<double, D> &st, double gm1, const double *n,
-                                                 Acoustic<D> &ac)
-{
-  constexpr int M = D + 2;
-  const double a = st.a, ir = st.inv_rho;
-  const double inv_a = 1. / a, inv_a2 = inv_a * inv_a;
-  double um = 0.0;
-#pragma unroll
-  for (int d = 0; d < D; ++d) um += st.V[d] * n[d];
-  const double ra = 0.5 * st.rho * inv_a;
-  const double cM2 = (0.5 * gm1 * st.v2) * inv_a2;
-  const double gm1_ov_rhoa = gm1 / (st.rho * a);
-  ac.rp[0] = ra;
-  ac.rm[0] = ra;
-#pragma unroll
-  for (int d = 0; d < D; ++d) {
-    ac.rp[1 + d] = ra * (st.V[d] + a * n[d]);
-    ac.rm[1 + d] = ra * (st.V[d] - a * n[d]);
-  }
-  ac.rp[M - 1] = ra * (st.H + a * um);
-  ac.rm[M - 1] = ra * (st.H - a * um);
-  ac.lp[0] = a * ir * (cM2 - um * inv_a);
-  ac.lm[0] = a * ir * (cM2 + um * inv_a);
-#pragma unroll
-  for (int d = 0; d < D; ++d) {
-    const double gva = gm1 * st.V[d] * inv_a;
-    ac.lp[1 + d] = ir * (n[d] - gva);
-    ac.lm[1 + d] = -ir * (n[d] + gva);
-  }
-  ac.lp[M - 1] = gm1_ov_rhoa;
-  ac.lm[M - 1] = gm1_ov_rhoa;
-}
-
 }  // namespace pdrd
