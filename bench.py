@@ -51,19 +51,20 @@ def algorithmic_bytes(wl, scatter: int):
     return cell, sweep
 
 
-def traffic_from_profile(workload: str):
-    """HBM bytes per cell-loop launch measured with rocprofv3 --pmc
-    (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel factor), if a summary
-    for this workload is committed under profiles/."""
+def profile_entry(workload: str) -> dict:
+    """Figures of the patch kernel measured with rocprofv3 --pmc, if a summary
+    for this workload is committed under profiles/: HBM bytes per launch
+    (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel factor) and FP64 flops
+    per launch (SQ_INSTS_VALU_FLOPS_FP64 x 64 lanes)."""
     path = os.path.join(ROOT, "profiles", "traffic.json")
     if not os.path.exists(path):
-        return None
+        return {}
     try:
         with open(path) as f:
             data = json.load(f)
-        return data.get(workload, {}).get("bytes_per_launch")
+        return data.get(workload, {})
     except (OSError, ValueError):
-        return None
+        return {}
 
 
 def cpu_baseline(wl, seconds: float):
@@ -173,7 +174,8 @@ def main():
 
     cell_bytes, sweep_bytes = algorithmic_bytes(wl, best[0])
     achieved = cell_bytes / (cell_ms * 1e-3) / 1e9
-    traffic = traffic_from_profile(wl.name)
+    prof = profile_entry(wl.name) if best[0] == 2 else {}
+    traffic = prof.get("bytes_per_launch")
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                 "kernel": "rd_patch_kernel (cell loop + nodal step)" if best[0] == 2 else "rd_cell_kernel (cell loop)",
@@ -197,6 +199,13 @@ def main():
                    "scatter": SCATTER_NAMES[best[0]], "colours": info["nb_colours"],
                    "parallelism": "single GPU"},
         "roofline": roofline,
+        # the FP64 vector pipe is the binding roof of this kernel: flops per
+        # launch from the committed rocprofv3 counters over the live launch time
+        "fp64": ({"achieved": round(prof["fp64_flops_per_launch"] / (cell_ms * 1e-3) / 1e12, 2),
+                  "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                  "frac": round(prof["fp64_flops_per_launch"] / (cell_ms * 1e-3) / 1e12 / FP64_PEAK_TFLOPS, 4),
+                  "flops_per_element": round(prof["fp64_flops_per_launch"] / E, 1)}
+                 if "fp64_flops_per_launch" in prof else None),
         "sweep_hbm_frac": round(sweep_bytes / (t / args.steps) / 1e9 / HBM_PEAK_GBS, 4),
         "nodal_pass_us": round(node_ms * 1e3, 2),  # shared-node fix-up with the patch assembly
     }

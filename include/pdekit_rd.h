@@ -140,6 +140,10 @@ int pdrd_add_cells(pdrd_method_t h, const pdrd_fe_desc *fe, const double *tables
 int pdrd_initialize(pdrd_method_t h);
 int pdrd_destroy(pdrd_method_t h);
 int pdrd_set_option(pdrd_method_t h, int option, int value);
+/* The value of an option; PDRD_OPT_SCATTER gives the assembly in use (the
+ * patch assembly falls back to 1 at pdrd_initialize when a patch does not fit
+ * its limits). */
+int pdrd_get_option(pdrd_method_t h, int option, int *value);
 /* HIP stream (hipStream_t) the method enqueues on; NULL = the method's own. */
 int pdrd_set_stream(pdrd_method_t h, void *stream);
 int pdrd_synchronize(pdrd_method_t h);
@@ -187,6 +191,22 @@ int pdrd_method_info(pdrd_method_t h, int64_t *info6);
  * with HIP events on the method's stream (ms). Without PDRD_OPT_TIMING,
  * pdrd_iterate reports its whole duration as cell_ms and 0 as node_ms. */
 int pdrd_last_timing(pdrd_method_t h, double *cell_ms, double *node_ms);
+
+/* Batches (no counterpart in the reference, whose driver loops run one
+ * problem at a time): n_iter iterations of each of `count` independent
+ * methods, with the patches of every method of one element type / physics /
+ * scheme in one launch per iteration and the shared-node sums of every method
+ * with the same number of equations in another; the iterations are captured
+ * once per call into a HIP graph on the batch's stream. Each method advances
+ * exactly as pdrd_iterate(methods[i], cfl[i], n_iter, exclude_dirichlet)
+ * would (same kernels, same order of sums, so the same bits). Methods must be
+ * initialised on the batch's device with the patch assembly; norms receives
+ * the norms of the methods one after the other (neq doubles each). */
+typedef struct pdrd_batch *pdrd_batch_t;
+int pdrd_batch_create(pdrd_batch_t *out);
+int pdrd_batch_destroy(pdrd_batch_t b);
+int pdrd_batch_iterate(pdrd_batch_t b, int count, const pdrd_method_t *methods, const double *cfl, int n_iter,
+                       int exclude_dirichlet, double *norms);
 
 const char *pdrd_last_error(void);
 const char *pdrd_version(void);

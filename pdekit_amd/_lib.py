@@ -39,7 +39,8 @@ EXPORTED = (
     "pdrd_set_stream", "pdrd_synchronize", "pdrd_set_field", "pdrd_get_field", "pdrd_field_device_ptr",
     "pdrd_add_strong_dirichlet", "pdrd_assemble_rhs", "pdrd_assemble_lhs_and_rhs", "pdrd_solve",
     "pdrd_compute_residual_norm", "pdrd_residual_norm_L2", "pdrd_iterate", "pdrd_check_status",
-    "pdrd_method_info", "pdrd_last_timing", "pdrd_last_error", "pdrd_version",
+    "pdrd_method_info", "pdrd_last_timing", "pdrd_last_error", "pdrd_version", "pdrd_get_option",
+    "pdrd_batch_create", "pdrd_batch_destroy", "pdrd_batch_iterate",
 )
 
 
@@ -107,6 +108,10 @@ def lib() -> C.CDLL:
         "pdrd_compute_residual_norm": [P, PD],
         "pdrd_residual_norm_L2": [P, I, PD],
         "pdrd_iterate": [P, D, I, I, PD],
+        "pdrd_get_option": [P, I, PI],
+        "pdrd_batch_create": [C.POINTER(P)],
+        "pdrd_batch_destroy": [P],
+        "pdrd_batch_iterate": [P, I, C.POINTER(P), PD, I, I, PD],
         "pdrd_check_status": [P],
         "pdrd_method_info": [P, PI64],
         "pdrd_last_timing": [P, PD, PD],

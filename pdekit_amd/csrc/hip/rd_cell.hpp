@@ -784,20 +784,17 @@ __device__ __forceinline__ void block_sum_store(double (&sq)[M + 1], double *red
   }
 }
 
+// One patch (local index pl of the block described by a) on one workgroup.
 template <int D, int NG, int ND, int NF, int NQ, bool AFFINE, int PH, int SC>
-__global__ __launch_bounds__((PatchSize<PhysTraits<PH>::M, ND>::value),
-                             (PatchMinWaves<PhysTraits<PH>::M, ND, AFFINE>::value)) void rd_patch_kernel(PatchArgs a)
+__device__ __forceinline__ void patch_body(const PatchArgs &a, uint32_t pl, double *lds)
 {
   constexpr int M = PhysTraits<PH>::M;
   constexpr int PATCH = PatchSize<M, ND>::value;
   constexpr int NPAIR = PATCH * ND;  // stride of one component in the pair sums
-  extern __shared__ double lds[];
   double *ul = lds;
   uint16_t *coff = reinterpret_cast<uint16_t *>(lds + static_cast<size_t>(a.maxl) * M);
   double *work = lds + static_cast<size_t>(a.maxl) * M + patch_coff_doubles(a.maxl);
 
-  const uint32_t pl = xcd_block_index();
-  if (pl >= a.np) return;
   const uint32_t p = a.first_patch + pl;
   const uint32_t s0 = a.pnode_off[p];
   const int nl = static_cast<int>(a.pnode_off[p + 1] - s0);
@@ -886,6 +883,33 @@ __global__ __launch_bounds__((PatchSize<PhysTraits<PH>::M, ND>::value),
   if (a.update && a.outputs) block_sum_store<M>(sq, work, a.norm_partial + static_cast<size_t>(p) * (M + 1));
 }
 
+template <int D, int NG, int ND, int NF, int NQ, bool AFFINE, int PH, int SC>
+__global__ __launch_bounds__((PatchSize<PhysTraits<PH>::M, ND>::value),
+                             (PatchMinWaves<PhysTraits<PH>::M, ND, AFFINE>::value)) void rd_patch_kernel(PatchArgs a)
+{
+  extern __shared__ double lds[];
+  const uint32_t pl = xcd_block_index();
+  if (pl >= a.np) return;
+  patch_body<D, NG, ND, NF, NQ, AFFINE, PH, SC>(a, pl, lds);
+}
+
+// Patches of several blocks (of several methods) of one element type /
+// physics / scheme in one launch: workgroup g runs patch g - first[i] of
+// args[i], first[i] <= g < first[i + 1]. The dynamic LDS is the largest of
+// the entries'.
+template <int D, int NG, int ND, int NF, int NQ, bool AFFINE, int PH, int SC>
+__global__ __launch_bounds__((PatchSize<PhysTraits<PH>::M, ND>::value),
+                             (PatchMinWaves<PhysTraits<PH>::M, ND, AFFINE>::value)) void
+rd_patch_batch_kernel(const PatchArgs *__restrict__ args, const uint32_t *__restrict__ first, int count)
+{
+  extern __shared__ double lds[];
+  const uint32_t g = xcd_block_index();
+  if (g >= first[count]) return;
+  int i = 0;
+  while (i + 1 < count && first[i + 1] <= g) ++i;
+  patch_body<D, NG, ND, NF, NQ, AFFINE, PH, SC>(args[i], g - first[i], lds);
+}
+
 struct FixupArgs
 {
   const uint32_t *__restrict__ nodes;     // [nb] shared nodes
@@ -897,14 +921,15 @@ struct FixupArgs
   double *__restrict__ norm_partial;      // [gridDim.x][M + 1]
 };
 
+// Sums of the shared nodes of one method: local block lb of nblk blocks
+// (grid-stride over the nodes), norm partial of the block at lb.
 template <int M>
-__global__ __launch_bounds__(256) void patch_fixup_kernel(FixupArgs f)
+__device__ __forceinline__ void fixup_body(const FixupArgs &f, uint32_t lb, uint32_t nblk, double *red)
 {
-  __shared__ double red[256 * (M + 1)];
   double sq[M + 1];
 #pragma unroll
   for (int k = 0; k <= M; ++k) sq[k] = 0.0;
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < f.nb; i += gridDim.x * blockDim.x) {
+  for (uint32_t i = lb * blockDim.x + threadIdx.x; i < f.nb; i += nblk * blockDim.x) {
     double acc[M + 1];
 #pragma unroll
     for (int k = 0; k <= M; ++k) acc[k] = 0.0;
@@ -919,7 +944,28 @@ __global__ __launch_bounds__(256) void patch_fixup_kernel(FixupArgs f)
     for (int k = 0; k < M; ++k) u_old[k] = f.pa.u[static_cast<size_t>(g) * M + k];
     finish_node<M>(f.pa, g, acc, u_old, sq);
   }
-  if (f.pa.update && f.pa.outputs) block_sum_store<M>(sq, red, f.norm_partial + static_cast<size_t>(blockIdx.x) * (M + 1));
+  if (f.pa.update && f.pa.outputs) block_sum_store<M>(sq, red, f.norm_partial + static_cast<size_t>(lb) * (M + 1));
+}
+
+template <int M>
+__global__ __launch_bounds__(256) void patch_fixup_kernel(FixupArgs f)
+{
+  __shared__ double red[256 * (M + 1)];
+  fixup_body<M>(f, blockIdx.x, gridDim.x, red);
+}
+
+// Shared-node sums of several methods in one launch: block g serves
+// args[i] as its block g - first[i] of first[i + 1] - first[i].
+template <int M>
+__global__ __launch_bounds__(256) void patch_fixup_batch_kernel(const FixupArgs *__restrict__ args,
+                                                                const uint32_t *__restrict__ first, int count)
+{
+  __shared__ double red[256 * (M + 1)];
+  const uint32_t g = blockIdx.x;
+  if (g >= first[count]) return;
+  int i = 0;
+  while (i + 1 < count && first[i + 1] <= g) ++i;
+  fixup_body<M>(args[i], g - first[i], first[i + 1] - first[i], red);
 }
 
 }  // namespace pdrd

@@ -9,6 +9,8 @@ namespace pdrd {
 
 using CellLauncher = hipError_t (*)(const CellArgs &, unsigned grid, unsigned block, hipStream_t);
 using PatchLauncher = hipError_t (*)(const PatchArgs &, unsigned grid, hipStream_t);
+using PatchBatchLauncher = hipError_t (*)(const PatchArgs *args, const uint32_t *first, int count, unsigned grid,
+                                          size_t lds_bytes, hipStream_t);
 
 struct CellKey
 {
@@ -20,6 +22,7 @@ struct KernelSet
 {
   CellLauncher cell = nullptr;    // element per thread, global scatter
   PatchLauncher patch = nullptr;  // patch assembly
+  PatchBatchLauncher patch_batch = nullptr;  // patch assembly of several blocks / methods
   int patch_size = 0;             // cells per patch workgroup
   size_t (*patch_lds_bytes)(int maxl) = nullptr;
   explicit operator bool() const { return cell != nullptr; }
@@ -54,6 +57,16 @@ hipError_t launch_patch(const PatchArgs &a, unsigned grid, hipStream_t s)
   return hipGetLastError();
 }
 
+template <int D, int NG, int ND, int NF, int NQ, bool AFF, int PH, int SC>
+hipError_t launch_patch_batch(const PatchArgs *args, const uint32_t *first, int count, unsigned grid, size_t lds,
+                              hipStream_t s)
+{
+  constexpr int M = PhysTraits<PH>::M;
+  hipLaunchKernelGGL((rd_patch_batch_kernel<D, NG, ND, NF, NQ, AFF, PH, SC>), dim3(grid),
+                     dim3(PatchSize<M, ND>::value), lds, s, args, first, count);
+  return hipGetLastError();
+}
+
 template <int NQ, int ND, int M>
 size_t patch_lds_bytes(int maxl)
 {
@@ -66,6 +79,7 @@ KernelSet kernel_set()
   KernelSet k;
   k.cell = &launch_cell<D, NG, ND, NF, NQ, AFF, PH, SC>;
   k.patch = &launch_patch<D, NG, ND, NF, NQ, AFF, PH, SC>;
+  k.patch_batch = &launch_patch_batch<D, NG, ND, NF, NQ, AFF, PH, SC>;
   k.patch_size = PatchSize<PhysTraits<PH>::M, ND>::value;
   k.patch_lds_bytes = &patch_lds_bytes<NQ, ND, PhysTraits<PH>::M>;
   return k;

@@ -351,6 +351,19 @@ pdrd::PatchArgs patch_args(const pdrd_method *h, const Block *b, int update, int
   return a;
 }
 
+pdrd::FixupArgs fixup_args(const pdrd_method *h, int update, int outputs, int excl, double cfl)
+{
+  pdrd::FixupArgs f{};
+  f.nodes = h->d_shared_nodes;
+  f.slot_off = h->d_shared_off;
+  f.slots = h->d_shared_slots;
+  f.slot_acc = h->d_slot_acc;
+  f.pa = patch_args(h, nullptr, update, outputs, excl, cfl);
+  f.nb = h->n_shared;
+  f.norm_partial = h->d_patch_partial + static_cast<size_t>(h->np_total) * (h->m + 1);
+  return f;
+}
+
 // Patch assembly of every block followed by the sums of the shared nodes.
 // update = 0: residual and wave speed of every node; update = 1: one
 // pseudo-time iteration (the strong Dirichlet values are applied in the
@@ -362,14 +375,7 @@ int launch_patches(pdrd_method *h, int update, int outputs, int excl, double cfl
     HIP_CHECK(b.ks.patch(a, b.np, h->stream));
   }
   if (h->n_shared == 0) return PDRD_OK;
-  pdrd::FixupArgs f{};
-  f.nodes = h->d_shared_nodes;
-  f.slot_off = h->d_shared_off;
-  f.slots = h->d_shared_slots;
-  f.slot_acc = h->d_slot_acc;
-  f.pa = patch_args(h, nullptr, update, outputs, excl, cfl);
-  f.nb = h->n_shared;
-  f.norm_partial = h->d_patch_partial + static_cast<size_t>(h->np_total) * (h->m + 1);
+  const pdrd::FixupArgs f = fixup_args(h, update, outputs, excl, cfl);
   const unsigned nb = fixup_blocks(h);
 #define FIX_CASE(MM)                                                                                    \
   case MM:                                                                                              \
@@ -1051,14 +1057,7 @@ int pdrd_iterate(pdrd_method_t h, double cfl, int n_iter, int exclude_dirichlet,
         HIP_CHECK(hipEventRecord(h->ev[1], h->stream));
         if (h->n_shared) {
           // fix-up only: re-enqueue through launch_patches would repeat the cells
-          pdrd::FixupArgs f{};
-          f.nodes = h->d_shared_nodes;
-          f.slot_off = h->d_shared_off;
-          f.slots = h->d_shared_slots;
-          f.slot_acc = h->d_slot_acc;
-          f.pa = patch_args(h, nullptr, 1, out, excl, cfl);
-          f.nb = h->n_shared;
-          f.norm_partial = h->d_patch_partial + static_cast<size_t>(h->np_total) * (h->m + 1);
+          const pdrd::FixupArgs f = fixup_args(h, 1, out, excl, cfl);
           const unsigned nbk = fixup_blocks(h);
           if (h->m == 1)
             hipLaunchKernelGGL(pdrd::patch_fixup_kernel<1>, dim3(nbk), dim3(256), 0, h->stream, f);
@@ -1145,12 +1144,246 @@ int pdrd_method_info(pdrd_method_t h, int64_t *info)
   return PDRD_OK;
 }
 
+int pdrd_get_option(pdrd_method_t h, int option, int *value)
+{
+  if (!h || !value) return set_error(PDRD_EINVAL, "null argument");
+  switch (option) {
+    case PDRD_OPT_SCATTER: *value = h->scatter_mode; return PDRD_OK;
+    case PDRD_OPT_PRECISION: *value = 0; return PDRD_OK;
+    case PDRD_OPT_BLOCK: *value = h->block_threads; return PDRD_OK;
+    case PDRD_OPT_TIMING: *value = h->timing ? 1 : 0; return PDRD_OK;
+  }
+  return set_error(PDRD_EINVAL, "unknown option");
+}
+
 int pdrd_last_timing(pdrd_method_t h, double *cell_ms, double *node_ms)
 {
   if (!h) return set_error(PDRD_EINVAL, "null method");
   if (cell_ms) *cell_ms = h->last_cell_ms;
   if (node_ms) *node_ms = h->last_node_ms;
   return PDRD_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Batches: the iterations of several methods in shared launches.
+// ---------------------------------------------------------------------------
+struct pdrd_batch
+{
+  int device = -1;
+  hipStream_t stream = nullptr;
+  pdrd::PatchArgs *d_pargs = nullptr;
+  pdrd::FixupArgs *d_fargs = nullptr;
+  uint32_t *d_first = nullptr;
+  size_t pargs_cap = 0, fargs_cap = 0, first_cap = 0;
+};
+
+namespace {
+
+template <typename T>
+int grow(T **p, size_t *cap, size_t need)
+{
+  if (need <= *cap) return PDRD_OK;
+  (void)hipFree(*p);
+  *p = nullptr;
+  *cap = 0;
+  const size_t n = std::max<size_t>(need, 64);
+  hipError_t e = hipMalloc(reinterpret_cast<void **>(p), n * sizeof(T));
+  if (e != hipSuccess) return set_error(PDRD_ENOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
+  *cap = n;
+  return PDRD_OK;
+}
+
+// One launch of the batch: a kernel and its slice of the argument arrays.
+struct BatchLaunch
+{
+  pdrd::PatchBatchLauncher patch = nullptr;  // null: shared-node sums
+  int m = 0;                                 // equations (shared-node sums)
+  size_t args0 = 0, first0 = 0;              // offsets in the argument / prefix arrays
+  int count = 0;
+  unsigned grid = 0;
+  size_t lds = 0;
+};
+
+int enqueue_batch(pdrd_batch *b, const std::vector<BatchLaunch> &launches, size_t args_half, size_t fargs_half,
+                  int outputs)
+{
+  for (const BatchLaunch &l : launches) {
+    if (l.grid == 0) continue;
+    if (l.patch) {
+      HIP_CHECK(l.patch(b->d_pargs + (outputs ? args_half : 0) + l.args0, b->d_first + l.first0, l.count, l.grid,
+                        l.lds, b->stream));
+      continue;
+    }
+    const pdrd::FixupArgs *fa = b->d_fargs + (outputs ? fargs_half : 0) + l.args0;
+#define FIXB_CASE(MM)                                                                                            \
+  case MM:                                                                                                       \
+    hipLaunchKernelGGL(pdrd::patch_fixup_batch_kernel<MM>, dim3(l.grid), dim3(256), 0, b->stream, fa,             \
+                       b->d_first + l.first0, l.count);                                                         \
+    break;
+    switch (l.m) {
+      FIXB_CASE(1)
+      FIXB_CASE(4)
+      FIXB_CASE(5)
+      default: return set_error(PDRD_ENOTSUP, "unsupported number of equations");
+    }
+#undef FIXB_CASE
+    HIP_CHECK(hipGetLastError());
+  }
+  return PDRD_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int pdrd_batch_create(pdrd_batch_t *out)
+{
+  PDRD_TRY
+  if (!out) return set_error(PDRD_EINVAL, "null argument");
+  pdrd_batch *b = new pdrd_batch();
+  hipError_t e = hipGetDevice(&b->device);
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
+  if (e != hipSuccess) {
+    delete b;
+    return set_error(PDRD_EHIP, std::string("pdrd_batch_create: ") + hipGetErrorString(e));
+  }
+  *out = b;
+  return PDRD_OK;
+  PDRD_CATCH
+}
+
+int pdrd_batch_destroy(pdrd_batch_t b)
+{
+  if (!b) return PDRD_OK;
+  (void)hipStreamSynchronize(b->stream);
+  (void)hipFree(b->d_pargs);
+  (void)hipFree(b->d_fargs);
+  (void)hipFree(b->d_first);
+  (void)hipStreamDestroy(b->stream);
+  delete b;
+  return PDRD_OK;
+}
+
+int pdrd_batch_iterate(pdrd_batch_t b, int count, const pdrd_method_t *methods, const double *cfl, int n_iter,
+                       int exclude_dirichlet, double *norms)
+{
+  PDRD_TRY
+  if (!b || !methods || !cfl || count < 1) return set_error(PDRD_EINVAL, "null or empty batch");
+  if (n_iter < 1) return set_error(PDRD_EINVAL, "n_iter must be >= 1");
+  const int excl = exclude_dirichlet ? 1 : 0;
+  for (int i = 0; i < count; ++i) {
+    pdrd_method *h = methods[i];
+    if (int rc = check_ready(h)) return rc;
+    if (h->device != b->device) return set_error(PDRD_EINVAL, "batch methods must live on the batch's device");
+    if (h->scatter_mode != 2) return set_error(PDRD_ENOTSUP, "batched iterations need the patch assembly");
+    for (int j = 0; j < i; ++j)
+      if (methods[j] == h) return set_error(PDRD_EINVAL, "a method appears twice in one batch");
+  }
+  for (int i = 0; i < count; ++i) HIP_CHECK(hipStreamSynchronize(methods[i]->stream));
+
+  // group the blocks of all methods by kernel, in order of first appearance
+  std::vector<BatchLaunch> launches;
+  std::vector<std::vector<std::pair<int, size_t>>> members;  // (method, block) per launch
+  for (int i = 0; i < count; ++i)
+    for (size_t k = 0; k < methods[i]->blocks.size(); ++k) {
+      const Block &bl = methods[i]->blocks[k];
+      size_t g = 0;
+      while (g < launches.size() && launches[g].patch != bl.ks.patch_batch) ++g;
+      if (g == launches.size()) {
+        BatchLaunch l;
+        l.patch = bl.ks.patch_batch;
+        launches.push_back(l);
+        members.emplace_back();
+      }
+      members[g].emplace_back(i, k);
+    }
+  const size_t npatch_launch = launches.size();
+  for (int mm : {1, 4, 5}) {
+    BatchLaunch l;
+    l.m = mm;
+    std::vector<std::pair<int, size_t>> mem;
+    for (int i = 0; i < count; ++i)
+      if (methods[i]->m == mm && methods[i]->n_shared) mem.emplace_back(i, 0);
+    if (mem.empty()) continue;
+    launches.push_back(l);
+    members.push_back(mem);
+  }
+  // argument arrays ([no outputs | outputs]) and prefix sums of workgroups
+  std::vector<pdrd::PatchArgs> pargs[2];
+  std::vector<pdrd::FixupArgs> fargs[2];
+  std::vector<uint32_t> first;
+  for (size_t g = 0; g < launches.size(); ++g) {
+    BatchLaunch &l = launches[g];
+    l.first0 = first.size();
+    l.count = static_cast<int>(members[g].size());
+    uint32_t acc = 0;
+    first.push_back(0);
+    if (g < npatch_launch) {
+      l.args0 = pargs[0].size();
+      for (const auto &mk : members[g]) {
+        pdrd_method *h = methods[mk.first];
+        const Block &bl = h->blocks[mk.second];
+        for (int o = 0; o < 2; ++o) pargs[o].push_back(patch_args(h, &bl, 1, o, excl, cfl[mk.first]));
+        l.lds = std::max(l.lds, bl.ks.patch_lds_bytes(bl.maxl));
+        acc += bl.np;
+        first.push_back(acc);
+      }
+    } else {
+      l.args0 = fargs[0].size();
+      for (const auto &mk : members[g]) {
+        pdrd_method *h = methods[mk.first];
+        for (int o = 0; o < 2; ++o) fargs[o].push_back(fixup_args(h, 1, o, excl, cfl[mk.first]));
+        acc += fixup_blocks(h);
+        first.push_back(acc);
+      }
+    }
+    l.grid = acc;
+  }
+  const size_t ph = pargs[0].size(), fh = fargs[0].size();
+  if (int rc = grow(&b->d_pargs, &b->pargs_cap, 2 * ph)) return rc;
+  if (int rc = grow(&b->d_fargs, &b->fargs_cap, 2 * fh)) return rc;
+  if (int rc = grow(&b->d_first, &b->first_cap, first.size())) return rc;
+  for (int o = 0; o < 2; ++o) {
+    if (ph) HIP_CHECK(hipMemcpy(b->d_pargs + o * ph, pargs[o].data(), ph * sizeof(pdrd::PatchArgs), hipMemcpyHostToDevice));
+    if (fh) HIP_CHECK(hipMemcpy(b->d_fargs + o * fh, fargs[o].data(), fh * sizeof(pdrd::FixupArgs), hipMemcpyHostToDevice));
+  }
+  HIP_CHECK(hipMemcpy(b->d_first, first.data(), first.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+
+  if (n_iter > 1) {
+    hipGraph_t g = nullptr;
+    hipGraphExec_t ge = nullptr;
+    HIP_CHECK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+    const int rc = enqueue_batch(b, launches, ph, fh, 0);
+    hipError_t e = hipStreamEndCapture(b->stream, &g);
+    if (rc) {
+      if (g) (void)hipGraphDestroy(g);
+      return rc;
+    }
+    if (e != hipSuccess) return hip_fail(e, "hipStreamEndCapture");
+    e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess) return hip_fail(e, "hipGraphInstantiate");
+    for (int it = 0; it < n_iter - 1; ++it) {
+      e = hipGraphLaunch(ge, b->stream);
+      if (e != hipSuccess) break;
+    }
+    (void)hipStreamSynchronize(b->stream);
+    (void)hipGraphExecDestroy(ge);
+    if (e != hipSuccess) return hip_fail(e, "hipGraphLaunch");
+  }
+  if (int rc = enqueue_batch(b, launches, ph, fh, 1)) return rc;
+  HIP_CHECK(hipStreamSynchronize(b->stream));
+  if (norms) {
+    size_t off = 0;
+    for (int i = 0; i < count; ++i) {
+      if (int rc = iterate_norm(methods[i], norms + off)) return rc;
+      off += methods[i]->m;
+    }
+  }
+  return PDRD_OK;
+  PDRD_CATCH
 }
 
 }  // extern "C"

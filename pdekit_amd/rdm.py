@@ -80,6 +80,11 @@ class PGRDMethodExplicit:
     def set_option(self, option: int, value: int):
         L.check(L.lib().pdrd_set_option(self._h, option, int(value)), "pdrd_set_option")
 
+    def get_option(self, option: int) -> int:
+        v = C.c_int(0)
+        L.check(L.lib().pdrd_get_option(self._h, option, C.byref(v)), "pdrd_get_option")
+        return v.value
+
     def __del__(self):
         h = getattr(self, "_h", None)
         if h is not None and h.value:
@@ -162,3 +167,36 @@ class PGRDMethodExplicit:
         n = C.c_double(0)
         L.check(L.lib().pdrd_last_timing(self._h, C.byref(c), C.byref(n)), "pdrd_last_timing")
         return c.value, n.value
+
+
+class Batch:
+    """Iterations of several independent methods in shared launches
+    (pdrd_batch_iterate): one launch per element type / physics / scheme and
+    iteration for the patches of all methods, one for their shared nodes.
+    Every method advances exactly as its own iterate() would."""
+
+    def __init__(self):
+        h = C.c_void_p()
+        L.check(L.lib().pdrd_batch_create(C.byref(h)), "pdrd_batch_create")
+        self._h = h
+
+    def iterate(self, methods, cfl, n_iter: int, exclude_dirichlet: bool = False):
+        """methods: PGRDMethodExplicit list; cfl: one value or one per method.
+        Returns the norm_L2 of the last residual of every method."""
+        n = len(methods)
+        hs = (C.c_void_p * n)(*[m._h.value for m in methods])
+        c = np.ascontiguousarray(np.broadcast_to(np.asarray(cfl, dtype=np.float64), (n,)))
+        norms = np.zeros(sum(m.neq for m in methods))
+        L.check(L.lib().pdrd_batch_iterate(self._h, n, hs, L.ptr(c), int(n_iter), int(exclude_dirichlet),
+                                           L.ptr(norms)), "pdrd_batch_iterate")
+        out, off = [], 0
+        for m in methods:
+            out.append(norms[off:off + m.neq].copy())
+            off += m.neq
+        return out
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            L.lib().pdrd_batch_destroy(h)
+            self._h = None

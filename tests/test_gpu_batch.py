@@ -1,0 +1,65 @@
+"""Batched iterations (pdrd_batch_iterate): several independent methods of
+different element types, orders, physics and schemes advanced in shared
+launches give bit for bit what each method's own pdrd_iterate gives."""
+import numpy as np
+import pytest
+
+from pdekit_amd import _lib as L
+from pdekit_amd.rdm import Batch, PGRDMethodExplicit
+from tests.helpers import CASES, quad_orders
+
+pytestmark = pytest.mark.gpu
+
+MIX = [("adv_p1", L.N), ("adv_p2", L.LDA), ("adv_p3", L.B), ("burgers_p2", L.B), ("burgers_q1", L.PSI),
+       ("euler2d_mixed", L.LDA), ("euler2d_p2", L.N), ("euler3d_p1", L.LDA), ("burgers_p1", L.B)]
+CFL = {L.ADVECTION2D: 0.9, L.BURGERS2D: 0.5, L.EULER2D: 0.3, L.EULER3D: 0.3}
+
+
+def _make(name, scheme):
+    build, physics, params, state = CASES[name]
+    mesh = build()
+    u = state(mesh)
+    m = PGRDMethodExplicit(physics, scheme, params=params if params else None)
+    m.configure_mesh_data(mesh).initialize_work_data(quad_order=quad_orders(mesh))
+    assert m.get_option(L.OPT_SCATTER) == 2
+    m.set_vec_function("solution", u)
+    bn = mesh.boundaries.get("xmin", mesh.boundaries.get("Edge3"))
+    if bn is not None and physics in (L.ADVECTION2D, L.BURGERS2D):
+        m.add_boundary_condition("StrongDirichlet", "inflow", bn, u[bn])
+    return m, CFL[physics]
+
+
+@pytest.mark.parametrize("n_iter", [1, 7])
+def test_batch_matches_single_methods(n_iter):
+    print(f"batch of {len(MIX)} methods, {n_iter} iterations", flush=True)
+    alone = [_make(n, s) for n, s in MIX]
+    mixed = [_make(n, s) for n, s in MIX]
+    ref_norms = [m.iterate(c, n_iter) for m, c in alone]
+    b = Batch()
+    norms = b.iterate([m for m, _ in mixed], [c for _, c in mixed], n_iter)
+    for (ma, _), (mb, _), na, nb, (name, _) in zip(alone, mixed, ref_norms, norms, MIX):
+        for f in ("solution", "residuals", "time_step", "wave_speed"):
+            a, c = ma.vec_function(f), mb.vec_function(f)
+            assert np.array_equal(a, c), (name, f, np.abs(a - c).max())
+        assert np.array_equal(na, nb), name
+    # a second call on a different subset and order reuses the batch buffers
+    sub = [mixed[5], mixed[0], mixed[3]]
+    for m, c in [alone[5], alone[0], alone[3]]:
+        m.iterate(c, 3)
+    b.iterate([m for m, _ in sub], [c for _, c in sub], 3)
+    for (ma, _), (mb, _) in zip([alone[5], alone[0], alone[3]], sub):
+        assert np.array_equal(ma.vec_function("solution"), mb.vec_function("solution"))
+
+
+def test_batch_rejects_bad_input():
+    m, c = _make("adv_p1", L.N)
+    b = Batch()
+    with pytest.raises(L.PdrdError):
+        b.iterate([m, m], [c, c], 2)
+    m2 = PGRDMethodExplicit(L.ADVECTION2D, L.N, params=(0.8, 0.6))
+    build = CASES["adv_p1"][0]
+    mesh = build()
+    m2.configure_mesh_data(mesh).initialize_work_data(quad_order=quad_orders(mesh), scatter=1)
+    m2.set_vec_function("solution", np.zeros((mesh.nb_nodes, 1)))
+    with pytest.raises(L.PdrdError):
+        b.iterate([m, m2], [c, c], 2)

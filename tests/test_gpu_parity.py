@@ -44,6 +44,7 @@ def test_assemble_rhs_matches_oracle(name, scheme, scatter):
     mesh, physics, params, u = _case(name)
     print(f"case {name} scheme {scheme} scatter {scatter} cells {mesh.nb_cells} nodes {mesh.nb_nodes}", flush=True)
     m = _method(mesh, physics, scheme, params, scatter=SCATTER[scatter])
+    assert m.get_option(L.OPT_SCATTER) == SCATTER[scatter]  # no silent fallback
     m.set_vec_function("solution", u)
     m.assemble_lhs_and_rhs(0.5)
     m.check_status()
