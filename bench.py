@@ -89,13 +89,68 @@ def cpu_baseline(wl, seconds: float):
                       f"oracle/rd_oracle.cpp, 1 thread, {t:.1f} s"}
 
 
+def serving_bench(args):
+    """The serving stream: seeded Poisson arrivals (20 requests/s, seed 5) of
+    steady advection P1-P3 and Burgers P2 solves to a residual norm of 1e-10,
+    served with continuous batching and, as the baseline, one at a time in
+    arrival order; p50 / p99 time-to-converge and throughput of both."""
+    from pdekit_amd import serving as S
+
+    sc = args.size_scale
+    reqs = S.request_stream(args.requests, rate=20.0, seed=5,
+                            adv_cells=(int(10_000 * sc), int(1_000_000 * sc)),
+                            burgers_cells=(int(50_000 * sc), int(500_000 * sc)), tol=1e-10)
+    t0 = time.perf_counter()
+    probs = {r.rid: S.build_problem(r) for r in reqs}
+    print(f"[bench] c5: {len(reqs)} requests, {sum(p.mesh.nb_cells for p in probs.values())} cells in all, "
+          f"meshes built in {time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
+    # warm-up: load the code objects of every kernel the stream uses
+    warm = S.Server(S.GpuExecutor(), chunk=2, max_iters=2)
+    kinds = {}
+    for r in reqs:
+        kinds.setdefault((r.family, r.order), r)
+    small = [S.Request(i, 0.0, k[0], k[1], 2_000, 1e-10) for i, k in enumerate(kinds)]
+    warm.run(small, {q.rid: S.build_problem(q) for q in small}, mode="batched")
+    out = {}
+    for mode in ("batched", "sequential"):
+        srv = S.Server(S.GpuExecutor(), chunk=50, max_iters=100_000)
+        res, peak = srv.run(reqs, probs, mode=mode)
+        out[mode] = S.summary(res)
+        out[mode]["peak_footprint_bytes"] = int(peak)
+        print(f"[bench] c5 {mode}: {out[mode]}", file=sys.stderr, flush=True)
+    b, q = out["batched"], out["sequential"]
+    line = {
+        "metric": "p99 time-to-converge (serving stream)",
+        "value": round(b["p99_s"], 4),
+        "unit": "s",
+        "n_gpus": 1,
+        "higher_is_better": False,
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "serving_c5", "requests": len(reqs), "rate_per_s": 20.0, "seed": 5,
+                   "size_scale": sc, "tol": 1e-10, "chunk": 50, "parallelism": "single GPU"},
+        "batched": b,
+        "sequential": q,
+        "p99_speedup_vs_sequential": round(q["p99_s"] / b["p99_s"], 3),
+        "p50_speedup_vs_sequential": round(q["p50_s"] / b["p50_s"], 3),
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--workload", default="c2", choices=["c1", "c2", "c3", "c4"])
+    ap.add_argument("--workload", default="c2", choices=["c1", "c2", "c3", "c4", "c5"],
+                    help="c1-c4: per-sweep throughput of one mesh; c5: the serving stream")
+    ap.add_argument("--requests", type=int, default=64, help="c5: requests in the stream")
+    ap.add_argument("--size-scale", type=float, default=0.1,
+                    help="c5: scale of the request sizes (1.0: advection 10k-1M, Burgers 50k-500k triangles)")
     ap.add_argument("--scatter", default="auto", choices=["auto", "atomic", "colour", "patch"])
+    ap.add_argument("--precision", default="plain", choices=["plain", "mixed"],
+                    help="plain FP64 cells, or FP32 interpolation / fluxes / K+ with FP64 solves and sums")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-calibration-mb", type=int, default=0,
@@ -119,6 +174,9 @@ def main():
         print(f"bench.py measures one GPU: this build has no multi-GPU path (--gpus {args.gpus} requested). "
               "Run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
+    if args.workload == "c5":
+        serving_bench(args)
+        return
 
     from pdekit_amd import _lib as L
     from pdekit_amd.rdm import PGRDMethodExplicit
@@ -129,6 +187,8 @@ def main():
     E = wl.nb_elements
     meth = PGRDMethodExplicit(wl.physics, wl.scheme, params=wl.params if wl.params else None)
     meth.configure_mesh_data(wl.mesh).initialize_work_data(quad_order=wl.quad_order)
+    if args.precision == "mixed":
+        meth.set_option(L.OPT_PRECISION, 1)
     meth.set_vec_function("solution", wl.state)
     if wl.dirichlet_nodes.size:
         meth.add_boundary_condition("StrongDirichlet", "inflow", wl.dirichlet_nodes, wl.dirichlet_values)
@@ -138,6 +198,8 @@ def main():
 
     # scatter: measured choice between FP64 atomics and colour-by-colour launches
     modes = {"atomic": [0], "colour": [1], "patch": [2], "auto": [2, 1, 0]}[args.scatter]
+    if args.precision == "mixed":
+        modes = [2]  # the mixed cells run with the patch assembly only
     best = None
     for mode in modes:
         meth.set_option(L.OPT_SCATTER, mode)
@@ -174,7 +236,7 @@ def main():
 
     cell_bytes, sweep_bytes = algorithmic_bytes(wl, best[0])
     achieved = cell_bytes / (cell_ms * 1e-3) / 1e9
-    prof = profile_entry(wl.name) if best[0] == 2 else {}
+    prof = profile_entry(wl.name) if (best[0] == 2 and args.precision == "plain") else {}
     traffic = prof.get("bytes_per_launch")
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
@@ -192,9 +254,9 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f64" if args.precision == "plain" else "f32+f64",
         "data": "synthetic",
-        "config": {"workload": wl.name, "elements": E, "nodes": wl.mesh.nb_nodes,
+        "config": {"workload": wl.name, "precision": args.precision, "elements": E, "nodes": wl.mesh.nb_nodes,
                    "order": wl.mesh.order, "scheme": ["PGLDA", "PGN", "PGB", "PSI"][wl.scheme],
                    "scatter": SCATTER_NAMES[best[0]], "colours": info["nb_colours"],
                    "parallelism": "single GPU"},

@@ -22,6 +22,7 @@
 
 #include <hip/hip_runtime.h>
 #include <cstdint>
+#include <type_traits>
 
 #include "rd_physics.hpp"
 
@@ -66,6 +67,10 @@ struct TabOffsets
   static constexpr int dVT = dVfT + NF * D * NQ;     // [NQ][D][ND]
   static constexpr int dVgT = dVT + NQ * D * ND;     // [NQ][D][NG]
   static constexpr int dev_size = dVgT + NQ * D * NG;
+  // then a float copy of entries [0, dev_size) (same offsets), read by the
+  // mixed-precision cells
+  static constexpr int f32 = dev_size;
+  static constexpr int dev_size_all = dev_size + (dev_size + 1) / 2;
 };
 
 // Workgroups are dealt round-robin to the 8 XCDs; give each XCD one
@@ -186,7 +191,14 @@ struct ULds
 // coordinates X and nodal state U. lds_div: this thread's slice of NQ*M
 // doubles of LDS (stride `stride` between entries) that carries the flux
 // divergence from the flux-node loop to the quadrature loop.
-template <int D, int NG, int ND, int NF, int NQ, bool AFFINE, int PH, int SC, class UA>
+//
+// PREC 0: everything in FP64. PREC 1 (mixed): the interpolation of the state
+// to the flux nodes and quadrature points, the fluxes (as differences from a
+// nodal reference state, see flux_delta) and their divergence, and the
+// upwind parameters K+ are evaluated in FP32; the sums S = sum K+, the solves
+// with S, the element residuals, wave speeds, and everything after them
+// (nodal sums, time step, update, norms) stay FP64.
+template <int D, int NG, int ND, int NF, int NQ, bool AFFINE, int PH, int SC, int PREC, class UA>
 __device__ __forceinline__ void cell_residual(const double *__restrict__ tab, const double (&X)[NG][D],
                                               const double (&U)[ND][PhysTraits<PH>::M], const UA &Uq,
                                               const PhysParams &pp, double (&phi)[ND][PhysTraits<PH>::M],
@@ -195,6 +207,19 @@ __device__ __forceinline__ void cell_residual(const double *__restrict__ tab, co
   constexpr int M = PhysTraits<PH>::M;
   static_assert(PhysTraits<PH>::DIM == D, "physics / element dimension mismatch");
   using O = TabOffsets<D, NG, ND, NF, NQ>;
+  using TK = std::conditional_t<PREC == 1, float, double>;  // type of the FP32-able parts
+  const float *__restrict__ tabf = reinterpret_cast<const float *>(tab + O::f32);
+  // mixed: reference state and nodal differences
+  double ub[M];
+  float dU[PREC == 1 ? ND : 1][M];
+  if constexpr (PREC == 1) {
+#pragma unroll
+    for (int k = 0; k < M; ++k) ub[k] = U[0][k];
+#pragma unroll
+    for (int n = 0; n < ND; ++n)
+#pragma unroll
+      for (int k = 0; k < M; ++k) dU[n][k] = static_cast<float>(U[n][k] - ub[k]);
+  }
 
   // ---- geometry ----------------------------------------------------------
   constexpr int NJ = AFFINE ? 1 : NQ;
@@ -219,6 +244,60 @@ __device__ __forceinline__ void cell_residual(const double *__restrict__ tab, co
   // The flux-node loop stays rolled: each trip reads one row of Vsf and one
   // [D][NQ] slab of dVf with scalar loads, so the table never has to be held
   // in (and spilled from) scalar registers as a whole.
+  if constexpr (PREC == 1) {
+    DeltaBase<PH, float> base;
+    delta_base<PH, float>(ub, pp, base);
+    float Jf[AFFINE ? 1 : NQ][D][D];
+#pragma unroll
+    for (int q = 0; q < (AFFINE ? 1 : NQ); ++q)
+#pragma unroll
+      for (int d = 0; d < D; ++d)
+#pragma unroll
+        for (int xi = 0; xi < D; ++xi) Jf[q][d][xi] = static_cast<float>(Jinv[q][d][xi]);
+    float divf[NQ][M];
+#pragma unroll
+    for (int q = 0; q < NQ; ++q)
+#pragma unroll
+      for (int k = 0; k < M; ++k) divf[q][k] = 0.0f;
+#pragma unroll 1
+    for (int f = 0; f < NF; ++f) {
+      const float *vs = tabf + O::Vsf + f * ND;
+      const float *dvf = tabf + O::dVfT + f * D * NQ;
+      float ds[M];
+#pragma unroll
+      for (int k = 0; k < M; ++k) {
+        float t = 0.0f;
+#pragma unroll
+        for (int n = 1; n < ND; ++n) t += vs[n] * dU[n][k];  // dU[0] = 0
+        ds[k] = t;
+      }
+      float F[D][M];
+      flux_delta<PH, float>(base, ds, pp, F, bad);
+#pragma unroll
+      for (int q = 0; q < NQ; ++q) {
+        const int jq = AFFINE ? 0 : q;
+        float b[D];
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+          float t = 0.0f;
+#pragma unroll
+          for (int xi = 0; xi < D; ++xi) t += Jf[jq][d][xi] * dvf[xi * NQ + q];
+          b[d] = t;
+        }
+#pragma unroll
+        for (int k = 0; k < M; ++k) {
+          float t = divf[q][k];
+#pragma unroll
+          for (int d = 0; d < D; ++d) t += b[d] * F[d][k];
+          divf[q][k] = t;
+        }
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < NQ; ++q)
+#pragma unroll
+      for (int k = 0; k < M; ++k) lds_div[(q * M + k) * stride] = static_cast<double>(divf[q][k]);
+  } else {
   double divr[NQ][M];
 #pragma unroll
   for (int q = 0; q < NQ; ++q)
@@ -283,6 +362,7 @@ __device__ __forceinline__ void cell_residual(const double *__restrict__ tab, co
   for (int q = 0; q < NQ; ++q)
 #pragma unroll
     for (int k = 0; k < M; ++k) lds_div[(q * M + k) * stride] = divr[q][k];
+  }
 
   // ---- element scheme ------------------------------------------------------
   constexpr bool NDIS = (SC != SC_LDA);
@@ -312,10 +392,17 @@ __device__ __forceinline__ void cell_residual(const double *__restrict__ tab, co
     const double *dvq = tab + O::dVT + q * D * ND;  // reference gradients [D][ND] at q
 
     if constexpr (M == 1) {
-      double vel0, vel1;
+      TK vel0, vel1;
       if constexpr (PH == PH_ADVECTION2D) {
-        vel0 = pp.p0;
-        vel1 = pp.p1;
+        vel0 = TK(pp.p0);
+        vel1 = TK(pp.p1);
+      } else if constexpr (PREC == 1) {
+        const float *vq = tabf + O::V + q * ND;
+        float t = 0.0f;
+#pragma unroll
+        for (int n = 1; n < ND; ++n) t += vq[n] * dU[n][0];
+        vel0 = static_cast<float>(ub[0]) + t;
+        vel1 = 1.0f;
       } else {
         double uq = 0.0;
 #pragma unroll
@@ -324,16 +411,19 @@ __device__ __forceinline__ void cell_residual(const double *__restrict__ tab, co
         vel1 = 1.0;
       }
       // k_n = a . grad(phi_n) = sum_xi c_xi dphi_n/dxi with c_xi = sum_x a_x dxi/dx
-      double c[D];
+      const TK *dvk = PREC == 1 ? reinterpret_cast<const TK *>(tabf + O::dVT + q * D * ND)
+                                : reinterpret_cast<const TK *>(dvq);
+      TK c[D];
 #pragma unroll
-      for (int xi = 0; xi < D; ++xi) c[xi] = vel0 * Jq[0][xi] + vel1 * Jq[1][xi];
+      for (int xi = 0; xi < D; ++xi) c[xi] = vel0 * TK(Jq[0][xi]) + vel1 * TK(Jq[1][xi]);
       double kp[ND];
       double S = 0.0, SU = 0.0;
 #pragma unroll
       for (int n = 0; n < ND; ++n) {
-        double kn = 0.0;
+        TK knt = TK(0);
 #pragma unroll
-        for (int xi = 0; xi < D; ++xi) kn += c[xi] * dvq[xi * ND + n];
+        for (int xi = 0; xi < D; ++xi) knt += c[xi] * dvk[xi * ND + n];
+        const double kn = knt;
         const double ak = fabs(kn);
         kp[n] = kn > 0.0 ? kn : 0.0;
         const double l = ak > 1.e-6 ? ak : 1.e-6;
@@ -350,26 +440,39 @@ __device__ __forceinline__ void cell_residual(const double *__restrict__ tab, co
         if constexpr (NDIS) dis[n][0] += wj * (kp[n] * (Uq(n, 0) - ub));
       }
     } else {
-      double uq[M];
+      TK uq[M];
+      if constexpr (PREC == 1) {
+        const float *vq = tabf + O::V + q * ND;
 #pragma unroll
-      for (int k = 0; k < M; ++k) {
-        double t = 0.0;
+        for (int k = 0; k < M; ++k) {
+          float t = 0.0f;
 #pragma unroll
-        for (int n = 0; n < ND; ++n) t += tab[O::V + q * ND + n] * Uq(n, k);
-        uq[k] = t;
+          for (int n = 1; n < ND; ++n) t += vq[n] * dU[n][k];
+          uq[k] = static_cast<float>(ub[k]) + t;
+        }
+      } else {
+#pragma unroll
+        for (int k = 0; k < M; ++k) {
+          double t = 0.0;
+#pragma unroll
+          for (int n = 0; n < ND; ++n) t += tab[O::V + q * ND + n] * Uq(n, k);
+          uq[k] = t;
+        }
       }
-      const EulerState<double, D> st = euler_state<double, D>(uq, pp.p0);
+      const EulerState<TK, D> st = euler_state<TK, D>(uq, TK(pp.p0));
       bad = bad || st.bad;
       const double gm1 = pp.p0 - 1.0;
       // physical gradients of the solution basis at q
-      double gphys[ND][D];
+      const TK *dvk = PREC == 1 ? reinterpret_cast<const TK *>(tabf + O::dVT + q * D * ND)
+                                : reinterpret_cast<const TK *>(dvq);
+      TK gphys[ND][D];
 #pragma unroll
       for (int n = 0; n < ND; ++n)
 #pragma unroll
         for (int x = 0; x < D; ++x) {
-          double t = 0.0;
+          TK t = TK(0);
 #pragma unroll
-          for (int xi = 0; xi < D; ++xi) t += Jq[x][xi] * dvq[xi * ND + n];
+          for (int xi = 0; xi < D; ++xi) t += TK(Jq[x][xi]) * dvk[xi * ND + n];
           gphys[n][x] = t;
         }
       // Compact K+ of every node. With the acoustic pairs written as
@@ -382,7 +485,7 @@ __device__ __forceinline__ void cell_residual(const double *__restrict__ tab, co
       //   P = l0v . x, Q = f_n . x, sg = dp + dm, dl = dp - dm,
       // and S = sum_n K+_n needs only sum l0, sum sg, w = sum dl n and
       // T = sum sg n n^T.
-      double l0[ND], sg[ND], dl[ND], um[ND], nh[ND][D];
+      TK l0[ND], sg[ND], dl[ND], um[ND], nh[ND][D];
       double sl0 = 0.0, ssg = 0.0, w[D], T[D][D];
 #pragma unroll
       for (int x = 0; x < D; ++x) {
@@ -392,34 +495,35 @@ __device__ __forceinline__ void cell_residual(const double *__restrict__ tab, co
       }
 #pragma unroll
       for (int n = 0; n < ND; ++n) {
-        double nn = 0.0;
+        TK nn = TK(0);
 #pragma unroll
         for (int x = 0; x < D; ++x) nn += gphys[n][x] * gphys[n][x];
-        const double norm = sqrt(nn);
-        const bool tiny = (D == 2) ? (norm < 1.e-9) : (norm == 0.0);
-        const double inorm = tiny ? 0.0 : 1.0 / norm;
-        double u_n = 0.0;
+        const TK norm = sqrt(nn);
+        const bool tiny = (D == 2) ? (norm < TK(1.e-9)) : (norm == TK(0));
+        const TK inorm = tiny ? TK(0) : TK(1) / norm;
+        TK u_n = TK(0);
 #pragma unroll
         for (int x = 0; x < D; ++x) {
           nh[n][x] = gphys[n][x] * inorm;
           u_n += st.V[x] * nh[n][x];
         }
         um[n] = u_n;
-        const double e0 = norm * u_n, ep = norm * (u_n + st.a), em = norm * (u_n - st.a);
-        lam[n] = fmax(lam[n], fmax(1.e-6, fmax(fabs(e0), fmax(fabs(ep), fabs(em)))));
-        const double p0 = fmax(0.0, e0);
-        const double dpn = tiny ? 0.0 : fmax(0.0, ep) - p0;
-        const double dmn = tiny ? 0.0 : fmax(0.0, em) - p0;
-        l0[n] = tiny ? 0.0 : p0;
+        const TK e0 = norm * u_n, ep = norm * (u_n + st.a), em = norm * (u_n - st.a);
+        lam[n] = fmax(lam[n], fmax(1.e-6, static_cast<double>(fmax(fabs(e0), fmax(fabs(ep), fabs(em))))));
+        const TK p0 = fmax(TK(0), e0);
+        const TK dpn = tiny ? TK(0) : fmax(TK(0), ep) - p0;
+        const TK dmn = tiny ? TK(0) : fmax(TK(0), em) - p0;
+        l0[n] = tiny ? TK(0) : p0;
         sg[n] = dpn + dmn;
         dl[n] = dpn - dmn;
         sl0 += l0[n];
         ssg += sg[n];
 #pragma unroll
         for (int x = 0; x < D; ++x) {
-          w[x] += dl[n] * nh[n][x];
+          // FP64 products: S must be the sum of the K+ that kplus applies
+          w[x] += double(dl[n]) * double(nh[n][x]);
 #pragma unroll
-          for (int y = 0; y < D; ++y) T[x][y] += sg[n] * nh[n][x] * nh[n][y];
+          for (int y = 0; y < D; ++y) T[x][y] += double(sg[n]) * double(nh[n][x]) * double(nh[n][y]);
         }
       }
       const double a = st.a;
@@ -628,7 +732,7 @@ __global__ __launch_bounds__(256) void rd_cell_kernel(CellArgs a)
   }
   double phi[ND][M];
   double wave;
-  cell_residual<D, NG, ND, NF, NQ, AFFINE, PH, SC>(a.tab, X, U, URegs<ND, M>{U}, a.pp, phi, wave, bad,
+  cell_residual<D, NG, ND, NF, NQ, AFFINE, PH, SC, 0>(a.tab, X, U, URegs<ND, M>{U}, a.pp, phi, wave, bad,
                                                    lds + threadIdx.x,
                                                    static_cast<int>(blockDim.x));
 #pragma unroll
@@ -785,7 +889,7 @@ __device__ __forceinline__ void block_sum_store(double (&sq)[M + 1], double *red
 }
 
 // One patch (local index pl of the block described by a) on one workgroup.
-template <int D, int NG, int ND, int NF, int NQ, bool AFFINE, int PH, int SC>
+template <int D, int NG, int ND, int NF, int NQ, bool AFFINE, int PH, int SC, int PREC>
 __device__ __forceinline__ void patch_body(const PatchArgs &a, uint32_t pl, double *lds)
 {
   constexpr int M = PhysTraits<PH>::M;
@@ -839,10 +943,10 @@ __device__ __forceinline__ void patch_body(const PatchArgs &a, uint32_t pl, doub
       for (int k = 0; k < M; ++k) U[n][k] = ul[row[n] * M + k];
     }
     if constexpr (M == 1)
-      cell_residual<D, NG, ND, NF, NQ, AFFINE, PH, SC>(a.tab, X, U, URegs<ND, M>{U}, a.pp, phi, wave, bad, work + tid,
+      cell_residual<D, NG, ND, NF, NQ, AFFINE, PH, SC, PREC>(a.tab, X, U, URegs<ND, M>{U}, a.pp, phi, wave, bad, work + tid,
                                                        PATCH);
     else
-      cell_residual<D, NG, ND, NF, NQ, AFFINE, PH, SC>(a.tab, X, U, ULds<ND, M>{ul, row}, a.pp, phi, wave, bad,
+      cell_residual<D, NG, ND, NF, NQ, AFFINE, PH, SC, PREC>(a.tab, X, U, ULds<ND, M>{ul, row}, a.pp, phi, wave, bad,
                                                        work + tid, PATCH);
   }
   __syncthreads();  // the work region changes from divergence slices to pair sums
@@ -883,21 +987,21 @@ __device__ __forceinline__ void patch_body(const PatchArgs &a, uint32_t pl, doub
   if (a.update && a.outputs) block_sum_store<M>(sq, work, a.norm_partial + static_cast<size_t>(p) * (M + 1));
 }
 
-template <int D, int NG, int ND, int NF, int NQ, bool AFFINE, int PH, int SC>
+template <int D, int NG, int ND, int NF, int NQ, bool AFFINE, int PH, int SC, int PREC>
 __global__ __launch_bounds__((PatchSize<PhysTraits<PH>::M, ND>::value),
                              (PatchMinWaves<PhysTraits<PH>::M, ND, AFFINE>::value)) void rd_patch_kernel(PatchArgs a)
 {
   extern __shared__ double lds[];
   const uint32_t pl = xcd_block_index();
   if (pl >= a.np) return;
-  patch_body<D, NG, ND, NF, NQ, AFFINE, PH, SC>(a, pl, lds);
+  patch_body<D, NG, ND, NF, NQ, AFFINE, PH, SC, PREC>(a, pl, lds);
 }
 
 // Patches of several blocks (of several methods) of one element type /
 // physics / scheme in one launch: workgroup g runs patch g - first[i] of
 // args[i], first[i] <= g < first[i + 1]. The dynamic LDS is the largest of
 // the entries'.
-template <int D, int NG, int ND, int NF, int NQ, bool AFFINE, int PH, int SC>
+template <int D, int NG, int ND, int NF, int NQ, bool AFFINE, int PH, int SC, int PREC>
 __global__ __launch_bounds__((PatchSize<PhysTraits<PH>::M, ND>::value),
                              (PatchMinWaves<PhysTraits<PH>::M, ND, AFFINE>::value)) void
 rd_patch_batch_kernel(const PatchArgs *__restrict__ args, const uint32_t *__restrict__ first, int count)
@@ -907,7 +1011,7 @@ rd_patch_batch_kernel(const PatchArgs *__restrict__ args, const uint32_t *__rest
   if (g >= first[count]) return;
   int i = 0;
   while (i + 1 < count && first[i + 1] <= g) ++i;
-  patch_body<D, NG, ND, NF, NQ, AFFINE, PH, SC>(args[i], g - first[i], lds);
+  patch_body<D, NG, ND, NF, NQ, AFFINE, PH, SC, PREC>(args[i], g - first[i], lds);
 }
 
 struct FixupArgs

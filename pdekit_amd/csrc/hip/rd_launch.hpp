@@ -23,6 +23,8 @@ struct KernelSet
   CellLauncher cell = nullptr;    // element per thread, global scatter
   PatchLauncher patch = nullptr;  // patch assembly
   PatchBatchLauncher patch_batch = nullptr;  // patch assembly of several blocks / methods
+  PatchLauncher patch_mixed = nullptr;       // the same with FP32 interpolation, fluxes and K+
+  PatchBatchLauncher patch_batch_mixed = nullptr;
   int patch_size = 0;             // cells per patch workgroup
   size_t (*patch_lds_bytes)(int maxl) = nullptr;
   explicit operator bool() const { return cell != nullptr; }
@@ -47,22 +49,22 @@ hipError_t launch_cell(const CellArgs &a, unsigned grid, unsigned block, hipStre
   return hipGetLastError();
 }
 
-template <int D, int NG, int ND, int NF, int NQ, bool AFF, int PH, int SC>
+template <int D, int NG, int ND, int NF, int NQ, bool AFF, int PH, int SC, int PREC>
 hipError_t launch_patch(const PatchArgs &a, unsigned grid, hipStream_t s)
 {
   constexpr int M = PhysTraits<PH>::M;
   const size_t lds = patch_lds_doubles<NQ, ND, M>(a.maxl) * sizeof(double);
-  hipLaunchKernelGGL((rd_patch_kernel<D, NG, ND, NF, NQ, AFF, PH, SC>), dim3(grid), dim3(PatchSize<M, ND>::value), lds, s,
-                     a);
+  hipLaunchKernelGGL((rd_patch_kernel<D, NG, ND, NF, NQ, AFF, PH, SC, PREC>), dim3(grid), dim3(PatchSize<M, ND>::value),
+                     lds, s, a);
   return hipGetLastError();
 }
 
-template <int D, int NG, int ND, int NF, int NQ, bool AFF, int PH, int SC>
+template <int D, int NG, int ND, int NF, int NQ, bool AFF, int PH, int SC, int PREC>
 hipError_t launch_patch_batch(const PatchArgs *args, const uint32_t *first, int count, unsigned grid, size_t lds,
                               hipStream_t s)
 {
   constexpr int M = PhysTraits<PH>::M;
-  hipLaunchKernelGGL((rd_patch_batch_kernel<D, NG, ND, NF, NQ, AFF, PH, SC>), dim3(grid),
+  hipLaunchKernelGGL((rd_patch_batch_kernel<D, NG, ND, NF, NQ, AFF, PH, SC, PREC>), dim3(grid),
                      dim3(PatchSize<M, ND>::value), lds, s, args, first, count);
   return hipGetLastError();
 }
@@ -78,8 +80,10 @@ KernelSet kernel_set()
 {
   KernelSet k;
   k.cell = &launch_cell<D, NG, ND, NF, NQ, AFF, PH, SC>;
-  k.patch = &launch_patch<D, NG, ND, NF, NQ, AFF, PH, SC>;
-  k.patch_batch = &launch_patch_batch<D, NG, ND, NF, NQ, AFF, PH, SC>;
+  k.patch = &launch_patch<D, NG, ND, NF, NQ, AFF, PH, SC, 0>;
+  k.patch_batch = &launch_patch_batch<D, NG, ND, NF, NQ, AFF, PH, SC, 0>;
+  k.patch_mixed = &launch_patch<D, NG, ND, NF, NQ, AFF, PH, SC, 1>;
+  k.patch_batch_mixed = &launch_patch_batch<D, NG, ND, NF, NQ, AFF, PH, SC, 1>;
   k.patch_size = PatchSize<PhysTraits<PH>::M, ND>::value;
   k.patch_lds_bytes = &patch_lds_bytes<NQ, ND, PhysTraits<PH>::M>;
   return k;

@@ -279,6 +279,7 @@ int field_size(const pdrd_method *h, int field, size_t *count, double **dptr)
 // accumulating into acc_res / acc_ws.
 int launch_cells(pdrd_method *h)
 {
+  if (h->precision) return set_error(PDRD_ENOTSUP, "the mixed-precision cells run with the patch assembly only");
   pdrd::CellArgs a{};
   a.coords = h->d_coords;
   a.u = h->d_u;
@@ -372,7 +373,7 @@ int launch_patches(pdrd_method *h, int update, int outputs, int excl, double cfl
 {
   for (Block &b : h->blocks) {
     const pdrd::PatchArgs a = patch_args(h, &b, update, outputs, excl, cfl);
-    HIP_CHECK(b.ks.patch(a, b.np, h->stream));
+    HIP_CHECK((h->precision ? b.ks.patch_mixed : b.ks.patch)(a, b.np, h->stream));
   }
   if (h->n_shared == 0) return PDRD_OK;
   const pdrd::FixupArgs f = fixup_args(h, update, outputs, excl, cfl);
@@ -683,6 +684,12 @@ static std::vector<double> device_tables(const pdrd_fe_desc &fe, const std::vect
   for (int q = 0; q < NQ; ++q)
     for (int xi = 0; xi < D; ++xi)
       for (int g = 0; g < NG; ++g) out.push_back(t[o_dVg + (static_cast<size_t>(xi) * NQ + q) * NG + g]);
+  // float copy of everything so far, two floats per double slot
+  const size_t n = out.size();
+  std::vector<float> f(n + (n & 1), 0.0f);
+  for (size_t i = 0; i < n; ++i) f[i] = static_cast<float>(out[i]);
+  out.resize(n + f.size() / 2);
+  std::memcpy(out.data() + n, f.data(), f.size() * sizeof(float));
   return out;
 }
 
@@ -862,7 +869,7 @@ int pdrd_set_option(pdrd_method_t h, int option, int value)
       h->scatter_mode = value;
       break;
     case PDRD_OPT_PRECISION:
-      if (value != 0) return set_error(PDRD_ENOTSUP, "only the plain FP64 path is built");
+      if (value != 0 && value != 1) return set_error(PDRD_EINVAL, "precision is 0 (FP64) or 1 (mixed)");
       h->precision = value;
       break;
     case PDRD_OPT_BLOCK:
@@ -1149,7 +1156,7 @@ int pdrd_get_option(pdrd_method_t h, int option, int *value)
   if (!h || !value) return set_error(PDRD_EINVAL, "null argument");
   switch (option) {
     case PDRD_OPT_SCATTER: *value = h->scatter_mode; return PDRD_OK;
-    case PDRD_OPT_PRECISION: *value = 0; return PDRD_OK;
+    case PDRD_OPT_PRECISION: *value = h->precision; return PDRD_OK;
     case PDRD_OPT_BLOCK: *value = h->block_threads; return PDRD_OK;
     case PDRD_OPT_TIMING: *value = h->timing ? 1 : 0; return PDRD_OK;
   }
@@ -1289,11 +1296,12 @@ int pdrd_batch_iterate(pdrd_batch_t b, int count, const pdrd_method_t *methods, 
   for (int i = 0; i < count; ++i)
     for (size_t k = 0; k < methods[i]->blocks.size(); ++k) {
       const Block &bl = methods[i]->blocks[k];
+      const pdrd::PatchBatchLauncher fn = methods[i]->precision ? bl.ks.patch_batch_mixed : bl.ks.patch_batch;
       size_t g = 0;
-      while (g < launches.size() && launches[g].patch != bl.ks.patch_batch) ++g;
+      while (g < launches.size() && launches[g].patch != fn) ++g;
       if (g == launches.size()) {
         BatchLaunch l;
-        l.patch = bl.ks.patch_batch;
+        l.patch = fn;
         launches.push_back(l);
         members.emplace_back();
       }
