@@ -151,6 +151,8 @@ def main():
     ap.add_argument("--scatter", default="auto", choices=["auto", "atomic", "colour", "patch"])
     ap.add_argument("--precision", default="plain", choices=["plain", "mixed"],
                     help="plain FP64 cells, or FP32 interpolation / fluxes / K+ with FP64 solves and sums")
+    ap.add_argument("--patch-cells", default="default", choices=["default", "auto", "64", "128", "256"],
+                    help="cells per patch workgroup; auto: pdekit_amd.autotune times 64/128/256 first")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-calibration-mb", type=int, default=0,
@@ -185,8 +187,17 @@ def main():
     t_setup = time.perf_counter()
     wl = BUILDERS[args.workload]()
     E = wl.nb_elements
+    cells = None
+    if args.patch_cells == "auto":
+        from pdekit_amd import autotune
+
+        cells = autotune.tune(wl.mesh, wl.physics, wl.scheme, wl.params, wl.quad_order, wl.state, wl.cfl,
+                              precision=1 if args.precision == "mixed" else 0)
+        print(f"[bench] autotuned cells per patch: {cells}", file=sys.stderr, flush=True)
+    elif args.patch_cells != "default":
+        cells = int(args.patch_cells)
     meth = PGRDMethodExplicit(wl.physics, wl.scheme, params=wl.params if wl.params else None)
-    meth.configure_mesh_data(wl.mesh).initialize_work_data(quad_order=wl.quad_order)
+    meth.configure_mesh_data(wl.mesh).initialize_work_data(quad_order=wl.quad_order, patch_cells=cells)
     if args.precision == "mixed":
         meth.set_option(L.OPT_PRECISION, 1)
     meth.set_vec_function("solution", wl.state)
@@ -259,6 +270,7 @@ def main():
         "config": {"workload": wl.name, "precision": args.precision, "elements": E, "nodes": wl.mesh.nb_nodes,
                    "order": wl.mesh.order, "scheme": ["PGLDA", "PGN", "PGB", "PSI"][wl.scheme],
                    "scatter": SCATTER_NAMES[best[0]], "colours": info["nb_colours"],
+                   "patch_cells": meth.get_option(L.OPT_PATCH_CELLS) if best[0] == 2 else None,
                    "parallelism": "single GPU"},
         "roofline": roofline,
         # the FP64 vector pipe is the binding roof of this kernel: flops per

@@ -63,12 +63,17 @@ enum pdrd_option {
                              2 = patch assembly (default): Morton-ordered patches of
                              cells summed in LDS, shared nodes summed by a fix-up pass;
                              1 and 2 are deterministic */
-  PDRD_OPT_PRECISION = 1, /* 0 = plain FP64, 1 = mixed: FP32 interpolation/fluxes,
-                             FP64 inverse, accumulation, update and norms */
+  PDRD_OPT_PRECISION = 1, /* 0 = plain FP64, 1 = mixed (patch assembly only): FP32
+                             interpolation, flux differences, divergence and K+ parameters;
+                             FP64 sums of K+, solves, residuals, update and norms */
   PDRD_OPT_BLOCK = 2,     /* threads per block of the cell kernels (64, 128, 192, 256) */
-  PDRD_OPT_TIMING = 3     /* 1: pdrd_iterate runs eagerly and times the cell loop and the
+  PDRD_OPT_TIMING = 3,    /* 1: pdrd_iterate runs eagerly and times the cell loop and the
                              nodal pass of every iteration with HIP events on the method's
                              stream; pdrd_last_timing then returns per-iteration means */
+  PDRD_OPT_PATCH_CELLS = 4 /* cells per patch workgroup, set before pdrd_initialize: 0 (each
+                             kernel's default: 256 scalar P1/P2, 128 otherwise), 64, 128 or
+                             256 (capped at the default); read back after pdrd_initialize
+                             it gives the largest in use */
 };
 
 /* ---- FE inputs (src/interpolation, src/mesh/point_set, shape_function) -- */

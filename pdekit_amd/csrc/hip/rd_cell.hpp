@@ -791,9 +791,10 @@ struct PatchArgs
   PhysParams pp;
 };
 
-// Cells per patch (= threads per workgroup): 256 for scalar physics on
-// elements of up to 6 nodes, 128 for the Euler systems (larger per-thread
-// state) and for P3 elements (more nodes per patch).
+// Largest (and default) number of cells per patch (= threads per
+// workgroup): 256 for scalar physics on elements of up to 6 nodes, 128 for
+// the Euler systems (larger per-thread state) and for P3 elements (more
+// nodes per patch). A block may use 64 or 128 instead (PDRD_OPT_PATCH_CELLS).
 template <int M, int ND>
 struct PatchSize
 {
@@ -810,27 +811,26 @@ struct PatchMinWaves
   static constexpr int value = (M == 1 && ND <= 6 && AFFINE) ? 5 : (M > 1 ? 2 : 1);
 };
 
-// Dynamic LDS of one patch workgroup, in doubles: node state [maxl][M], the
-// pair offsets of the nodes [maxl] (u16), then a work region shared in turn
-// by the flux-divergence slices, the (cell, node) pair sums and the norm
-// reduction.
+// Dynamic LDS of one patch workgroup of P cells, in doubles: node state
+// [maxl][M], the pair offsets of the nodes [maxl] (u16), then a work region
+// shared in turn by the flux-divergence slices, the (cell, node) pair sums
+// and the norm reduction.
 template <int NQ, int ND, int M>
-__host__ __device__ constexpr size_t patch_work_doubles()
+__host__ __device__ constexpr size_t patch_work_doubles(int P)
 {
-  constexpr int P = PatchSize<M, ND>::value;
-  constexpr size_t div = static_cast<size_t>(NQ) * M * P;
-  constexpr size_t pairs = static_cast<size_t>(P) * ND * (M + 1);
-  constexpr size_t red = static_cast<size_t>(P) * (M + 1);
-  constexpr size_t w = div > pairs ? div : pairs;
+  const size_t div = static_cast<size_t>(NQ) * M * P;
+  const size_t pairs = static_cast<size_t>(P) * ND * (M + 1);
+  const size_t red = static_cast<size_t>(P) * (M + 1);
+  const size_t w = div > pairs ? div : pairs;
   return w > red ? w : red;
 }
 
 __host__ __device__ constexpr size_t patch_coff_doubles(int maxl) { return (static_cast<size_t>(maxl) + 3) / 4; }
 
 template <int NQ, int ND, int M>
-__host__ __device__ constexpr size_t patch_lds_doubles(int maxl)
+__host__ __device__ constexpr size_t patch_lds_doubles(int maxl, int P)
 {
-  return static_cast<size_t>(maxl) * M + patch_coff_doubles(maxl) + patch_work_doubles<NQ, ND, M>();
+  return static_cast<size_t>(maxl) * M + patch_coff_doubles(maxl) + patch_work_doubles<NQ, ND, M>(P);
 }
 
 // Finishes node g from its sums acc[M + 1] (residual sum, wave-speed sum);
@@ -893,8 +893,8 @@ template <int D, int NG, int ND, int NF, int NQ, bool AFFINE, int PH, int SC, in
 __device__ __forceinline__ void patch_body(const PatchArgs &a, uint32_t pl, double *lds)
 {
   constexpr int M = PhysTraits<PH>::M;
-  constexpr int PATCH = PatchSize<M, ND>::value;
-  constexpr int NPAIR = PATCH * ND;  // stride of one component in the pair sums
+  const int PATCH = blockDim.x;     // cells per patch of this block (a multiple of 64, <= PatchSize)
+  const int NPAIR = PATCH * ND;      // stride of one component in the pair sums
   double *ul = lds;
   uint16_t *coff = reinterpret_cast<uint16_t *>(lds + static_cast<size_t>(a.maxl) * M);
   double *work = lds + static_cast<size_t>(a.maxl) * M + patch_coff_doubles(a.maxl);

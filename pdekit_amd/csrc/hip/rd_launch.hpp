@@ -8,9 +8,9 @@
 namespace pdrd {
 
 using CellLauncher = hipError_t (*)(const CellArgs &, unsigned grid, unsigned block, hipStream_t);
-using PatchLauncher = hipError_t (*)(const PatchArgs &, unsigned grid, hipStream_t);
+using PatchLauncher = hipError_t (*)(const PatchArgs &, unsigned grid, int cells, hipStream_t);
 using PatchBatchLauncher = hipError_t (*)(const PatchArgs *args, const uint32_t *first, int count, unsigned grid,
-                                          size_t lds_bytes, hipStream_t);
+                                          int cells, size_t lds_bytes, hipStream_t);
 
 struct CellKey
 {
@@ -25,8 +25,8 @@ struct KernelSet
   PatchBatchLauncher patch_batch = nullptr;  // patch assembly of several blocks / methods
   PatchLauncher patch_mixed = nullptr;       // the same with FP32 interpolation, fluxes and K+
   PatchBatchLauncher patch_batch_mixed = nullptr;
-  int patch_size = 0;             // cells per patch workgroup
-  size_t (*patch_lds_bytes)(int maxl) = nullptr;
+  int patch_size = 0;             // largest (default) cells per patch workgroup
+  size_t (*patch_lds_bytes)(int maxl, int cells) = nullptr;
   explicit operator bool() const { return cell != nullptr; }
 };
 
@@ -50,29 +50,27 @@ hipError_t launch_cell(const CellArgs &a, unsigned grid, unsigned block, hipStre
 }
 
 template <int D, int NG, int ND, int NF, int NQ, bool AFF, int PH, int SC, int PREC>
-hipError_t launch_patch(const PatchArgs &a, unsigned grid, hipStream_t s)
+hipError_t launch_patch(const PatchArgs &a, unsigned grid, int cells, hipStream_t s)
 {
   constexpr int M = PhysTraits<PH>::M;
-  const size_t lds = patch_lds_doubles<NQ, ND, M>(a.maxl) * sizeof(double);
-  hipLaunchKernelGGL((rd_patch_kernel<D, NG, ND, NF, NQ, AFF, PH, SC, PREC>), dim3(grid), dim3(PatchSize<M, ND>::value),
-                     lds, s, a);
+  const size_t lds = patch_lds_doubles<NQ, ND, M>(a.maxl, cells) * sizeof(double);
+  hipLaunchKernelGGL((rd_patch_kernel<D, NG, ND, NF, NQ, AFF, PH, SC, PREC>), dim3(grid), dim3(cells), lds, s, a);
   return hipGetLastError();
 }
 
 template <int D, int NG, int ND, int NF, int NQ, bool AFF, int PH, int SC, int PREC>
-hipError_t launch_patch_batch(const PatchArgs *args, const uint32_t *first, int count, unsigned grid, size_t lds,
-                              hipStream_t s)
+hipError_t launch_patch_batch(const PatchArgs *args, const uint32_t *first, int count, unsigned grid, int cells,
+                              size_t lds, hipStream_t s)
 {
-  constexpr int M = PhysTraits<PH>::M;
-  hipLaunchKernelGGL((rd_patch_batch_kernel<D, NG, ND, NF, NQ, AFF, PH, SC, PREC>), dim3(grid),
-                     dim3(PatchSize<M, ND>::value), lds, s, args, first, count);
+  hipLaunchKernelGGL((rd_patch_batch_kernel<D, NG, ND, NF, NQ, AFF, PH, SC, PREC>), dim3(grid), dim3(cells), lds, s,
+                     args, first, count);
   return hipGetLastError();
 }
 
 template <int NQ, int ND, int M>
-size_t patch_lds_bytes(int maxl)
+size_t patch_lds_bytes(int maxl, int cells)
 {
-  return patch_lds_doubles<NQ, ND, M>(maxl) * sizeof(double);
+  return patch_lds_doubles<NQ, ND, M>(maxl, cells) * sizeof(double);
 }
 
 template <int D, int NG, int ND, int NF, int NQ, bool AFF, int PH, int SC>

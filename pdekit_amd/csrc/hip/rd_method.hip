@@ -43,6 +43,7 @@ struct Block
   uint16_t *d_lconn = nullptr;   // [nd][ne] patch-local node indices
   uint32_t first_patch = 0, np = 0;
   int maxl = 0;
+  int cells = 0;  // cells per patch (threads per workgroup)
 };
 
 }  // namespace
@@ -62,6 +63,7 @@ struct pdrd_method
   bool patch_ok = false;
   std::string patch_reason;
   int precision = 0;
+  int patch_cells = 0;  // requested cells per patch (0: each kernel's default)
   int block_threads = 256;
   int timing = 0;
 
@@ -365,16 +367,23 @@ pdrd::FixupArgs fixup_args(const pdrd_method *h, int update, int outputs, int ex
   return f;
 }
 
+// The patch kernels of every block (plain or mixed precision).
+int launch_patch_blocks(pdrd_method *h, int update, int outputs, int excl, double cfl)
+{
+  for (Block &b : h->blocks) {
+    const pdrd::PatchArgs a = patch_args(h, &b, update, outputs, excl, cfl);
+    HIP_CHECK((h->precision ? b.ks.patch_mixed : b.ks.patch)(a, b.np, b.cells, h->stream));
+  }
+  return PDRD_OK;
+}
+
 // Patch assembly of every block followed by the sums of the shared nodes.
 // update = 0: residual and wave speed of every node; update = 1: one
 // pseudo-time iteration (the strong Dirichlet values are applied in the
 // nodal step), with res / dt / ws / norm partials when outputs is set.
 int launch_patches(pdrd_method *h, int update, int outputs, int excl, double cfl)
 {
-  for (Block &b : h->blocks) {
-    const pdrd::PatchArgs a = patch_args(h, &b, update, outputs, excl, cfl);
-    HIP_CHECK((h->precision ? b.ks.patch_mixed : b.ks.patch)(a, b.np, h->stream));
-  }
+  if (int rc = launch_patch_blocks(h, update, outputs, excl, cfl)) return rc;
   if (h->n_shared == 0) return PDRD_OK;
   const pdrd::FixupArgs f = fixup_args(h, update, outputs, excl, cfl);
   const unsigned nb = fixup_blocks(h);
@@ -763,7 +772,7 @@ int pdrd_initialize(pdrd_method_t h)
       x.nd = b.fe.nd;
       x.geo = b.geo.data();
       x.sol = b.sol.data();
-      x.patch_size = b.ks.patch_size;
+      x.patch_size = h->patch_cells ? std::min(h->patch_cells, b.ks.patch_size) : b.ks.patch_size;
       pin.push_back(x);
     }
     pdrd::PatchPlan plan;
@@ -776,7 +785,7 @@ int pdrd_initialize(pdrd_method_t h)
     }
     if (h->patch_ok) {
       for (size_t i = 0; i < h->blocks.size(); ++i)
-        if (h->blocks[i].ks.patch_lds_bytes(plan.blocks[i].maxl) > 65536) {
+        if (h->blocks[i].ks.patch_lds_bytes(plan.blocks[i].maxl, pin[i].patch_size) > 65536) {
           h->patch_ok = false;
           h->patch_reason = "patch LDS above 64 KiB";
         }
@@ -788,6 +797,7 @@ int pdrd_initialize(pdrd_method_t h)
         b.first_patch = o.first_patch;
         b.np = o.np;
         b.maxl = o.maxl;
+        b.cells = pin[i].patch_size;
         if ((rc = dev_alloc(h, &b.d_pgconn, o.gconn.size()))) { free_device(h); return rc; }
         if ((rc = dev_alloc(h, &b.d_lconn, o.lconn.size()))) { free_device(h); return rc; }
         HIP_CHECK(hipMemcpy(b.d_pgconn, o.gconn.data(), o.gconn.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
@@ -880,6 +890,12 @@ int pdrd_set_option(pdrd_method_t h, int option, int value)
     case PDRD_OPT_TIMING:
       if (value != 0 && value != 1) return set_error(PDRD_EINVAL, "timing is 0 or 1");
       h->timing = value;
+      break;
+    case PDRD_OPT_PATCH_CELLS:
+      if (h->ready) return set_error(PDRD_EINVAL, "cells per patch are set before pdrd_initialize");
+      if (value != 0 && value != 64 && value != 128 && value != 256)
+        return set_error(PDRD_EINVAL, "cells per patch is 0 (default), 64, 128 or 256");
+      h->patch_cells = value;
       break;
     default: return set_error(PDRD_EINVAL, "unknown option");
   }
@@ -1057,10 +1073,7 @@ int pdrd_iterate(pdrd_method_t h, double cfl, int n_iter, int exclude_dirichlet,
       if (h->scatter_mode == 2) {
         // the cell loop and the nodal step of patch-only nodes are one kernel;
         // the second interval is the shared-node fix-up
-        for (Block &b : h->blocks) {
-          const pdrd::PatchArgs a = patch_args(h, &b, 1, out, excl, cfl);
-          HIP_CHECK(b.ks.patch(a, b.np, h->stream));
-        }
+        if (int rc = launch_patch_blocks(h, 1, out, excl, cfl)) return rc;
         HIP_CHECK(hipEventRecord(h->ev[1], h->stream));
         if (h->n_shared) {
           // fix-up only: re-enqueue through launch_patches would repeat the cells
@@ -1159,6 +1172,12 @@ int pdrd_get_option(pdrd_method_t h, int option, int *value)
     case PDRD_OPT_PRECISION: *value = h->precision; return PDRD_OK;
     case PDRD_OPT_BLOCK: *value = h->block_threads; return PDRD_OK;
     case PDRD_OPT_TIMING: *value = h->timing ? 1 : 0; return PDRD_OK;
+    case PDRD_OPT_PATCH_CELLS: {
+      int c = 0;
+      for (const Block &b : h->blocks) c = std::max(c, b.cells);
+      *value = h->ready ? c : h->patch_cells;
+      return PDRD_OK;
+    }
   }
   return set_error(PDRD_EINVAL, "unknown option");
 }
@@ -1207,6 +1226,7 @@ struct BatchLaunch
 {
   pdrd::PatchBatchLauncher patch = nullptr;  // null: shared-node sums
   int m = 0;                                 // equations (shared-node sums)
+  int cells = 0;                             // threads per workgroup of a patch launch
   size_t args0 = 0, first0 = 0;              // offsets in the argument / prefix arrays
   int count = 0;
   unsigned grid = 0;
@@ -1220,7 +1240,7 @@ int enqueue_batch(pdrd_batch *b, const std::vector<BatchLaunch> &launches, size_
     if (l.grid == 0) continue;
     if (l.patch) {
       HIP_CHECK(l.patch(b->d_pargs + (outputs ? args_half : 0) + l.args0, b->d_first + l.first0, l.count, l.grid,
-                        l.lds, b->stream));
+                        l.cells, l.lds, b->stream));
       continue;
     }
     const pdrd::FixupArgs *fa = b->d_fargs + (outputs ? fargs_half : 0) + l.args0;
@@ -1298,10 +1318,11 @@ int pdrd_batch_iterate(pdrd_batch_t b, int count, const pdrd_method_t *methods, 
       const Block &bl = methods[i]->blocks[k];
       const pdrd::PatchBatchLauncher fn = methods[i]->precision ? bl.ks.patch_batch_mixed : bl.ks.patch_batch;
       size_t g = 0;
-      while (g < launches.size() && launches[g].patch != fn) ++g;
+      while (g < launches.size() && (launches[g].patch != fn || launches[g].cells != bl.cells)) ++g;
       if (g == launches.size()) {
         BatchLaunch l;
         l.patch = fn;
+        l.cells = bl.cells;
         launches.push_back(l);
         members.emplace_back();
       }
@@ -1334,7 +1355,7 @@ int pdrd_batch_iterate(pdrd_batch_t b, int count, const pdrd_method_t *methods, 
         pdrd_method *h = methods[mk.first];
         const Block &bl = h->blocks[mk.second];
         for (int o = 0; o < 2; ++o) pargs[o].push_back(patch_args(h, &bl, 1, o, excl, cfl[mk.first]));
-        l.lds = std::max(l.lds, bl.ks.patch_lds_bytes(bl.maxl));
+        l.lds = std::max(l.lds, bl.ks.patch_lds_bytes(bl.maxl, bl.cells));
         acc += bl.np;
         first.push_back(acc);
       }

@@ -51,10 +51,11 @@ class PGRDMethodExplicit:
         self.mesh = mesh
         return self
 
-    def initialize_work_data(self, quad_order=4, point_set=L.WARPBLEND, scatter=None):
+    def initialize_work_data(self, quad_order=4, point_set=L.WARPBLEND, scatter=None, patch_cells=None):
         """Builds the FE tables of every element type (one quadrature order, or a
         dict shape -> order) and uploads mesh and tables. scatter: None (the
-        library's default, patch assembly), 0 FP64 atomics, 1 colours, 2 patches."""
+        library's default, patch assembly), 0 FP64 atomics, 1 colours, 2 patches.
+        patch_cells: None (default), 64 / 128 / 256 cells per patch workgroup."""
         if self.mesh is None:
             raise RuntimeError("configure_mesh_data first")
         m = self.mesh
@@ -74,6 +75,8 @@ class PGRDMethodExplicit:
                     "pdrd_add_cells")
         if scatter is not None:
             L.check(lib.pdrd_set_option(h, L.OPT_SCATTER, int(scatter)), "pdrd_set_option")
+        if patch_cells is not None:
+            L.check(lib.pdrd_set_option(h, L.OPT_PATCH_CELLS, int(patch_cells)), "pdrd_set_option")
         L.check(lib.pdrd_initialize(h), "pdrd_initialize")
         return self
 
