@@ -779,7 +779,8 @@ struct PatchArgs
   double *__restrict__ res;                 // [N][M]
   double *__restrict__ dt;                  // [N]
   double *__restrict__ ws;                  // [N]
-  double *__restrict__ slot_acc;            // [slots][M + 1] partial sums of shared nodes
+  double *__restrict__ slot_acc;            // [shared slots][M + 1] partial sums, grouped by node
+  const uint32_t *__restrict__ slot_pos;    // [slots] row of a shared slot in slot_acc
   double *__restrict__ norm_partial;        // [np][M + 1]
   int *__restrict__ status;
   uint32_t ne, np, first_patch;             // first_patch: offset of this block's patches
@@ -978,7 +979,7 @@ __device__ __forceinline__ void patch_body(const PatchArgs &a, uint32_t pl, doub
     if (l < ninner) {
       finish_node<M>(a, a.pnodes[s0 + l], acc, ul + l * M, sq);
     } else {
-      double *slot = a.slot_acc + static_cast<size_t>(s0 + l) * (M + 1);
+      double *slot = a.slot_acc + static_cast<size_t>(a.slot_pos[s0 + l]) * (M + 1);
 #pragma unroll
       for (int k = 0; k <= M; ++k) slot[k] = acc[k];
     }
@@ -1018,8 +1019,7 @@ struct FixupArgs
 {
   const uint32_t *__restrict__ nodes;     // [nb] shared nodes
   const uint32_t *__restrict__ slot_off;  // [nb + 1]
-  const uint32_t *__restrict__ slots;     // slot ids, ascending per node
-  const double *__restrict__ slot_acc;
+  const double *__restrict__ slot_acc;    // rows of node i: [slot_off[i], slot_off[i + 1])
   PatchArgs pa;                           // node outputs as in the patch kernel
   uint32_t nb;
   double *__restrict__ norm_partial;      // [gridDim.x][M + 1]
@@ -1038,7 +1038,7 @@ __device__ __forceinline__ void fixup_body(const FixupArgs &f, uint32_t lb, uint
 #pragma unroll
     for (int k = 0; k <= M; ++k) acc[k] = 0.0;
     for (uint32_t s = f.slot_off[i]; s < f.slot_off[i + 1]; ++s) {
-      const double *row = f.slot_acc + static_cast<size_t>(f.slots[s]) * (M + 1);
+      const double *row = f.slot_acc + static_cast<size_t>(s) * (M + 1);
 #pragma unroll
       for (int k = 0; k <= M; ++k) acc[k] += row[k];
     }

@@ -82,7 +82,7 @@ struct pdrd_method
   uint32_t np_total = 0, n_shared = 0;
   uint32_t *d_pnode_off = nullptr, *d_pnodes = nullptr, *d_pn_inner = nullptr;
   uint16_t *d_pn_coff = nullptr;
-  uint32_t *d_shared_nodes = nullptr, *d_shared_off = nullptr, *d_shared_slots = nullptr;
+  uint32_t *d_shared_nodes = nullptr, *d_shared_off = nullptr, *d_slot_pos = nullptr;
   double *d_slot_acc = nullptr, *d_patch_partial = nullptr;
   size_t bytes = 0;
 
@@ -343,6 +343,7 @@ pdrd::PatchArgs patch_args(const pdrd_method *h, const Block *b, int update, int
   a.dt = h->d_dt;
   a.ws = h->d_ws;
   a.slot_acc = h->d_slot_acc;
+  a.slot_pos = h->d_slot_pos;
   a.norm_partial = h->d_patch_partial;
   a.status = h->d_status;
   a.update = update;
@@ -359,7 +360,6 @@ pdrd::FixupArgs fixup_args(const pdrd_method *h, int update, int outputs, int ex
   pdrd::FixupArgs f{};
   f.nodes = h->d_shared_nodes;
   f.slot_off = h->d_shared_off;
-  f.slots = h->d_shared_slots;
   f.slot_acc = h->d_slot_acc;
   f.pa = patch_args(h, nullptr, update, outputs, excl, cfl);
   f.nb = h->n_shared;
@@ -539,7 +539,7 @@ void free_device(pdrd_method *h)
     b.d_lconn = nullptr;
   }
   uint32_t **u32s[] = {&h->d_pnode_off, &h->d_pnodes, &h->d_pn_inner, &h->d_shared_nodes, &h->d_shared_off,
-                       &h->d_shared_slots};
+                       &h->d_slot_pos};
   for (uint32_t **p : u32s) {
     (void)hipFree(*p);
     *p = nullptr;
@@ -803,6 +803,9 @@ int pdrd_initialize(pdrd_method_t h)
         HIP_CHECK(hipMemcpy(b.d_pgconn, o.gconn.data(), o.gconn.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
         HIP_CHECK(hipMemcpy(b.d_lconn, o.lconn.data(), o.lconn.size() * sizeof(uint16_t), hipMemcpyHostToDevice));
       }
+      // the row of every shared slot in the node-grouped partial sums
+      std::vector<uint32_t> slot_pos(plan.pnodes.size(), 0);
+      for (size_t k = 0; k < plan.shared_slots.size(); ++k) slot_pos[plan.shared_slots[k]] = static_cast<uint32_t>(k);
       h->np_total = static_cast<uint32_t>(plan.pn_inner.size());
       h->n_shared = static_cast<uint32_t>(plan.shared_nodes.size());
       auto up32 = [&](uint32_t **d, const std::vector<uint32_t> &v) -> int {
@@ -814,13 +817,13 @@ int pdrd_initialize(pdrd_method_t h)
       };
       if ((rc = up32(&h->d_pnode_off, plan.pnode_off)) || (rc = up32(&h->d_pnodes, plan.pnodes)) ||
           (rc = up32(&h->d_pn_inner, plan.pn_inner)) || (rc = up32(&h->d_shared_nodes, plan.shared_nodes)) ||
-          (rc = up32(&h->d_shared_off, plan.shared_off)) || (rc = up32(&h->d_shared_slots, plan.shared_slots))) {
+          (rc = up32(&h->d_shared_off, plan.shared_off)) || (rc = up32(&h->d_slot_pos, slot_pos))) {
         free_device(h);
         return rc;
       }
       if ((rc = dev_alloc(h, &h->d_pn_coff, plan.pn_coff.size()))) { free_device(h); return rc; }
       HIP_CHECK(hipMemcpy(h->d_pn_coff, plan.pn_coff.data(), plan.pn_coff.size() * sizeof(uint16_t), hipMemcpyHostToDevice));
-      if ((rc = dev_alloc(h, &h->d_slot_acc, static_cast<size_t>(plan.pnodes.size()) * (h->m + 1)))) {
+      if ((rc = dev_alloc(h, &h->d_slot_acc, static_cast<size_t>(plan.shared_slots.size()) * (h->m + 1)))) {
         free_device(h);
         return rc;
       }
