@@ -70,10 +70,14 @@ enum pdrd_option {
   PDRD_OPT_TIMING = 3,    /* 1: pdrd_iterate runs eagerly and times the cell loop and the
                              nodal pass of every iteration with HIP events on the method's
                              stream; pdrd_last_timing then returns per-iteration means */
-  PDRD_OPT_PATCH_CELLS = 4 /* cells per patch workgroup, set before pdrd_initialize: 0 (each
+  PDRD_OPT_PATCH_CELLS = 4, /* cells per patch workgroup, set before pdrd_initialize: 0 (each
                              kernel's default: 256 scalar P1/P2, 128 otherwise), 64, 128 or
                              256 (capped at the default); read back after pdrd_initialize
                              it gives the largest in use */
+  PDRD_OPT_REFRESH = 5     /* mixed precision: every this many iterations (default 50) the
+                             defect correction R64(u) - Rmixed(u) is recomputed (two sweeps)
+                             and added to the mixed residual of the updates, so the iteration
+                             converges to the FP64 steady state; 0 disables it */
 };
 
 /* ---- FE inputs (src/interpolation, src/mesh/point_set, shape_function) -- */

@@ -776,6 +776,7 @@ struct PatchArgs
   const double *__restrict__ vol;           // [N]
   const int *__restrict__ dir_index;        // [N] row in dir_vals or -1
   const double *__restrict__ dir_vals;      // [ndir][M]
+  const double *__restrict__ corr;          // [N][M] added to the residual in the update, or null
   double *__restrict__ res;                 // [N][M]
   double *__restrict__ dt;                  // [N]
   double *__restrict__ ws;                  // [N]
@@ -850,7 +851,7 @@ __device__ __forceinline__ void finish_node(const PatchArgs &a, uint32_t g, cons
 #pragma unroll
     for (int k = 0; k < M; ++k) {
       const size_t j = static_cast<size_t>(g) * M + k;
-      const double r = acc[k] * iv;
+      const double r = a.corr ? acc[k] * iv + a.corr[j] : acc[k] * iv;
       const double u0 = (di >= 0) ? a.dir_vals[static_cast<size_t>(di) * M + k] : u_old[k];
       a.u[j] = u0 - d * r;
       if (a.outputs) a.res[j] = r;

@@ -63,6 +63,11 @@ struct pdrd_method
   bool patch_ok = false;
   std::string patch_reason;
   int precision = 0;
+  // defect correction of the mixed cells: every `refresh` iterations
+  // corr = R64(u) - Rmixed(u), added to the mixed residual in the update
+  int refresh = 50;
+  int since_refresh = 1 << 30;
+  double *d_corr = nullptr;
   int patch_cells = 0;  // requested cells per patch (0: each kernel's default)
   int block_threads = 256;
   int timing = 0;
@@ -115,6 +120,13 @@ __global__ void finalize_kernel(uint32_t nn, int m, double *__restrict__ acc_res
   }
   ws[i] = acc_ws[i];
   acc_ws[i] = 0.0;
+}
+
+// corr -= res (corr holds the FP64 residual, res the mixed one)
+__global__ void corr_kernel(size_t n, double *__restrict__ corr, const double *__restrict__ res)
+{
+  const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) corr[i] -= res[i];
 }
 
 __global__ void dirichlet_kernel(uint32_t n, int m, const uint32_t *__restrict__ nodes,
@@ -339,6 +351,7 @@ pdrd::PatchArgs patch_args(const pdrd_method *h, const Block *b, int update, int
   a.vol = h->d_vol;
   a.dir_index = h->d_dir_index;
   a.dir_vals = h->d_dir_vals;
+  a.corr = (update && h->precision == 1 && h->refresh > 0) ? h->d_corr : nullptr;
   a.res = h->d_res;
   a.dt = h->d_dt;
   a.ws = h->d_ws;
@@ -547,6 +560,8 @@ void free_device(pdrd_method *h)
   (void)hipFree(h->d_pn_coff);
   (void)hipFree(h->d_slot_acc);
   (void)hipFree(h->d_patch_partial);
+  (void)hipFree(h->d_corr);
+  h->d_corr = nullptr;
   (void)hipFree(h->d_dir_index);
   h->d_pn_coff = nullptr;
   h->d_slot_acc = nullptr;
@@ -584,6 +599,32 @@ int enqueue_iteration(pdrd_method *h, double cfl, int outputs, int excl)
   if (rc) return rc;
   return launch_node_pass(h, cfl, outputs, excl);
 }
+
+// Defect correction of the mixed cells: corr = R64(u) - Rmixed(u) at the
+// current state (two residual sweeps), so that the mixed iteration's fixed
+// point is the FP64 steady state.
+int refresh_correction(pdrd_method *h)
+{
+  const size_t nm = static_cast<size_t>(h->nn) * h->m;
+  if (!h->d_corr) {
+    if (int rc = dev_alloc(h, &h->d_corr, nm)) return rc;
+    drop_graph(h);  // captured iterations must see the correction
+  }
+  const int prec = h->precision;
+  h->precision = 0;
+  int rc = launch_patches(h, 0, 0, 0, 0.0);
+  h->precision = prec;
+  if (rc) return rc;
+  HIP_CHECK(hipMemcpyAsync(h->d_corr, h->d_res, nm * sizeof(double), hipMemcpyDeviceToDevice, h->stream));
+  if ((rc = launch_patches(h, 0, 0, 0, 0.0))) return rc;
+  hipLaunchKernelGGL(corr_kernel, dim3(static_cast<unsigned>((nm + 255) / 256)), dim3(256), 0, h->stream, nm,
+                     h->d_corr, h->d_res);
+  HIP_CHECK(hipGetLastError());
+  h->since_refresh = 0;
+  return PDRD_OK;
+}
+
+bool corrects(const pdrd_method *h) { return h->precision == 1 && h->refresh > 0 && h->scatter_mode == 2; }
 
 int iterate_norm(pdrd_method *h, double *norm)
 {
@@ -884,6 +925,7 @@ int pdrd_set_option(pdrd_method_t h, int option, int value)
     case PDRD_OPT_PRECISION:
       if (value != 0 && value != 1) return set_error(PDRD_EINVAL, "precision is 0 (FP64) or 1 (mixed)");
       h->precision = value;
+      h->since_refresh = 1 << 30;
       break;
     case PDRD_OPT_BLOCK:
       if (value < 64 || value > 256 || (value % 64) != 0)
@@ -893,6 +935,11 @@ int pdrd_set_option(pdrd_method_t h, int option, int value)
     case PDRD_OPT_TIMING:
       if (value != 0 && value != 1) return set_error(PDRD_EINVAL, "timing is 0 or 1");
       h->timing = value;
+      break;
+    case PDRD_OPT_REFRESH:
+      if (value < 0) return set_error(PDRD_EINVAL, "refresh period must be >= 0");
+      h->refresh = value;
+      h->since_refresh = 1 << 30;
       break;
     case PDRD_OPT_PATCH_CELLS:
       if (h->ready) return set_error(PDRD_EINVAL, "cells per patch are set before pdrd_initialize");
@@ -1072,6 +1119,9 @@ int pdrd_iterate(pdrd_method_t h, double cfl, int n_iter, int exclude_dirichlet,
     double cell = 0.0, node = 0.0;
     for (int it = 0; it < n_iter; ++it) {
       const int out = (it == n_iter - 1) ? 1 : 0;
+      if (corrects(h) && h->since_refresh >= h->refresh)
+        if (int rc = refresh_correction(h)) return rc;
+      if (corrects(h)) ++h->since_refresh;
       HIP_CHECK(hipEventRecord(h->ev[0], h->stream));
       if (h->scatter_mode == 2) {
         // the cell loop and the nodal step of patch-only nodes are one kernel;
@@ -1109,6 +1159,9 @@ int pdrd_iterate(pdrd_method_t h, double cfl, int n_iter, int exclude_dirichlet,
     if (norm) return iterate_norm(h, norm);
     return PDRD_OK;
   }
+  const bool corr = corrects(h);
+  if (corr && h->since_refresh >= h->refresh)
+    if (int rc = refresh_correction(h)) return rc;
   HIP_CHECK(hipEventRecord(h->ev[0], h->stream));
   if (n_iter > 1) {
     if (!h->graph || h->graph_cfl != cfl) {
@@ -1127,9 +1180,17 @@ int pdrd_iterate(pdrd_method_t h, double cfl, int n_iter, int exclude_dirichlet,
       if (e != hipSuccess) return hip_fail(e, "hipGraphInstantiate");
       h->graph_cfl = cfl;
     }
-    for (int it = 0; it < n_iter - 1; ++it) HIP_CHECK(hipGraphLaunch(h->graph, h->stream));
+    for (int it = 0; it < n_iter - 1; ++it) {
+      if (corr && h->since_refresh >= h->refresh)
+        if (int rc = refresh_correction(h)) return rc;
+      HIP_CHECK(hipGraphLaunch(h->graph, h->stream));
+      if (corr) ++h->since_refresh;
+    }
   }
+  if (corr && h->since_refresh >= h->refresh)
+    if (int rc = refresh_correction(h)) return rc;
   if (int rc = enqueue_iteration(h, cfl, 1, excl)) return rc;
+  if (corr) ++h->since_refresh;
   HIP_CHECK(hipEventRecord(h->ev[1], h->stream));
   HIP_CHECK(hipEventSynchronize(h->ev[1]));
   float ms = 0.f;
@@ -1175,6 +1236,7 @@ int pdrd_get_option(pdrd_method_t h, int option, int *value)
     case PDRD_OPT_PRECISION: *value = h->precision; return PDRD_OK;
     case PDRD_OPT_BLOCK: *value = h->block_threads; return PDRD_OK;
     case PDRD_OPT_TIMING: *value = h->timing ? 1 : 0; return PDRD_OK;
+    case PDRD_OPT_REFRESH: *value = h->refresh; return PDRD_OK;
     case PDRD_OPT_PATCH_CELLS: {
       int c = 0;
       for (const Block &b : h->blocks) c = std::max(c, b.cells);
@@ -1308,6 +1370,8 @@ int pdrd_batch_iterate(pdrd_batch_t b, int count, const pdrd_method_t *methods, 
     if (int rc = check_ready(h)) return rc;
     if (h->device != b->device) return set_error(PDRD_EINVAL, "batch methods must live on the batch's device");
     if (h->scatter_mode != 2) return set_error(PDRD_ENOTSUP, "batched iterations need the patch assembly");
+    if (corrects(h))
+      return set_error(PDRD_ENOTSUP, "batched iterations of mixed-precision methods need PDRD_OPT_REFRESH = 0");
     for (int j = 0; j < i; ++j)
       if (methods[j] == h) return set_error(PDRD_EINVAL, "a method appears twice in one batch");
   }
