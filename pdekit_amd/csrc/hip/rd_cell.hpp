@@ -462,19 +462,8 @@ __device__ __forceinline__ void cell_residual(const double *__restrict__ tab, co
       const EulerState<TK, D> st = euler_state<TK, D>(uq, TK(pp.p0));
       bad = bad || st.bad;
       const double gm1 = pp.p0 - 1.0;
-      // physical gradients of the solution basis at q
       const TK *dvk = PREC == 1 ? reinterpret_cast<const TK *>(tabf + O::dVT + q * D * ND)
                                 : reinterpret_cast<const TK *>(dvq);
-      TK gphys[ND][D];
-#pragma unroll
-      for (int n = 0; n < ND; ++n)
-#pragma unroll
-        for (int x = 0; x < D; ++x) {
-          TK t = TK(0);
-#pragma unroll
-          for (int xi = 0; xi < D; ++xi) t += TK(Jq[x][xi]) * dvk[xi * ND + n];
-          gphys[n][x] = t;
-        }
       // Compact K+ of every node. With the acoustic pairs written as
       //   r+- = (rho / 2a) (r0 +- a e_n),  l+- = (l0v +- a f_n) / (rho a),
       //   r0 = (1, V, H), e_n = (0, n, V.n), l0v = (gm1 |V|^2 / 2, -gm1 V, gm1),
@@ -485,7 +474,48 @@ __device__ __forceinline__ void cell_residual(const double *__restrict__ tab, co
       //   P = l0v . x, Q = f_n . x, sg = dp + dm, dl = dp - dm,
       // and S = sum_n K+_n needs only sum l0, sum sg, w = sum dl n and
       // T = sum sg n n^T.
-      TK l0[ND], sg[ND], dl[ND], um[ND], nh[ND][D];
+      struct NodeK
+      {
+        TK l0, sg, dl, um, nh[D], emax;
+      };
+      // the K+ parameters of node n from the physical gradient of its basis
+      // function (recomputed where keeping them for all nodes costs spills)
+      auto node_k = [&](const int n) {
+        NodeK k;
+        TK g[D];
+#pragma unroll
+        for (int x = 0; x < D; ++x) {
+          TK t = TK(0);
+#pragma unroll
+          for (int xi = 0; xi < D; ++xi) t += TK(Jq[x][xi]) * dvk[xi * ND + n];
+          g[x] = t;
+        }
+        TK nn = TK(0);
+#pragma unroll
+        for (int x = 0; x < D; ++x) nn += g[x] * g[x];
+        const TK norm = sqrt(nn);
+        const bool tiny = (D == 2) ? (norm < TK(1.e-9)) : (norm == TK(0));
+        const TK inorm = tiny ? TK(0) : TK(1) / norm;
+        TK u_n = TK(0);
+#pragma unroll
+        for (int x = 0; x < D; ++x) {
+          k.nh[x] = g[x] * inorm;
+          u_n += st.V[x] * k.nh[x];
+        }
+        k.um = u_n;
+        const TK e0 = norm * u_n, ep = norm * (u_n + st.a), em = norm * (u_n - st.a);
+        k.emax = fmax(fabs(e0), fmax(fabs(ep), fabs(em)));
+        const TK p0 = fmax(TK(0), e0);
+        const TK dpn = tiny ? TK(0) : fmax(TK(0), ep) - p0;
+        const TK dmn = tiny ? TK(0) : fmax(TK(0), em) - p0;
+        k.l0 = tiny ? TK(0) : p0;
+        k.sg = dpn + dmn;
+        k.dl = dpn - dmn;
+        return k;
+      };
+      // 3D: recompute in the application instead of keeping ND sets
+      constexpr bool KEEP = (D == 2);
+      NodeK nk[KEEP ? ND : 1];
       double sl0 = 0.0, ssg = 0.0, w[D], T[D][D];
 #pragma unroll
       for (int x = 0; x < D; ++x) {
@@ -495,35 +525,17 @@ __device__ __forceinline__ void cell_residual(const double *__restrict__ tab, co
       }
 #pragma unroll
       for (int n = 0; n < ND; ++n) {
-        TK nn = TK(0);
-#pragma unroll
-        for (int x = 0; x < D; ++x) nn += gphys[n][x] * gphys[n][x];
-        const TK norm = sqrt(nn);
-        const bool tiny = (D == 2) ? (norm < TK(1.e-9)) : (norm == TK(0));
-        const TK inorm = tiny ? TK(0) : TK(1) / norm;
-        TK u_n = TK(0);
-#pragma unroll
-        for (int x = 0; x < D; ++x) {
-          nh[n][x] = gphys[n][x] * inorm;
-          u_n += st.V[x] * nh[n][x];
-        }
-        um[n] = u_n;
-        const TK e0 = norm * u_n, ep = norm * (u_n + st.a), em = norm * (u_n - st.a);
-        lam[n] = fmax(lam[n], fmax(1.e-6, static_cast<double>(fmax(fabs(e0), fmax(fabs(ep), fabs(em))))));
-        const TK p0 = fmax(TK(0), e0);
-        const TK dpn = tiny ? TK(0) : fmax(TK(0), ep) - p0;
-        const TK dmn = tiny ? TK(0) : fmax(TK(0), em) - p0;
-        l0[n] = tiny ? TK(0) : p0;
-        sg[n] = dpn + dmn;
-        dl[n] = dpn - dmn;
-        sl0 += l0[n];
-        ssg += sg[n];
+        const NodeK k = node_k(n);
+        if constexpr (KEEP) nk[n] = k;
+        lam[n] = fmax(lam[n], fmax(1.e-6, static_cast<double>(k.emax)));
+        sl0 += k.l0;
+        ssg += k.sg;
 #pragma unroll
         for (int x = 0; x < D; ++x) {
           // FP64 products: S must be the sum of the K+ that kplus applies
-          w[x] += double(dl[n]) * double(nh[n][x]);
+          w[x] += double(k.dl) * double(k.nh[x]);
 #pragma unroll
-          for (int y = 0; y < D; ++y) T[x][y] += double(sg[n]) * double(nh[n][x]) * double(nh[n][y]);
+          for (int y = 0; y < D; ++y) T[x][y] += double(k.sg) * double(k.nh[x]) * double(k.nh[y]);
         }
       }
       const double a = st.a;
@@ -533,19 +545,25 @@ __device__ __forceinline__ void cell_residual(const double *__restrict__ tab, co
 #pragma unroll
       for (int x = 0; x < D; ++x) l0v[1 + x] = -gm1 * st.V[x];
       l0v[M - 1] = gm1;
-      auto kplus = [&](const int n, const double (&v)[M], double (&out)[M]) {
+      auto kplus = [&](const NodeK &k, const double (&v)[M], double (&out)[M]) {
         double P = 0.0;
 #pragma unroll
-        for (int k = 0; k < M; ++k) P += l0v[k] * v[k];
-        double Q = -um[n] * v[0];
+        for (int c = 0; c < M; ++c) P += l0v[c] * v[c];
+        double Q = -k.um * v[0];
 #pragma unroll
-        for (int x = 0; x < D; ++x) Q += nh[n][x] * v[1 + x];
-        const double al = (sg[n] * P + a * dl[n] * Q) * i2a2;
-        const double be = 0.5 * sg[n] * Q + dl[n] * P * i2a;
-        out[0] = l0[n] * v[0] + al;
+        for (int x = 0; x < D; ++x) Q += k.nh[x] * v[1 + x];
+        const double al = (k.sg * P + a * k.dl * Q) * i2a2;
+        const double be = 0.5 * k.sg * Q + k.dl * P * i2a;
+        out[0] = k.l0 * v[0] + al;
 #pragma unroll
-        for (int x = 0; x < D; ++x) out[1 + x] = l0[n] * v[1 + x] + al * st.V[x] + be * nh[n][x];
-        out[M - 1] = l0[n] * v[M - 1] + al * st.H + be * um[n];
+        for (int x = 0; x < D; ++x) out[1 + x] = k.l0 * v[1 + x] + al * st.V[x] + be * k.nh[x];
+        out[M - 1] = k.l0 * v[M - 1] + al * st.H + be * k.um;
+      };
+      auto node_at = [&](const int n) -> NodeK {
+        if constexpr (KEEP)
+          return nk[n];
+        else
+          return node_k(n);
       };
       // S = sl0 I + ssg/2a^2 r0 l0v^T + 1/2 sum sg e f^T + 1/2a (r0 (F w)^T + (E w) l0v^T)
       double r0[M], ew[M], fw[M];
@@ -603,7 +621,7 @@ __device__ __forceinline__ void cell_residual(const double *__restrict__ tab, co
           double un[M], z[M];
 #pragma unroll
           for (int k = 0; k < M; ++k) un[k] = Uq(n, k);
-          kplus(n, un, z);
+          kplus(node_at(n), un, z);
 #pragma unroll
           for (int k = 0; k < M; ++k) B[k][1] += z[k];
         }
@@ -614,15 +632,16 @@ __device__ __forceinline__ void cell_residual(const double *__restrict__ tab, co
       for (int k = 0; k < M; ++k) x[k] = B[k][0] * wj;
 #pragma unroll
       for (int n = 0; n < ND; ++n) {
+        const NodeK kn = node_at(n);
         double t[M];
-        kplus(n, x, t);
+        kplus(kn, x, t);
 #pragma unroll
         for (int k = 0; k < M; ++k) lda[n][k] += t[k];
         if constexpr (NDIS) {
           double du[M];
 #pragma unroll
           for (int k = 0; k < M; ++k) du[k] = wj * (Uq(n, k) - B[k][R - 1]);
-          kplus(n, du, t);
+          kplus(kn, du, t);
 #pragma unroll
           for (int k = 0; k < M; ++k) dis[n][k] += t[k];
         }
@@ -805,12 +824,14 @@ struct PatchSize
 
 // Waves per SIMD the register allocation of a patch kernel must leave room
 // for: 5 for the scalar affine P1 / P2 cells (at most 96 VGPRs, measured
-// faster than 4 waves); the other cells keep the compiler's choice, since
-// capping them spills.
+// faster than 4 waves); 2 for the 2D Euler cells (little or no scratch,
+// measured faster than 1 wave); the others keep the compiler's choice (the
+// 3D Euler cells run as fast at 1 wave, where capping them at 2 spills a
+// few hundred bytes per thread to scratch memory).
 template <int M, int ND, bool AFFINE>
 struct PatchMinWaves
 {
-  static constexpr int value = (M == 1 && ND <= 6 && AFFINE) ? 5 : (M > 1 ? 2 : 1);
+  static constexpr int value = (M == 1 && ND <= 6 && AFFINE) ? 5 : (M == 4 ? 2 : 1);
 };
 
 // Dynamic LDS of one patch workgroup of P cells, in doubles: node state
