@@ -77,12 +77,10 @@ enum pdrd_option {
   PDRD_OPT_REFRESH = 5,    /* mixed precision: every this many iterations (default 50) the
                              defect correction R64(u) - Rmixed(u) is recomputed (two sweeps)
                              and added to the mixed residual of the updates; 0 disables it */
-  PDRD_OPT_PIPELINE = 6    /* 1: pdrd_iterate finishes the nodes shared between patches of
-                             one sweep in the next sweep's gather and runs one fix-up after
-                             the last sweep; 0: a fix-up launch after every sweep; 2
-                             (default): 1 for meshes of up to 131 072 cells (launch-bound),
-                             0 above (where the longer gather costs more than the launch).
-                             Same results either way. */
+  PDRD_OPT_PIPELINE = 6    /* 1 (default): pdrd_iterate finishes the nodes shared between
+                             patches of one sweep in the next sweep's gather and runs one
+                             fix-up after the last sweep; 0: a fix-up launch after every
+                             sweep. Same results either way. */
 };
 
 /* ---- FE inputs (src/interpolation, src/mesh/point_set, shape_function) -- */

@@ -25,7 +25,6 @@ namespace {
 
 constexpr int kNormBlock = 256;
 constexpr int kNormMaxBlocks = 1024;
-constexpr uint64_t kPipelineMaxCells = 131072;  // default PDRD_OPT_PIPELINE: pipelined up to this size
 
 struct Block
 {
@@ -64,7 +63,7 @@ struct pdrd_method
   bool patch_ok = false;
   std::string patch_reason;
   int precision = 0;
-  int pipeline = 2;  // pipelined sweeps in pdrd_iterate (PDRD_OPT_PIPELINE; 2: by mesh size)
+  int pipeline = 1;  // pipelined sweeps in pdrd_iterate (PDRD_OPT_PIPELINE)
   // defect correction of the mixed cells: every `refresh` iterations
   // corr = R64(u) - Rmixed(u), added to the mixed residual in the update
   int refresh = 50;
@@ -716,18 +715,6 @@ int iterate_norm(pdrd_method *h, double *norm)
   return reduce_partials(h, norm_blocks(h->nn), norm);
 }
 
-// Measured on MI355X: pipelining saves the fix-up launch of every sweep
-// (C1, 32 768 cells: 26.5 -> 19.8 us per iteration) but lengthens the
-// dependent load chain of the gather (C2, 1 048 576 cells: 57 -> 108 us per
-// patch sweep), so by default it is used for small meshes only.
-bool pipelines(const pdrd_method *h)
-{
-  if (h->pipeline != 2) return h->pipeline == 1;
-  uint64_t cells = 0;
-  for (const Block &b : h->blocks) cells += b.ne;
-  return cells <= kPipelineMaxCells;
-}
-
 // n pipelined sweeps; timing: eager, HIP events around every sweep (cell
 // loop) and around the final fix-up (counted as the nodal pass).
 int pipelined_iterate(pdrd_method *h, double cfl, int n, int excl, bool timing)
@@ -1111,7 +1098,7 @@ int pdrd_set_option(pdrd_method_t h, int option, int value)
       h->timing = value;
       break;
     case PDRD_OPT_PIPELINE:
-      if (value < 0 || value > 2) return set_error(PDRD_EINVAL, "pipeline is 0, 1 or 2 (by mesh size)");
+      if (value != 0 && value != 1) return set_error(PDRD_EINVAL, "pipeline is 0 or 1");
       h->pipeline = value;
       break;
     case PDRD_OPT_REFRESH:
@@ -1292,7 +1279,7 @@ int pdrd_iterate(pdrd_method_t h, double cfl, int n_iter, int exclude_dirichlet,
   if (int rc = check_ready(h)) return rc;
   if (n_iter < 1) return set_error(PDRD_EINVAL, "n_iter must be >= 1");
   const int excl = exclude_dirichlet ? 1 : 0;
-  if (h->scatter_mode == 2 && h->n_shared > 0 && !corrects(h) && pipelines(h)) {
+  if (h->scatter_mode == 2 && h->n_shared > 0 && !corrects(h) && h->pipeline) {
     if (int rc = pipelined_iterate(h, cfl, n_iter, excl, h->timing)) return rc;
     if (norm) return iterate_norm(h, norm);
     return PDRD_OK;
