@@ -74,13 +74,10 @@ enum pdrd_option {
                              kernel's default: 256 scalar P1/P2, 128 otherwise), 64, 128 or
                              256 (capped at the default); read back after pdrd_initialize
                              it gives the largest in use */
-  PDRD_OPT_REFRESH = 5,    /* mixed precision: every this many iterations (default 50) the
+  PDRD_OPT_REFRESH = 5     /* mixed precision: every this many iterations (default 50) the
                              defect correction R64(u) - Rmixed(u) is recomputed (two sweeps)
-                             and added to the mixed residual of the updates; 0 disables it */
-  PDRD_OPT_PIPELINE = 6    /* 1 (default): pdrd_iterate finishes the nodes shared between
-                             patches of one sweep in the next sweep's gather and runs one
-                             fix-up after the last sweep; 0: a fix-up launch after every
-                             sweep. Same results either way. */
+                             and added to the mixed residual of the updates, so the iteration
+                             converges to the FP64 steady state; 0 disables it */
 };
 
 /* ---- FE inputs (src/interpolation, src/mesh/point_set, shape_function) -- */

@@ -801,16 +801,6 @@ struct PatchArgs
   double *__restrict__ ws;                  // [N]
   double *__restrict__ slot_acc;            // [shared slots][M + 1] partial sums, grouped by node
   const uint32_t *__restrict__ slot_pos;    // [slots] row of a shared slot in slot_acc
-  // Pipelined sweeps: the shared nodes of the previous sweep are finished in
-  // this sweep's gather, by every patch that holds them (same arithmetic, so
-  // the same values), from that sweep's partial sums instead of by a fix-up
-  // launch in between.
-  const double *__restrict__ pend_acc;      // previous sweep's partial sums (slot_acc layout), or null
-  const double *__restrict__ pend_u;        // shared nodes' state before that sweep
-  double *__restrict__ pend_out;            // [nshared][M] their state after it (first holder writes)
-  const uint32_t *__restrict__ slot_sid;    // [slots] shared-node index of a shared slot
-  const uint32_t *__restrict__ shared_off;  // [nshared + 1] first row of each shared node
-  int pend_by_node;                         // pend_u is [N][M] (1) or [nshared][M] (0)
   double *__restrict__ norm_partial;        // [np][M + 1]
   int *__restrict__ status;
   uint32_t ne, np, first_patch;             // first_patch: offset of this block's patches
@@ -864,26 +854,6 @@ template <int NQ, int ND, int M>
 __host__ __device__ constexpr size_t patch_lds_doubles(int maxl, int P)
 {
   return static_cast<size_t>(maxl) * M + patch_coff_doubles(maxl) + patch_work_doubles<NQ, ND, M>(P);
-}
-
-// The updated state of node g from its sums acc[M + 1] and its state u_old
-// before the sweep: the arithmetic of finish_node's update, without stores.
-template <int M>
-__device__ __forceinline__ void updated_state(const PatchArgs &a, uint32_t g, const double *acc, const double *u_old,
-                                              double (&u_new)[M])
-{
-  const double v = a.vol[g];
-  const double iv = 1.0 / v;
-  const double w = acc[M];
-  const double d = a.cfl * v / w;
-  const int di = a.dir_index[g];
-#pragma unroll
-  for (int k = 0; k < M; ++k) {
-    const size_t j = static_cast<size_t>(g) * M + k;
-    const double r = a.corr ? acc[k] * iv + a.corr[j] : acc[k] * iv;
-    const double u0 = (di >= 0) ? a.dir_vals[static_cast<size_t>(di) * M + k] : u_old[k];
-    u_new[k] = u0 - d * r;
-  }
 }
 
 // Finishes node g from its sums acc[M + 1] (residual sum, wave-speed sum);
@@ -975,35 +945,11 @@ __device__ __forceinline__ void patch_body(const PatchArgs &a, uint32_t pl, doub
 #pragma unroll
     for (int n = 0; n < ND; ++n) lc[n] = a.lconn[static_cast<size_t>(n) * a.ne + e];
   }
-  const int ninner = static_cast<int>(a.pn_inner[p]);
   for (int l = tid; l < nl; l += PATCH) {
     const size_t g = a.pnodes[s0 + l];
+#pragma unroll
+    for (int k = 0; k < M; ++k) ul[l * M + k] = a.u[g * M + k];
     coff[l] = a.pn_coff[s0 + l];
-    if (a.pend_acc && l >= ninner) {
-      // a shared node of the previous sweep: finish it here
-      const uint32_t sid = a.slot_sid[s0 + l];
-      const uint32_t b0 = a.shared_off[sid], b1 = a.shared_off[sid + 1];
-      double tot[M + 1];
-#pragma unroll
-      for (int k = 0; k <= M; ++k) tot[k] = 0.0;
-      for (uint32_t r = b0; r < b1; ++r)
-#pragma unroll
-        for (int k = 0; k <= M; ++k) tot[k] += a.pend_acc[static_cast<size_t>(r) * (M + 1) + k];
-      double uo[M], un[M];
-      const size_t src = a.pend_by_node ? g : static_cast<size_t>(sid);
-#pragma unroll
-      for (int k = 0; k < M; ++k) uo[k] = a.pend_u[src * M + k];
-      updated_state<M>(a, static_cast<uint32_t>(g), tot, uo, un);
-#pragma unroll
-      for (int k = 0; k < M; ++k) ul[l * M + k] = un[k];
-      if (a.slot_pos[s0 + l] == b0) {
-#pragma unroll
-        for (int k = 0; k < M; ++k) a.pend_out[static_cast<size_t>(sid) * M + k] = un[k];
-      }
-    } else {
-#pragma unroll
-      for (int k = 0; k < M; ++k) ul[l * M + k] = a.u[g * M + k];
-    }
   }
   __syncthreads();
 
@@ -1038,6 +984,7 @@ __device__ __forceinline__ void patch_body(const PatchArgs &a, uint32_t pl, doub
   }
   __syncthreads();
 
+  const int ninner = static_cast<int>(a.pn_inner[p]);
   const int npair = ecount * ND;
   double sq[M + 1];
 #pragma unroll
@@ -1096,8 +1043,6 @@ struct FixupArgs
   const uint32_t *__restrict__ slot_off;  // [nb + 1]
   const double *__restrict__ slot_acc;    // rows of node i: [slot_off[i], slot_off[i + 1])
   PatchArgs pa;                           // node outputs as in the patch kernel
-  const double *__restrict__ u_old;       // state before the sweep: pa.u ([N][M]) or [nb][M]
-  int u_old_by_node;
   uint32_t nb;
   double *__restrict__ norm_partial;      // [gridDim.x][M + 1]
 };
@@ -1120,10 +1065,9 @@ __device__ __forceinline__ void fixup_body(const FixupArgs &f, uint32_t lb, uint
       for (int k = 0; k <= M; ++k) acc[k] += row[k];
     }
     const uint32_t g = f.nodes[i];
-    const size_t src = f.u_old_by_node ? static_cast<size_t>(g) : static_cast<size_t>(i);
     double u_old[M];
 #pragma unroll
-    for (int k = 0; k < M; ++k) u_old[k] = f.u_old[src * M + k];
+    for (int k = 0; k < M; ++k) u_old[k] = f.pa.u[static_cast<size_t>(g) * M + k];
     finish_node<M>(f.pa, g, acc, u_old, sq);
   }
   if (f.pa.update && f.pa.outputs) block_sum_store<M>(sq, red, f.norm_partial + static_cast<size_t>(lb) * (M + 1));

@@ -63,7 +63,6 @@ struct pdrd_method
   bool patch_ok = false;
   std::string patch_reason;
   int precision = 0;
-  int pipeline = 1;  // pipelined sweeps in pdrd_iterate (PDRD_OPT_PIPELINE)
   // defect correction of the mixed cells: every `refresh` iterations
   // corr = R64(u) - Rmixed(u), added to the mixed residual in the update
   int refresh = 50;
@@ -90,12 +89,6 @@ struct pdrd_method
   uint16_t *d_pn_coff = nullptr;
   uint32_t *d_shared_nodes = nullptr, *d_shared_off = nullptr, *d_slot_pos = nullptr;
   double *d_slot_acc = nullptr, *d_patch_partial = nullptr;
-  // pipelined sweeps: second partial-sum buffer, shared-node states, slot -> shared index
-  double *d_slot_acc2 = nullptr, *d_ush[2] = {nullptr, nullptr};
-  uint32_t *d_slot_sid = nullptr;
-  hipGraphExec_t graph2 = nullptr;  // two pipelined sweeps
-  double graph2_cfl = 0.0;
-  int graph2_excl = -1;
   size_t bytes = 0;
 
   hipStream_t stream = nullptr;
@@ -364,8 +357,6 @@ pdrd::PatchArgs patch_args(const pdrd_method *h, const Block *b, int update, int
   a.ws = h->d_ws;
   a.slot_acc = h->d_slot_acc;
   a.slot_pos = h->d_slot_pos;
-  a.slot_sid = h->d_slot_sid;
-  a.shared_off = h->d_shared_off;
   a.norm_partial = h->d_patch_partial;
   a.status = h->d_status;
   a.update = update;
@@ -383,8 +374,6 @@ pdrd::FixupArgs fixup_args(const pdrd_method *h, int update, int outputs, int ex
   f.nodes = h->d_shared_nodes;
   f.slot_off = h->d_shared_off;
   f.slot_acc = h->d_slot_acc;
-  f.u_old = h->d_u;
-  f.u_old_by_node = 1;
   f.pa = patch_args(h, nullptr, update, outputs, excl, cfl);
   f.nb = h->n_shared;
   f.norm_partial = h->d_patch_partial + static_cast<size_t>(h->np_total) * (h->m + 1);
@@ -422,69 +411,6 @@ int launch_patches(pdrd_method *h, int update, int outputs, int excl, double cfl
     default: return set_error(PDRD_ENOTSUP, "unsupported number of equations");
   }
 #undef FIX_CASE
-  HIP_CHECK(hipGetLastError());
-  return PDRD_OK;
-}
-
-// Pipelined sweeps of the patch assembly: sweep k writes its shared-node
-// partial sums to one of two buffers, and sweep k + 1 finishes those nodes
-// in its gather (every holder, same arithmetic) from them and from their
-// state before sweep k, the first holder keeping the new state in one of two
-// per-shared-node arrays; one fix-up after the last sweep finishes its shared
-// nodes and writes them to u. Sweeps k >= 2 alternate between two fixed
-// configurations, so two of them are captured in one HIP graph.
-struct SweepCfg
-{
-  double *slots_out;
-  const double *pend_acc;
-  const double *pend_u;
-  int by_node;
-  double *pend_out;
-};
-
-SweepCfg sweep_cfg(const pdrd_method *h, int k)
-{
-  double *A = h->d_slot_acc, *B = h->d_slot_acc2, *X = h->d_ush[0], *Y = h->d_ush[1];
-  SweepCfg c;
-  c.slots_out = (k % 2 == 0) ? A : B;
-  c.pend_acc = (k == 0) ? nullptr : ((k - 1) % 2 == 0 ? A : B);
-  c.pend_u = (k == 1) ? h->d_u : ((k % 2 == 0) ? X : Y);
-  c.by_node = (k == 1) ? 1 : 0;
-  c.pend_out = (k % 2 == 1) ? X : Y;
-  return c;
-}
-
-int launch_sweep(pdrd_method *h, int k, int outputs, int excl, double cfl)
-{
-  const SweepCfg c = sweep_cfg(h, k);
-  for (Block &b : h->blocks) {
-    pdrd::PatchArgs a = patch_args(h, &b, 1, outputs, excl, cfl);
-    a.slot_acc = c.slots_out;
-    a.pend_acc = c.pend_acc;
-    a.pend_u = c.pend_u;
-    a.pend_by_node = c.by_node;
-    a.pend_out = c.pend_out;
-    HIP_CHECK((h->precision ? b.ks.patch_mixed : b.ks.patch)(a, b.np, b.cells, h->stream));
-  }
-  return PDRD_OK;
-}
-
-int launch_final_fixup(pdrd_method *h, int k_last, int outputs, int excl, double cfl)
-{
-  const SweepCfg c = sweep_cfg(h, k_last);
-  pdrd::FixupArgs f = fixup_args(h, 1, outputs, excl, cfl);
-  f.slot_acc = c.slots_out;
-  if (k_last > 0) {
-    f.u_old = c.pend_out;
-    f.u_old_by_node = 0;
-  }
-  const unsigned nb = fixup_blocks(h);
-  switch (h->m) {
-    case 1: hipLaunchKernelGGL(pdrd::patch_fixup_kernel<1>, dim3(nb), dim3(256), 0, h->stream, f); break;
-    case 4: hipLaunchKernelGGL(pdrd::patch_fixup_kernel<4>, dim3(nb), dim3(256), 0, h->stream, f); break;
-    case 5: hipLaunchKernelGGL(pdrd::patch_fixup_kernel<5>, dim3(nb), dim3(256), 0, h->stream, f); break;
-    default: return set_error(PDRD_ENOTSUP, "unsupported number of equations");
-  }
   HIP_CHECK(hipGetLastError());
   return PDRD_OK;
 }
@@ -565,8 +491,6 @@ int check_ready(pdrd_method *h)
 
 void drop_graph(pdrd_method *h)
 {
-  if (h->graph2) (void)hipGraphExecDestroy(h->graph2);
-  h->graph2 = nullptr;
   if (h->graph) {
     (void)hipGraphExecDestroy(h->graph);
     h->graph = nullptr;
@@ -635,12 +559,6 @@ void free_device(pdrd_method *h)
   }
   (void)hipFree(h->d_pn_coff);
   (void)hipFree(h->d_slot_acc);
-  (void)hipFree(h->d_slot_acc2);
-  (void)hipFree(h->d_ush[0]);
-  (void)hipFree(h->d_ush[1]);
-  (void)hipFree(h->d_slot_sid);
-  h->d_slot_acc2 = h->d_ush[0] = h->d_ush[1] = nullptr;
-  h->d_slot_sid = nullptr;
   (void)hipFree(h->d_patch_partial);
   (void)hipFree(h->d_corr);
   h->d_corr = nullptr;
@@ -713,76 +631,6 @@ int iterate_norm(pdrd_method *h, double *norm)
   if (h->scatter_mode == 2)
     return reduce_partials(h, h->np_total + (h->n_shared ? fixup_blocks(h) : 0), norm, h->d_patch_partial);
   return reduce_partials(h, norm_blocks(h->nn), norm);
-}
-
-// n pipelined sweeps; timing: eager, HIP events around every sweep (cell
-// loop) and around the final fix-up (counted as the nodal pass).
-int pipelined_iterate(pdrd_method *h, double cfl, int n, int excl, bool timing)
-{
-  if (timing) {
-    double cell = 0.0;
-    for (int k = 0; k < n; ++k) {
-      HIP_CHECK(hipEventRecord(h->ev[0], h->stream));
-      if (int rc = launch_sweep(h, k, k == n - 1 ? 1 : 0, excl, cfl)) return rc;
-      HIP_CHECK(hipEventRecord(h->ev[1], h->stream));
-      HIP_CHECK(hipEventSynchronize(h->ev[1]));
-      float a = 0.f;
-      HIP_CHECK(hipEventElapsedTime(&a, h->ev[0], h->ev[1]));
-      cell += a;
-    }
-    HIP_CHECK(hipEventRecord(h->ev[1], h->stream));
-    if (int rc = launch_final_fixup(h, n - 1, 1, excl, cfl)) return rc;
-    HIP_CHECK(hipEventRecord(h->ev[2], h->stream));
-    HIP_CHECK(hipEventSynchronize(h->ev[2]));
-    float b = 0.f;
-    HIP_CHECK(hipEventElapsedTime(&b, h->ev[1], h->ev[2]));
-    h->last_cell_ms = cell / n;
-    h->last_node_ms = b / n;
-    return PDRD_OK;
-  }
-  HIP_CHECK(hipEventRecord(h->ev[0], h->stream));
-  int k = 0;
-  while (k < n && k < 2) {
-    if (int rc = launch_sweep(h, k, k == n - 1 ? 1 : 0, excl, cfl)) return rc;
-    ++k;
-  }
-  if (k + 1 < n - 1) {
-    if (!h->graph2 || h->graph2_cfl != cfl || h->graph2_excl != excl) {
-      if (h->graph2) (void)hipGraphExecDestroy(h->graph2);
-      h->graph2 = nullptr;
-      hipGraph_t g = nullptr;
-      HIP_CHECK(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
-      int rc = launch_sweep(h, 2, 0, excl, cfl);
-      if (!rc) rc = launch_sweep(h, 3, 0, excl, cfl);
-      hipError_t e = hipStreamEndCapture(h->stream, &g);
-      if (rc) {
-        if (g) (void)hipGraphDestroy(g);
-        return rc;
-      }
-      if (e != hipSuccess) return hip_fail(e, "hipStreamEndCapture");
-      e = hipGraphInstantiate(&h->graph2, g, nullptr, nullptr, 0);
-      (void)hipGraphDestroy(g);
-      if (e != hipSuccess) return hip_fail(e, "hipGraphInstantiate");
-      h->graph2_cfl = cfl;
-      h->graph2_excl = excl;
-    }
-    while (k + 1 < n - 1) {
-      HIP_CHECK(hipGraphLaunch(h->graph2, h->stream));
-      k += 2;
-    }
-  }
-  while (k < n) {
-    if (int rc = launch_sweep(h, k, k == n - 1 ? 1 : 0, excl, cfl)) return rc;
-    ++k;
-  }
-  if (int rc = launch_final_fixup(h, n - 1, 1, excl, cfl)) return rc;
-  HIP_CHECK(hipEventRecord(h->ev[1], h->stream));
-  HIP_CHECK(hipEventSynchronize(h->ev[1]));
-  float ms = 0.f;
-  HIP_CHECK(hipEventElapsedTime(&ms, h->ev[0], h->ev[1]));
-  h->last_cell_ms = ms;
-  h->last_node_ms = 0.0;
-  return PDRD_OK;
 }
 
 }  // namespace
@@ -1016,18 +864,9 @@ int pdrd_initialize(pdrd_method_t h)
       }
       if ((rc = dev_alloc(h, &h->d_pn_coff, plan.pn_coff.size()))) { free_device(h); return rc; }
       HIP_CHECK(hipMemcpy(h->d_pn_coff, plan.pn_coff.data(), plan.pn_coff.size() * sizeof(uint16_t), hipMemcpyHostToDevice));
-      if ((rc = dev_alloc(h, &h->d_slot_acc, static_cast<size_t>(plan.shared_slots.size()) * (h->m + 1))) ||
-          (rc = dev_alloc(h, &h->d_slot_acc2, static_cast<size_t>(plan.shared_slots.size()) * (h->m + 1))) ||
-          (rc = dev_alloc(h, &h->d_ush[0], static_cast<size_t>(h->n_shared) * h->m)) ||
-          (rc = dev_alloc(h, &h->d_ush[1], static_cast<size_t>(h->n_shared) * h->m))) {
+      if ((rc = dev_alloc(h, &h->d_slot_acc, static_cast<size_t>(plan.shared_slots.size()) * (h->m + 1)))) {
         free_device(h);
         return rc;
-      }
-      {
-        std::vector<uint32_t> sid(plan.pnodes.size(), 0);
-        for (uint32_t i = 0; i < h->n_shared; ++i)
-          for (uint32_t k = plan.shared_off[i]; k < plan.shared_off[i + 1]; ++k) sid[plan.shared_slots[k]] = i;
-        if ((rc = up32(&h->d_slot_sid, sid))) { free_device(h); return rc; }
       }
       if ((rc = dev_alloc(h, &h->d_patch_partial, static_cast<size_t>(h->np_total + kNormMaxBlocks) * (h->m + 1)))) {
         free_device(h);
@@ -1096,10 +935,6 @@ int pdrd_set_option(pdrd_method_t h, int option, int value)
     case PDRD_OPT_TIMING:
       if (value != 0 && value != 1) return set_error(PDRD_EINVAL, "timing is 0 or 1");
       h->timing = value;
-      break;
-    case PDRD_OPT_PIPELINE:
-      if (value != 0 && value != 1) return set_error(PDRD_EINVAL, "pipeline is 0 or 1");
-      h->pipeline = value;
       break;
     case PDRD_OPT_REFRESH:
       if (value < 0) return set_error(PDRD_EINVAL, "refresh period must be >= 0");
@@ -1279,11 +1114,6 @@ int pdrd_iterate(pdrd_method_t h, double cfl, int n_iter, int exclude_dirichlet,
   if (int rc = check_ready(h)) return rc;
   if (n_iter < 1) return set_error(PDRD_EINVAL, "n_iter must be >= 1");
   const int excl = exclude_dirichlet ? 1 : 0;
-  if (h->scatter_mode == 2 && h->n_shared > 0 && !corrects(h) && h->pipeline) {
-    if (int rc = pipelined_iterate(h, cfl, n_iter, excl, h->timing)) return rc;
-    if (norm) return iterate_norm(h, norm);
-    return PDRD_OK;
-  }
   if (h->timing) {
     // eager iterations with events around the cell loop and the nodal pass
     double cell = 0.0, node = 0.0;
@@ -1407,7 +1237,6 @@ int pdrd_get_option(pdrd_method_t h, int option, int *value)
     case PDRD_OPT_BLOCK: *value = h->block_threads; return PDRD_OK;
     case PDRD_OPT_TIMING: *value = h->timing ? 1 : 0; return PDRD_OK;
     case PDRD_OPT_REFRESH: *value = h->refresh; return PDRD_OK;
-    case PDRD_OPT_PIPELINE: *value = h->pipeline; return PDRD_OK;
     case PDRD_OPT_PATCH_CELLS: {
       int c = 0;
       for (const Block &b : h->blocks) c = std::max(c, b.cells);
