@@ -368,11 +368,10 @@ __device__ __forceinline__ void cell_residual(const double *__restrict__ tab, co
   constexpr bool NDIS = (SC != SC_LDA);
   double lda[ND][M];
   double dis[NDIS ? ND : 1][M];
-  double lam[ND];
+  double lmax = 0.0;  // max over nodes and quadrature points of the node's wave speed
   double fint[M];
 #pragma unroll
   for (int n = 0; n < ND; ++n) {
-    lam[n] = 0.0;
 #pragma unroll
     for (int k = 0; k < M; ++k) lda[n][k] = 0.0;
   }
@@ -425,9 +424,9 @@ __device__ __forceinline__ void cell_residual(const double *__restrict__ tab, co
         for (int xi = 0; xi < D; ++xi) knt += c[xi] * dvk[xi * ND + n];
         const double kn = knt;
         const double ak = fabs(kn);
-        kp[n] = kn > 0.0 ? kn : 0.0;
+        kp[n] = fmax(kn, 0.0);
         const double l = ak > 1.e-6 ? ak : 1.e-6;
-        lam[n] = l > lam[n] ? l : lam[n];
+        lmax = fmax(lmax, l);
         S += kp[n];
         SU += kp[n] * Uq(n, 0);
       }
@@ -527,7 +526,7 @@ __device__ __forceinline__ void cell_residual(const double *__restrict__ tab, co
       for (int n = 0; n < ND; ++n) {
         const NodeK k = node_k(n);
         if constexpr (KEEP) nk[n] = k;
-        lam[n] = fmax(lam[n], fmax(1.e-6, static_cast<double>(k.emax)));
+        lmax = fmax(lmax, fmax(1.e-6, static_cast<double>(k.emax)));
         sl0 += k.l0;
         ssg += k.sg;
 #pragma unroll
@@ -711,9 +710,6 @@ __device__ __forceinline__ void cell_residual(const double *__restrict__ tab, co
     }
   }
 
-  double lmax = lam[0];
-#pragma unroll
-  for (int n = 1; n < ND; ++n) lmax = fmax(lmax, lam[n]);
   wave = vol * lmax;
 }
 
