@@ -28,11 +28,24 @@ class OrcBlock(C.Structure):
     ]
 
 
+WEAK_WALL, WEAK_SUBINLET, WEAK_SUBOUTLET = 0, 1, 2
+
+
+class OrcWeak(C.Structure):
+    _fields_ = [
+        ("type", C.c_int), ("params", C.c_double * 4), ("order", C.c_int), ("nfacet", C.c_uint32),
+        ("nfn", C.c_int), ("nfv", C.c_int), ("nq", C.c_int),
+        ("nodes", C.POINTER(C.c_uint32)), ("verts", C.POINTER(C.c_uint32)), ("inside", C.POINTER(C.c_double)),
+        ("w", C.POINTER(C.c_double)), ("V", C.POINTER(C.c_double)),
+    ]
+
+
 class OrcProblem(C.Structure):
     _fields_ = [
         ("dim", C.c_int), ("physics", C.c_int), ("scheme", C.c_int), ("nblocks", C.c_int),
         ("nv", C.c_uint32), ("nn", C.c_uint32), ("coords", C.POINTER(C.c_double)),
         ("params", C.c_double * 4), ("blocks", C.POINTER(OrcBlock)),
+        ("nweak", C.c_int), ("weak", C.POINTER(OrcWeak)),
     ]
 
 
@@ -104,9 +117,32 @@ class Problem:
         for i, v in enumerate(params[:4]):
             p.params[i] = v
         p.blocks = blocks
+        p.nweak = 0
         self.p = p
         self.nn = mesh.nb_nodes
         self.m = lib().orc_neq(physics)
+        self._weak = []
+
+    def add_weak(self, bc_type, params, order, nodes, verts, inside, w, V):
+        """A weak BC over facets: nodes [nf][nfn], verts [nf][nfv], inside
+        [nf][dim] (a point of each facet's cell), facet rule w[nq], V[nq][nfn]."""
+        arrs = [np.ascontiguousarray(nodes, dtype=np.uint32), np.ascontiguousarray(verts, dtype=np.uint32),
+                np.ascontiguousarray(inside, dtype=np.float64), np.ascontiguousarray(w, dtype=np.float64),
+                np.ascontiguousarray(V, dtype=np.float64)]
+        self._keep += arrs
+        wk = OrcWeak()
+        wk.type, wk.order = bc_type, order
+        for i, v in enumerate(list(params)[:4]):
+            wk.params[i] = v
+        wk.nfacet, wk.nfn = arrs[0].shape[0], arrs[0].shape[1]
+        wk.nfv, wk.nq = arrs[1].shape[1], arrs[3].size
+        wk.nodes, wk.verts = _p(arrs[0], C.c_uint32), _p(arrs[1], C.c_uint32)
+        wk.inside, wk.w, wk.V = _p(arrs[2]), _p(arrs[3]), _p(arrs[4])
+        self._weak.append(wk)
+        arr = (OrcWeak * len(self._weak))(*self._weak)
+        self._keep.append(arr)
+        self.p.nweak = len(self._weak)
+        self.p.weak = arr
 
     def dual_volume(self):
         v = np.zeros(self.nn)

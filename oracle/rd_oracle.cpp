@@ -620,6 +620,145 @@ int orc_cell_residuals(const orc_problem *p, int block, const double *u, double 
   return rc;
 }
 
+namespace {
+
+// WeakBC::apply with the concrete BC's compute_on_element_weak
+// (WeakBC.hpp:225-352, 786-821; WeakWall.hpp "Stefano's version";
+// WeakSubInlet.hpp, WeakSubOutlet.hpp): per facet, the corrective residual
+// integrated over the facet, split over its nodes by the facet basis,
+// rescaled to the facet total, divided by the dual volume; the wave speed
+// gains V_dual * max over the facet's points of max |eigenvalue|.
+int apply_weak(const orc_problem *p, const orc_weak &bc, const double *u, const double *vdual, double *res,
+               double *ws)
+{
+  const int m = neq_of(p->physics);
+  const int dim = p->dim;
+  if (p->physics != ORC_EULER2D && p->physics != ORC_EULER3D) return -1;
+  const double R_gas = 287.046;
+  int rc = 0;
+  std::vector<double> bres(static_cast<size_t>(bc.nfn) * m);
+  for (uint32_t f = 0; f < bc.nfacet; ++f) {
+    const uint32_t *fn = bc.nodes + static_cast<size_t>(f) * bc.nfn;
+    const uint32_t *fv = bc.verts + static_cast<size_t>(f) * bc.nfv;
+    double x[3][3] = {{0}};
+    for (int i = 0; i < bc.nfv; ++i)
+      for (int d = 0; d < dim; ++d) x[i][d] = p->coords[static_cast<size_t>(fv[i]) * dim + d];
+    double nrm[3] = {0, 0, 0}, jac;
+    if (dim == 2) {
+      const double tx = x[1][0] - x[0][0], ty = x[1][1] - x[0][1];
+      const double len = std::sqrt(tx * tx + ty * ty);
+      nrm[0] = ty / len;
+      nrm[1] = -tx / len;
+      jac = 0.5 * len;
+    } else {
+      const double a[3] = {x[1][0] - x[0][0], x[1][1] - x[0][1], x[1][2] - x[0][2]};
+      const double b[3] = {x[2][0] - x[0][0], x[2][1] - x[0][1], x[2][2] - x[0][2]};
+      const double c[3] = {a[1] * b[2] - a[2] * b[1], a[2] * b[0] - a[0] * b[2], a[0] * b[1] - a[1] * b[0]};
+      const double len = std::sqrt(c[0] * c[0] + c[1] * c[1] + c[2] * c[2]);
+      for (int d = 0; d < 3; ++d) nrm[d] = c[d] / len;
+      jac = 0.25 * len;
+    }
+    double dot = 0.0;
+    for (int d = 0; d < dim; ++d) dot += nrm[d] * (x[0][d] - bc.inside[static_cast<size_t>(f) * dim + d]);
+    if (dot < 0)
+      for (int d = 0; d < dim; ++d) nrm[d] = -nrm[d];
+    const double relax = (bc.type == ORC_WEAK_WALL && bc.order > 1) ? 1. / (3. * bc.order) : 1.0;
+    std::fill(bres.begin(), bres.end(), 0.0);
+    double total[MAXM] = {0};
+    double coeff = 0.0;
+    for (int q = 0; q < bc.nq; ++q) {
+      const double wq = jac * bc.w[q];
+      double uq[MAXM] = {0};
+      for (int n = 0; n < bc.nfn; ++n)
+        for (int e = 0; e < m; ++e) uq[e] += bc.V[q * bc.nfn + n] * u[static_cast<size_t>(fn[n]) * m + e];
+      Props pr;
+      properties(p, uq, pr);
+      if (pr.bad) rc = -5;
+      Mat R, L;
+      double Dv[MAXM];
+      eigen_structure(p->physics, pr, nrm, R, L, Dv);
+      double eig_max = 1.e-9;
+      for (int e = 0; e < m; ++e) eig_max = std::max(std::abs(Dv[e]), eig_max);
+      coeff = std::max(coeff, eig_max);
+      double dF[MAXM] = {0};
+      if (bc.type == ORC_WEAK_WALL) {
+        double F[3][MAXM], Fn[MAXM] = {0}, Gn[MAXM] = {0};
+        flux(p, uq, pr, F);
+        for (int e = 0; e < m; ++e)
+          for (int d = 0; d < dim; ++d) Fn[e] += F[d][e] * nrm[d];
+        double g[MAXM];
+        double vn = 0.0, v2n = 0.0;
+        for (int d = 0; d < dim; ++d) vn += pr.V[d] * nrm[d];
+        g[0] = pr.rho;
+        for (int d = 0; d < dim; ++d) {
+          const double vt = pr.V[d] - vn * nrm[d];
+          v2n += vt * vt;
+          g[1 + d] = vt * pr.rho;
+        }
+        g[dim + 1] = uq[dim + 1] + 0.5 * pr.rho * (v2n - pr.v2);
+        Props pg;
+        properties(p, g, pg);
+        flux(p, g, pg, F);
+        for (int e = 0; e < m; ++e)
+          for (int d = 0; d < dim; ++d) Gn[e] += F[d][e] * nrm[d];
+        for (int e = 0; e < m; ++e) dF[e] = Gn[e] - Fn[e];
+      } else {
+        double g[MAXM];
+        if (bc.type == ORC_WEAK_SUBINLET) {
+          const double Ma2 = pr.v2 / (pr.a * pr.a);
+          const double fac = 1. + 0.5 * pr.gm1 * Ma2;
+          const double Tg = bc.params[0] / fac;
+          const double pg = bc.params[1] / std::pow(fac, pr.gamma / pr.gm1);
+          g[0] = pg / (R_gas * Tg);
+          g[1] = std::sqrt(Ma2) * std::sqrt(pr.gamma * R_gas * Tg) * g[0];
+          for (int d = 1; d < dim; ++d) g[1 + d] = 0.0;
+          double r2u2 = 0.0;
+          for (int d = 0; d < dim; ++d) r2u2 += g[1 + d] * g[1 + d];
+          g[dim + 1] = pg / pr.gm1 + 0.5 * (r2u2 / g[0]);
+        } else {
+          for (int e = 0; e < m - 1; ++e) g[e] = uq[e];
+          double r2u2 = 0.0;
+          for (int d = 0; d < dim; ++d) r2u2 += uq[1 + d] * uq[1 + d];
+          g[m - 1] = bc.params[0] / pr.gm1 + 0.5 * r2u2 / uq[0];
+        }
+        double dm[MAXM];
+        for (int e = 0; e < m; ++e) {
+          dm[e] = std::min(Dv[e], 0.0);
+          g[e] -= uq[e];
+        }
+        Mat K;
+        product_RDL(m, R, dm, L, K);
+        for (int i = 0; i < m; ++i) {
+          double t = 0.0;
+          for (int j = 0; j < m; ++j) t += K[i][j] * g[j];
+          dF[i] = t;
+        }
+      }
+      for (int e = 0; e < m; ++e) total[e] += wq * dF[e];
+      for (int n = 0; n < bc.nfn; ++n)
+        for (int e = 0; e < m; ++e) bres[n * m + e] += relax * bc.V[q * bc.nfn + n] * wq * dF[e];
+    }
+    // rescale so that the nodal parts sum to the facet total
+    for (int e = 0; e < m; ++e) {
+      double sum = 0.0;
+      for (int n = 0; n < bc.nfn; ++n) sum += bres[n * m + e];
+      if (std::abs(sum) > 1.e-14) {
+        const double sc = total[e] / sum;
+        for (int n = 0; n < bc.nfn; ++n) bres[n * m + e] *= sc;
+      }
+    }
+    for (int n = 0; n < bc.nfn; ++n) {
+      const uint32_t v = fn[n];
+      ws[v] += vdual[v] * coeff;
+      const double iv = 1. / vdual[v];
+      for (int e = 0; e < m; ++e) res[static_cast<size_t>(v) * m + e] += iv * bres[n * m + e];
+    }
+  }
+  return rc;
+}
+
+}  // namespace
+
 int orc_assemble_rhs(const orc_problem *p, const double *u, const double *vdual, double *res, double *ws)
 {
   const int m = neq_of(p->physics);
@@ -640,6 +779,10 @@ int orc_assemble_rhs(const orc_problem *p, const double *u, const double *vdual,
         ws[v] += out.wave;
       }
     }
+  }
+  for (int k = 0; k < p->nweak; ++k) {
+    const int r = apply_weak(p, p->weak[k], u, vdual, res, ws);
+    if (r) rc = r;
   }
   return rc;
 }

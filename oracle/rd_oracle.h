@@ -33,6 +33,26 @@ typedef struct orc_block
   const uint32_t *geo_conn, *sol_conn;
 } orc_block;
 
+/* Weak boundary conditions (Euler): WeakWall, WeakSubInlet, WeakSubOutlet. */
+enum { ORC_WEAK_WALL = 0, ORC_WEAK_SUBINLET = 1, ORC_WEAK_SUBOUTLET = 2 };
+
+/* One weak BC over its boundary facets. Facet rule on the reference facet:
+ * w[nq] (edge [-1,1] / triangle of area 2), V[nq][nfn] the values of the
+ * facet's Lagrange nodes. The facet's normal and measure are computed here
+ * from its vertices, oriented away from `inside` (a point of its cell). */
+typedef struct orc_weak
+{
+  int type;
+  double params[4]; /* SubInlet: total temperature, total pressure; SubOutlet: pressure */
+  int order;        /* solution order (WeakWall's relaxation coefficient) */
+  uint32_t nfacet;
+  int nfn, nfv, nq;
+  const uint32_t *nodes; /* [nfacet][nfn] solution nodes */
+  const uint32_t *verts; /* [nfacet][nfv] vertices */
+  const double *inside;  /* [nfacet][dim] */
+  const double *w, *V;
+} orc_weak;
+
 typedef struct orc_problem
 {
   int dim, physics, scheme, nblocks;
@@ -40,6 +60,8 @@ typedef struct orc_problem
   const double *coords; /* [nv][dim] */
   double params[4];     /* advection: ax, ay; Euler: gamma, R */
   const orc_block *blocks;
+  int nweak; /* weak BCs applied after the cell loop, in this order */
+  const orc_weak *weak;
 } orc_problem;
 
 int orc_neq(int physics);

@@ -28,6 +28,7 @@ ADVECTION2D, BURGERS2D, EULER2D, EULER3D = 0, 1, 2, 3
 LDA, N, B, PSI = 0, 1, 2, 3
 SOLUTION, RESIDUAL, TIME_STEP, WAVE_SPEED, DUAL_VOLUME = 0, 1, 2, 3, 4
 OPT_SCATTER, OPT_PRECISION, OPT_BLOCK, OPT_TIMING, OPT_PATCH_CELLS, OPT_REFRESH = 0, 1, 2, 3, 4, 5
+WEAK_WALL, WEAK_SUBINLET, WEAK_SUBOUTLET = 0, 1, 2
 
 # every symbol include/pdekit_rd.h declares
 EXPORTED = (
@@ -40,7 +41,7 @@ EXPORTED = (
     "pdrd_add_strong_dirichlet", "pdrd_assemble_rhs", "pdrd_assemble_lhs_and_rhs", "pdrd_solve",
     "pdrd_compute_residual_norm", "pdrd_residual_norm_L2", "pdrd_iterate", "pdrd_check_status",
     "pdrd_method_info", "pdrd_last_timing", "pdrd_last_error", "pdrd_version", "pdrd_get_option",
-    "pdrd_batch_create", "pdrd_batch_destroy", "pdrd_batch_iterate",
+    "pdrd_batch_create", "pdrd_batch_destroy", "pdrd_batch_iterate", "pdrd_add_weak_bc", "pdrd_weak_bc_info",
 )
 
 
@@ -109,6 +110,8 @@ def lib() -> C.CDLL:
         "pdrd_residual_norm_L2": [P, I, PD],
         "pdrd_iterate": [P, D, I, I, PD],
         "pdrd_get_option": [P, I, PI],
+        "pdrd_add_weak_bc": [P, I, U32, PU32, PD, I, I],
+        "pdrd_weak_bc_info": [P, I, PI64],
         "pdrd_batch_create": [C.POINTER(P)],
         "pdrd_batch_destroy": [P],
         "pdrd_batch_iterate": [P, I, C.POINTER(P), PD, I, I, PD],

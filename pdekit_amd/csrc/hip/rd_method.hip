@@ -14,10 +14,12 @@
 
 #include "../../../include/pdekit_rd.h"
 #include "../host/capi_common.hpp"
+#include "../host/facets.hpp"
 #include "../host/fe_tables.hpp"
 #include "../host/mesh.hpp"
 #include "../host/patches.hpp"
 #include "rd_launch.hpp"
+#include "rd_weakbc.hpp"
 
 using pdrd::set_error;
 
@@ -46,6 +48,24 @@ struct Block
   int cells = 0;  // cells per patch (threads per workgroup)
 };
 
+// A weak boundary condition as the caller gives it, and on the device.
+struct WeakSpec
+{
+  int type = 0;
+  std::vector<uint32_t> nodes;
+  double params[2] = {0, 0};
+  int quad_order = 4;
+};
+
+struct WeakDev
+{
+  int type = 0, nfn = 0, nq = 0;
+  uint32_t nf = 0;
+  double relax = 1.0, params[2] = {0, 0};
+  uint32_t *d_nodes = nullptr, *d_rows = nullptr;
+  double *d_normal = nullptr, *d_jac = nullptr, *d_w = nullptr, *d_V = nullptr;
+};
+
 }  // namespace
 
 struct pdrd_method
@@ -63,6 +83,8 @@ struct pdrd_method
   bool patch_ok = false;
   std::string patch_reason;
   int precision = 0;
+  std::vector<WeakSpec> weak_specs;  // weak BCs added before pdrd_initialize
+  std::vector<WeakDev> weak;
   // defect correction of the mixed cells: every `refresh` iterations
   // corr = R64(u) - Rmixed(u), added to the mixed residual in the update
   int refresh = 50;
@@ -390,6 +412,45 @@ int launch_patch_blocks(pdrd_method *h, int update, int outputs, int excl, doubl
   return PDRD_OK;
 }
 
+// The weak boundary conditions' facet kernels: after the patch kernels of a
+// sweep (whose facet nodes are all shared, so their state is still the
+// sweep's start state) and before the shared-node sums.
+int launch_weak(pdrd_method *h, hipStream_t s)
+{
+  for (const WeakDev &w : h->weak) {
+    if (w.nf == 0) continue;
+    pdrd::WeakArgs a{};
+    a.nodes = w.d_nodes;
+    a.rows = w.d_rows;
+    a.normal = w.d_normal;
+    a.jac = w.d_jac;
+    a.w = w.d_w;
+    a.V = w.d_V;
+    a.u = h->d_u;
+    a.vol = h->d_vol;
+    a.slot_acc = h->d_slot_acc;
+    a.status = h->d_status;
+    a.nf = w.nf;
+    a.nq = w.nq;
+    a.type = w.type;
+    a.relax = w.relax;
+    a.params[0] = w.params[0];
+    a.params[1] = w.params[1];
+    a.gamma = h->params[0];
+    const dim3 grid((w.nf + 127) / 128);
+    if (h->dim == 2 && w.nfn == 2)
+      hipLaunchKernelGGL((pdrd::weak_bc_kernel<2, 2>), grid, dim3(128), 0, s, a);
+    else if (h->dim == 2 && w.nfn == 3)
+      hipLaunchKernelGGL((pdrd::weak_bc_kernel<2, 3>), grid, dim3(128), 0, s, a);
+    else if (h->dim == 3 && w.nfn == 3)
+      hipLaunchKernelGGL((pdrd::weak_bc_kernel<3, 3>), grid, dim3(128), 0, s, a);
+    else
+      return set_error(PDRD_ENOTSUP, "weak boundary condition on this facet kind");
+    HIP_CHECK(hipGetLastError());
+  }
+  return PDRD_OK;
+}
+
 // Patch assembly of every block followed by the sums of the shared nodes.
 // update = 0: residual and wave speed of every node; update = 1: one
 // pseudo-time iteration (the strong Dirichlet values are applied in the
@@ -397,6 +458,7 @@ int launch_patch_blocks(pdrd_method *h, int update, int outputs, int excl, doubl
 int launch_patches(pdrd_method *h, int update, int outputs, int excl, double cfl)
 {
   if (int rc = launch_patch_blocks(h, update, outputs, excl, cfl)) return rc;
+  if (int rc = launch_weak(h, h->stream)) return rc;
   if (h->n_shared == 0) return PDRD_OK;
   const pdrd::FixupArgs f = fixup_args(h, update, outputs, excl, cfl);
   const unsigned nb = fixup_blocks(h);
@@ -557,6 +619,15 @@ void free_device(pdrd_method *h)
     (void)hipFree(*p);
     *p = nullptr;
   }
+  for (WeakDev &w : h->weak) {
+    (void)hipFree(w.d_nodes);
+    (void)hipFree(w.d_rows);
+    (void)hipFree(w.d_normal);
+    (void)hipFree(w.d_jac);
+    (void)hipFree(w.d_w);
+    (void)hipFree(w.d_V);
+  }
+  h->weak.clear();
   (void)hipFree(h->d_pn_coff);
   (void)hipFree(h->d_slot_acc);
   (void)hipFree(h->d_patch_partial);
@@ -816,10 +887,34 @@ int pdrd_initialize(pdrd_method_t h)
       x.patch_size = h->patch_cells ? std::min(h->patch_cells, b.ks.patch_size) : b.ks.patch_size;
       pin.push_back(x);
     }
+    // boundary facets of the weak BCs; their nodes get one extra partial-sum
+    // row per facet
+    std::vector<pdrd::FacetSet> fsets;
+    std::vector<uint32_t> extra(h->nn, 0);
+    if (!h->weak_specs.empty()) {
+      std::vector<pdrd::FacetBlockIn> fin;
+      for (const Block &b : h->blocks) {
+        pdrd::FacetBlockIn x;
+        x.shape = b.fe.shape;
+        x.order = b.fe.sol_order;
+        x.ng = b.fe.ng;
+        x.nd = b.fe.nd;
+        x.ne = b.ne;
+        x.geo = b.geo.data();
+        x.sol = b.sol.data();
+        fin.push_back(x);
+      }
+      for (const WeakSpec &ws : h->weak_specs) {
+        std::vector<char> in_set(h->nn, 0);
+        for (uint32_t v : ws.nodes) in_set[v] = 1;
+        fsets.push_back(pdrd::boundary_facets(h->dim, h->coords.data(), fin, in_set));
+        for (uint32_t v : fsets.back().nodes) extra[v]++;
+      }
+    }
     pdrd::PatchPlan plan;
     h->patch_ok = true;
     try {
-      plan = pdrd::build_patch_plan(h->dim, h->nv, h->coords.data(), h->nn, pin);
+      plan = pdrd::build_patch_plan(h->dim, h->nv, h->coords.data(), h->nn, pin, fsets.empty() ? nullptr : &extra);
     } catch (const std::exception &ex) {
       h->patch_ok = false;
       h->patch_reason = ex.what();
@@ -846,7 +941,8 @@ int pdrd_initialize(pdrd_method_t h)
       }
       // the row of every shared slot in the node-grouped partial sums
       std::vector<uint32_t> slot_pos(plan.pnodes.size(), 0);
-      for (size_t k = 0; k < plan.shared_slots.size(); ++k) slot_pos[plan.shared_slots[k]] = static_cast<uint32_t>(k);
+      for (size_t k = 0; k < plan.shared_slots.size(); ++k)
+        if (plan.shared_slots[k] != UINT32_MAX) slot_pos[plan.shared_slots[k]] = static_cast<uint32_t>(k);
       h->np_total = static_cast<uint32_t>(plan.pn_inner.size());
       h->n_shared = static_cast<uint32_t>(plan.shared_nodes.size());
       auto up32 = [&](uint32_t **d, const std::vector<uint32_t> &v) -> int {
@@ -872,7 +968,51 @@ int pdrd_initialize(pdrd_method_t h)
         free_device(h);
         return rc;
       }
+      // weak BCs: the row of every (facet, node) pair after the node's patch rows
+      if (!fsets.empty()) {
+        std::vector<uint32_t> next(h->nn, UINT32_MAX);
+        for (uint32_t i = 0; i < h->n_shared; ++i) {
+          const uint32_t v = plan.shared_nodes[i];
+          next[v] = plan.shared_off[i + 1] - extra[v];
+        }
+        for (size_t k = 0; k < fsets.size(); ++k) {
+          const pdrd::FacetSet &fs = fsets[k];
+          const WeakSpec &ws = h->weak_specs[k];
+          WeakDev wd;
+          wd.type = ws.type;
+          wd.nfn = fs.nfn;
+          wd.nf = static_cast<uint32_t>(fs.size());
+          wd.params[0] = ws.params[0];
+          wd.params[1] = ws.params[1];
+          const int forder = (fs.nfn > fs.nfv) ? 2 : 1;
+          wd.relax = (ws.type == PDRD_WEAK_WALL && forder > 1) ? 1.0 / (3.0 * forder) : 1.0;
+          std::vector<uint32_t> rows(fs.nodes.size());
+          for (size_t j = 0; j < fs.nodes.size(); ++j) rows[j] = next[fs.nodes[j]]++;
+          std::vector<double> w, V;
+          if (wd.nf) pdrd::facet_tables(h->dim, fs.nfn, ws.quad_order + 1, w, V);
+          wd.nq = static_cast<int>(w.size());
+          auto upd = [&](double **d, const std::vector<double> &v) -> int {
+            int r = dev_alloc(h, d, v.size());
+            if (r) return r;
+            if (!v.empty() && hipMemcpy(*d, v.data(), v.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess)
+              return set_error(PDRD_EHIP, "hipMemcpy of the boundary facets");
+            return PDRD_OK;
+          };
+          h->weak.push_back(wd);
+          WeakDev &wr = h->weak.back();
+          if ((rc = up32(&wr.d_nodes, fs.nodes)) || (rc = up32(&wr.d_rows, rows)) ||
+              (rc = upd(&wr.d_normal, fs.normal)) || (rc = upd(&wr.d_jac, fs.jac)) || (rc = upd(&wr.d_w, w)) ||
+              (rc = upd(&wr.d_V, V))) {
+            free_device(h);
+            return rc;
+          }
+        }
+      }
     } else {
+      if (!h->weak_specs.empty()) {
+        free_device(h);
+        return set_error(PDRD_ENOTSUP, "weak boundary conditions need the patch assembly: " + h->patch_reason);
+      }
       h->scatter_mode = 1;
     }
   }
@@ -920,6 +1060,8 @@ int pdrd_set_option(pdrd_method_t h, int option, int value)
       if (value < 0 || value > 2) return set_error(PDRD_EINVAL, "scatter mode is 0, 1 or 2");
       if (value == 2 && h->ready && !h->patch_ok)
         return set_error(PDRD_ENOTSUP, "patch assembly unavailable: " + h->patch_reason);
+      if (value != 2 && !h->weak_specs.empty())
+        return set_error(PDRD_ENOTSUP, "weak boundary conditions run with the patch assembly only");
       h->scatter_mode = value;
       break;
     case PDRD_OPT_PRECISION:
@@ -1006,6 +1148,43 @@ int pdrd_field_device_ptr(pdrd_method_t h, int field, double **dptr)
   if (!dptr) return set_error(PDRD_EINVAL, "null output");
   size_t n;
   return field_size(h, field, &n, dptr);
+}
+
+int pdrd_add_weak_bc(pdrd_method_t h, int type, uint32_t n, const uint32_t *nodes, const double *params,
+                     int nparams, int quad_order)
+{
+  PDRD_TRY
+  if (!h || (n && !nodes)) return set_error(PDRD_EINVAL, "null argument");
+  if (h->ready) return set_error(PDRD_EINVAL, "weak boundary conditions are added before pdrd_initialize");
+  if (h->physics != pdrd::PH_EULER2D && h->physics != pdrd::PH_EULER3D)
+    return set_error(PDRD_ENOTSUP, "weak boundary conditions are built for the Euler equations");
+  if (type < PDRD_WEAK_WALL || type > PDRD_WEAK_SUBOUTLET) return set_error(PDRD_EINVAL, "unknown weak BC type");
+  const int need = (type == PDRD_WEAK_WALL) ? 0 : (type == PDRD_WEAK_SUBINLET ? 2 : 1);
+  if (nparams < need || (need && !params)) return set_error(PDRD_EINVAL, "missing weak BC parameters");
+  if (quad_order < 1) return set_error(PDRD_EINVAL, "quadrature order must be >= 1");
+  if (h->scatter_mode != 2) return set_error(PDRD_ENOTSUP, "weak boundary conditions run with the patch assembly only");
+  WeakSpec w;
+  w.type = type;
+  w.quad_order = quad_order;
+  for (int i = 0; i < need; ++i) w.params[i] = params[i];
+  for (uint32_t i = 0; i < n; ++i) {
+    if (nodes[i] >= h->nn) return set_error(PDRD_EINVAL, "weak BC node out of range");
+    w.nodes.push_back(nodes[i]);
+  }
+  h->weak_specs.push_back(std::move(w));
+  return PDRD_OK;
+  PDRD_CATCH
+}
+
+int pdrd_weak_bc_info(pdrd_method_t h, int k, int64_t *info3)
+{
+  if (!h || !info3) return set_error(PDRD_EINVAL, "null argument");
+  if (!h->ready) return set_error(PDRD_EINVAL, "not initialized");
+  if (k < 0 || k >= static_cast<int>(h->weak.size())) return set_error(PDRD_EINVAL, "no such weak BC");
+  info3[0] = h->weak[k].nf;
+  info3[1] = h->weak[k].nfn;
+  info3[2] = h->weak[k].nq;
+  return PDRD_OK;
 }
 
 int pdrd_add_strong_dirichlet(pdrd_method_t h, uint32_t n, const uint32_t *nodes, const double *values)
@@ -1128,6 +1307,7 @@ int pdrd_iterate(pdrd_method_t h, double cfl, int n_iter, int exclude_dirichlet,
         // the second interval is the shared-node fix-up
         if (int rc = launch_patch_blocks(h, 1, out, excl, cfl)) return rc;
         HIP_CHECK(hipEventRecord(h->ev[1], h->stream));
+        if (int rc = launch_weak(h, h->stream)) return rc;
         if (h->n_shared) {
           // fix-up only: re-enqueue through launch_patches would repeat the cells
           const pdrd::FixupArgs f = fixup_args(h, 1, out, excl, cfl);
@@ -1299,9 +1479,16 @@ struct BatchLaunch
 };
 
 int enqueue_batch(pdrd_batch *b, const std::vector<BatchLaunch> &launches, size_t args_half, size_t fargs_half,
-                  int outputs)
+                  int outputs, int count, const pdrd_method_t *methods)
 {
+  bool weak_done = false;
   for (const BatchLaunch &l : launches) {
+    if (!l.patch && !weak_done) {
+      // the weak BCs of every method, after all patch launches
+      for (int i = 0; i < count; ++i)
+        if (int rc = launch_weak(methods[i], b->stream)) return rc;
+      weak_done = true;
+    }
     if (l.grid == 0) continue;
     if (l.patch) {
       HIP_CHECK(l.patch(b->d_pargs + (outputs ? args_half : 0) + l.args0, b->d_first + l.first0, l.count, l.grid,
@@ -1451,7 +1638,7 @@ int pdrd_batch_iterate(pdrd_batch_t b, int count, const pdrd_method_t *methods, 
     hipGraph_t g = nullptr;
     hipGraphExec_t ge = nullptr;
     HIP_CHECK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
-    const int rc = enqueue_batch(b, launches, ph, fh, 0);
+    const int rc = enqueue_batch(b, launches, ph, fh, 0, count, methods);
     hipError_t e = hipStreamEndCapture(b->stream, &g);
     if (rc) {
       if (g) (void)hipGraphDestroy(g);
@@ -1469,7 +1656,7 @@ int pdrd_batch_iterate(pdrd_batch_t b, int count, const pdrd_method_t *methods, 
     (void)hipGraphExecDestroy(ge);
     if (e != hipSuccess) return hip_fail(e, "hipGraphLaunch");
   }
-  if (int rc = enqueue_batch(b, launches, ph, fh, 1)) return rc;
+  if (int rc = enqueue_batch(b, launches, ph, fh, 1, count, methods)) return rc;
   HIP_CHECK(hipStreamSynchronize(b->stream));
   if (norms) {
     size_t off = 0;

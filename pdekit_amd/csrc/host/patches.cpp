@@ -36,8 +36,9 @@ uint64_t spread2(uint64_t x)
 }  // namespace
 
 PatchPlan build_patch_plan(int dim, uint32_t nv, const double *coords, uint32_t nn,
-                           const std::vector<PatchBlockIn> &blocks)
+                           const std::vector<PatchBlockIn> &blocks, const std::vector<uint32_t> *extra_rows)
 {
+  auto extra = [&](uint32_t v) -> uint32_t { return extra_rows ? (*extra_rows)[v] : 0u; };
   PatchPlan plan;
   double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
   for (uint32_t v = 0; v < nv; ++v)
@@ -102,7 +103,7 @@ PatchPlan build_patch_plan(int dim, uint32_t nv, const double *coords, uint32_t 
   std::vector<int32_t> pos(nn, -1);
   for (size_t p = 0; p < np; ++p) {
     std::vector<uint32_t> inner, shared;
-    for (uint32_t v : pn[p]) (cnt[v] == 1 ? inner : shared).push_back(v);
+    for (uint32_t v : pn[p]) (cnt[v] == 1 && extra(v) == 0 ? inner : shared).push_back(v);
     plan.pn_inner[p] = static_cast<uint32_t>(inner.size());
     pn[p] = inner;
     pn[p].insert(pn[p].end(), shared.begin(), shared.end());
@@ -149,14 +150,14 @@ PatchPlan build_patch_plan(int dim, uint32_t nv, const double *coords, uint32_t 
   // 4. slots of every shared node, in patch order
   std::vector<uint32_t> shared_id(nn, UINT32_MAX);
   for (uint32_t v = 0; v < nn; ++v)
-    if (cnt[v] > 1) {
+    if (cnt[v] > 1 || (cnt[v] == 1 && extra(v) > 0)) {
       shared_id[v] = static_cast<uint32_t>(plan.shared_nodes.size());
       plan.shared_nodes.push_back(v);
     }
   plan.shared_off.assign(plan.shared_nodes.size() + 1, 0);
-  for (uint32_t v : plan.shared_nodes) plan.shared_off[shared_id[v] + 1] = cnt[v];
+  for (uint32_t v : plan.shared_nodes) plan.shared_off[shared_id[v] + 1] = cnt[v] + extra(v);
   for (size_t i = 0; i < plan.shared_nodes.size(); ++i) plan.shared_off[i + 1] += plan.shared_off[i];
-  plan.shared_slots.assign(plan.shared_off.back(), 0);
+  plan.shared_slots.assign(plan.shared_off.back(), UINT32_MAX);
   std::vector<uint32_t> fillp(plan.shared_off.begin(), plan.shared_off.end() - 1);
   for (size_t p = 0; p < np; ++p)
     for (uint32_t l = plan.pn_inner[p]; l < plan.pnode_off[p + 1] - plan.pnode_off[p]; ++l) {

@@ -44,7 +44,7 @@ struct PatchPlan
   std::vector<uint16_t> pn_coff;
   std::vector<uint32_t> shared_nodes;  // nodes held by several patches, ascending
   std::vector<uint32_t> shared_off;    // [nshared + 1]
-  std::vector<uint32_t> shared_slots;  // slots of each shared node, ascending
+  std::vector<uint32_t> shared_slots;  // slots of each shared node, ascending (then extra rows)
   int max_rank = 0;                    // most cells of one patch sharing one node
 };
 
@@ -54,7 +54,12 @@ struct PatchPlan
 // (cell, node) pairs of each patch. Throws std::invalid_argument when a patch
 // has more than PATCH_MAX_NODES nodes, a node is held by more than
 // PATCH_MAX_RANK cells of one patch, or a patch has 65536 pairs or more.
+// extra_rows (optional, per node): further partial sums a node receives
+// besides its patches' (weak boundary facets); such nodes are treated as
+// shared, and their extra rows follow the patch rows in shared_slots as
+// UINT32_MAX entries.
 PatchPlan build_patch_plan(int dim, uint32_t nv, const double *coords, uint32_t nn,
-                           const std::vector<PatchBlockIn> &blocks);
+                           const std::vector<PatchBlockIn> &blocks,
+                           const std::vector<uint32_t> *extra_rows = nullptr);
 
 }  // namespace pdrd

@@ -42,7 +42,9 @@ class PGRDMethodExplicit:
         else:
             self.nb_params = 0
         self.neq = NEQ[self.physics]
-        self._h = None
+        self._raw = None     # C handle; pdrd_initialize runs at the first use that needs the device
+        self._ready = False
+        self.quad_order = 4
         self.mesh: Mesh | None = None
         self.dirichlet_nodes = np.zeros(0, dtype=np.uint32)
 
@@ -64,7 +66,9 @@ class PGRDMethodExplicit:
         coords = np.ascontiguousarray(m.coords, dtype=np.float64)
         L.check(lib.pdrd_create(m.dim, coords.shape[0], L.ptr(coords), m.nb_nodes, self.physics, self.scheme,
                                 L.ptr(self.params), self.nb_params, C.byref(h)), "pdrd_create")
-        self._h = h
+        self._raw = h
+        self._ready = False
+        self.quad_order = max(quad_order.values()) if isinstance(quad_order, dict) else quad_order
         self.tables = {}
         for blk in m.blocks:
             q = quad_order[blk.shape] if isinstance(quad_order, dict) else quad_order
@@ -77,11 +81,22 @@ class PGRDMethodExplicit:
             L.check(lib.pdrd_set_option(h, L.OPT_SCATTER, int(scatter)), "pdrd_set_option")
         if patch_cells is not None:
             L.check(lib.pdrd_set_option(h, L.OPT_PATCH_CELLS, int(patch_cells)), "pdrd_set_option")
-        L.check(lib.pdrd_initialize(h), "pdrd_initialize")
         return self
 
+    @property
+    def _h(self):
+        """The C handle, initialised (device upload, colouring, patch plan) on
+        first use, so that weak boundary conditions can still be added after
+        initialize_work_data as in the reference's call order."""
+        if self._raw is None:
+            raise RuntimeError("initialize_work_data first")
+        if not self._ready:
+            L.check(L.lib().pdrd_initialize(self._raw), "pdrd_initialize")
+            self._ready = True
+        return self._raw
+
     def set_option(self, option: int, value: int):
-        L.check(L.lib().pdrd_set_option(self._h, option, int(value)), "pdrd_set_option")
+        L.check(L.lib().pdrd_set_option(self._raw, option, int(value)), "pdrd_set_option")
 
     def get_option(self, option: int) -> int:
         v = C.c_int(0)
@@ -89,10 +104,10 @@ class PGRDMethodExplicit:
         return v.value
 
     def __del__(self):
-        h = getattr(self, "_h", None)
+        h = getattr(self, "_raw", None)
         if h is not None and h.value:
             L.lib().pdrd_destroy(h)
-            self._h = None
+            self._raw = None
 
     def destroy(self):
         self.__del__()
@@ -114,8 +129,24 @@ class PGRDMethodExplicit:
         return a
 
     # -- boundary conditions --------------------------------------------------------------
-    def add_boundary_condition(self, bc_type: str, name: str, nodes, values):
-        """'StrongDirichlet': values [n][m] or a callable f(node_coords[n][dim]) -> [n][m]."""
+    WEAK = {"WeakWall": L.WEAK_WALL, "WeakSubInlet": L.WEAK_SUBINLET, "WeakSubOutlet": L.WEAK_SUBOUTLET}
+
+    def add_boundary_condition(self, bc_type: str, name: str, nodes, values=None, params=None):
+        """'StrongDirichlet': values [n][m] or a callable f(node_coords[n][dim]) -> [n][m].
+        'WeakWall', 'WeakSubInlet' (params: total_temperature, total_pressure),
+        'WeakSubOutlet' (params: total_pressure): Euler only, added before the
+        first use of the method's device data."""
+        if bc_type in self.WEAK:
+            nodes = np.ascontiguousarray(nodes, dtype=np.uint32)
+            params = params or {}
+            keys = {"WeakWall": [], "WeakSubInlet": ["total_temperature", "total_pressure"],
+                    "WeakSubOutlet": ["total_pressure"]}[bc_type]
+            par = np.array([float(params[k]) for k in keys] + [0.0, 0.0])
+            if self._ready:
+                raise RuntimeError("weak boundary conditions are added before the method's first use")
+            L.check(L.lib().pdrd_add_weak_bc(self._raw, self.WEAK[bc_type], nodes.size, L.ptr(nodes, C.c_uint32),
+                                             L.ptr(par), len(keys), int(self.quad_order)), "pdrd_add_weak_bc")
+            return
         if bc_type != "StrongDirichlet":
             raise NotImplementedError(f"boundary condition {bc_type} is not built on the GPU path")
         nodes = np.ascontiguousarray(nodes, dtype=np.uint32)
@@ -164,6 +195,11 @@ class PGRDMethodExplicit:
         L.check(L.lib().pdrd_method_info(self._h, L.ptr(a, C.c_int64)), "pdrd_method_info")
         return dict(nb_nodes=int(a[0]), neq=int(a[1]), nb_blocks=int(a[2]), nb_cells=int(a[3]),
                     nb_colours=int(a[4]), device_bytes=int(a[5]))
+
+    def weak_bc_info(self, k: int) -> dict:
+        a = np.zeros(3, dtype=np.int64)
+        L.check(L.lib().pdrd_weak_bc_info(self._h, int(k), L.ptr(a, C.c_int64)), "pdrd_weak_bc_info")
+        return dict(facets=int(a[0]), nodes_per_facet=int(a[1]), quadrature_points=int(a[2]))
 
     def last_timing(self):
         c = C.c_double(0)
