@@ -28,12 +28,12 @@ class OrcBlock(C.Structure):
     ]
 
 
-WEAK_WALL, WEAK_SUBINLET, WEAK_SUBOUTLET = 0, 1, 2
+WEAK_WALL, WEAK_SUBINLET, WEAK_SUBOUTLET, WEAK_FARFIELD, WEAK_SYMMETRY, WEAK_SUPERINLET, WEAK_DIRICHLET = range(7)
 
 
 class OrcWeak(C.Structure):
     _fields_ = [
-        ("type", C.c_int), ("params", C.c_double * 4), ("order", C.c_int), ("nfacet", C.c_uint32),
+        ("type", C.c_int), ("params", C.c_double * 5), ("order", C.c_int), ("nfacet", C.c_uint32),
         ("nfn", C.c_int), ("nfv", C.c_int), ("nq", C.c_int),
         ("nodes", C.POINTER(C.c_uint32)), ("verts", C.POINTER(C.c_uint32)), ("inside", C.POINTER(C.c_double)),
         ("w", C.POINTER(C.c_double)), ("V", C.POINTER(C.c_double)),
@@ -132,7 +132,7 @@ class Problem:
         self._keep += arrs
         wk = OrcWeak()
         wk.type, wk.order = bc_type, order
-        for i, v in enumerate(list(params)[:4]):
+        for i, v in enumerate(list(params)[:5]):
             wk.params[i] = v
         wk.nfacet, wk.nfn = arrs[0].shape[0], arrs[0].shape[1]
         wk.nfv, wk.nq = arrs[1].shape[1], arrs[3].size

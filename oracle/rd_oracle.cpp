@@ -633,7 +633,8 @@ int apply_weak(const orc_problem *p, const orc_weak &bc, const double *u, const 
 {
   const int m = neq_of(p->physics);
   const int dim = p->dim;
-  if (p->physics != ORC_EULER2D && p->physics != ORC_EULER3D) return -1;
+  const bool euler = (p->physics == ORC_EULER2D || p->physics == ORC_EULER3D);
+  if (!euler && bc.type != ORC_WEAK_DIRICHLET) return -1;
   const double R_gas = 287.046;
   int rc = 0;
   std::vector<double> bres(static_cast<size_t>(bc.nfn) * m);
@@ -677,11 +678,15 @@ int apply_weak(const orc_problem *p, const orc_weak &bc, const double *u, const 
       Mat R, L;
       double Dv[MAXM];
       eigen_structure(p->physics, pr, nrm, R, L, Dv);
-      double eig_max = 1.e-9;
+      // WeakFarfield keeps one running maximum from 1e-8 over its points,
+      // the others restart from 1e-9 at every point
+      double eig_max = (bc.type == ORC_WEAK_FARFIELD) ? std::max(coeff, 1.e-8) : 1.e-9;
       for (int e = 0; e < m; ++e) eig_max = std::max(std::abs(Dv[e]), eig_max);
       coeff = std::max(coeff, eig_max);
       double dF[MAXM] = {0};
-      if (bc.type == ORC_WEAK_WALL) {
+      if (bc.type == ORC_WEAK_SUPERINLET) {
+        continue;  // wave speed only: its residual is reset to zero (WeakSuperInlet.hpp)
+      } else if (bc.type == ORC_WEAK_WALL) {
         double F[3][MAXM], Fn[MAXM] = {0}, Gn[MAXM] = {0};
         flux(p, uq, pr, F);
         for (int e = 0; e < m; ++e)
@@ -704,7 +709,15 @@ int apply_weak(const orc_problem *p, const orc_weak &bc, const double *u, const 
         for (int e = 0; e < m; ++e) dF[e] = Gn[e] - Fn[e];
       } else {
         double g[MAXM];
-        if (bc.type == ORC_WEAK_SUBINLET) {
+        if (bc.type == ORC_WEAK_FARFIELD || bc.type == ORC_WEAK_DIRICHLET) {
+          for (int e = 0; e < m; ++e) g[e] = bc.params[e];
+        } else if (bc.type == ORC_WEAK_SYMMETRY) {
+          double vn = 0.0;
+          for (int d = 0; d < dim; ++d) vn += nrm[d] * pr.V[d];
+          g[0] = uq[0];
+          g[m - 1] = uq[m - 1];
+          for (int d = 0; d < dim; ++d) g[1 + d] = uq[0] * (pr.V[d] - 2. * vn * nrm[d]);
+        } else if (bc.type == ORC_WEAK_SUBINLET) {
           const double Ma2 = pr.v2 / (pr.a * pr.a);
           const double fac = 1. + 0.5 * pr.gm1 * Ma2;
           const double Tg = bc.params[0] / fac;

@@ -33,8 +33,18 @@ typedef struct orc_block
   const uint32_t *geo_conn, *sol_conn;
 } orc_block;
 
-/* Weak boundary conditions (Euler): WeakWall, WeakSubInlet, WeakSubOutlet. */
-enum { ORC_WEAK_WALL = 0, ORC_WEAK_SUBINLET = 1, ORC_WEAK_SUBOUTLET = 2 };
+/* Weak boundary conditions: WeakWall, WeakSubInlet, WeakSubOutlet,
+ * WeakFarfield, WeakSymmetry, WeakSuperInlet (Euler) and WeakDirichlet (any
+ * physics, a constant boundary state). */
+enum {
+  ORC_WEAK_WALL = 0,
+  ORC_WEAK_SUBINLET = 1,
+  ORC_WEAK_SUBOUTLET = 2,
+  ORC_WEAK_FARFIELD = 3,
+  ORC_WEAK_SYMMETRY = 4,
+  ORC_WEAK_SUPERINLET = 5,
+  ORC_WEAK_DIRICHLET = 6
+};
 
 /* One weak BC over its boundary facets. Facet rule on the reference facet:
  * w[nq] (edge [-1,1] / triangle of area 2), V[nq][nfn] the values of the
@@ -43,7 +53,8 @@ enum { ORC_WEAK_WALL = 0, ORC_WEAK_SUBINLET = 1, ORC_WEAK_SUBOUTLET = 2 };
 typedef struct orc_weak
 {
   int type;
-  double params[4]; /* SubInlet: total temperature, total pressure; SubOutlet: pressure */
+  double params[5]; /* SubInlet: total temperature, total pressure; SubOutlet: pressure;
+                       Farfield / SuperInlet / Dirichlet: the boundary state */
   int order;        /* solution order (WeakWall's relaxation coefficient) */
   uint32_t nfacet;
   int nfn, nfv, nq;

@@ -137,6 +137,8 @@ def add_oracle_weak(P, mesh, bc_type, nodes, params=(), quad_order=4):
     from oracle import pyoracle as O
     fn, fv, ins = boundary_facets(mesh, nodes)
     w, V = facet_rule(mesh.dim, fn.shape[1], quad_order + 1)
-    P.add_weak({"WeakWall": O.WEAK_WALL, "WeakSubInlet": O.WEAK_SUBINLET, "WeakSubOutlet": O.WEAK_SUBOUTLET}[bc_type],
-               params, mesh.order, fn, fv, ins, w, V)
+    kinds = {"WeakWall": O.WEAK_WALL, "WeakSubInlet": O.WEAK_SUBINLET, "WeakSubOutlet": O.WEAK_SUBOUTLET,
+             "WeakFarfield": O.WEAK_FARFIELD, "WeakSymmetry": O.WEAK_SYMMETRY, "WeakSuperInlet": O.WEAK_SUPERINLET,
+             "WeakDirichlet": O.WEAK_DIRICHLET}
+    P.add_weak(kinds[bc_type], params, mesh.order, fn, fv, ins, w, V)
     return fn

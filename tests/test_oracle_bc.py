@@ -140,3 +140,43 @@ def test_wall_3d():
     g = _state(u[0], v - (v @ n) * n, (GAMMA - 1) * (u[4] - 0.5 * u[0] * v @ v))
     dF = _flux_n(g, n) - _flux_n(u, n)
     assert np.allclose(dres.sum(0), area.sum() * dF, rtol=1e-12, atol=1e-13)
+
+
+def test_farfield_symmetry_superinlet_dirichlet():
+    mesh = rectangle(5, 4, split=2, jitter=0.1, seed=7)
+    b = mesh.boundaries
+    u = _state(1.05, (0.4, 0.12), 0.75)
+    ref = _state(1.0, (0.5, 0.0), 1.0 / GAMMA)
+    n = np.array([0.0, 1.0])  # ymax
+    Km, eig = _kminus(u, n)
+    for kind, par, g in (("WeakFarfield", tuple(ref), ref), ("WeakDirichlet", tuple(ref), ref)):
+        dres, _, _ = _boundary_part(mesh, u, [(kind, b["ymax"], par)])
+        assert np.allclose(dres.sum(0), 1.0 * Km @ (g - u), rtol=1e-11, atol=1e-13)
+    v = u[1:3] / u[0]
+    g = np.concatenate([[u[0]], u[0] * (v - 2 * (v @ n) * n), [u[3]]])
+    dres, _, _ = _boundary_part(mesh, u, [("WeakSymmetry", b["ymax"], ())])
+    assert np.allclose(dres.sum(0), 1.0 * Km @ (g - u), rtol=1e-11, atol=1e-13)
+    # the supersonic inlet only adds to the wave speed
+    dres, dws, vol = _boundary_part(mesh, u, [("WeakSuperInlet", b["ymax"], tuple(ref))])
+    assert np.abs(dres).max() == 0.0 and dws.max() > 0
+
+
+def test_scalar_weak_dirichlet():
+    """Linear advection a = (0.8, 0.6): on the inflow side x = 0 (n = (-1, 0),
+    a.n = -0.8) the correction is -0.8 (g - u) per unit length, on the
+    outflow side x = 1 it vanishes."""
+    mesh = rectangle(6, 5, split=2, jitter=0.1, seed=2)
+    b = mesh.boundaries
+    P0 = oracle_problem(mesh, O.ADVECTION2D, O.N, (0.8, 0.6))
+    P1 = oracle_problem(mesh, O.ADVECTION2D, O.N, (0.8, 0.6))
+    add_oracle_weak(P1, mesh, "WeakDirichlet", b["xmin"], (1.0,))
+    add_oracle_weak(P1, mesh, "WeakDirichlet", b["xmax"], (1.0,))
+    U = np.full((mesh.nb_nodes, 1), 0.25)
+    r0, w0, _ = P0.assemble_rhs(U)
+    r1, w1, _ = P1.assemble_rhs(U)
+    vol = P0.dual_volume()
+    d = (r1 - r0)[:, 0] * vol
+    fn_in, _, _ = boundary_facets(mesh, b["xmin"])
+    fn_out, _, _ = boundary_facets(mesh, b["xmax"])
+    assert abs(d[np.unique(fn_in)].sum() - (-0.8) * (1.0 - 0.25)) < 1e-13
+    assert np.abs(d[np.setdiff1d(np.unique(fn_out), np.unique(fn_in))]).max() < 1e-15
