@@ -163,17 +163,28 @@ int pdrd_get_field(pdrd_method_t h, int field, double *host);
 /* Device address of a field for callers that keep data resident. */
 int pdrd_field_device_ptr(pdrd_method_t h, int field, double **dptr);
 
-/* add_boundary_condition("WeakWall" | "WeakSubInlet" | "WeakSubOutlet", name,
- * boundary) of the Euler equations (src/solver/rdm/bc/WeakWall.hpp,
- * WeakSubInlet.hpp, WeakSubOutlet.hpp over WeakBC::apply, WeakBC.hpp:225-352):
- * the boundary is given by its solution nodes; its facets are the cell facets
- * on the domain boundary whose nodes all belong to it (triangles P1/P2,
- * quadrilaterals Q1, tetrahedra P1). params: WeakSubInlet total temperature,
- * total pressure; WeakSubOutlet pressure. quad_order: the method's quadrature
- * order (the facet rule has order quad_order + 1). Added before
- * pdrd_initialize; applied in every sweep after the cells, with the patch
- * assembly. */
-enum pdrd_weak_bc { PDRD_WEAK_WALL = 0, PDRD_WEAK_SUBINLET = 1, PDRD_WEAK_SUBOUTLET = 2 };
+/* add_boundary_condition("WeakWall" | "WeakSubInlet" | "WeakSubOutlet" |
+ * "WeakFarfield" | "WeakSymmetry" | "WeakSuperInlet" | "WeakDirichlet", name,
+ * boundary) (src/solver/rdm/bc/WeakWall.hpp, WeakSubInlet.hpp,
+ * WeakSubOutlet.hpp, WeakFarfield.hpp, WeakSymmetry.hpp, WeakSuperInlet.hpp,
+ * WeakDirichletBC.hpp over WeakBC::apply, WeakBC.hpp:225-352): the boundary is
+ * given by its solution nodes; its facets are the cell facets on the domain
+ * boundary whose nodes all belong to it (triangles P1/P2, quadrilaterals Q1,
+ * tetrahedra P1). WeakDirichlet applies to every physics, the others to the
+ * Euler equations. params: WeakSubInlet total temperature, total pressure;
+ * WeakSubOutlet pressure; WeakFarfield and WeakDirichlet the boundary state
+ * (m values); none for the others. quad_order: the method's quadrature order
+ * (the facet rule has order quad_order + 1). Added before pdrd_initialize;
+ * applied in every sweep after the cells, with the patch assembly. */
+enum pdrd_weak_bc {
+  PDRD_WEAK_WALL = 0,
+  PDRD_WEAK_SUBINLET = 1,
+  PDRD_WEAK_SUBOUTLET = 2,
+  PDRD_WEAK_FARFIELD = 3,
+  PDRD_WEAK_SYMMETRY = 4,
+  PDRD_WEAK_SUPERINLET = 5,
+  PDRD_WEAK_DIRICHLET = 6
+};
 int pdrd_add_weak_bc(pdrd_method_t h, int type, uint32_t n, const uint32_t *nodes, const double *params,
                      int nparams, int quad_order);
 /* After pdrd_initialize: facets, nodes per facet, facet quadrature points of

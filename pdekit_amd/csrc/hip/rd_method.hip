@@ -53,7 +53,7 @@ struct WeakSpec
 {
   int type = 0;
   std::vector<uint32_t> nodes;
-  double params[2] = {0, 0};
+  double params[5] = {0, 0, 0, 0, 0};
   int quad_order = 4;
 };
 
@@ -61,7 +61,7 @@ struct WeakDev
 {
   int type = 0, nfn = 0, nq = 0;
   uint32_t nf = 0;
-  double relax = 1.0, params[2] = {0, 0};
+  double relax = 1.0, params[5] = {0, 0, 0, 0, 0};
   uint32_t *d_nodes = nullptr, *d_rows = nullptr;
   double *d_normal = nullptr, *d_jac = nullptr, *d_w = nullptr, *d_V = nullptr;
 };
@@ -434,18 +434,26 @@ int launch_weak(pdrd_method *h, hipStream_t s)
     a.nq = w.nq;
     a.type = w.type;
     a.relax = w.relax;
-    a.params[0] = w.params[0];
-    a.params[1] = w.params[1];
-    a.gamma = h->params[0];
+    for (int i = 0; i < 5; ++i) a.params[i] = w.params[i];
+    a.pp.p0 = h->params[0];
+    a.pp.p1 = h->params[1];
     const dim3 grid((w.nf + 127) / 128);
-    if (h->dim == 2 && w.nfn == 2)
-      hipLaunchKernelGGL((pdrd::weak_bc_kernel<2, 2>), grid, dim3(128), 0, s, a);
-    else if (h->dim == 2 && w.nfn == 3)
-      hipLaunchKernelGGL((pdrd::weak_bc_kernel<2, 3>), grid, dim3(128), 0, s, a);
-    else if (h->dim == 3 && w.nfn == 3)
-      hipLaunchKernelGGL((pdrd::weak_bc_kernel<3, 3>), grid, dim3(128), 0, s, a);
-    else
-      return set_error(PDRD_ENOTSUP, "weak boundary condition on this facet kind");
+    const int key = h->physics * 8 + w.nfn;
+#define WEAK_CASE(PH, NFN)                                                                   \
+  case PH * 8 + NFN:                                                                         \
+    hipLaunchKernelGGL((pdrd::weak_bc_kernel<PH, NFN>), grid, dim3(128), 0, s, a);           \
+    break;
+    switch (key) {
+      WEAK_CASE(pdrd::PH_ADVECTION2D, 2)
+      WEAK_CASE(pdrd::PH_ADVECTION2D, 3)
+      WEAK_CASE(pdrd::PH_BURGERS2D, 2)
+      WEAK_CASE(pdrd::PH_BURGERS2D, 3)
+      WEAK_CASE(pdrd::PH_EULER2D, 2)
+      WEAK_CASE(pdrd::PH_EULER2D, 3)
+      WEAK_CASE(pdrd::PH_EULER3D, 3)
+      default: return set_error(PDRD_ENOTSUP, "weak boundary condition on this facet kind");
+    }
+#undef WEAK_CASE
     HIP_CHECK(hipGetLastError());
   }
   return PDRD_OK;
@@ -982,8 +990,7 @@ int pdrd_initialize(pdrd_method_t h)
           wd.type = ws.type;
           wd.nfn = fs.nfn;
           wd.nf = static_cast<uint32_t>(fs.size());
-          wd.params[0] = ws.params[0];
-          wd.params[1] = ws.params[1];
+          for (int i = 0; i < 5; ++i) wd.params[i] = ws.params[i];
           const int forder = (fs.nfn > fs.nfv) ? 2 : 1;
           wd.relax = (ws.type == PDRD_WEAK_WALL && forder > 1) ? 1.0 / (3.0 * forder) : 1.0;
           std::vector<uint32_t> rows(fs.nodes.size());
@@ -1156,10 +1163,15 @@ int pdrd_add_weak_bc(pdrd_method_t h, int type, uint32_t n, const uint32_t *node
   PDRD_TRY
   if (!h || (n && !nodes)) return set_error(PDRD_EINVAL, "null argument");
   if (h->ready) return set_error(PDRD_EINVAL, "weak boundary conditions are added before pdrd_initialize");
-  if (h->physics != pdrd::PH_EULER2D && h->physics != pdrd::PH_EULER3D)
-    return set_error(PDRD_ENOTSUP, "weak boundary conditions are built for the Euler equations");
-  if (type < PDRD_WEAK_WALL || type > PDRD_WEAK_SUBOUTLET) return set_error(PDRD_EINVAL, "unknown weak BC type");
-  const int need = (type == PDRD_WEAK_WALL) ? 0 : (type == PDRD_WEAK_SUBINLET ? 2 : 1);
+  if (type < PDRD_WEAK_WALL || type > PDRD_WEAK_DIRICHLET) return set_error(PDRD_EINVAL, "unknown weak BC type");
+  if (h->physics != pdrd::PH_EULER2D && h->physics != pdrd::PH_EULER3D && type != PDRD_WEAK_DIRICHLET)
+    return set_error(PDRD_ENOTSUP, "only WeakDirichlet applies to scalar physics");
+  // parameters: WeakSubInlet T_tot, p_tot; WeakSubOutlet p; WeakFarfield and
+  // WeakDirichlet the boundary state; the others none
+  int need = 0;
+  if (type == PDRD_WEAK_SUBINLET) need = 2;
+  if (type == PDRD_WEAK_SUBOUTLET) need = 1;
+  if (type == PDRD_WEAK_FARFIELD || type == PDRD_WEAK_DIRICHLET) need = h->m;
   if (nparams < need || (need && !params)) return set_error(PDRD_EINVAL, "missing weak BC parameters");
   if (quad_order < 1) return set_error(PDRD_EINVAL, "quadrature order must be >= 1");
   if (h->scatter_mode != 2) return set_error(PDRD_ENOTSUP, "weak boundary conditions run with the patch assembly only");

@@ -129,23 +129,38 @@ class PGRDMethodExplicit:
         return a
 
     # -- boundary conditions --------------------------------------------------------------
-    WEAK = {"WeakWall": L.WEAK_WALL, "WeakSubInlet": L.WEAK_SUBINLET, "WeakSubOutlet": L.WEAK_SUBOUTLET}
+    WEAK = {"WeakWall": L.WEAK_WALL, "WeakSubInlet": L.WEAK_SUBINLET, "WeakSubOutlet": L.WEAK_SUBOUTLET,
+            "WeakFarfield": L.WEAK_FARFIELD, "WeakSymmetry": L.WEAK_SYMMETRY, "WeakSuperInlet": L.WEAK_SUPERINLET,
+            "WeakDirichlet": L.WEAK_DIRICHLET}
 
     def add_boundary_condition(self, bc_type: str, name: str, nodes, values=None, params=None):
         """'StrongDirichlet': values [n][m] or a callable f(node_coords[n][dim]) -> [n][m].
-        'WeakWall', 'WeakSubInlet' (params: total_temperature, total_pressure),
-        'WeakSubOutlet' (params: total_pressure): Euler only, added before the
-        first use of the method's device data."""
+        Weak conditions, added before the first use of the method's device data:
+        'WeakWall', 'WeakSymmetry', 'WeakSuperInlet' (no parameters),
+        'WeakSubInlet' (params: total_temperature, total_pressure),
+        'WeakSubOutlet' (params: total_pressure), 'WeakFarfield' (values: the
+        reference state, m values) for the Euler equations; 'WeakDirichlet'
+        (values: the boundary state, m values) for every physics."""
         if bc_type in self.WEAK:
             nodes = np.ascontiguousarray(nodes, dtype=np.uint32)
-            params = params or {}
-            keys = {"WeakWall": [], "WeakSubInlet": ["total_temperature", "total_pressure"],
-                    "WeakSubOutlet": ["total_pressure"]}[bc_type]
-            par = np.array([float(params[k]) for k in keys] + [0.0, 0.0])
+            if bc_type in ("WeakFarfield", "WeakDirichlet"):
+                if values is None:
+                    raise ValueError(f"{bc_type} needs values: the boundary state")
+                par = np.ascontiguousarray(np.asarray(values, dtype=np.float64).reshape(-1))
+                if par.size != self.neq:
+                    raise ValueError(f"{bc_type} takes one state of {self.neq} values")
+                npar = par.size
+            else:
+                params = params or {}
+                keys = {"WeakWall": [], "WeakSymmetry": [], "WeakSuperInlet": [],
+                        "WeakSubInlet": ["total_temperature", "total_pressure"],
+                        "WeakSubOutlet": ["total_pressure"]}[bc_type]
+                par = np.array([float(params[k]) for k in keys] + [0.0, 0.0])
+                npar = len(keys)
             if self._ready:
                 raise RuntimeError("weak boundary conditions are added before the method's first use")
             L.check(L.lib().pdrd_add_weak_bc(self._raw, self.WEAK[bc_type], nodes.size, L.ptr(nodes, C.c_uint32),
-                                             L.ptr(par), len(keys), int(self.quad_order)), "pdrd_add_weak_bc")
+                                             L.ptr(par), npar, int(self.quad_order)), "pdrd_add_weak_bc")
             return
         if bc_type != "StrongDirichlet":
             raise NotImplementedError(f"boundary condition {bc_type} is not built on the GPU path")

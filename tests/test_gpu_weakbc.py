@@ -102,3 +102,79 @@ def test_bump_channel_converges():
     m.check_status()
     print(f"bump channel 48x24 LDA: residual norm {first:.3e} after 50 it -> {last:.3e} after 6050 it", flush=True)
     assert np.isfinite(last) and last < 1e-2 * first
+
+
+# WeakSymmetry, WeakFarfield, WeakSuperInlet and WeakDirichlet of the Euler
+# equations, and WeakDirichlet of the scalar physics
+def _bcs_far(mesh):
+    b = mesh.boundaries
+    far = np.array([1.0, 0.5, 0.02, 0.0, 0.0][:mesh.dim + 1] + [0.0])
+    far[-1] = P_INF / (GAMMA - 1) + 0.5 * (far[1:-1] ** 2).sum() / far[0]
+    out = [("WeakSymmetry", b["ymin"], None), ("WeakFarfield", b["ymax"], far),
+           ("WeakSuperInlet", b["xmin"], None), ("WeakDirichlet", b["xmax"], 1.02 * far)]
+    if mesh.dim == 3:
+        out += [("WeakSymmetry", b["zmin"], None), ("WeakFarfield", b["zmax"], far)]
+    return out
+
+
+@pytest.mark.parametrize("name", sorted(MESHES))
+@pytest.mark.parametrize("scheme", [L.LDA, L.N])
+def test_weak_far_bcs_match_oracle(name, scheme):
+    print(f"weak far-field BCs {name} scheme {scheme}", flush=True)
+    mesh = MESHES[name]()
+    phys = L.EULER2D if mesh.dim == 2 else L.EULER3D
+    u = euler_state(mesh, dim=mesh.dim)
+    m = PGRDMethodExplicit(phys, scheme, params=(GAMMA, 287.05))
+    q = quad_orders(mesh)
+    m.configure_mesh_data(mesh).initialize_work_data(quad_order=q)
+    P = oracle_problem(mesh, phys, scheme, (GAMMA, 287.05))
+    for kind, nodes, val in _bcs_far(mesh):
+        m.add_boundary_condition(kind, kind, nodes, values=val)
+        add_oracle_weak(P, mesh, kind, nodes, () if val is None else tuple(val), quad_order=max(q.values()))
+    m.set_vec_function("solution", u)
+    m.assemble_lhs_and_rhs(0.3)
+    m.check_status()
+    ref, ref_ws, rc = P.assemble_rhs(u)
+    assert rc == 0
+    assert rel_err(m.vec_function("residuals"), ref) <= 1e-12
+    assert rel_err(m.vec_function("wave_speed"), ref_ws) <= 1e-12
+    m.iterate(0.3, 10)
+    uo, res, _, _, rc = P.iterate(u, 0.3, 10)
+    assert rc == 0
+    assert rel_err(m.vec_function("solution"), uo) <= 1e-12
+
+
+SCALAR = {
+    "advection_p1": (lambda: rectangle(12, 10, split=2, jitter=0.1, seed=3), L.ADVECTION2D, (0.8, 0.6)),
+    "advection_mixed": (lambda: rectangle(12, 10, split=2, quad_rows=4), L.ADVECTION2D, (0.8, 0.6)),
+    "burgers_p2": (lambda: rectangle(8, 8, split=2, order=2, jitter=0.05, seed=5), L.BURGERS2D, (0.0, 0.0)),
+}
+
+
+@pytest.mark.parametrize("name", sorted(SCALAR))
+@pytest.mark.parametrize("scheme", [L.LDA, L.N])
+def test_scalar_weak_dirichlet_matches_oracle(name, scheme):
+    print(f"scalar WeakDirichlet {name} scheme {scheme}", flush=True)
+    make, phys, par = SCALAR[name]
+    mesh = make()
+    b = mesh.boundaries
+    x = mesh.node_coords
+    u = (0.5 + 0.3 * np.sin(3 * x[:, 0]) * np.cos(2 * x[:, 1]))[:, None].copy()
+    m = PGRDMethodExplicit(phys, scheme, params=par)
+    q = quad_orders(mesh)
+    m.configure_mesh_data(mesh).initialize_work_data(quad_order=q)
+    P = oracle_problem(mesh, phys, scheme, par)
+    for side, val in (("xmin", 1.0), ("ymin", -0.5), ("xmax", 0.25)):
+        m.add_boundary_condition("WeakDirichlet", side, b[side], values=[val])
+        add_oracle_weak(P, mesh, "WeakDirichlet", b[side], (val,), quad_order=max(q.values()))
+    m.set_vec_function("solution", u)
+    m.assemble_lhs_and_rhs(0.3)
+    m.check_status()
+    ref, ref_ws, rc = P.assemble_rhs(u)
+    assert rc == 0
+    assert rel_err(m.vec_function("residuals"), ref) <= 1e-12
+    assert rel_err(m.vec_function("wave_speed"), ref_ws) <= 1e-12
+    m.iterate(0.3, 10)
+    uo, res, _, _, rc = P.iterate(u, 0.3, 10)
+    assert rc == 0
+    assert rel_err(m.vec_function("solution"), uo) <= 1e-12
