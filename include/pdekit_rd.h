@@ -130,6 +130,29 @@ int pdrd_mesh_sol_conn(pdrd_meshgen_t m, uint32_t *conn, int64_t *size);
 int pdrd_mesh_boundary_name(pdrd_meshgen_t m, int idx, char *name, int cap);
 int pdrd_mesh_boundary_nodes(pdrd_meshgen_t m, const char *name, uint32_t *nodes, int64_t *n);
 
+/* ---- Gmsh 2.2 ASCII (src/mesh/io/gmsh/GmshReader.hpp, GmshReader.cpp:60-243,
+ * GmshWriter.hpp:145-330) -------------------------------------------------- */
+/* GmshReader::read_mesh_from_file: cells are the elements of the highest
+ * dimension (triangles, quadrilaterals, tetrahedra of order 1-3, file order);
+ * vertices are the file's nodes (ids 1..n); high-order cells must be
+ * straight-sided and are kept by their corners (pdrd_mesh_clone then gives
+ * the solution nodes in Gmsh's local order); boundaries are the physical
+ * groups of dimension dim - 1, named by $PhysicalNames. */
+int pdrd_mesh_read_gmsh(const char *path, pdrd_meshgen_t *out);
+/* GmshWriter::write_mesh_to_file(mesh, dofs, file) of the cloned solution
+ * nodes: $Nodes, boundary facets and cells in the element type of the
+ * solution order (P1-P3 in 2D, P1-P2 tetrahedra). */
+int pdrd_mesh_write_gmsh(pdrd_meshgen_t m, const char *path);
+/* GmshWriter::append_nodal_function_to_file: values[n][m] as m $NodeData
+ * blocks "<field>_<k>" (scalar != 0 and m == 1: "<field>"). */
+int pdrd_gmsh_append_node_data(const char *path, const char *field, int m, uint32_t n, const double *values,
+                               double time, uint32_t step, int scalar);
+/* GmshReader::read_nodal_function_from_file: the $NodeData blocks named
+ * names[f] (without quotes) fill column f of values[n][nfields]; *found =
+ * how many of the names were present. */
+int pdrd_gmsh_read_node_data(const char *path, int nfields, const char *const *names, uint32_t n, double *values,
+                             int *found);
+
 /* ---- the method --------------------------------------------------------- */
 typedef struct pdrd_method *pdrd_method_t;
 

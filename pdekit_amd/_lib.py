@@ -42,6 +42,7 @@ EXPORTED = (
     "pdrd_compute_residual_norm", "pdrd_residual_norm_L2", "pdrd_iterate", "pdrd_check_status",
     "pdrd_method_info", "pdrd_last_timing", "pdrd_last_error", "pdrd_version", "pdrd_get_option",
     "pdrd_batch_create", "pdrd_batch_destroy", "pdrd_batch_iterate", "pdrd_add_weak_bc", "pdrd_weak_bc_info",
+    "pdrd_mesh_read_gmsh", "pdrd_mesh_write_gmsh", "pdrd_gmsh_append_node_data", "pdrd_gmsh_read_node_data",
 )
 
 
@@ -92,6 +93,10 @@ def lib() -> C.CDLL:
         "pdrd_mesh_sol_conn": [P, PU32, PI64],
         "pdrd_mesh_boundary_name": [P, I, C.c_char_p, I],
         "pdrd_mesh_boundary_nodes": [P, C.c_char_p, PU32, PI64],
+        "pdrd_mesh_read_gmsh": [C.c_char_p, C.POINTER(P)],
+        "pdrd_mesh_write_gmsh": [P, C.c_char_p],
+        "pdrd_gmsh_append_node_data": [C.c_char_p, C.c_char_p, I, U32, PD, D, U32, I],
+        "pdrd_gmsh_read_node_data": [C.c_char_p, I, C.POINTER(C.c_char_p), U32, PD, PI],
         "pdrd_create": [I, U32, PD, U32, I, I, PD, I, C.POINTER(P)],
         "pdrd_add_cells": [P, C.POINTER(FeDesc), PD, U32, PU32, PU32],
         "pdrd_initialize": [P],

@@ -7,6 +7,7 @@
 #include "../../include/pdekit_rd.h"
 #include "host/capi_common.hpp"
 #include "host/fe_tables.hpp"
+#include "host/gmsh.hpp"
 #include "host/mesh.hpp"
 
 struct pdrd_meshgen
@@ -241,6 +242,58 @@ int pdrd_mesh_boundary_nodes(pdrd_meshgen_t m, const char *name, uint32_t *nodes
   const auto v = pdrd::boundary_nodes(m->geo, m->sol, name);
   if (n) *n = static_cast<int64_t>(v.size());
   if (nodes) std::memcpy(nodes, v.data(), v.size() * sizeof(uint32_t));
+  return PDRD_OK;
+  PDRD_CATCH
+}
+
+int pdrd_mesh_read_gmsh(const char *path, pdrd_meshgen_t *out)
+{
+  PDRD_TRY
+  if (!path || !out) return set_error(PDRD_EINVAL, "null argument");
+  auto *m = new pdrd_meshgen();
+  try {
+    m->geo = pdrd::read_gmsh(path);
+  } catch (...) {
+    delete m;
+    throw;
+  }
+  *out = m;
+  return PDRD_OK;
+  PDRD_CATCH
+}
+
+int pdrd_mesh_write_gmsh(pdrd_meshgen_t m, const char *path)
+{
+  PDRD_TRY
+  if (!m || !path) return set_error(PDRD_EINVAL, "null argument");
+  if (!m->cloned) return set_error(PDRD_EINVAL, "mesh not cloned");
+  pdrd::write_gmsh(path, m->geo, m->sol);
+  return PDRD_OK;
+  PDRD_CATCH
+}
+
+int pdrd_gmsh_append_node_data(const char *path, const char *field, int m, uint32_t n, const double *values,
+                               double time, uint32_t step, int scalar)
+{
+  PDRD_TRY
+  if (!path || !field || (n && !values) || m < 1) return set_error(PDRD_EINVAL, "bad argument");
+  pdrd::append_gmsh_node_data(path, field, m, n, values, time, step, scalar != 0);
+  return PDRD_OK;
+  PDRD_CATCH
+}
+
+int pdrd_gmsh_read_node_data(const char *path, int nfields, const char *const *names, uint32_t n, double *values,
+                             int *found)
+{
+  PDRD_TRY
+  if (!path || nfields < 1 || !names || (n && !values)) return set_error(PDRD_EINVAL, "bad argument");
+  std::vector<std::string> nm;
+  for (int f = 0; f < nfields; ++f) {
+    if (!names[f]) return set_error(PDRD_EINVAL, "null field name");
+    nm.emplace_back(names[f]);
+  }
+  const int c = pdrd::read_gmsh_node_data(path, nm, n, values);
+  if (found) *found = c;
   return PDRD_OK;
   PDRD_CATCH
 }

@@ -43,6 +43,7 @@ class Mesh:
     node_coords: np.ndarray       # [N][dim]
     blocks: list = field(default_factory=list)
     boundaries: dict = field(default_factory=dict)  # name -> solution node ids
+    generator: object = field(default=None, repr=False, compare=False)  # the MeshGen it was built from
 
     @property
     def nb_cells(self) -> int:
@@ -63,6 +64,19 @@ class MeshGen:
         L.check(lib.pdrd_mesh_generate(kind, L.ptr(ip, C.c_int64), L.ptr(rp), C.byref(h)), "pdrd_mesh_generate")
         self._h = h
         self._cloned = False
+
+    @classmethod
+    def from_gmsh(cls, path: str) -> "MeshGen":
+        """GmshReader::read_mesh_from_file (Gmsh 2.2 ASCII, straight-sided cells)."""
+        self = cls.__new__(cls)
+        h = C.c_void_p()
+        L.check(L.lib().pdrd_mesh_read_gmsh(str(path).encode(), C.byref(h)), "pdrd_mesh_read_gmsh")
+        self._h = h
+        self._cloned = False
+        return self
+
+    def write_gmsh(self, path: str):
+        L.check(L.lib().pdrd_mesh_write_gmsh(self._h, str(path).encode()), "pdrd_mesh_write_gmsh")
 
     def __del__(self):
         h = getattr(self, "_h", None)
@@ -134,7 +148,7 @@ class MeshGen:
                     "pdrd_mesh_boundary_nodes")
             bnd[name.value.decode()] = ids
         return Mesh(dim=dim, coords=coords, cell_shape=shapes, order=p, nb_nodes=nn, node_coords=ncoords,
-                    blocks=blocks, boundaries=bnd)
+                    blocks=blocks, boundaries=bnd, generator=self)
 
 
 def unit_quad_triag(N: int, order: int = 1, to_unit_square: bool = True) -> Mesh:
