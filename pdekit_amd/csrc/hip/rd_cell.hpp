@@ -810,12 +810,14 @@ struct PatchArgs
 
 // Largest (and default) number of cells per patch (= threads per
 // workgroup): 256 for scalar physics on elements of up to 6 nodes, 128 for
-// the Euler systems (larger per-thread state) and for P3 elements (more
-// nodes per patch). A block may use 64 or 128 instead (PDRD_OPT_PATCH_CELLS).
+// the Euler systems (larger per-thread state) and for P3 / Q2 elements (more
+// nodes per patch), 64 for the 3D Euler equations on P2 tetrahedra (whose
+// (cell, node) pair sums of 128 cells alone would fill 60 KB of LDS). A block
+// may use 64 or 128 instead (PDRD_OPT_PATCH_CELLS).
 template <int M, int ND>
 struct PatchSize
 {
-  static constexpr int value = (M == 1 && ND <= 6) ? 256 : 128;
+  static constexpr int value = (M == 1 && ND <= 6) ? 256 : ((M == 5 && ND > 4) ? 64 : 128);
 };
 
 // Waves per SIMD the register allocation of a patch kernel must leave room

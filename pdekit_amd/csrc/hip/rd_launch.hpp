@@ -32,13 +32,17 @@ struct KernelSet
 
 // Element types built: (shape, P, dim, ng, nd, nf, quadrature points, affine)
 //   triangle P1 / P4 rule (6 points), P2 / P4 rule, P3 / P6 rule (12 points),
-//   quadrilateral Q1 / P4 rule (8 points), tetrahedron P1 / P2 rule (4 points)
+//   quadrilateral Q1 and Q2 / P4 rule (8 points), tetrahedron P1 / P2 rule
+//   (4 points), tetrahedron P2 / P2 rule and P4 rule (11 points)
 #define PDRD_ELEMENT_TYPES(X)                      \
   X(0, 1, 2, 3, 3, 6, 6, true)                     \
   X(0, 2, 2, 3, 6, 10, 6, true)                    \
   X(0, 3, 2, 3, 10, 15, 12, true)                  \
   X(1, 1, 2, 4, 4, 9, 8, false)                    \
-  X(2, 1, 3, 4, 4, 10, 4, true)
+  X(1, 2, 2, 4, 9, 16, 8, false)                   \
+  X(2, 1, 3, 4, 4, 10, 4, true)                    \
+  X(2, 2, 3, 4, 10, 20, 4, true)                   \
+  X(2, 2, 3, 4, 10, 20, 11, true)
 
 template <int D, int NG, int ND, int NF, int NQ, bool AFF, int PH, int SC>
 hipError_t launch_cell(const CellArgs &a, unsigned grid, unsigned block, hipStream_t s)

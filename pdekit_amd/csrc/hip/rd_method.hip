@@ -451,6 +451,7 @@ int launch_weak(pdrd_method *h, hipStream_t s)
       WEAK_CASE(pdrd::PH_EULER2D, 2)
       WEAK_CASE(pdrd::PH_EULER2D, 3)
       WEAK_CASE(pdrd::PH_EULER3D, 3)
+      WEAK_CASE(pdrd::PH_EULER3D, 6)
       default: return set_error(PDRD_ENOTSUP, "weak boundary condition on this facet kind");
     }
 #undef WEAK_CASE

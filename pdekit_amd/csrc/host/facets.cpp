@@ -16,27 +16,45 @@ struct LocalFacet
 {
   int nv;        // vertices
   int v[3];      // local vertex indices
-  int mid;       // local index of the P2 mid-edge node, or -1
+  int nm;        // P2 mid-edge nodes on the facet
+  int mid[3];    // their local indices: edge (v0,v1), then (v1,v2), (v2,v0)
 };
 
 // Local facets of the reference cells in the node order of reference_nodes:
-// triangle edges (0,1), (1,2), (2,0) with P2 mid nodes 3, 4, 5; quadrilateral
-// edges (0,1), (1,2), (2,3), (3,0); tetrahedron faces opposite each vertex.
+// triangle edges (0,1), (1,2), (2,0) with P2 mid nodes 3, 4, 5;
+// quadrilateral edges (0,1), (1,2), (2,3), (3,0) with Q2 mid nodes 4..7;
+// tetrahedron faces opposite each vertex, whose P2 mid nodes are those of the
+// edges (0,1), (1,2), (2,0), (3,0), (3,2), (3,1) (local 4..9).
 std::vector<LocalFacet> local_facets(int shape, int order)
 {
+  if (order > 2) throw std::invalid_argument("weak boundary conditions: cells of order 1 or 2");
   if (shape == 0) {
-    if (order > 2) throw std::invalid_argument("weak boundary conditions: triangles of order 1 or 2");
-    std::vector<LocalFacet> f = {{2, {0, 1, 0}, -1}, {2, {1, 2, 0}, -1}, {2, {2, 0, 0}, -1}};
+    std::vector<LocalFacet> f = {{2, {0, 1, 0}, 0, {0}}, {2, {1, 2, 0}, 0, {0}}, {2, {2, 0, 0}, 0, {0}}};
     if (order == 2)
-      for (int e = 0; e < 3; ++e) f[e].mid = 3 + e;
+      for (int e = 0; e < 3; ++e) { f[e].nm = 1; f[e].mid[0] = 3 + e; }
     return f;
   }
   if (shape == 1) {
-    if (order != 1) throw std::invalid_argument("weak boundary conditions: quadrilaterals of order 1");
-    return {{2, {0, 1, 0}, -1}, {2, {1, 2, 0}, -1}, {2, {2, 3, 0}, -1}, {2, {3, 0, 0}, -1}};
+    std::vector<LocalFacet> f = {{2, {0, 1, 0}, 0, {0}}, {2, {1, 2, 0}, 0, {0}}, {2, {2, 3, 0}, 0, {0}},
+                                 {2, {3, 0, 0}, 0, {0}}};
+    if (order == 2)
+      for (int e = 0; e < 4; ++e) { f[e].nm = 1; f[e].mid[0] = 4 + e; }
+    return f;
   }
-  if (order != 1) throw std::invalid_argument("weak boundary conditions: tetrahedra of order 1");
-  return {{3, {1, 2, 3}, -1}, {3, {0, 2, 3}, -1}, {3, {0, 1, 3}, -1}, {3, {0, 1, 2}, -1}};
+  std::vector<LocalFacet> f = {{3, {1, 2, 3}, 0, {0}}, {3, {0, 2, 3}, 0, {0}}, {3, {0, 1, 3}, 0, {0}},
+                               {3, {0, 1, 2}, 0, {0}}};
+  if (order == 2) {
+    const int e[6][2] = {{0, 1}, {1, 2}, {2, 0}, {3, 0}, {3, 2}, {3, 1}};
+    for (LocalFacet &lf : f) {
+      lf.nm = 3;
+      for (int k = 0; k < 3; ++k) {
+        const int a = lf.v[k], b = lf.v[(k + 1) % 3];
+        for (int j = 0; j < 6; ++j)
+          if ((e[j][0] == a && e[j][1] == b) || (e[j][0] == b && e[j][1] == a)) lf.mid[k] = 4 + j;
+      }
+    }
+  }
+  return f;
 }
 
 // Gauss-Legendre nodes and weights on [-1, 1] (Newton on P_n in long double).
@@ -94,10 +112,10 @@ FacetSet boundary_facets(int dim, const double *coords, const std::vector<FacetB
       const uint32_t *cs = b.sol + static_cast<size_t>(c) * b.nd;
       for (const LocalFacet &lf : lfs) {
         if (count[key_of(cv, lf)] != 1) continue;
-        const int nfn = lf.nv + (lf.mid >= 0 ? 1 : 0);
-        uint32_t fn[4];
+        const int nfn = lf.nv + lf.nm;
+        uint32_t fn[6];
         for (int i = 0; i < lf.nv; ++i) fn[i] = cs[lf.v[i]];
-        if (lf.mid >= 0) fn[lf.nv] = cs[lf.mid];
+        for (int i = 0; i < lf.nm; ++i) fn[lf.nv + i] = cs[lf.mid[i]];
         bool all = true;
         for (int i = 0; i < nfn; ++i) all = all && in_set[fn[i]];
         if (!all) continue;
@@ -168,9 +186,20 @@ void facet_tables(int dim, int nfn, int order, std::vector<double> &w, std::vect
   V.clear();
   for (size_t q = 0; q < w.size(); ++q) {
     const double a = xi[2 * q], b = xi[2 * q + 1];
-    V.push_back(-0.5 * (a + b));
-    V.push_back(0.5 * (1.0 + a));
-    V.push_back(0.5 * (1.0 + b));
+    const double L0 = -0.5 * (a + b), L1 = 0.5 * (1.0 + a), L2 = 0.5 * (1.0 + b);
+    if (nfn == 3) {
+      V.push_back(L0);
+      V.push_back(L1);
+      V.push_back(L2);
+    } else {
+      // P2 face: vertices, then the mid nodes of (0,1), (1,2), (2,0)
+      V.push_back(L0 * (2.0 * L0 - 1.0));
+      V.push_back(L1 * (2.0 * L1 - 1.0));
+      V.push_back(L2 * (2.0 * L2 - 1.0));
+      V.push_back(4.0 * L0 * L1);
+      V.push_back(4.0 * L1 * L2);
+      V.push_back(4.0 * L2 * L0);
+    }
   }
 }
 

@@ -26,7 +26,7 @@ struct FacetSet
 {
   int nfn = 0;                   // solution nodes per facet
   int nfv = 0;                   // vertices per facet
-  std::vector<uint32_t> nodes;   // [nf][nfn]: vertices first, then (P2 edge) the mid node
+  std::vector<uint32_t> nodes;   // [nf][nfn]: vertices first, then the P2 mid-edge nodes
   std::vector<uint32_t> verts;   // [nf][nfv]
   std::vector<double> normal;    // [nf][dim] outward unit normal
   std::vector<double> jac;       // [nf] |dx/dxi| of the facet map (straight facets)
@@ -34,9 +34,9 @@ struct FacetSet
 };
 
 // Facets on the domain boundary (held by one cell only) whose solution nodes
-// all lie in `in_set` (indexed by solution node). Supported cells: triangles
-// P1 / P2, quadrilaterals Q1, tetrahedra P1; throws std::invalid_argument for
-// others or for mixed facet kinds.
+// all lie in `in_set` (indexed by solution node). Supported cells: triangles,
+// quadrilaterals and tetrahedra of order 1 or 2; throws std::invalid_argument
+// for others or for mixed facet kinds.
 FacetSet boundary_facets(int dim, const double *coords, const std::vector<FacetBlockIn> &blocks,
                          const std::vector<char> &in_set);
 
@@ -44,7 +44,8 @@ FacetSet boundary_facets(int dim, const double *coords, const std::vector<FacetB
 // (max(2, order / 2 + 1) points, QdPointSetLineGauss.cpp) or the triangle
 // Gauss rule (reference triangle, weights sum to 2) for faces; V[q][n] the
 // Lagrange values of the facet's nodes (line P1: -1, 1; line P2: -1, 1, 0;
-// triangle P1: the reference vertices).
+// triangle P1: the reference vertices; triangle P2: vertices, then the mid
+// nodes of edges (0,1), (1,2), (2,0)).
 void facet_tables(int dim, int nfn, int order, std::vector<double> &w, std::vector<double> &V);
 
 }  // namespace pdrd

@@ -203,24 +203,26 @@ std::vector<double> quad_nodes(int p, PointSet ps)
   const int n = (p + 1) * (p + 1);
   std::vector<double> c(2 * n);
   if (p == 0) { c[0] = 0; c[1] = 0; return c; }
-  if (p > 2 && ps == PointSet::Warpblend) fail("warp-blend quadrilateral nodes above Q2 are not built");
-  // positions along an edge
-  std::vector<double> t(p - 1 > 0 ? p - 1 : 0);
-  for (int i = 1; i < p; ++i) t[i - 1] = -1. + i * (2. / p);
+  // 1D positions: equidistant, or for warp-blend from Q3 on the
+  // Gauss-Lobatto-Legendre points (WarpblendStdRegionQuad.cpp:79-110)
+  std::vector<double> pt(p + 1);
+  for (int i = 0; i <= p; ++i) pt[i] = -1. + i * (2. / p);
+  if (ps == PointSet::Warpblend && p >= 3) {
+    const std::vector<double> g = gll_interior(p);
+    for (int i = 1; i < p; ++i) pt[i] = (std::fabs(g[i - 1]) < 1e-16) ? 0.0 : g[i - 1];
+  }
   int in = 0;
-  auto put = [&](double x, double y) { c[2 * in] = x; c[2 * in + 1] = y; ++in; };
-  // peeled squares from outside in
-  double lo = -1., hi = 1.;
-  int cur = p;
-  const double h = 2. / p;
+  auto put = [&](int i, int j) { c[2 * in] = pt[i]; c[2 * in + 1] = pt[j]; ++in; };
+  // peeled squares from outside in: corners, then the nodes of each edge
+  int lo = 0, hi = p, cur = p;
   while (cur >= 0) {
     if (cur == 0) { put(lo, lo); break; }
     put(lo, lo); put(hi, lo); put(hi, hi); put(lo, hi);
-    for (int i = 1; i < cur; ++i) put(lo + i * h, lo);
-    for (int i = 1; i < cur; ++i) put(hi, lo + i * h);
-    for (int i = 1; i < cur; ++i) put(hi - i * h, hi);
-    for (int i = 1; i < cur; ++i) put(lo, hi - i * h);
-    lo += h; hi -= h; cur -= 2;
+    for (int i = 1; i < cur; ++i) put(lo + i, lo);
+    for (int i = 1; i < cur; ++i) put(hi, lo + i);
+    for (int i = 1; i < cur; ++i) put(hi - i, hi);
+    for (int i = 1; i < cur; ++i) put(lo, hi - i);
+    ++lo; --hi; cur -= 2;
   }
   if (in != n) fail("quad lattice size mismatch");
   return c;
@@ -228,8 +230,7 @@ std::vector<double> quad_nodes(int p, PointSet ps)
 
 std::vector<double> tetra_nodes(int p, PointSet ps)
 {
-  if (p > 2) fail("tetrahedral nodes above P2 are not built");
-  (void)ps;
+  if (p > 3) fail("tetrahedral nodes above P3 are not built");
   const int n = (p + 1) * (p + 2) * (p + 3) / 6;
   std::vector<double> c(3 * n);
   if (p == 0) { c[0] = c[1] = c[2] = -0.5; return c; }
@@ -239,13 +240,38 @@ std::vector<double> tetra_nodes(int p, PointSet ps)
     for (int d = 0; d < 3; ++d) c[3 * in + d] = v[k][d];
     ++in;
   }
-  if (p == 2) {
-    const int e[6][2] = {{0, 1}, {1, 2}, {2, 0}, {3, 0}, {3, 2}, {3, 1}};
-    for (int k = 0; k < 6; ++k) {
-      for (int d = 0; d < 3; ++d) c[3 * in + d] = 0.5 * (v[e[k][0]][d] + v[e[k][1]][d]);
+  // edge nodes from each edge's first vertex to its second
+  // (EquidistStdRegionTetra.cpp:589-646): fractions k / p, or for warp-blend
+  // the Gauss-Lobatto-Legendre points, where the warp & blend construction
+  // (WarpblendStdRegionTetra.cpp:88-386) puts the nodes of every edge
+  std::vector<double> frac(p > 1 ? p - 1 : 0);
+  for (int k = 1; k < p; ++k) frac[k - 1] = static_cast<double>(k) / p;
+  if (ps == PointSet::Warpblend && p >= 3) {
+    const std::vector<double> g = gll_interior(p);
+    for (int k = 1; k < p; ++k) frac[k - 1] = 0.5 * (1.0 + g[k - 1]);
+  }
+  const int e[6][2] = {{0, 1}, {1, 2}, {2, 0}, {3, 0}, {3, 2}, {3, 1}};
+  for (int k = 0; k < 6; ++k)
+    for (int i = 0; i < p - 1; ++i) {
+      for (int d = 0; d < 3; ++d) {
+        const double a = v[e[k][0]][d], b = v[e[k][1]][d];
+        c[3 * in + d] = (p == 2) ? 0.5 * (a + b) : a + frac[i] * (b - a);
+      }
+      ++in;
+    }
+  if (p == 3) {
+    // one node per face at its centroid, faces (0,2,1), (0,1,3), (0,3,2),
+    // (3,1,2) (EquidistStdRegionTetra.cpp:648-1020); the warp & blend
+    // construction leaves face centroids in place at P3 (its face warp is
+    // symmetric under the face's rotations and the other faces' blends
+    // vanish there)
+    const int f[4][3] = {{0, 2, 1}, {0, 1, 3}, {0, 3, 2}, {3, 1, 2}};
+    for (int k = 0; k < 4; ++k) {
+      for (int d = 0; d < 3; ++d) c[3 * in + d] = (v[f[k][0]][d] + v[f[k][1]][d] + v[f[k][2]][d]) / 3.0;
       ++in;
     }
   }
+  if (in != n) fail("tetra lattice size mismatch");
   return c;
 }
 
@@ -381,6 +407,17 @@ void quadrature(Shape s, int order, std::vector<double> &xi, std::vector<double>
         add3(-2. / 3., 0.0, -2. / 3., 0.6);
         add3(0.0, -2. / 3., -2. / 3., 0.6);
         return;
+      case 4: {
+        // the 11-point rule with a negative centre weight, at the
+        // reference's 10 digits (QdPointSetTetraGauss.cpp:268-331)
+        const double a = -0.8571428571, b = 0.5714285714, c = -0.2011928476, d = -0.7988071523;
+        const double w1 = 0.0609777777, w2 = 0.1991111111;
+        add3(-0.5, -0.5, -0.5, -0.1052444444);
+        add3(a, a, a, w1); add3(a, a, b, w1); add3(a, b, a, w1); add3(b, a, a, w1);
+        add3(c, c, d, w2); add3(c, d, c, w2); add3(d, c, c, w2);
+        add3(c, d, d, w2); add3(d, c, d, w2); add3(d, d, c, w2);
+        return;
+      }
       default: fail("tetrahedral quadrature order not tabulated");
     }
   }

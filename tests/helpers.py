@@ -77,6 +77,15 @@ CASES = {
     "euler3d_p1": (lambda: box_tetra(5, 5, 4, extent=(1.0, 1.0, 0.5), jitter=0.1, seed=4,
                                      bump=(0.5, 0.08, 25.0, 0.5, 0.5)), L.EULER3D, (1.4, 287.05),
                    lambda m: euler_state(m, dim=3)),
+    "adv_q2": (lambda: rectangle(8, 7, split=1, order=2, jitter=0.1, seed=3), L.ADVECTION2D, (0.8, 0.6),
+               advection_state),
+    "burgers_q2_mixed": (lambda: rectangle(10, 10, split=2, quad_rows=4, order=2, jitter=0.1, seed=6), L.BURGERS2D,
+                         (), burgers_state),
+    "euler2d_q2_mixed": (lambda: rectangle(12, 6, extent=(-0.5, 1.5, 0.0, 1.0), split=2, quad_rows=3, order=2,
+                                           bump=(1.0, 0.1, 25.0, 0.5)), L.EULER2D, (1.4, 287.05), euler_state),
+    "euler3d_p2": (lambda: box_tetra(3, 3, 2, extent=(1.0, 1.0, 0.5), order=2, jitter=0.1, seed=4,
+                                     bump=(0.5, 0.08, 25.0, 0.5, 0.5)), L.EULER3D, (1.4, 287.05),
+                   lambda m: euler_state(m, dim=3)),
 }
 
 
@@ -86,6 +95,9 @@ CASES = {
 # ---------------------------------------------------------------------------
 _LOCAL_FACETS = {L.TRIAG: [(0, 1), (1, 2), (2, 0)], L.QUAD: [(0, 1), (1, 2), (2, 3), (3, 0)],
                  L.TETRA: [(1, 2, 3), (0, 2, 3), (0, 1, 3), (0, 1, 2)]}
+
+
+_TET_EDGES = [(0, 1), (1, 2), (2, 0), (3, 0), (3, 2), (3, 1)]
 
 
 def boundary_facets(mesh, nodes):
@@ -109,6 +121,12 @@ def boundary_facets(mesh, nodes):
                 nd = [int(cs[i]) for i in lf]
                 if mesh.order == 2 and b.shape == L.TRIAG:
                     nd.append(int(cs[3 + e]))
+                if mesh.order == 2 and b.shape == L.QUAD:
+                    nd.append(int(cs[4 + e]))
+                if mesh.order == 2 and b.shape == L.TETRA:
+                    for k in range(3):
+                        pair = {lf[k], lf[(k + 1) % 3]}
+                        nd.append(int(cs[4 + [set(t) for t in _TET_EDGES].index(pair)]))
                 if not in_set[nd].all():
                     continue
                 fn.append(nd)
@@ -130,7 +148,14 @@ def facet_rule(dim, nfn, order):
     from pdekit_amd.fe import quadrature
     xi, w = quadrature(L.TRIAG, order)
     a, b = xi[:, 0], xi[:, 1]
-    return w, np.stack([-0.5 * (a + b), 0.5 * (1 + a), 0.5 * (1 + b)], 1)
+    lam = np.stack([-0.5 * (a + b), 0.5 * (1 + a), 0.5 * (1 + b)], 1)
+    if nfn == 3:
+        return w, lam
+    # P2 face: the quadratic Lagrange basis by solving on its six nodes
+    nodes = np.array([[1, 0, 0], [0, 1, 0], [0, 0, 1], [.5, .5, 0], [0, .5, .5], [.5, 0, .5]])
+    mono = lambda l: np.stack([l[:, 0] ** 2, l[:, 1] ** 2, l[:, 2] ** 2, l[:, 0] * l[:, 1], l[:, 1] * l[:, 2],
+                               l[:, 2] * l[:, 0]], 1)
+    return w, mono(lam) @ np.linalg.inv(mono(nodes))
 
 
 def add_oracle_weak(P, mesh, bc_type, nodes, params=(), quad_order=4):

@@ -66,12 +66,12 @@ def test_quad_rules(order):
             assert abs(np.sum(w * xi[:, 0] ** a * xi[:, 1] ** b) - exact) < 1e-13
 
 
-@pytest.mark.parametrize("order", [1, 2, 3])
+@pytest.mark.parametrize("order", [1, 2, 3, 4])
 def test_tetra_rules(order):
     xi, w = quadrature(L.TETRA, order)
     assert abs(w.sum() - 4.0 / 3.0) < 3e-8
-    # the tabulated P2 points carry 10 significant digits
-    tol = 1e-9 if order == 2 else 1e-13
+    # the tabulated P2 and P4 points carry 10 significant digits
+    tol = 1e-9 if order in (2, 4) else 1e-13
     for a in range(order + 1):
         for b in range(order + 1 - a):
             for c in range(order + 1 - a - b):
@@ -145,7 +145,8 @@ def _numpy_lagrange(shape, p, nodes, pts):
     return Vq @ inv, [d @ inv for d in dVq]
 
 
-CASES = [(L.TRIAG, 1, 4), (L.TRIAG, 2, 4), (L.TRIAG, 3, 6), (L.QUAD, 1, 4), (L.TETRA, 1, 2)]
+CASES = [(L.TRIAG, 1, 4), (L.TRIAG, 2, 4), (L.TRIAG, 3, 6), (L.QUAD, 1, 4), (L.QUAD, 2, 4), (L.TETRA, 1, 2),
+         (L.TETRA, 2, 2), (L.TETRA, 2, 4)]
 
 
 @pytest.mark.parametrize("shape,p,q", CASES)
@@ -209,3 +210,109 @@ def test_node_orders():
     edges = [(0, 1), (1, 2), (2, 0), (3, 0), (3, 2), (3, 1)]
     assert np.allclose(t[:4], v)
     assert np.allclose(t[4:], [(v[a] + v[b]) / 2 for a, b in edges])
+
+
+def test_quad_warpblend_q3_nodes():
+    """Q3 warp-blend (the flux nodes of Q2): the peeled lattice on the
+    Gauss-Lobatto points -1, -1/sqrt(5), 1/sqrt(5), 1 (WarpblendStdRegionQuad.cpp:79-256);
+    Q2 and below equal the equidistant lattice."""
+    g = [-1.0, -1 / np.sqrt(5), 1 / np.sqrt(5), 1.0]
+    expect = [(0, 0), (3, 0), (3, 3), (0, 3), (1, 0), (2, 0), (3, 1), (3, 2), (2, 3), (1, 3), (0, 2), (0, 1),
+              (1, 1), (2, 1), (2, 2), (1, 2)]
+    x = reference_nodes(L.QUAD, 3, L.WARPBLEND)
+    assert np.abs(x - np.array([[g[i], g[j]] for i, j in expect])).max() < 1e-15
+    e = reference_nodes(L.QUAD, 3, L.EQUIDIST)
+    assert np.abs(e - np.array([[-1 + 2 * i / 3, -1 + 2 * j / 3] for i, j in expect])).max() < 1e-15
+    assert np.allclose(reference_nodes(L.QUAD, 2, L.EQUIDIST), reference_nodes(L.QUAD, 2, L.WARPBLEND))
+
+
+def _gll(p):
+    x = np.polynomial.legendre.Legendre.basis(p).deriv().roots()
+    return np.concatenate([[-1.0], np.sort(x.real), [1.0]])
+
+
+def _warp(p, rout):
+    """1D warp of the published 3D construction (evalwarp): the equidistant
+    Lagrange interpolant of (LGL - equidistant) with the end-point factors of
+    each basis function left out, i.e. the displacement divided by 1 - r^2,
+    pointwise."""
+    req = np.linspace(-1, 1, p + 1)
+    rlgl = _gll(p)
+    warp = np.zeros_like(rout)
+    for i in range(p + 1):
+        d = np.full_like(rout, rlgl[i] - req[i])
+        for j in range(1, p):
+            if j != i:
+                d = d * (rout - req[j]) / (req[i] - req[j])
+        if i != 0:
+            d = -d / (req[i] - req[0])
+        if i != p:
+            d = d / (req[i] - req[p])
+        warp += d
+    return warp
+
+
+def _warp_shift_face(p, alpha, L1, L2, L3):
+    b1, b2, b3 = L2 * L3, L1 * L3, L1 * L2
+    w1 = 4 * b1 * _warp(p, L3 - L2) * (1 + (alpha * L1) ** 2)
+    w2 = 4 * b2 * _warp(p, L1 - L3) * (1 + (alpha * L2) ** 2)
+    w3 = 4 * b3 * _warp(p, L2 - L1) * (1 + (alpha * L3) ** 2)
+    c, s = np.cos(2 * np.pi / 3), np.sin(2 * np.pi / 3)
+    return w1 + c * w2 + c * w3, s * w2 - s * w3
+
+
+def _nodes3d(x, p, alpha):
+    """Warp & blend nodes of the tetrahedron (Hesthaven and Warburton, Nodal
+    Discontinuous Galerkin Methods, 2008, section 10.1), applied pointwise to
+    the equidistant nodes x [n][3] of the reference tetrahedron."""
+    r, s, t = x.T
+    L1, L2, L3, L4 = (1 + t) / 2, (1 + s) / 2, -(1 + r + s + t) / 2, (1 + r) / 2
+    v1 = np.array([-1, -1 / np.sqrt(3), -1 / np.sqrt(6)])
+    v2 = np.array([1, -1 / np.sqrt(3), -1 / np.sqrt(6)])
+    v3 = np.array([0, 2 / np.sqrt(3), -1 / np.sqrt(6)])
+    v4 = np.array([0, 0, 3 / np.sqrt(6)])
+    t1 = np.array([v2 - v1, v2 - v1, v3 - v2, v3 - v1])
+    t2 = np.array([v3 - (v1 + v2) / 2, v4 - (v1 + v2) / 2, v4 - (v2 + v3) / 2, v4 - (v1 + v3) / 2])
+    t1 /= np.linalg.norm(t1, axis=1)[:, None]
+    t2 /= np.linalg.norm(t2, axis=1)[:, None]
+    XYZ = np.outer(L3, v1) + np.outer(L4, v2) + np.outer(L2, v3) + np.outer(L1, v4)
+    shift = np.zeros_like(XYZ)
+    tol = 1e-10
+    for f, (La, Lb, Lc, Ld) in enumerate([(L1, L2, L3, L4), (L2, L1, L3, L4), (L3, L1, L4, L2), (L4, L1, L3, L2)]):
+        w1, w2 = _warp_shift_face(p, alpha, Lb, Lc, Ld)
+        blend = Lb * Lc * Ld
+        denom = (Lb + La / 2) * (Lc + La / 2) * (Ld + La / 2)
+        ok = denom > tol
+        blend[ok] = (1 + (alpha * La[ok]) ** 2) * blend[ok] / denom[ok]
+        shift += np.outer(blend * w1, t1[f]) + np.outer(blend * w2, t2[f])
+        fix = (La < tol) & (((Lb > tol).astype(int) + (Lc > tol) + (Ld > tol)) < 3)
+        shift[fix] = np.outer(w1[fix], t1[f]) + np.outer(w2[fix], t2[f])
+    XYZ += shift
+    # back to the reference tetrahedron through barycentric coordinates
+    M = np.stack([v1 - v4, v2 - v4, v3 - v4], 1)
+    lam = np.linalg.solve(M, (XYZ - v4).T).T
+    ref = np.array([[-1, -1, -1], [1, -1, -1], [-1, 1, -1], [-1, -1, 1]], float)
+    return lam[:, :1] * ref[0] + lam[:, 1:2] * ref[1] + lam[:, 2:3] * ref[2] + (1 - lam.sum(1))[:, None] * ref[3]
+
+
+def test_tetra_p3_nodes():
+    """P3 tetrahedron (the flux nodes of P2 tetrahedra): equidistant nodes in
+    the order vertices, edges (0,1), (1,2), (2,0), (3,0), (3,2), (3,1) from
+    first to second vertex, faces (0,2,1), (0,1,3), (0,3,2), (3,1,2)
+    (EquidistStdRegionTetra.cpp:522-1020); warp-blend nodes against an
+    independent restatement of the published construction with the
+    reference's alpha for P3 (0.0, WarpblendStdRegionTetra.cpp:22-24)."""
+    v = np.array([[-1, -1, -1], [1, -1, -1], [-1, 1, -1], [-1, -1, 1]], float)
+    expect = list(v)
+    for a, b in [(0, 1), (1, 2), (2, 0), (3, 0), (3, 2), (3, 1)]:
+        for k in (1, 2):
+            expect.append(v[a] + k / 3 * (v[b] - v[a]))
+    for f in [(0, 2, 1), (0, 1, 3), (0, 3, 2), (3, 1, 2)]:
+        expect.append(v[list(f)].mean(0))
+    e = reference_nodes(L.TETRA, 3, L.EQUIDIST)
+    assert np.abs(e - np.array(expect)).max() < 1e-15
+    w = reference_nodes(L.TETRA, 3, L.WARPBLEND)
+    assert np.abs(w - _nodes3d(e, 3, 0.0)).max() < 1e-14
+    # and the construction is not the identity: the edge nodes move to LGL
+    assert np.abs(w - e).max() > 0.1
+    assert np.allclose(reference_nodes(L.TETRA, 2, L.EQUIDIST), reference_nodes(L.TETRA, 2, L.WARPBLEND))

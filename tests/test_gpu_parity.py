@@ -133,3 +133,21 @@ def test_negative_pressure_is_reported():
         m.check_status()
     assert "pressure" in str(ei.value)
     m.destroy()
+
+
+@pytest.mark.parametrize("scheme", SCHEMES)
+def test_p2_tetrahedra_with_the_p4_rule(scheme):
+    """P2 tetrahedra with the 11-point rule of order 4 (the other instantiated
+    rule of the element, besides the 4-point rule of the parity cases)."""
+    mesh, physics, params, u = _case("euler3d_p2")
+    q = {L.TETRA: 4}
+    m = PGRDMethodExplicit(physics, scheme, params=params)
+    m.configure_mesh_data(mesh).initialize_work_data(quad_order=q)
+    m.set_vec_function("solution", u)
+    m.assemble_lhs_and_rhs(0.5)
+    m.check_status()
+    ref, ref_ws, rc = oracle_problem(mesh, physics, scheme, params, qorders=q).assemble_rhs(u)
+    assert rc == 0
+    assert rel_err(m.vec_function("residuals"), ref) <= TOL
+    assert rel_err(m.vec_function("wave_speed"), ref_ws) <= TOL
+    m.destroy()

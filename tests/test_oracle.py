@@ -22,7 +22,7 @@ import pytest
 
 from oracle import pyoracle as O
 from pdekit_amd import _lib as L
-from pdekit_amd.mesh import rectangle, unit_quad_triag
+from pdekit_amd.mesh import box_tetra, rectangle, unit_quad_triag
 from tests.helpers import euler_state, oracle_problem
 
 A = (0.8, 0.6)
@@ -92,11 +92,19 @@ def test_burgers_p1_conservation_is_exact(scheme):
     assert np.abs(phi[:, :, 0].sum(1) - exact).max() <= 1e-13 * np.abs(exact).max()
 
 
-@pytest.mark.parametrize("name", ["p2", "q1", "euler"])
+@pytest.mark.parametrize("name", ["p2", "q1", "q2", "euler", "euler3d_p2"])
 def test_schemes_share_the_element_total(name):
     """The N / B dissipation and the PSI limiter redistribute without changing
     the element total, which equals the LDA total."""
-    if name == "p2":
+    if name == "q2":
+        mesh = rectangle(4, 3, split=1, order=2, jitter=0.15, seed=5)
+        phys, par = O.BURGERS2D, ()
+        u = np.random.default_rng(3).uniform(0.2, 1.5, (mesh.nb_nodes, 1))
+    elif name == "euler3d_p2":
+        mesh = box_tetra(2, 2, 2, order=2, jitter=0.1, seed=4)
+        phys, par = O.EULER3D, (1.4, 287.05)
+        u = euler_state(mesh, dim=3)
+    elif name == "p2":
         mesh = rectangle(5, 4, split=2, order=2, jitter=0.15, seed=5)
         phys, par = O.BURGERS2D, ()
         u = np.random.default_rng(3).uniform(0.2, 1.5, (mesh.nb_nodes, 1))
@@ -119,8 +127,11 @@ def test_schemes_share_the_element_total(name):
         assert np.abs(t - tot[0]).max() <= 1e-12 * scale
 
 
-def test_linearity_preservation():
-    mesh = rectangle(8, 7, split=2, jitter=0.2, seed=8, order=2)
+@pytest.mark.parametrize("quads", [False, True])
+def test_linearity_preservation(quads):
+    """P2 triangles, or Q2 quadrilaterals on bilinear cells: a linear exact
+    solution of a . grad u = 0 is represented exactly and LDA / B leave it."""
+    mesh = rectangle(8, 7, split=1 if quads else 2, jitter=0.2, seed=8, order=2)
     x, y = mesh.node_coords.T
     u = 0.3 + (-A[1] * x + A[0] * y)  # a . grad u = 0
     for sch in (O.LDA, O.B):
@@ -186,11 +197,20 @@ def test_small_inverses(n):
         assert np.abs(a @ inv - np.eye(n)).max() < 1e-12
 
 
+@pytest.mark.parametrize("kind", ["mixed_p1", "mixed_p2_q2", "tet_p2"])
 @pytest.mark.parametrize("scheme", [O.LDA, O.N, O.B, O.PSI])
-def test_euler_free_stream_is_preserved(scheme):
-    mesh = rectangle(8, 6, extent=(-0.5, 1.5, 0, 1), split=2, quad_rows=3, bump=(1.0, 0.1, 25.0, 0.5))
-    u = euler_state(mesh, pert=0.0)
-    P = oracle_problem(mesh, O.EULER2D, scheme, (1.4, 287.05))
+def test_euler_free_stream_is_preserved(scheme, kind):
+    if kind == "tet_p2":
+        mesh = box_tetra(3, 3, 2, extent=(1.0, 1.0, 0.5), order=2, jitter=0.1, seed=4,
+                         bump=(0.5, 0.08, 25.0, 0.5, 0.5))
+        u = euler_state(mesh, pert=0.0, dim=3)
+        phys = O.EULER3D
+    else:
+        mesh = rectangle(8, 6, extent=(-0.5, 1.5, 0, 1), split=2, quad_rows=3, bump=(1.0, 0.1, 25.0, 0.5),
+                         order=2 if kind == "mixed_p2_q2" else 1)
+        u = euler_state(mesh, pert=0.0)
+        phys = O.EULER2D
+    P = oracle_problem(mesh, phys, scheme, (1.4, 287.05))
     res, ws, rc = P.assemble_rhs(u)
     assert rc == 0
     assert np.abs(res).max() < 1e-12 * ws.max()
@@ -203,6 +223,10 @@ def test_dual_volumes():
         assert abs(P.dual_volume().sum() - nd * 1.0) < 1e-13
     mesh = rectangle(6, 5, split=1, jitter=0.2, seed=1)
     assert abs(oracle_problem(mesh, O.ADVECTION2D, O.LDA, A).dual_volume().sum() - 4.0) < 1e-13
+    mesh = rectangle(6, 5, split=1, jitter=0.2, seed=1, order=2)
+    assert abs(oracle_problem(mesh, O.ADVECTION2D, O.LDA, A).dual_volume().sum() - 9.0) < 1e-13
+    mesh = box_tetra(3, 3, 2, extent=(1.0, 1.0, 0.5), order=2, jitter=0.1, seed=4)
+    assert abs(oracle_problem(mesh, O.EULER3D, O.LDA, (1.4, 287.05)).dual_volume().sum() - 10 * 0.5) < 1e-13
 
 
 @pytest.mark.parametrize("scheme", [O.N, O.LDA])
