@@ -822,14 +822,15 @@ struct PatchSize
 
 // Waves per SIMD the register allocation of a patch kernel must leave room
 // for: 5 for the scalar affine P1 / P2 cells (at most 96 VGPRs, measured
-// faster than 4 waves); 2 for the 2D Euler cells (little or no scratch,
-// measured faster than 1 wave); the others keep the compiler's choice (the
-// 3D Euler cells run as fast at 1 wave, where capping them at 2 spills a
-// few hundred bytes per thread to scratch memory).
+// faster than 4 waves); 2 for the 2D Euler cells of up to 4 nodes (little or
+// no scratch, measured faster than 1 wave); the others keep the compiler's
+// choice (the 3D Euler cells run as fast at 1 wave, where capping them at 2
+// spills a few hundred bytes per thread to scratch memory; the 2D Euler P2 /
+// Q2 / P3 cells spill 250-1600 bytes per thread at 2 waves).
 template <int M, int ND, bool AFFINE>
 struct PatchMinWaves
 {
-  static constexpr int value = (M == 1 && ND <= 6 && AFFINE) ? 5 : (M == 4 ? 2 : 1);
+  static constexpr int value = (M == 1 && ND <= 6 && AFFINE) ? 5 : ((M == 4 && ND <= 4) ? 2 : 1);
 };
 
 // Dynamic LDS of one patch workgroup of P cells, in doubles: node state

@@ -920,20 +920,31 @@ int pdrd_initialize(pdrd_method_t h)
         for (uint32_t v : fsets.back().nodes) extra[v]++;
       }
     }
+    // A block whose patches need more than 64 KiB of LDS at its default size
+    // is planned again with half as many cells per patch (down to 64); a size
+    // the caller chose is kept, and the method then falls back to colouring.
     pdrd::PatchPlan plan;
-    h->patch_ok = true;
-    try {
-      plan = pdrd::build_patch_plan(h->dim, h->nv, h->coords.data(), h->nn, pin, fsets.empty() ? nullptr : &extra);
-    } catch (const std::exception &ex) {
-      h->patch_ok = false;
-      h->patch_reason = ex.what();
-    }
-    if (h->patch_ok) {
+    for (;;) {
+      h->patch_ok = true;
+      try {
+        plan = pdrd::build_patch_plan(h->dim, h->nv, h->coords.data(), h->nn, pin, fsets.empty() ? nullptr : &extra);
+      } catch (const std::exception &ex) {
+        h->patch_ok = false;
+        h->patch_reason = ex.what();
+        break;
+      }
+      bool shrunk = false;
       for (size_t i = 0; i < h->blocks.size(); ++i)
         if (h->blocks[i].ks.patch_lds_bytes(plan.blocks[i].maxl, pin[i].patch_size) > 65536) {
-          h->patch_ok = false;
-          h->patch_reason = "patch LDS above 64 KiB";
+          if (!h->patch_cells && pin[i].patch_size > 64) {
+            pin[i].patch_size /= 2;
+            shrunk = true;
+          } else {
+            h->patch_ok = false;
+            h->patch_reason = "patch LDS above 64 KiB";
+          }
         }
+      if (!h->patch_ok || !shrunk) break;
     }
     if (h->patch_ok) {
       for (size_t i = 0; i < h->blocks.size(); ++i) {
