@@ -326,9 +326,16 @@ void cell(const orc_problem *p, const orc_block &b, uint32_t c, const double *u,
     const uint32_t v = b.geo_conn[static_cast<size_t>(c) * ng + g];
     for (int d = 0; d < D; ++d) X[g][d] = p->coords[static_cast<size_t>(v) * D + d];
   }
+  // nodal blending coefficient (PGB) and artificial viscosity (PGLDA) of the
+  // cell (PGRDMExplicitCellWorker.hpp:507-533)
+  double theta_n[MAXN], nu_n[MAXN];
+  const bool ext_theta = p->blend && p->scheme == ORC_B;
+  const bool visc = p->art_visc && p->scheme == ORC_LDA;
   for (int n = 0; n < nd; ++n) {
     const uint32_t v = b.sol_conn[static_cast<size_t>(c) * nd + n];
     for (int e = 0; e < m; ++e) U[n][e] = u[static_cast<size_t>(v) * m + e];
+    theta_n[n] = ext_theta ? p->blend[v] : 0.0;
+    nu_n[n] = visc ? p->art_visc[v] : 0.0;
   }
   out.bad = false;
 
@@ -415,6 +422,12 @@ void cell(const orc_problem *p, const orc_block &b, uint32_t c, const double *u,
   double elem_volume = 0.0;
   double flux_integral[MAXM] = {0, 0, 0, 0, 0};
 
+  // PGB with an external blending coefficient skips the dissipation when it
+  // is below 1e-6 at every node of the cell (PGB.hpp:141-148)
+  double max_theta = 0.0;
+  for (int n = 0; n < nd; ++n) max_theta = std::max(max_theta, theta_n[n]);
+  const bool skip_dissipation = ext_theta && max_theta < 1.e-6;
+
   static thread_local Mat Kp[MAXN];
   for (int q = 0; q < nq; ++q) {
     const double wj = b.w[q] * detJ[q];
@@ -425,6 +438,21 @@ void cell(const orc_problem *p, const orc_block &b, uint32_t c, const double *u,
     Mat sumKp;
     for (int i = 0; i < m; ++i)
       for (int j = 0; j < m; ++j) sumKp[i][j] = 0.0;
+    // artificial viscosity at q (the nodal field interpolated like the
+    // solution) and the physical solution gradient there
+    double nu_q = 0.0;
+    double gradu[3][MAXM];
+    if (visc) {
+      for (int n = 0; n < nd; ++n) nu_q += b.V[q * nd + n] * nu_n[n];
+      for (int x = 0; x < D; ++x)
+        for (int e = 0; e < m; ++e) gradu[x][e] = 0.0;
+      for (int n = 0; n < nd; ++n)
+        for (int x = 0; x < D; ++x) {
+          double gx = 0.0;
+          for (int xi = 0; xi < D; ++xi) gx += invJ[q][x][xi] * b.dV[(static_cast<size_t>(xi) * nq + q) * nd + n];
+          for (int e = 0; e < m; ++e) gradu[x][e] += gx * U[n][e];
+        }
+    }
     for (int n = 0; n < nd; ++n) {
       double gref[3], g[3];
       for (int xi = 0; xi < D; ++xi) gref[xi] = b.dV[(static_cast<size_t>(xi) * nq + q) * nd + n];
@@ -433,6 +461,13 @@ void cell(const orc_problem *p, const orc_block &b, uint32_t c, const double *u,
         for (int xi = 0; xi < D; ++xi) s += invJ[q][x][xi] * gref[xi];
         g[x] = s;
       }
+      // shock-capturing term of PGLDA: w |J| nu grad(phi_n) . grad(u)
+      // (PGLDA.hpp:245-256)
+      if (visc && nu_q != 0.0)
+        for (int x = 0; x < D; ++x) {
+          const double cx = wj * nu_q * g[x];
+          for (int e = 0; e < m; ++e) lda[n][e] += cx * gradu[x][e];
+        }
       Mat R, L;
       double Dv[MAXM], Dp[MAXM];
       eigen_structure(p->physics, pr, g, R, L, Dv);
@@ -473,7 +508,7 @@ void cell(const orc_problem *p, const orc_block &b, uint32_t c, const double *u,
         for (int j = 0; j < m; ++j) s += KI[i][j] * r[j];
         lda[n][i] += wj * s;
       }
-      if (p->scheme == ORC_LDA) continue;
+      if (p->scheme == ORC_LDA || skip_dissipation) continue;
       // dissipation: sum_{k != n} Kp[n] inv Kp[k] (U_n - U_k)
       for (int k = 0; k < nd; ++k) {
         if (k == n) continue;
@@ -500,6 +535,10 @@ void cell(const orc_problem *p, const orc_block &b, uint32_t c, const double *u,
   } else if (p->scheme == ORC_N) {
     for (int n = 0; n < nd; ++n)
       for (int e = 0; e < m; ++e) out.phi[n][e] = lda[n][e] + ndis[n][e];
+  } else if (p->scheme == ORC_B && ext_theta) {
+    // the external nodal coefficient (PGB.hpp:467-477)
+    for (int n = 0; n < nd; ++n)
+      for (int e = 0; e < m; ++e) out.phi[n][e] = lda[n][e] + theta_n[n] * ndis[n][e];
   } else if (p->scheme == ORC_B) {
     // cell-wise theta from equation 0 only, +1e-6 in the denominator
     double theta[MAXM];
@@ -806,6 +845,144 @@ void orc_norm_L2(const double *f, uint32_t n, int m, double *out)
   for (uint32_t i = 0; i < n; ++i)
     for (int e = 0; e < m; ++e) out[e] += f[static_cast<size_t>(i) * m + e] * f[static_cast<size_t>(i) * m + e];
   for (int e = 0; e < m; ++e) out[e] = std::sqrt(out[e] / n);
+}
+
+namespace {
+
+// MeshFunctionTools::interp_from_cells_to_nodes_p1_max (MeshFunctionTools.hpp:53-330,
+// conforming meshes): every vertex takes the largest value of the cells
+// around it, every other node the P1 / Q1 interpolation of its cell's
+// vertex values.
+void cells_to_nodes_max(const orc_problem *p, const std::vector<std::vector<double>> &fc, const double *const *p1w,
+                        const int *nvert, double *f)
+{
+  for (uint32_t i = 0; i < p->nn; ++i) f[i] = 0.0;
+  for (int bi = 0; bi < p->nblocks; ++bi) {
+    const orc_block &b = p->blocks[bi];
+    for (uint32_t c = 0; c < b.ne; ++c)
+      for (int a = 0; a < nvert[bi]; ++a) {
+        const uint32_t v = b.sol_conn[static_cast<size_t>(c) * b.nd + a];
+        f[v] = std::max(f[v], fc[bi][c]);
+      }
+  }
+  for (int bi = 0; bi < p->nblocks; ++bi) {
+    const orc_block &b = p->blocks[bi];
+    const int nv = nvert[bi];
+    for (uint32_t c = 0; c < b.ne; ++c) {
+      const uint32_t *cn = b.sol_conn + static_cast<size_t>(c) * b.nd;
+      for (int n = nv; n < b.nd; ++n) {
+        double s = 0.0;
+        for (int a = 0; a < nv; ++a) s += p1w[bi][n * nv + a] * f[cn[a]];
+        f[cn[n]] = s;
+      }
+    }
+  }
+}
+
+}  // namespace
+
+int orc_lf_blend(const orc_problem *p, const double *u, const double *const *p1w, const int *nvert, double *theta)
+{
+  const int m = neq_of(p->physics);
+  // theta_nodal = max over the cells of a node of
+  //   max_v |u_v - u_avg| / (|u_v| + |u_avg| + 1e-12), first component
+  std::vector<double> tn(p->nn, 0.0);
+  for (int bi = 0; bi < p->nblocks; ++bi) {
+    const orc_block &b = p->blocks[bi];
+    for (uint32_t c = 0; c < b.ne; ++c) {
+      const uint32_t *cn = b.sol_conn + static_cast<size_t>(c) * b.nd;
+      double avg = 0.0;
+      for (int n = 0; n < b.nd; ++n) avg += u[static_cast<size_t>(cn[n]) * m];
+      avg /= b.nd;
+      double ksi = 0.0;
+      for (int n = 0; n < b.nd; ++n) {
+        const double un = u[static_cast<size_t>(cn[n]) * m];
+        ksi = std::max(ksi, std::abs(un - avg) / (std::abs(un) + std::abs(avg) + 1.e-12));
+      }
+      for (int n = 0; n < b.nd; ++n) tn[cn[n]] = std::max(tn[cn[n]], ksi);
+    }
+  }
+  // cell value 1 - max over its nodes
+  std::vector<std::vector<double>> fc(p->nblocks);
+  for (int bi = 0; bi < p->nblocks; ++bi) {
+    const orc_block &b = p->blocks[bi];
+    fc[bi].resize(b.ne);
+    for (uint32_t c = 0; c < b.ne; ++c) {
+      double t = 0.0;
+      for (int n = 0; n < b.nd; ++n) t = std::max(t, tn[b.sol_conn[static_cast<size_t>(c) * b.nd + n]]);
+      fc[bi][c] = 1. - t;
+    }
+  }
+  cells_to_nodes_max(p, fc, p1w, nvert, theta);
+  return 0;
+}
+
+int orc_art_visc(const orc_problem *p, const double *u, const orc_av_rule *rules, const double *const *p1w,
+                 const int *nvert, double kappa, double factor, double *nu)
+{
+  const int m = neq_of(p->physics);
+  const int D = p->dim;
+  std::vector<std::vector<double>> fc(p->nblocks);
+  for (int bi = 0; bi < p->nblocks; ++bi) {
+    const orc_block &b = p->blocks[bi];
+    const orc_av_rule &r = rules[bi];
+    fc[bi].resize(b.ne);
+    for (uint32_t c = 0; c < b.ne; ++c) {
+      double X[8][3];
+      for (int g = 0; g < b.ng; ++g) {
+        const uint32_t v = b.geo_conn[static_cast<size_t>(c) * b.ng + g];
+        for (int d = 0; d < D; ++d) X[g][d] = p->coords[static_cast<size_t>(v) * D + d];
+      }
+      const uint32_t *cn = b.sol_conn + static_cast<size_t>(c) * b.nd;
+      // u_h at the points, its projection onto the polynomials of degree
+      // < p (the modal expansion without its leading terms), both
+      // integrated with w |J|
+      double uq[64], wj[64], coef[16];
+      double vol = 0.0;
+      for (int q = 0; q < r.nq; ++q) {
+        double J[9];
+        for (int xi = 0; xi < D; ++xi)
+          for (int x = 0; x < D; ++x) {
+            double s = 0.0;
+            for (int g = 0; g < b.ng; ++g) s += r.dVg[(static_cast<size_t>(xi) * r.nq + q) * b.ng + g] * X[g][x];
+            J[xi * D + x] = s;
+          }
+        const double det = (D == 2) ? J[0] * J[3] - J[1] * J[2] : det3(J, 0, 1, 2, 0, 1, 2, 3);
+        wj[q] = r.w[q] * det;
+        vol += wj[q];
+        double s = 0.0;
+        for (int n = 0; n < b.nd; ++n) s += r.V[q * b.nd + n] * u[static_cast<size_t>(cn[n]) * m];
+        uq[q] = s;
+      }
+      for (int k = 0; k < r.npm; ++k) {
+        double s = 0.0;
+        for (int q = 0; q < r.nq; ++q) s += r.w[q] * r.psi[q * r.npm + k] * uq[q];
+        coef[k] = s;
+      }
+      double nom = 0.0, denom = 0.0;
+      for (int q = 0; q < r.nq; ++q) {
+        double pr = 0.0;
+        for (int k = 0; k < r.npm; ++k) pr += coef[k] * r.psi[q * r.npm + k];
+        nom += wj[q] * (uq[q] - pr) * (uq[q] - pr);
+        denom += wj[q] * uq[q] * uq[q];
+      }
+      const double Se = (denom <= 1.e-10) ? 0.0 : nom / denom;
+      const double se = std::log(Se) / std::log(10.);
+      const double h = std::pow(vol, 1. / D);
+      const double eps0 = h / r.p;
+      const double s0 = 1. / (r.p * r.p * r.p * r.p);
+      double v;
+      if (se < s0 - kappa)
+        v = 0.0;
+      else if (se > s0 + kappa)
+        v = factor * eps0;
+      else
+        v = factor * 0.5 * eps0 * (1. + std::sin(M_PI * (se - s0) / (2. * kappa)));
+      fc[bi][c] = v;
+    }
+  }
+  cells_to_nodes_max(p, fc, p1w, nvert, nu);
+  return 0;
 }
 
 int orc_iterate(const orc_problem *p, double cfl, int n_iter, uint32_t nb_dirichlet, const uint32_t *dnodes,

@@ -73,6 +73,11 @@ typedef struct orc_problem
   const orc_block *blocks;
   int nweak; /* weak BCs applied after the cell loop, in this order */
   const orc_weak *weak;
+  /* set_blending_coeff: nodal blending coefficient [N] that PGB uses instead
+   * of its cell-wise theta, or NULL; set_artificial_viscosity: nodal
+   * artificial viscosity [N] of PGLDA's shock-capturing term, or NULL */
+  const double *blend;
+  const double *art_visc;
 } orc_problem;
 
 int orc_neq(int physics);
@@ -99,6 +104,27 @@ int orc_cell_residuals(const orc_problem *p, int block, const double *u, double 
 int orc_iterate(const orc_problem *p, double cfl, int n_iter, uint32_t nb_dirichlet,
                 const uint32_t *dirichlet_nodes, const double *dirichlet_values, double *u,
                 double *res, double *dt, double *norms);
+
+/* Providers of the two stabilisation inputs. Per block: p1w[nd][nvert], the
+ * P1 / Q1 shape functions of the cell's vertices at its solution nodes
+ * (vertices are the first nvert nodes). */
+/* RDBlendingCoeff "LF_Blend" (LFBlendingCoeffImpl.hpp:79-136) followed by
+ * MeshFunctionTools::interp_from_cells_to_nodes_p1_max: theta[N]. */
+int orc_lf_blend(const orc_problem *p, const double *u, const double *const *p1w, const int *nvert,
+                 double *theta);
+/* The smoothness-indicator quadrature of one block: rule of order
+ * max(2p, 1) with weights w[nq], nodal basis V[nq][nd], geometry derivatives
+ * dVg[dim][nq][ng], and psi[nq][npm] an orthonormal basis (under the rule)
+ * of the polynomials of degree < p on the reference simplex. */
+typedef struct orc_av_rule
+{
+  int p, nq, npm;
+  const double *w, *V, *dVg, *psi;
+} orc_av_rule;
+/* ArtificialViscosity::compute_artificial_viscosity_nodal
+ * (ArtificialViscosity.hpp:371-560) on simplices: nu[N]. */
+int orc_art_visc(const orc_problem *p, const double *u, const orc_av_rule *rules, const double *const *p1w,
+                 const int *nvert, double kappa, double factor, double *nu);
 
 /* norm_L2 per field: sqrt(sum_i r_i^2 / N) (MeshFunctionNorm.hpp). */
 void orc_norm_L2(const double *f, uint32_t n, int m, double *out);
