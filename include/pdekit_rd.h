@@ -55,7 +55,9 @@ enum pdrd_field {
   PDRD_RESIDUAL = 1,   /* [N][m] */
   PDRD_TIME_STEP = 2,  /* [N]    */
   PDRD_WAVE_SPEED = 3, /* [N]    sum of element wave speeds (RDTimeUpdate) */
-  PDRD_DUAL_VOLUME = 4 /* [N]    */
+  PDRD_DUAL_VOLUME = 4, /* [N]   */
+  PDRD_BLEND_COEFF = 5, /* [N]   nodal blending coefficient of the B scheme (once set or computed) */
+  PDRD_ART_VISC = 6     /* [N]   nodal artificial viscosity of the LDA scheme (once set or computed) */
 };
 /* Options of a method. */
 enum pdrd_option {
@@ -213,6 +215,27 @@ int pdrd_add_weak_bc(pdrd_method_t h, int type, uint32_t n, const uint32_t *node
 /* After pdrd_initialize: facets, nodes per facet, facet quadrature points of
  * weak BC k (in the order added). */
 int pdrd_weak_bc_info(pdrd_method_t h, int k, int64_t *info3);
+
+/* Nodal inputs of the cell loop (PGRDMExplicitCellWorker.hpp:307-330, 501-533).
+ * set_blending_coeff(f): PGB takes theta_i = f[node i] instead of its
+ * cell-wise theta, Phi_i = lda_i + theta_i n_i, and keeps the LDA residuals
+ * of a cell whose nodes are all below 1e-6 (PGB.hpp:141-148, 467-477).
+ * set_artificial_viscosity(nu): PGLDA adds w |J| nu_q grad(phi_i) . grad(u)
+ * at every quadrature point, nu_q interpolated from the nodes like the state
+ * (PGLDA.hpp:245-256). theta / nu: host [N], copied to the device; NULL
+ * switches the input off. Other schemes ignore them; both run with the FP64
+ * cells (not with PDRD_OPT_PRECISION = 1) and outside batches. */
+int pdrd_set_blending_coeff(pdrd_method_t h, const double *theta);
+int pdrd_set_artificial_viscosity(pdrd_method_t h, const double *nu);
+/* Providers, on the device from the current solution, each storing its
+ * result as the method's input (and switching it on):
+ * RDBlendingCoeff::calculate with "LF_Blend" (LFBlendingCoeffImpl.hpp:79-136
+ * + MeshFunctionTools::interp_from_cells_to_nodes_p1_max, RDBlendingCoeff.hpp:
+ * 150-165); ArtificialViscosity::compute_artificial_viscosity_nodal(kappa,
+ * factor) (ArtificialViscosity.hpp:371-560; triangles and tetrahedra). */
+enum pdrd_blend_kind { PDRD_BLEND_LF = 0 };
+int pdrd_compute_blending_coeff(pdrd_method_t h, int kind);
+int pdrd_compute_artificial_viscosity(pdrd_method_t h, double kappa, double factor);
 
 /* add_boundary_condition("StrongDirichlet", ...) with precomputed values:
  * u[nodes[k]] = values[k][0..m) after every cell loop

@@ -260,17 +260,17 @@ def p1_weights(shape, nodes):
 
 def lower_degree_basis(shape, p, xi, w):
     """An orthonormal basis, under the rule (xi, w), of the polynomials of
-    total degree < p on the reference simplex: [nq][npm] (QR of the
-    weighted monomials)."""
+    total degree < p on the reference simplex: [nq][npm] (Cholesky of the
+    Gram matrix of the monomials; the rule may carry negative weights)."""
     if shape == 1:
         raise ValueError("the smoothness indicator is built for simplices")
     xi = np.asarray(xi, dtype=np.float64)
     dim = xi.shape[1]
     exps = [e for e in np.ndindex(*([p] * dim)) if sum(e) < p]
     M = np.stack([np.prod([xi[:, d] ** e[d] for d in range(dim)], axis=0) for e in exps], 1)
-    sw = np.sqrt(np.asarray(w))
-    _, R = np.linalg.qr(sw[:, None] * M)
-    return np.ascontiguousarray(M @ np.linalg.inv(R))
+    G = M.T @ (np.asarray(w)[:, None] * M)
+    Lc = np.linalg.cholesky(G)
+    return np.ascontiguousarray(np.linalg.solve(Lc, M.T).T)
 
 
 def invert(a):

@@ -26,7 +26,8 @@ TRIAG, QUAD, TETRA = 0, 1, 2
 EQUIDIST, WARPBLEND = 0, 1
 ADVECTION2D, BURGERS2D, EULER2D, EULER3D = 0, 1, 2, 3
 LDA, N, B, PSI = 0, 1, 2, 3
-SOLUTION, RESIDUAL, TIME_STEP, WAVE_SPEED, DUAL_VOLUME = 0, 1, 2, 3, 4
+SOLUTION, RESIDUAL, TIME_STEP, WAVE_SPEED, DUAL_VOLUME, BLEND_COEFF, ART_VISC = 0, 1, 2, 3, 4, 5, 6
+BLEND_LF = 0
 OPT_SCATTER, OPT_PRECISION, OPT_BLOCK, OPT_TIMING, OPT_PATCH_CELLS, OPT_REFRESH = 0, 1, 2, 3, 4, 5
 WEAK_WALL, WEAK_SUBINLET, WEAK_SUBOUTLET, WEAK_FARFIELD, WEAK_SYMMETRY, WEAK_SUPERINLET, WEAK_DIRICHLET = range(7)
 
@@ -43,6 +44,8 @@ EXPORTED = (
     "pdrd_method_info", "pdrd_last_timing", "pdrd_last_error", "pdrd_version", "pdrd_get_option",
     "pdrd_batch_create", "pdrd_batch_destroy", "pdrd_batch_iterate", "pdrd_add_weak_bc", "pdrd_weak_bc_info",
     "pdrd_mesh_read_gmsh", "pdrd_mesh_write_gmsh", "pdrd_gmsh_append_node_data", "pdrd_gmsh_read_node_data",
+    "pdrd_set_blending_coeff", "pdrd_set_artificial_viscosity", "pdrd_compute_blending_coeff",
+    "pdrd_compute_artificial_viscosity",
 )
 
 
@@ -120,6 +123,10 @@ def lib() -> C.CDLL:
         "pdrd_batch_create": [C.POINTER(P)],
         "pdrd_batch_destroy": [P],
         "pdrd_batch_iterate": [P, I, C.POINTER(P), PD, I, I, PD],
+        "pdrd_set_blending_coeff": [P, PD],
+        "pdrd_set_artificial_viscosity": [P, PD],
+        "pdrd_compute_blending_coeff": [P, I],
+        "pdrd_compute_artificial_viscosity": [P, D, D],
         "pdrd_check_status": [P],
         "pdrd_method_info": [P, PI64],
         "pdrd_last_timing": [P, PD, PD],

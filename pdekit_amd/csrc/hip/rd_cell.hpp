@@ -43,6 +43,7 @@ struct CellArgs
   uint32_t ne;     // elements in the block (SoA stride)
   int atomic;      // 1: FP64 atomics, 0: plain read-modify-write (one colour)
   PhysParams pp;
+  const double *__restrict__ nodal;  // [N] blending coefficient (B) or artificial viscosity (LDA), or null
 };
 
 template <int D, int NG, int ND, int NF, int NQ>
@@ -187,6 +188,30 @@ struct ULds
   __device__ __forceinline__ double operator()(int n, int k) const { return ul[row[n] * M + k]; }
 };
 
+// A nodal scalar input of the cell (external blending coefficient of the B
+// scheme, artificial viscosity of the LDA scheme): none, from the patch's
+// LDS rows, or from global memory by node id.
+struct NodalNone
+{
+  __device__ __forceinline__ double operator()(int) const { return 0.0; }
+};
+
+template <int ND>
+struct NodalLds
+{
+  const double *v;       // [local node]
+  const int (&row)[ND];
+  __device__ __forceinline__ double operator()(int n) const { return v[row[n]]; }
+};
+
+template <int ND>
+struct NodalGlobal
+{
+  const double *__restrict__ v;  // [N]
+  const uint32_t (&nid)[ND];
+  __device__ __forceinline__ double operator()(int n) const { return v[nid[n]]; }
+};
+
 // Residuals phi[ND][M] and wave speed of one element from its vertex
 // coordinates X and nodal state U. lds_div: this thread's slice of NQ*M
 // doubles of LDS (stride `stride` between entries) that carries the flux
@@ -197,12 +222,17 @@ struct ULds
 // nodal reference state, see flux_delta) and their divergence, and the
 // upwind parameters K+ are evaluated in FP32; the sums S = sum K+, the solves
 // with S, the element residuals, wave speeds, and everything after them
-// (nodal sums, time step, update, norms) stay FP64.
-template <int D, int NG, int ND, int NF, int NQ, bool AFFINE, int PH, int SC, int PREC, class UA>
+// (nodal sums, time step, update, norms) stay FP64. PREC 2: FP64 with the
+// nodal input `nodal` -- for the B scheme the external blending coefficient
+// (set_blending_coeff, PGB.hpp:141-148, 467-477), for the LDA scheme the
+// artificial viscosity of the shock-capturing term (PGLDA.hpp:245-256).
+template <int D, int NG, int ND, int NF, int NQ, bool AFFINE, int PH, int SC, int PREC, class UA,
+          class NA = NodalNone>
 __device__ __forceinline__ void cell_residual(const double *__restrict__ tab, const double (&X)[NG][D],
                                               const double (&U)[ND][PhysTraits<PH>::M], const UA &Uq,
                                               const PhysParams &pp, double (&phi)[ND][PhysTraits<PH>::M],
-                                              double &wave, bool &bad, double *lds_div, int stride)
+                                              double &wave, bool &bad, double *lds_div, int stride,
+                                              const NA &nodal = NA{})
 {
   constexpr int M = PhysTraits<PH>::M;
   static_assert(PhysTraits<PH>::DIM == D, "physics / element dimension mismatch");
@@ -389,6 +419,44 @@ __device__ __forceinline__ void cell_residual(const double *__restrict__ tab, co
 #pragma unroll
     for (int k = 0; k < M; ++k) fint[k] += wj * r[k];
     const double *dvq = tab + O::dVT + q * D * ND;  // reference gradients [D][ND] at q
+
+    if constexpr (PREC == 2 && SC == SC_LDA) {
+      // shock capturing: w |J| nu_q grad(phi_n) . grad(u_k), nu interpolated
+      // from the nodes like the state, added before the LDA share of q
+      double nuq = 0.0;
+#pragma unroll
+      for (int n = 0; n < ND; ++n) nuq += tab[O::V + q * ND + n] * nodal(n);
+      if (nuq != 0.0) {
+        double gu[D][M];
+#pragma unroll
+        for (int x = 0; x < D; ++x)
+#pragma unroll
+          for (int k = 0; k < M; ++k) gu[x][k] = 0.0;
+#pragma unroll
+        for (int n = 0; n < ND; ++n) {
+#pragma unroll
+          for (int x = 0; x < D; ++x) {
+            double gx = 0.0;
+#pragma unroll
+            for (int xi = 0; xi < D; ++xi) gx += Jq[x][xi] * dvq[xi * ND + n];
+#pragma unroll
+            for (int k = 0; k < M; ++k) gu[x][k] += gx * Uq(n, k);
+          }
+        }
+#pragma unroll
+        for (int n = 0; n < ND; ++n) {
+#pragma unroll
+          for (int x = 0; x < D; ++x) {
+            double gx = 0.0;
+#pragma unroll
+            for (int xi = 0; xi < D; ++xi) gx += Jq[x][xi] * dvq[xi * ND + n];
+            const double cx = wj * nuq * gx;
+#pragma unroll
+            for (int k = 0; k < M; ++k) lda[n][k] += cx * gu[x][k];
+          }
+        }
+      }
+    }
 
     if constexpr (M == 1) {
       TK vel0, vel1;
@@ -683,6 +751,21 @@ __device__ __forceinline__ void cell_residual(const double *__restrict__ tab, co
     for (int n = 0; n < ND; ++n)
 #pragma unroll
       for (int k = 0; k < M; ++k) phi[n][k] = lda[n][k] + dis[n][k];
+  } else if constexpr (SC == SC_B && PREC == 2) {
+    // external nodal coefficient; below 1e-6 at every node the cell keeps
+    // its LDA residuals
+    double th[ND];
+    double tmax = 0.0;
+#pragma unroll
+    for (int n = 0; n < ND; ++n) {
+      th[n] = nodal(n);
+      tmax = fmax(tmax, th[n]);
+    }
+    const bool skip = tmax < 1.e-6;
+#pragma unroll
+    for (int n = 0; n < ND; ++n)
+#pragma unroll
+      for (int k = 0; k < M; ++k) phi[n][k] = skip ? lda[n][k] : lda[n][k] + th[n] * dis[n][k];
   } else if constexpr (SC == SC_B) {
     double an = 0.0;
 #pragma unroll
@@ -726,7 +809,7 @@ __device__ __forceinline__ void gather_coords(const CellArgs &a, uint32_t e, dou
 
 // Element-per-thread cell loop with a global scatter: FP64 atomics or, one
 // colour per launch, plain read-modify-write.
-template <int D, int NG, int ND, int NF, int NQ, bool AFFINE, int PH, int SC>
+template <int D, int NG, int ND, int NF, int NQ, bool AFFINE, int PH, int SC, int PREC = 0>
 __global__ __launch_bounds__(256) void rd_cell_kernel(CellArgs a)
 {
   constexpr int M = PhysTraits<PH>::M;
@@ -747,9 +830,13 @@ __global__ __launch_bounds__(256) void rd_cell_kernel(CellArgs a)
   }
   double phi[ND][M];
   double wave;
-  cell_residual<D, NG, ND, NF, NQ, AFFINE, PH, SC, 0>(a.tab, X, U, URegs<ND, M>{U}, a.pp, phi, wave, bad,
-                                                   lds + threadIdx.x,
-                                                   static_cast<int>(blockDim.x));
+  if constexpr (PREC == 2)
+    cell_residual<D, NG, ND, NF, NQ, AFFINE, PH, SC, 2>(a.tab, X, U, URegs<ND, M>{U}, a.pp, phi, wave, bad,
+                                                     lds + threadIdx.x, static_cast<int>(blockDim.x),
+                                                     NodalGlobal<ND>{a.nodal, nid});
+  else
+    cell_residual<D, NG, ND, NF, NQ, AFFINE, PH, SC, 0>(a.tab, X, U, URegs<ND, M>{U}, a.pp, phi, wave, bad,
+                                                     lds + threadIdx.x, static_cast<int>(blockDim.x));
 #pragma unroll
   for (int n = 0; n < ND; ++n) {
     const size_t b = static_cast<size_t>(nid[n]);
@@ -798,6 +885,7 @@ struct PatchArgs
   double *__restrict__ slot_acc;            // [shared slots][M + 1] partial sums, grouped by node
   const uint32_t *__restrict__ slot_pos;    // [slots] row of a shared slot in slot_acc
   double *__restrict__ norm_partial;        // [np][M + 1]
+  const double *__restrict__ nodal;         // [N] blending coefficient (B) / artificial viscosity (LDA), PREC 2
   int *__restrict__ status;
   uint32_t ne, np, first_patch;             // first_patch: offset of this block's patches
   int maxl;                                 // LDS rows of node state
@@ -853,6 +941,14 @@ template <int NQ, int ND, int M>
 __host__ __device__ constexpr size_t patch_lds_doubles(int maxl, int P)
 {
   return static_cast<size_t>(maxl) * M + patch_coff_doubles(maxl) + patch_work_doubles<NQ, ND, M>(P);
+}
+
+// With a nodal input (PREC 2) its values follow the work region, one per
+// local node.
+template <int NQ, int ND, int M, int PREC>
+__host__ __device__ constexpr size_t patch_lds_doubles_prec(int maxl, int P)
+{
+  return patch_lds_doubles<NQ, ND, M>(maxl, P) + (PREC == 2 ? static_cast<size_t>(maxl) : 0);
 }
 
 // Finishes node g from its sums acc[M + 1] (residual sum, wave-speed sum);
@@ -944,11 +1040,13 @@ __device__ __forceinline__ void patch_body(const PatchArgs &a, uint32_t pl, doub
 #pragma unroll
     for (int n = 0; n < ND; ++n) lc[n] = a.lconn[static_cast<size_t>(n) * a.ne + e];
   }
+  double *nodal_l = work + patch_work_doubles<NQ, ND, M>(PATCH);  // PREC 2 only
   for (int l = tid; l < nl; l += PATCH) {
     const size_t g = a.pnodes[s0 + l];
 #pragma unroll
     for (int k = 0; k < M; ++k) ul[l * M + k] = a.u[g * M + k];
     coff[l] = a.pn_coff[s0 + l];
+    if constexpr (PREC == 2) nodal_l[l] = a.nodal[g];
   }
   __syncthreads();
 
@@ -964,7 +1062,13 @@ __device__ __forceinline__ void patch_body(const PatchArgs &a, uint32_t pl, doub
 #pragma unroll
       for (int k = 0; k < M; ++k) U[n][k] = ul[row[n] * M + k];
     }
-    if constexpr (M == 1)
+    if constexpr (PREC == 2 && M == 1)
+      cell_residual<D, NG, ND, NF, NQ, AFFINE, PH, SC, PREC>(a.tab, X, U, URegs<ND, M>{U}, a.pp, phi, wave, bad, work + tid,
+                                                       PATCH, NodalLds<ND>{nodal_l, row});
+    else if constexpr (PREC == 2)
+      cell_residual<D, NG, ND, NF, NQ, AFFINE, PH, SC, PREC>(a.tab, X, U, ULds<ND, M>{ul, row}, a.pp, phi, wave, bad,
+                                                       work + tid, PATCH, NodalLds<ND>{nodal_l, row});
+    else if constexpr (M == 1)
       cell_residual<D, NG, ND, NF, NQ, AFFINE, PH, SC, PREC>(a.tab, X, U, URegs<ND, M>{U}, a.pp, phi, wave, bad, work + tid,
                                                        PATCH);
     else

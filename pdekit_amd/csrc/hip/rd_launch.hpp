@@ -25,6 +25,10 @@ struct KernelSet
   PatchBatchLauncher patch_batch = nullptr;  // patch assembly of several blocks / methods
   PatchLauncher patch_mixed = nullptr;       // the same with FP32 interpolation, fluxes and K+
   PatchBatchLauncher patch_batch_mixed = nullptr;
+  // with the nodal input of the scheme (LDA: artificial viscosity, B:
+  // external blending coefficient); null for N and PSI, which take none
+  CellLauncher cell_nodal = nullptr;
+  PatchLauncher patch_nodal = nullptr;
   int patch_size = 0;             // largest (default) cells per patch workgroup
   size_t (*patch_lds_bytes)(int maxl, int cells) = nullptr;
   explicit operator bool() const { return cell != nullptr; }
@@ -44,12 +48,12 @@ struct KernelSet
   X(2, 2, 3, 4, 10, 20, 4, true)                   \
   X(2, 2, 3, 4, 10, 20, 11, true)
 
-template <int D, int NG, int ND, int NF, int NQ, bool AFF, int PH, int SC>
+template <int D, int NG, int ND, int NF, int NQ, bool AFF, int PH, int SC, int PREC = 0>
 hipError_t launch_cell(const CellArgs &a, unsigned grid, unsigned block, hipStream_t s)
 {
   constexpr int M = PhysTraits<PH>::M;
   const size_t lds = static_cast<size_t>(NQ) * M * block * sizeof(double);
-  hipLaunchKernelGGL((rd_cell_kernel<D, NG, ND, NF, NQ, AFF, PH, SC>), dim3(grid), dim3(block), lds, s, a);
+  hipLaunchKernelGGL((rd_cell_kernel<D, NG, ND, NF, NQ, AFF, PH, SC, PREC>), dim3(grid), dim3(block), lds, s, a);
   return hipGetLastError();
 }
 
@@ -57,7 +61,7 @@ template <int D, int NG, int ND, int NF, int NQ, bool AFF, int PH, int SC, int P
 hipError_t launch_patch(const PatchArgs &a, unsigned grid, int cells, hipStream_t s)
 {
   constexpr int M = PhysTraits<PH>::M;
-  const size_t lds = patch_lds_doubles<NQ, ND, M>(a.maxl, cells) * sizeof(double);
+  const size_t lds = patch_lds_doubles_prec<NQ, ND, M, PREC>(a.maxl, cells) * sizeof(double);
   hipLaunchKernelGGL((rd_patch_kernel<D, NG, ND, NF, NQ, AFF, PH, SC, PREC>), dim3(grid), dim3(cells), lds, s, a);
   return hipGetLastError();
 }
@@ -86,6 +90,10 @@ KernelSet kernel_set()
   k.patch_batch = &launch_patch_batch<D, NG, ND, NF, NQ, AFF, PH, SC, 0>;
   k.patch_mixed = &launch_patch<D, NG, ND, NF, NQ, AFF, PH, SC, 1>;
   k.patch_batch_mixed = &launch_patch_batch<D, NG, ND, NF, NQ, AFF, PH, SC, 1>;
+  if constexpr (SC == SC_LDA || SC == SC_B) {
+    k.cell_nodal = &launch_cell<D, NG, ND, NF, NQ, AFF, PH, SC, 2>;
+    k.patch_nodal = &launch_patch<D, NG, ND, NF, NQ, AFF, PH, SC, 2>;
+  }
   k.patch_size = PatchSize<PhysTraits<PH>::M, ND>::value;
   k.patch_lds_bytes = &patch_lds_bytes<NQ, ND, PhysTraits<PH>::M>;
   return k;

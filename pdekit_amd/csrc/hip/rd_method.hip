@@ -19,6 +19,7 @@
 #include "../host/mesh.hpp"
 #include "../host/patches.hpp"
 #include "rd_launch.hpp"
+#include "rd_stab.hpp"
 #include "rd_weakbc.hpp"
 
 using pdrd::set_error;
@@ -46,6 +47,10 @@ struct Block
   uint32_t first_patch = 0, np = 0;
   int maxl = 0;
   int cells = 0;  // cells per patch (threads per workgroup)
+  // providers of the nodal inputs: vertex functions at the nodes [nd][nvert];
+  // smoothness-indicator rule w, V, dVg, psi packed
+  double *d_p1w = nullptr, *d_av = nullptr;
+  int av_nq = 0, av_npm = 0;
 };
 
 // A weak boundary condition as the caller gives it, and on the device.
@@ -90,6 +95,12 @@ struct pdrd_method
   int refresh = 50;
   int since_refresh = 1 << 30;
   double *d_corr = nullptr;
+  // nodal inputs of the cell loop: the external blending coefficient of the
+  // B scheme (set_blending_coeff) and the artificial viscosity of the LDA
+  // scheme (set_artificial_viscosity), device [N] each, and a scratch [N]
+  // for their providers
+  double *d_blend = nullptr, *d_visc = nullptr, *d_stab_tmp = nullptr;
+  bool use_blend = false, use_visc = false;
   int patch_cells = 0;  // requested cells per patch (0: each kernel's default)
   int block_threads = 256;
   int timing = 0;
@@ -307,8 +318,27 @@ int field_size(const pdrd_method *h, int field, size_t *count, double **dptr)
     case PDRD_TIME_STEP: *count = h->nn; *dptr = h->d_dt; return PDRD_OK;
     case PDRD_WAVE_SPEED: *count = h->nn; *dptr = h->d_ws; return PDRD_OK;
     case PDRD_DUAL_VOLUME: *count = h->nn; *dptr = h->d_vol; return PDRD_OK;
+    case PDRD_BLEND_COEFF:
+      if (!h->d_blend) return set_error(PDRD_EINVAL, "no blending coefficient has been set or computed");
+      *count = h->nn;
+      *dptr = h->d_blend;
+      return PDRD_OK;
+    case PDRD_ART_VISC:
+      if (!h->d_visc) return set_error(PDRD_EINVAL, "no artificial viscosity has been set or computed");
+      *count = h->nn;
+      *dptr = h->d_visc;
+      return PDRD_OK;
   }
   return set_error(PDRD_EINVAL, "unknown field");
+}
+
+// The nodal input the cells of this method take, or null: the blending
+// coefficient for the B scheme, the artificial viscosity for the LDA scheme.
+const double *nodal_input(const pdrd_method *h)
+{
+  if (h->scheme == pdrd::SC_B && h->use_blend) return h->d_blend;
+  if (h->scheme == pdrd::SC_LDA && h->use_visc) return h->d_visc;
+  return nullptr;
 }
 
 // Cell loop of the atomic / colour modes over every block (and colour),
@@ -317,6 +347,7 @@ int launch_cells(pdrd_method *h)
 {
   if (h->precision) return set_error(PDRD_ENOTSUP, "the mixed-precision cells run with the patch assembly only");
   pdrd::CellArgs a{};
+  a.nodal = nodal_input(h);
   a.coords = h->d_coords;
   a.u = h->d_u;
   a.acc_res = h->d_acc_res;
@@ -333,14 +364,16 @@ int launch_cells(pdrd_method *h)
       a.atomic = 1;
       a.first = 0;
       a.count = b.ne;
-      HIP_CHECK(b.ks.cell(a, grid_for(b.ne, h->block_threads), h->block_threads, h->stream));
+      HIP_CHECK((a.nodal ? b.ks.cell_nodal : b.ks.cell)(a, grid_for(b.ne, h->block_threads), h->block_threads,
+                                                        h->stream));
     } else {
       a.atomic = 0;
       for (size_t c = 0; c + 1 < b.colour_start.size(); ++c) {
         a.first = b.colour_start[c];
         a.count = b.colour_start[c + 1] - b.colour_start[c];
         if (a.count == 0) continue;
-        HIP_CHECK(b.ks.cell(a, grid_for(a.count, h->block_threads), h->block_threads, h->stream));
+        HIP_CHECK((a.nodal ? b.ks.cell_nodal : b.ks.cell)(a, grid_for(a.count, h->block_threads), h->block_threads,
+                                                          h->stream));
       }
     }
   }
@@ -380,6 +413,7 @@ pdrd::PatchArgs patch_args(const pdrd_method *h, const Block *b, int update, int
   a.slot_acc = h->d_slot_acc;
   a.slot_pos = h->d_slot_pos;
   a.norm_partial = h->d_patch_partial;
+  a.nodal = nodal_input(h);
   a.status = h->d_status;
   a.update = update;
   a.outputs = outputs;
@@ -402,12 +436,14 @@ pdrd::FixupArgs fixup_args(const pdrd_method *h, int update, int outputs, int ex
   return f;
 }
 
-// The patch kernels of every block (plain or mixed precision).
+// The patch kernels of every block (plain, mixed precision, or with the
+// nodal input of the scheme).
 int launch_patch_blocks(pdrd_method *h, int update, int outputs, int excl, double cfl)
 {
   for (Block &b : h->blocks) {
     const pdrd::PatchArgs a = patch_args(h, &b, update, outputs, excl, cfl);
-    HIP_CHECK((h->precision ? b.ks.patch_mixed : b.ks.patch)(a, b.np, b.cells, h->stream));
+    const pdrd::PatchLauncher f = a.nodal ? b.ks.patch_nodal : (h->precision ? b.ks.patch_mixed : b.ks.patch);
+    HIP_CHECK(f(a, b.np, b.cells, h->stream));
   }
   return PDRD_OK;
 }
@@ -616,6 +652,10 @@ void free_device(pdrd_method *h)
     (void)hipFree(b.d_sconn);
     (void)hipFree(b.d_pgconn);
     (void)hipFree(b.d_lconn);
+    (void)hipFree(b.d_p1w);
+    (void)hipFree(b.d_av);
+    b.d_p1w = nullptr;
+    b.d_av = nullptr;
     b.d_tab = nullptr;
     b.d_gconn = nullptr;
     b.d_sconn = nullptr;
@@ -642,6 +682,11 @@ void free_device(pdrd_method *h)
   (void)hipFree(h->d_patch_partial);
   (void)hipFree(h->d_corr);
   h->d_corr = nullptr;
+  for (double **q : {&h->d_blend, &h->d_visc, &h->d_stab_tmp}) {
+    (void)hipFree(*q);
+    *q = nullptr;
+  }
+  h->use_blend = h->use_visc = false;
   (void)hipFree(h->d_dir_index);
   h->d_pn_coff = nullptr;
   h->d_slot_acc = nullptr;
@@ -935,7 +980,10 @@ int pdrd_initialize(pdrd_method_t h)
       }
       bool shrunk = false;
       for (size_t i = 0; i < h->blocks.size(); ++i)
-        if (h->blocks[i].ks.patch_lds_bytes(plan.blocks[i].maxl, pin[i].patch_size) > 65536) {
+        // (room for one more double per node: the nodal input of the scheme)
+        if (h->blocks[i].ks.patch_lds_bytes(plan.blocks[i].maxl, pin[i].patch_size) +
+                static_cast<size_t>(plan.blocks[i].maxl) * sizeof(double) >
+            65536) {
           if (!h->patch_cells && pin[i].patch_size > 64) {
             pin[i].patch_size /= 2;
             shrunk = true;
@@ -1085,6 +1133,8 @@ int pdrd_set_option(pdrd_method_t h, int option, int value)
       break;
     case PDRD_OPT_PRECISION:
       if (value != 0 && value != 1) return set_error(PDRD_EINVAL, "precision is 0 (FP64) or 1 (mixed)");
+      if (value == 1 && nodal_input(h))
+        return set_error(PDRD_ENOTSUP, "the blending coefficient / artificial viscosity run with the FP64 cells");
       h->precision = value;
       h->since_refresh = 1 << 30;
       break;
@@ -1209,6 +1259,193 @@ int pdrd_weak_bc_info(pdrd_method_t h, int k, int64_t *info3)
   info3[1] = h->weak[k].nfn;
   info3[2] = h->weak[k].nq;
   return PDRD_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+// Per-block tables of the providers, built at their first use.
+int stab_tables(pdrd_method *h, Block &b, bool smoothness)
+{
+  const pdrd::Shape shape = static_cast<pdrd::Shape>(b.fe.shape);
+  const int p = b.fe.sol_order;
+  const pdrd::PointSet ps = static_cast<pdrd::PointSet>(b.fe.point_set);
+  if (b.fe.nd > pdrd::STAB_MAX_ND) return set_error(PDRD_ENOTSUP, "element too large for the providers");
+  if (!b.d_p1w) {
+    const std::vector<double> p1w = pdrd::vertex_functions(shape, pdrd::reference_nodes(shape, p, ps));
+    if (int rc = dev_alloc(h, &b.d_p1w, p1w.size())) return rc;
+    HIP_CHECK(hipMemcpy(b.d_p1w, p1w.data(), p1w.size() * sizeof(double), hipMemcpyHostToDevice));
+  }
+  if (smoothness && !b.d_av) {
+    if (shape == pdrd::Shape::Quad)
+      return set_error(PDRD_ENOTSUP, "the smoothness indicator is built for triangles and tetrahedra");
+    const pdrd::ElementTables t = pdrd::make_element_tables(shape, 1, p, std::max(2 * p, 1), ps);
+    int npm = 0;
+    const std::vector<double> psi = pdrd::lower_degree_basis(shape, p, t.qp, t.w, &npm);
+    if (t.nq > pdrd::STAB_MAX_NQ || npm > pdrd::STAB_MAX_NPM)
+      return set_error(PDRD_ENOTSUP, "smoothness-indicator rule too large");
+    std::vector<double> pack;
+    pack.insert(pack.end(), t.w.begin(), t.w.end());
+    pack.insert(pack.end(), t.V.begin(), t.V.end());
+    pack.insert(pack.end(), t.dVg.begin(), t.dVg.end());
+    pack.insert(pack.end(), psi.begin(), psi.end());
+    if (int rc = dev_alloc(h, &b.d_av, pack.size())) return rc;
+    HIP_CHECK(hipMemcpy(b.d_av, pack.data(), pack.size() * sizeof(double), hipMemcpyHostToDevice));
+    b.av_nq = t.nq;
+    b.av_npm = npm;
+  }
+  return PDRD_OK;
+}
+
+pdrd::StabArgs stab_args(const pdrd_method *h, const Block &b)
+{
+  pdrd::StabArgs a{};
+  a.sconn = b.d_sconn;
+  a.gconn = b.d_gconn;
+  a.coords = h->d_coords;
+  a.u = h->d_u;
+  a.p1w = b.d_p1w;
+  a.ne = b.ne;
+  a.m = h->m;
+  a.nd = b.fe.nd;
+  a.ng = b.fe.ng;
+  a.dim = h->dim;
+  a.nvert = pdrd::shape_nb_vertices(static_cast<pdrd::Shape>(b.fe.shape));
+  a.p = b.fe.sol_order;
+  if (b.d_av) {
+    a.nq = b.av_nq;
+    a.npm = b.av_npm;
+    a.w = b.d_av;
+    a.V = a.w + a.nq;
+    a.dVg = a.V + static_cast<size_t>(a.nq) * a.nd;
+    a.psi = a.dVg + static_cast<size_t>(h->dim) * a.nq * a.ng;
+  }
+  return a;
+}
+
+int stab_fill(pdrd_method *h, double *d, double v)
+{
+  hipLaunchKernelGGL(pdrd::stab_fill_kernel, dim3(grid_for(h->nn, 256)), dim3(256), 0, h->stream, d, h->nn, v);
+  HIP_CHECK(hipGetLastError());
+  return PDRD_OK;
+}
+
+// P1 / Q1 interpolation of the vertex values in `out` to the other nodes.
+int stab_interp(pdrd_method *h, double *out)
+{
+  for (Block &b : h->blocks) {
+    if (b.fe.sol_order == 1) continue;
+    pdrd::StabArgs a = stab_args(h, b);
+    a.out = out;
+    hipLaunchKernelGGL(pdrd::p1_interp_kernel, dim3(grid_for(b.ne, 256)), dim3(256), 0, h->stream, a);
+    HIP_CHECK(hipGetLastError());
+  }
+  return PDRD_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int pdrd_set_blending_coeff(pdrd_method_t h, const double *theta)
+{
+  if (int rc = check_ready(h)) return rc;
+  if (!theta) {
+    if (h->use_blend) drop_graph(h);
+    h->use_blend = false;
+    return PDRD_OK;
+  }
+  if (h->precision && h->scheme == pdrd::SC_B)
+    return set_error(PDRD_ENOTSUP, "the blending coefficient runs with the FP64 cells");
+  if (!h->d_blend)
+    if (int rc = dev_alloc(h, &h->d_blend, h->nn)) return rc;
+  HIP_CHECK(hipMemcpyAsync(h->d_blend, theta, h->nn * sizeof(double), hipMemcpyHostToDevice, h->stream));
+  HIP_CHECK(hipStreamSynchronize(h->stream));
+  if (!h->use_blend) drop_graph(h);
+  h->use_blend = true;
+  return PDRD_OK;
+}
+
+int pdrd_set_artificial_viscosity(pdrd_method_t h, const double *nu)
+{
+  if (int rc = check_ready(h)) return rc;
+  if (!nu) {
+    if (h->use_visc) drop_graph(h);
+    h->use_visc = false;
+    return PDRD_OK;
+  }
+  if (h->precision && h->scheme == pdrd::SC_LDA)
+    return set_error(PDRD_ENOTSUP, "the artificial viscosity runs with the FP64 cells");
+  if (!h->d_visc)
+    if (int rc = dev_alloc(h, &h->d_visc, h->nn)) return rc;
+  HIP_CHECK(hipMemcpyAsync(h->d_visc, nu, h->nn * sizeof(double), hipMemcpyHostToDevice, h->stream));
+  HIP_CHECK(hipStreamSynchronize(h->stream));
+  if (!h->use_visc) drop_graph(h);
+  h->use_visc = true;
+  return PDRD_OK;
+}
+
+int pdrd_compute_blending_coeff(pdrd_method_t h, int kind)
+{
+  PDRD_TRY
+  if (int rc = check_ready(h)) return rc;
+  if (kind != PDRD_BLEND_LF) return set_error(PDRD_EINVAL, "unknown blending coefficient");
+  if (h->precision && h->scheme == pdrd::SC_B)
+    return set_error(PDRD_ENOTSUP, "the blending coefficient runs with the FP64 cells");
+  for (Block &b : h->blocks)
+    if (int rc = stab_tables(h, b, false)) return rc;
+  if (!h->d_blend)
+    if (int rc = dev_alloc(h, &h->d_blend, h->nn)) return rc;
+  if (!h->d_stab_tmp)
+    if (int rc = dev_alloc(h, &h->d_stab_tmp, h->nn)) return rc;
+  if (int rc = stab_fill(h, h->d_stab_tmp, 0.0)) return rc;
+  if (int rc = stab_fill(h, h->d_blend, 0.0)) return rc;
+  for (Block &b : h->blocks) {
+    pdrd::StabArgs a = stab_args(h, b);
+    a.out = h->d_stab_tmp;
+    hipLaunchKernelGGL(pdrd::lf_indicator_kernel, dim3(grid_for(b.ne, 256)), dim3(256), 0, h->stream, a);
+    HIP_CHECK(hipGetLastError());
+  }
+  for (Block &b : h->blocks) {
+    pdrd::StabArgs a = stab_args(h, b);
+    a.in = h->d_stab_tmp;
+    a.out = h->d_blend;
+    hipLaunchKernelGGL(pdrd::lf_cell_kernel, dim3(grid_for(b.ne, 256)), dim3(256), 0, h->stream, a);
+    HIP_CHECK(hipGetLastError());
+  }
+  if (int rc = stab_interp(h, h->d_blend)) return rc;
+  if (!h->use_blend) drop_graph(h);
+  h->use_blend = true;
+  return PDRD_OK;
+  PDRD_CATCH
+}
+
+int pdrd_compute_artificial_viscosity(pdrd_method_t h, double kappa, double factor)
+{
+  PDRD_TRY
+  if (int rc = check_ready(h)) return rc;
+  if (!(kappa > 0.0) || !(factor >= 0.0)) return set_error(PDRD_EINVAL, "kappa must be > 0 and factor >= 0");
+  if (h->precision && h->scheme == pdrd::SC_LDA)
+    return set_error(PDRD_ENOTSUP, "the artificial viscosity runs with the FP64 cells");
+  for (Block &b : h->blocks)
+    if (int rc = stab_tables(h, b, true)) return rc;
+  if (!h->d_visc)
+    if (int rc = dev_alloc(h, &h->d_visc, h->nn)) return rc;
+  if (int rc = stab_fill(h, h->d_visc, 0.0)) return rc;
+  for (Block &b : h->blocks) {
+    pdrd::StabArgs a = stab_args(h, b);
+    a.out = h->d_visc;
+    a.kappa = kappa;
+    a.factor = factor;
+    hipLaunchKernelGGL(pdrd::av_cell_kernel, dim3(grid_for(b.ne, 256)), dim3(256), 0, h->stream, a);
+    HIP_CHECK(hipGetLastError());
+  }
+  if (int rc = stab_interp(h, h->d_visc)) return rc;
+  if (!h->use_visc) drop_graph(h);
+  h->use_visc = true;
+  return PDRD_OK;
+  PDRD_CATCH
 }
 
 int pdrd_add_strong_dirichlet(pdrd_method_t h, uint32_t n, const uint32_t *nodes, const double *values)
@@ -1583,6 +1820,9 @@ int pdrd_batch_iterate(pdrd_batch_t b, int count, const pdrd_method_t *methods, 
     if (h->scatter_mode != 2) return set_error(PDRD_ENOTSUP, "batched iterations need the patch assembly");
     if (corrects(h))
       return set_error(PDRD_ENOTSUP, "batched iterations of mixed-precision methods need PDRD_OPT_REFRESH = 0");
+    if (nodal_input(h))
+      return set_error(PDRD_ENOTSUP, "batched iterations of methods with a blending coefficient / artificial "
+                                     "viscosity are not built");
     for (int j = 0; j < i; ++j)
       if (methods[j] == h) return set_error(PDRD_EINVAL, "a method appears twice in one batch");
   }

@@ -1,5 +1,6 @@
 #include "fe_tables.hpp"
 
+#include <array>
 #include <cmath>
 #include <stdexcept>
 
@@ -577,6 +578,66 @@ void pack(const ElementTables &t, double *out)
   for (const auto *p : parts) {
     for (double v : *p) out[off++] = v;
   }
+}
+
+std::vector<double> lower_degree_basis(Shape s, int p, const std::vector<double> &qp, const std::vector<double> &w,
+                                       int *npm)
+{
+  if (s == Shape::Quad) fail("the smoothness indicator is built for simplices");
+  const int dim = shape_dim(s);
+  const int nq = static_cast<int>(w.size());
+  std::vector<std::array<int, 3>> exps;
+  for (int i = 0; i < p; ++i)
+    for (int j = 0; j + i < p; ++j)
+      if (dim == 2)
+        exps.push_back({i, j, 0});
+      else
+        for (int k = 0; i + j + k < p; ++k) exps.push_back({i, j, k});
+  const int n = static_cast<int>(exps.size());
+  std::vector<long double> B(static_cast<size_t>(nq) * n);
+  for (int q = 0; q < nq; ++q)
+    for (int k = 0; k < n; ++k) {
+      long double v = 1.0L;
+      for (int d = 0; d < dim; ++d)
+        for (int e = 0; e < exps[k][d]; ++e) v *= qp[static_cast<size_t>(q) * dim + d];
+      B[static_cast<size_t>(q) * n + k] = v;
+    }
+  auto dot = [&](int a, int b) {
+    long double t = 0.0L;
+    for (int q = 0; q < nq; ++q) t += w[q] * B[static_cast<size_t>(q) * n + a] * B[static_cast<size_t>(q) * n + b];
+    return t;
+  };
+  for (int k = 0; k < n; ++k) {
+    for (int pass = 0; pass < 2; ++pass)
+      for (int j = 0; j < k; ++j) {
+        const long double c = dot(k, j);
+        for (int q = 0; q < nq; ++q) B[static_cast<size_t>(q) * n + k] -= c * B[static_cast<size_t>(q) * n + j];
+      }
+    const long double nrm = std::sqrt(dot(k, k));
+    if (!(nrm > 0.0L)) fail("degenerate lower-degree basis");
+    for (int q = 0; q < nq; ++q) B[static_cast<size_t>(q) * n + k] /= nrm;
+  }
+  *npm = n;
+  return std::vector<double>(B.begin(), B.end());
+}
+
+std::vector<double> vertex_functions(Shape s, const std::vector<double> &pts)
+{
+  const int dim = shape_dim(s);
+  const size_t np = pts.size() / dim;
+  std::vector<double> out;
+  for (size_t i = 0; i < np; ++i) {
+    const double x = pts[i * dim], y = pts[i * dim + 1];
+    if (s == Shape::Triag) {
+      out.insert(out.end(), {-(x + y) / 2, (1 + x) / 2, (1 + y) / 2});
+    } else if (s == Shape::Quad) {
+      out.insert(out.end(), {(1 - x) * (1 - y) / 4, (1 + x) * (1 - y) / 4, (1 + x) * (1 + y) / 4, (1 - x) * (1 + y) / 4});
+    } else {
+      const double z = pts[i * dim + 2];
+      out.insert(out.end(), {-(1 + x + y + z) / 2, (1 + x) / 2, (1 + y) / 2, (1 + z) / 2});
+    }
+  }
+  return out;
 }
 
 }  // namespace pdrd

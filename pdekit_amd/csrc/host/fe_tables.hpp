@@ -93,6 +93,19 @@ struct ElementTables
 ElementTables make_element_tables(Shape s, int geo_order, int sol_order, int quad_order,
                                   PointSet ps);
 
+// An orthonormal basis, under the rule (qp[nq][dim], w[nq]), of the
+// polynomials of total degree < p on the reference simplex: psi[nq][npm]
+// (modified Gram-Schmidt, twice, of the monomials in long double). The L2
+// projection onto them is what the smoothness indicator of
+// ArtificialViscosity.hpp:371-540 takes from the modal expansion by leaving
+// out its leading terms.
+std::vector<double> lower_degree_basis(Shape s, int p, const std::vector<double> &qp, const std::vector<double> &w,
+                                       int *npm);
+
+// P1 / Q1 shape functions of the reference vertices at reference points
+// pts[n][dim]: [n][nvert].
+std::vector<double> vertex_functions(Shape s, const std::vector<double> &pts);
+
 // Packed layout used by the C ABI: the arrays above concatenated in this
 // order: qp, w, Vg, dVg, V, dV, Vf, dVf, Vgf, Vsf, sol_nodes, flux_nodes.
 size_t packed_size(const ElementTables &t);

@@ -21,7 +21,8 @@ PHYSICS = {"Advection2D": L.ADVECTION2D, "Burgers2D": L.BURGERS2D, "Euler2DCons"
 SCHEMES = {"PGLDA": L.LDA, "PGN": L.N, "PGB": L.B, "PSI": L.PSI}
 NEQ = {L.ADVECTION2D: 1, L.BURGERS2D: 1, L.EULER2D: 4, L.EULER3D: 5}
 FIELDS = {"solution": L.SOLUTION, "residuals": L.RESIDUAL, "time_step": L.TIME_STEP,
-          "wave_speed": L.WAVE_SPEED, "dual_volume": L.DUAL_VOLUME}
+          "wave_speed": L.WAVE_SPEED, "dual_volume": L.DUAL_VOLUME, "blending_coeff": L.BLEND_COEFF,
+          "artificial_viscosity": L.ART_VISC}
 
 
 class PGRDMethodExplicit:
@@ -210,6 +211,41 @@ class PGRDMethodExplicit:
         L.check(L.lib().pdrd_method_info(self._h, L.ptr(a, C.c_int64)), "pdrd_method_info")
         return dict(nb_nodes=int(a[0]), neq=int(a[1]), nb_blocks=int(a[2]), nb_cells=int(a[3]),
                     nb_colours=int(a[4]), device_bytes=int(a[5]))
+
+    # -- nodal inputs of the cell loop ------------------------------------------------------
+    def set_blending_coeff(self, theta):
+        """set_blending_coeff: nodal theta [N] that the B scheme uses instead
+        of its cell-wise theta (PGB.hpp:141-148, 467-477); None switches it off."""
+        if theta is None:
+            L.check(L.lib().pdrd_set_blending_coeff(self._h, None), "pdrd_set_blending_coeff")
+            return
+        t = np.ascontiguousarray(np.asarray(theta, dtype=np.float64).reshape(self.mesh.nb_nodes))
+        L.check(L.lib().pdrd_set_blending_coeff(self._h, L.ptr(t)), "pdrd_set_blending_coeff")
+
+    def set_artificial_viscosity(self, nu):
+        """set_artificial_viscosity: nodal nu [N] of the LDA scheme's
+        shock-capturing term (PGLDA.hpp:245-256); None switches it off."""
+        if nu is None:
+            L.check(L.lib().pdrd_set_artificial_viscosity(self._h, None), "pdrd_set_artificial_viscosity")
+            return
+        v = np.ascontiguousarray(np.asarray(nu, dtype=np.float64).reshape(self.mesh.nb_nodes))
+        L.check(L.lib().pdrd_set_artificial_viscosity(self._h, L.ptr(v)), "pdrd_set_artificial_viscosity")
+
+    def compute_blending_coeff(self, kind: str = "LF_Blend") -> np.ndarray:
+        """RDBlendingCoeff::calculate(kind) on the device from the current
+        solution, kept as the method's blending coefficient; returns it."""
+        if kind != "LF_Blend":
+            raise NotImplementedError(f"blending coefficient {kind} is not built on the GPU path")
+        L.check(L.lib().pdrd_compute_blending_coeff(self._h, L.BLEND_LF), "pdrd_compute_blending_coeff")
+        return self.vec_function("blending_coeff")
+
+    def compute_artificial_viscosity(self, kappa: float = 5.0, factor: float = 1.0) -> np.ndarray:
+        """ArtificialViscosity::compute_artificial_viscosity_nodal on the
+        device from the current solution, kept as the method's artificial
+        viscosity; returns it."""
+        L.check(L.lib().pdrd_compute_artificial_viscosity(self._h, float(kappa), float(factor)),
+                "pdrd_compute_artificial_viscosity")
+        return self.vec_function("artificial_viscosity")
 
     def weak_bc_info(self, k: int) -> dict:
         a = np.zeros(3, dtype=np.int64)

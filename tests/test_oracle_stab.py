@@ -209,3 +209,38 @@ def test_art_visc_provider_high_order_projection(p):
     vids = np.unique(cn[:, :3])
     assert np.abs(got[vids] - vert[vids]).max() <= 1e-10 * vert.max()
     assert vert.max() > 0
+
+
+def test_art_visc_provider_p2_tetrahedra():
+    """P2 tetrahedra (rule of order 4, with a negative weight): a state linear
+    on every cell has no high modes; otherwise the vertices carry the largest
+    cell value, here checked against an independent least-squares projection
+    onto P1 in physical coordinates."""
+    mesh = box_tetra(3, 3, 2, order=2, jitter=0.1, seed=4)
+    P = oracle_problem(mesh, O.EULER3D, O.LDA, (1.4, 287.05))
+    x = mesh.node_coords
+    u = np.zeros((mesh.nb_nodes, 5))
+    u[:, 0] = 1.0 + 0.5 * x[:, 0] - 0.25 * x[:, 1] + 0.1 * x[:, 2]
+    u[:, 4] = 2.5
+    assert np.abs(P.art_visc(u)).max() == 0.0
+    u[:, 0] = 1.0 + 0.3 * np.sin(9 * x[:, 0] + 2 * x[:, 1] - 3 * x[:, 2])
+    t = element_tables(L.TETRA, 2, 4)
+    conn, geo = mesh.blocks[0].sol_conn, mesh.blocks[0].geo_conn
+    Se = np.zeros(conn.shape[0])
+    vol = np.zeros(conn.shape[0])
+    for c in range(conn.shape[0]):
+        X = mesh.coords[geo[c]]
+        xq = t.Vg @ X
+        vol[c] = abs(np.linalg.det(X[1:] - X[0])) / 6
+        wq = t.w * vol[c] / (4.0 / 3.0)
+        uq = t.V @ u[conn[c], 0]
+        A = np.concatenate([np.ones((xq.shape[0], 1)), xq], 1)
+        G = A.T @ (wq[:, None] * A)
+        coef = np.linalg.solve(G, A.T @ (wq * uq))
+        Se[c] = (wq * (uq - A @ coef) ** 2).sum() / (wq * uq ** 2).sum()
+    vert = _cell_to_node(mesh, [_nu_of(Se, vol, 3, 2)])
+    got = P.art_visc(u)
+    vids = np.unique(conn[:, :4])
+    # the rule of order 4 carries 10 significant digits
+    assert vert.max() > 0
+    assert np.abs(got[vids] - vert[vids]).max() <= 1e-8 * vert.max()
