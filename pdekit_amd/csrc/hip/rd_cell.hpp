@@ -398,7 +398,10 @@ __device__ __forceinline__ void cell_residual(const double *__restrict__ tab, co
   constexpr bool NDIS = (SC != SC_LDA);
   double lda[ND][M];
   double dis[NDIS ? ND : 1][M];
-  double lmax = 0.0;  // max over nodes and quadrature points of the node's wave speed
+  // max over nodes and quadrature points of |eigenvalue|; the floor 1e-6 of
+  // every (node, point) term is applied once at the end (max(1e-6, max x) =
+  // max over x of max(1e-6, x))
+  double lmax = 0.0;
   double fint[M];
 #pragma unroll
   for (int n = 0; n < ND; ++n) {
@@ -491,10 +494,8 @@ __device__ __forceinline__ void cell_residual(const double *__restrict__ tab, co
 #pragma unroll
         for (int xi = 0; xi < D; ++xi) knt += c[xi] * dvk[xi * ND + n];
         const double kn = knt;
-        const double ak = fabs(kn);
         kp[n] = fmax(kn, 0.0);
-        const double l = ak > 1.e-6 ? ak : 1.e-6;
-        lmax = fmax(lmax, l);
+        lmax = fmax(lmax, fabs(kn));
         S += kp[n];
         SU += kp[n] * Uq(n, 0);
       }
@@ -594,7 +595,7 @@ __device__ __forceinline__ void cell_residual(const double *__restrict__ tab, co
       for (int n = 0; n < ND; ++n) {
         const NodeK k = node_k(n);
         if constexpr (KEEP) nk[n] = k;
-        lmax = fmax(lmax, fmax(1.e-6, static_cast<double>(k.emax)));
+        lmax = fmax(lmax, static_cast<double>(k.emax));
         sl0 += k.l0;
         ssg += k.sg;
 #pragma unroll
@@ -793,7 +794,7 @@ __device__ __forceinline__ void cell_residual(const double *__restrict__ tab, co
     }
   }
 
-  wave = vol * lmax;
+  wave = vol * fmax(lmax, 1.e-6);
 }
 
 template <int D, int NG, int NQ, int M>
