@@ -79,6 +79,7 @@ def lib():
             "orc_euler_eigen": [C.c_int, C.c_double, PD, PD, PD, PD, PD],
             "orc_euler_kplus": [C.c_int, C.c_double, PD, PD, PD, PD],
             "orc_lf_blend": [PP, PD, C.POINTER(PD), C.POINTER(C.c_int), PD],
+            "orc_cell_jacobians": [PP, C.c_int, PD, PD, PD, PD],
             "orc_art_visc": [PP, PD, C.POINTER(OrcAvRule), C.POINTER(PD), C.POINTER(C.c_int), C.c_double,
                              C.c_double, PD],
         }.items():
@@ -228,6 +229,17 @@ class Problem:
         wave = np.zeros(b.ne)
         rc = lib().orc_cell_residuals(C.byref(self.p), block, _p(u), _p(phi), _p(wave))
         return phi, wave, rc
+
+    def cell_jacobians(self, block, u):
+        """compute_adv_res_and_jacobian of every cell of one block:
+        phi[ne][nd][m], jac[ne][nd*m][nd*m], wave[ne], rc."""
+        u = np.ascontiguousarray(u, dtype=np.float64).reshape(self.nn, self.m)
+        b = self.p.blocks[block]
+        phi = np.zeros((b.ne, b.nd, self.m))
+        jac = np.zeros((b.ne, b.nd * self.m, b.nd * self.m))
+        wave = np.zeros(b.ne)
+        rc = lib().orc_cell_jacobians(C.byref(self.p), block, _p(u), _p(phi), _p(jac), _p(wave))
+        return phi, jac, wave, rc
 
     def iterate(self, u, cfl, n_iter, dir_nodes=None, dir_values=None):
         u = np.array(u, dtype=np.float64).reshape(self.nn, self.m)

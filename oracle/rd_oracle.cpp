@@ -316,7 +316,16 @@ struct CellOut
   bool bad;
 };
 
-void cell(const orc_problem *p, const orc_block &b, uint32_t c, const double *u, CellOut &out)
+// One cell. With jac (ND*m x ND*m, row-major) the implicit forms of
+// compute_adv_res_and_jacobian: PGLDA.hpp:315-505 (LDA residual, J_mn =
+// sum_q w K+_m (sum K+)^-1 K_n), PGN.hpp:294-400 (Phi_m = sum_{n!=m} w K+_m
+// (sum K-)^-1 K-_n (u_m - u_n), J_mn = -w K+_m (sum K-)^-1 K-_n,
+// J_mm = w (K+_m - K+_m (sum K-)^-1 K-_m)), PGB.hpp:503-800 (the B residual,
+// J = J_LDA + theta_m J_stab with J_stab,mn = -w K+_m (sum K+)^-1 K+_n,
+// J_stab,mm = w (K+_m - K+_m (sum K+)^-1 K+_m)). Returns false for PSI,
+// which has no implicit form.
+bool cell(const orc_problem *p, const orc_block &b, uint32_t c, const double *u, CellOut &out,
+          double *jac = nullptr)
 {
   const int D = p->dim, m = neq_of(p->physics);
   const int ng = b.ng, nd = b.nd, nf = b.nf, nq = b.nq;
@@ -429,6 +438,19 @@ void cell(const orc_problem *p, const orc_block &b, uint32_t c, const double *u,
   const bool skip_dissipation = ext_theta && max_theta < 1.e-6;
 
   static thread_local Mat Kp[MAXN];
+  // implicit forms: full K_n and K-_n, the Jacobian and its stabilisation part
+  static thread_local Mat Kf[MAXN], Km[MAXN];
+  const bool implicit = jac != nullptr;
+  if (implicit && p->scheme == ORC_PSI) return false;
+  const int ndm = nd * m;
+  std::vector<double> jstab;
+  if (implicit) {
+    std::fill(jac, jac + static_cast<size_t>(ndm) * ndm, 0.0);
+    if (p->scheme == ORC_B) jstab.assign(static_cast<size_t>(ndm) * ndm, 0.0);
+  }
+  double nres[MAXN][MAXM];
+  for (int n = 0; n < nd; ++n)
+    for (int e = 0; e < m; ++e) nres[n][e] = 0.0;
   for (int q = 0; q < nq; ++q) {
     const double wj = b.w[q] * detJ[q];
     elem_volume += wj;
@@ -478,8 +500,16 @@ void cell(const orc_problem *p, const orc_block &b, uint32_t c, const double *u,
       }
       ws[n] = std::max(ws[n], lmax);
       product_RDL(m, R, Dp, L, Kp[n]);
+      if (implicit) {
+        double Dm[MAXM];
+        for (int e = 0; e < m; ++e) Dm[e] = std::min(0.0, Dv[e]);
+        product_RDL(m, R, Dv, L, Kf[n]);
+        product_RDL(m, R, Dm, L, Km[n]);
+      }
+      // the implicit N scheme inverts the sum of the K-
+      const Mat &Ks = (implicit && p->scheme == ORC_N) ? Km[n] : Kp[n];
       for (int i = 0; i < m; ++i)
-        for (int j = 0; j < m; ++j) sumKp[i][j] += Kp[n][i][j];
+        for (int j = 0; j < m; ++j) sumKp[i][j] += Ks[i][j];
     }
     double a[MAXM * MAXM], inv[MAXM * MAXM];
     for (int i = 0; i < m; ++i)
@@ -487,6 +517,92 @@ void cell(const orc_problem *p, const orc_block &b, uint32_t c, const double *u,
     invert(m, a, inv);
     const double *r = divF[q];
     for (int e = 0; e < m; ++e) flux_integral[e] += wj * r[e];
+    if (implicit) {
+      auto jadd = [&](std::vector<double> *dst, int mm, int nn, const double (&B)[MAXM][MAXM], double f) {
+        for (int i = 0; i < m; ++i)
+          for (int j = 0; j < m; ++j) {
+            const size_t k = static_cast<size_t>(mm * m + i) * ndm + nn * m + j;
+            if (dst)
+              (*dst)[k] += f * B[i][j];
+            else
+              jac[k] += f * B[i][j];
+          }
+      };
+      for (int mm = 0; mm < nd; ++mm) {
+        // beta_m = K+_m (sum)^-1
+        double KI[MAXM][MAXM];
+        for (int i = 0; i < m; ++i)
+          for (int j = 0; j < m; ++j) {
+            double s2 = 0.0;
+            for (int k = 0; k < m; ++k) s2 += Kp[mm][i][k] * inv[k * m + j];
+            KI[i][j] = s2;
+          }
+        if (p->scheme != ORC_N) {
+          // LDA residual (after the shock-capturing term above)
+          for (int i = 0; i < m; ++i) {
+            double s2 = 0.0;
+            for (int j = 0; j < m; ++j) s2 += KI[i][j] * r[j];
+            lda[mm][i] += wj * s2;
+          }
+        }
+        for (int nn = 0; nn < nd; ++nn) {
+          const Mat &Kn = (p->scheme == ORC_LDA || p->scheme == ORC_B) ? Kf[nn] : Km[nn];
+          double P[MAXM][MAXM];
+          for (int i = 0; i < m; ++i)
+            for (int j = 0; j < m; ++j) {
+              double s2 = 0.0;
+              for (int k = 0; k < m; ++k) s2 += KI[i][k] * Kn[k][j];
+              P[i][j] = s2;
+            }
+          if (p->scheme == ORC_LDA || p->scheme == ORC_B) {
+            jadd(nullptr, mm, nn, P, wj);
+          } else {
+            if (nn != mm) {
+              double du[MAXM];
+              for (int e = 0; e < m; ++e) du[e] = U[mm][e] - U[nn][e];
+              for (int i = 0; i < m; ++i) {
+                double s2 = 0.0;
+                for (int j = 0; j < m; ++j) s2 += P[i][j] * du[j];
+                nres[mm][i] += wj * s2;
+              }
+              jadd(nullptr, mm, nn, P, -wj);
+            } else {
+              double D2[MAXM][MAXM];
+              for (int i = 0; i < m; ++i)
+                for (int j = 0; j < m; ++j) D2[i][j] = Kp[mm][i][j] - P[i][j];
+              jadd(nullptr, mm, nn, D2, wj);
+            }
+          }
+        }
+        if (p->scheme == ORC_B && !skip_dissipation) {
+          for (int nn = 0; nn < nd; ++nn) {
+            double P[MAXM][MAXM];
+            for (int i = 0; i < m; ++i)
+              for (int j = 0; j < m; ++j) {
+                double s2 = 0.0;
+                for (int k = 0; k < m; ++k) s2 += KI[i][k] * Kp[nn][k][j];
+                P[i][j] = s2;
+              }
+            if (nn != mm) {
+              double du[MAXM];
+              for (int e = 0; e < m; ++e) du[e] = U[mm][e] - U[nn][e];
+              for (int i = 0; i < m; ++i) {
+                double s2 = 0.0;
+                for (int j = 0; j < m; ++j) s2 += P[i][j] * du[j];
+                ndis[mm][i] += wj * s2;
+              }
+              jadd(&jstab, mm, nn, P, -wj);
+            } else {
+              double D2[MAXM][MAXM];
+              for (int i = 0; i < m; ++i)
+                for (int j = 0; j < m; ++j) D2[i][j] = Kp[mm][i][j] - P[i][j];
+              jadd(&jstab, mm, nn, D2, wj);
+            }
+          }
+        }
+      }
+      continue;
+    }
     // K+_n (sum K+)^-1 r
     double Ir[MAXM];
     for (int i = 0; i < m; ++i) {
@@ -529,7 +645,26 @@ void cell(const orc_problem *p, const orc_block &b, uint32_t c, const double *u,
     (void)Ir;
   }
 
-  if (p->scheme == ORC_LDA) {
+  if (implicit && p->scheme == ORC_N) {
+    for (int n = 0; n < nd; ++n)
+      for (int e = 0; e < m; ++e) out.phi[n][e] = nres[n][e];
+  } else if (implicit && p->scheme == ORC_B) {
+    double th[MAXN];
+    if (ext_theta) {
+      for (int n = 0; n < nd; ++n) th[n] = theta_n[n];
+    } else {
+      double abs_phiN = 0.0;
+      for (int n = 0; n < nd; ++n) abs_phiN += std::fabs(lda[n][0] + ndis[n][0]);
+      const double t0 = std::fabs(flux_integral[0]) / (abs_phiN + 1.e-6);
+      for (int n = 0; n < nd; ++n) th[n] = t0;
+    }
+    for (int n = 0; n < nd; ++n)
+      for (int e = 0; e < m; ++e) out.phi[n][e] = lda[n][e] + th[n] * ndis[n][e];
+    if (!skip_dissipation)
+      for (int mm = 0; mm < ndm; ++mm)
+        for (int k = 0; k < ndm; ++k)
+          jac[static_cast<size_t>(mm) * ndm + k] += th[mm / m] * jstab[static_cast<size_t>(mm) * ndm + k];
+  } else if (p->scheme == ORC_LDA) {
     for (int n = 0; n < nd; ++n)
       for (int e = 0; e < m; ++e) out.phi[n][e] = lda[n][e];
   } else if (p->scheme == ORC_N) {
@@ -569,6 +704,7 @@ void cell(const orc_problem *p, const orc_block &b, uint32_t c, const double *u,
   double wmax = ws[0];
   for (int n = 1; n < nd; ++n) wmax = std::max(wmax, ws[n]);
   out.wave = elem_volume * wmax;
+  return true;
 }
 
 }  // namespace
@@ -641,6 +777,23 @@ int orc_dual_volume(const orc_problem *p, double *vdual)
     }
   }
   return 0;
+}
+
+int orc_cell_jacobians(const orc_problem *p, int block, const double *u, double *phi, double *jac, double *wave)
+{
+  const int m = neq_of(p->physics);
+  const orc_block &b = p->blocks[block];
+  const size_t ndm = static_cast<size_t>(b.nd) * m;
+  int rc = 0;
+  CellOut out;
+  for (uint32_t c = 0; c < b.ne; ++c) {
+    if (!cell(p, b, c, u, out, jac + c * ndm * ndm)) return -1;
+    if (out.bad) rc = -5;
+    for (int n = 0; n < b.nd; ++n)
+      for (int e = 0; e < m; ++e) phi[(static_cast<size_t>(c) * b.nd + n) * m + e] = out.phi[n][e];
+    wave[c] = out.wave;
+  }
+  return rc;
 }
 
 int orc_cell_residuals(const orc_problem *p, int block, const double *u, double *phi, double *wave)

@@ -96,6 +96,12 @@ int orc_assemble_rhs(const orc_problem *p, const double *u, const double *vdual,
 int orc_cell_residuals(const orc_problem *p, int block, const double *u, double *phi,
                        double *wave);
 
+/* compute_adv_res_and_jacobian of every cell of one block (the implicit
+ * forms of PGLDA / PGN / PGB): phi[ne][nd][m], jac[ne][nd*m][nd*m]
+ * (row-major, row = node * m + equation), wave[ne]. Returns -1 for PSI. */
+int orc_cell_jacobians(const orc_problem *p, int block, const double *u, double *phi, double *jac,
+                       double *wave);
+
 /* n explicit iterations of assemble_lhs_and_rhs(CFL) + solve() with strong
  * Dirichlet values g[nb][m] written into u after every cell loop
  * (StrongDirichletBC::apply). norms[n][m] receives norm_L2(residual) over all
