@@ -237,6 +237,15 @@ enum pdrd_blend_kind { PDRD_BLEND_LF = 0 };
 int pdrd_compute_blending_coeff(pdrd_method_t h, int kind);
 int pdrd_compute_artificial_viscosity(pdrd_method_t h, double kappa, double factor);
 
+/* compute_adv_res_and_jacobian (PGLDA.hpp:315-505, PGN.hpp:294-400,
+ * PGB.hpp:503-800): for every cell of block `block` (the order of its
+ * pdrd_add_cells call) the implicit form's element residual phi[ne][nd][m],
+ * the element Jacobian jac[ne][nd*m][nd*m] (row-major, row = node * m +
+ * equation) and the element wave speed wave[ne], from the current solution.
+ * Host buffers; a NULL one is not copied. PSI has no implicit form
+ * (PDRD_ENOTSUP). */
+int pdrd_element_jacobians(pdrd_method_t h, int block, double *phi, double *jac, double *wave);
+
 /* add_boundary_condition("StrongDirichlet", ...) with precomputed values:
  * u[nodes[k]] = values[k][0..m) after every cell loop
  * (StrongDirichletBC::apply, StrongDirichletBC.hpp:202-233). */

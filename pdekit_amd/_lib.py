@@ -45,7 +45,7 @@ EXPORTED = (
     "pdrd_batch_create", "pdrd_batch_destroy", "pdrd_batch_iterate", "pdrd_add_weak_bc", "pdrd_weak_bc_info",
     "pdrd_mesh_read_gmsh", "pdrd_mesh_write_gmsh", "pdrd_gmsh_append_node_data", "pdrd_gmsh_read_node_data",
     "pdrd_set_blending_coeff", "pdrd_set_artificial_viscosity", "pdrd_compute_blending_coeff",
-    "pdrd_compute_artificial_viscosity",
+    "pdrd_compute_artificial_viscosity", "pdrd_element_jacobians",
 )
 
 
@@ -127,6 +127,7 @@ def lib() -> C.CDLL:
         "pdrd_set_artificial_viscosity": [P, PD],
         "pdrd_compute_blending_coeff": [P, I],
         "pdrd_compute_artificial_viscosity": [P, D, D],
+        "pdrd_element_jacobians": [P, I, PD, PD, PD],
         "pdrd_check_status": [P],
         "pdrd_method_info": [P, PI64],
         "pdrd_last_timing": [P, PD, PD],

@@ -1,0 +1,451 @@
+// Element residuals and Jacobians of the implicit forms of the schemes
+// (compute_adv_res_and_jacobian):
+//   LDA (PGLDA.hpp:315-505): Phi_m = sum_q w K+_m S+^-1 r_q (+ the
+//     shock-capturing term), J_mn = sum_q w K+_m S+^-1 K_n;
+//   N (PGN.hpp:294-400): Phi_m = sum_q sum_{n != m} w K+_m S-^-1 K-_n (u_m - u_n),
+//     J_mn = -w K+_m S-^-1 K-_n, J_mm = w (K+_m - K+_m S-^-1 K-_m);
+//   B (PGB.hpp:503-800): the B residual lda + theta_m n_m, J = J_LDA +
+//     theta_m J_stab, J_stab,mn = -w K+_m S+^-1 K+_n, J_stab,mm =
+//     w (K+_m - K+_m S+^-1 K+_m);
+// with S+- = sum_n K+-_n, K_n = A(u_q) . grad phi_n, theta the nodal input or
+// the cell-wise theta of equation 0. One thread per cell, FP64. K+- of the
+// Euler equations come from the compact form l0 x + alpha r0 + beta e_n (see
+// cell_residual) applied to the unit vectors; K = K+ + K-. The flux
+// divergence at the quadrature points is the P+1 flux element's, as in
+// cell_residual. Element blocks are written row-major (row = node * m +
+// equation) to the caller's cell slot perm[e]; the Jacobian is accumulated
+// in place, the stabilisation part of B in a scratch block of the same size.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+#include "rd_cell.hpp"
+
+namespace pdrd {
+
+struct JacArgs
+{
+  const double *__restrict__ tab;
+  const uint32_t *__restrict__ gconn;  // [NG][ne]
+  const uint32_t *__restrict__ sconn;  // [ND][ne]
+  const uint32_t *__restrict__ perm;   // [ne] caller cell index of each block position
+  const double *__restrict__ coords;
+  const double *__restrict__ u;
+  const double *__restrict__ nodal;    // [N] blending coefficient (B) / artificial viscosity (LDA), or null
+  double *__restrict__ phi;            // [ne][ND][M]
+  double *__restrict__ jac;            // [ne][ND M][ND M]
+  double *__restrict__ jstab;          // [ne][ND M][ND M] scratch (B)
+  double *__restrict__ wave;           // [ne]
+  int *__restrict__ status;
+  uint32_t ne;
+  PhysParams pp;
+};
+
+// Gauss-Jordan inverse with partial pivoting of a small matrix.
+template <int M>
+__device__ __forceinline__ void small_inverse(double (&A)[M][M], double (&X)[M][M])
+{
+#pragma unroll
+  for (int i = 0; i < M; ++i)
+#pragma unroll
+    for (int j = 0; j < M; ++j) X[i][j] = (i == j) ? 1.0 : 0.0;
+  for (int c = 0; c < M; ++c) {
+    int piv = c;
+    double best = fabs(A[c][c]);
+    for (int r = c + 1; r < M; ++r)
+      if (fabs(A[r][c]) > best) {
+        best = fabs(A[r][c]);
+        piv = r;
+      }
+    if (piv != c)
+      for (int k = 0; k < M; ++k) {
+        const double t = A[c][k];
+        A[c][k] = A[piv][k];
+        A[piv][k] = t;
+        const double s = X[c][k];
+        X[c][k] = X[piv][k];
+        X[piv][k] = s;
+      }
+    const double d = 1.0 / A[c][c];
+    for (int k = 0; k < M; ++k) {
+      A[c][k] *= d;
+      X[c][k] *= d;
+    }
+    for (int r = 0; r < M; ++r) {
+      if (r == c) continue;
+      const double f = A[r][c];
+      for (int k = 0; k < M; ++k) {
+        A[r][k] -= f * A[c][k];
+        X[r][k] -= f * X[c][k];
+      }
+    }
+  }
+}
+
+template <int D, int NG, int ND, int NF, int NQ, bool AFFINE, int PH, int SC>
+__global__ __launch_bounds__(64) void rd_jacobian_kernel(JacArgs a)
+{
+  constexpr int M = PhysTraits<PH>::M;
+  constexpr int NDM = ND * M;
+  using O = TabOffsets<D, NG, ND, NF, NQ>;
+  const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= a.ne) return;
+  const double *__restrict__ tab = a.tab;
+  bool bad = false;
+  double X[NG][D];
+#pragma unroll
+  for (int g = 0; g < NG; ++g) {
+    const size_t v = a.gconn[static_cast<size_t>(g) * a.ne + e];
+#pragma unroll
+    for (int d = 0; d < D; ++d) X[g][d] = a.coords[v * D + d];
+  }
+  double U[ND][M], nodal[ND];
+#pragma unroll
+  for (int n = 0; n < ND; ++n) {
+    const size_t v = a.sconn[static_cast<size_t>(n) * a.ne + e];
+#pragma unroll
+    for (int k = 0; k < M; ++k) U[n][k] = a.u[v * M + k];
+    nodal[n] = a.nodal ? a.nodal[v] : 0.0;
+  }
+  const uint32_t c = a.perm[e];
+  double *jac = a.jac + static_cast<size_t>(c) * NDM * NDM;
+  double *jst = a.jstab + static_cast<size_t>(c) * NDM * NDM;
+  for (int i = 0; i < NDM * NDM; ++i) {
+    jac[i] = 0.0;
+    if (SC == SC_B) jst[i] = 0.0;
+  }
+  const bool ext_theta = (SC == SC_B) && a.nodal;
+  const bool visc = (SC == SC_LDA) && a.nodal;
+  double tmax = 0.0;
+#pragma unroll
+  for (int n = 0; n < ND; ++n) tmax = fmax(tmax, nodal[n]);
+  const bool skip = ext_theta && tmax < 1.e-6;
+
+  // geometry at every quadrature point
+  constexpr int NJ = AFFINE ? 1 : NQ;
+  double Jinv[NJ][D][D], detJ[NJ];
+#pragma unroll
+  for (int q = 0; q < NJ; ++q) {
+    double J[D][D];
+#pragma unroll
+    for (int xi = 0; xi < D; ++xi)
+#pragma unroll
+      for (int x = 0; x < D; ++x) {
+        double s = 0.0;
+#pragma unroll
+        for (int g = 0; g < NG; ++g) s += tab[O::dVgT + (q * D + xi) * NG + g] * X[g][x];
+        J[xi][x] = s;
+      }
+    detJ[q] = det_inv<D>(J, Jinv[q]);
+  }
+
+  // flux divergence of the P+1 flux element at the quadrature points
+  double divr[NQ][M];
+#pragma unroll
+  for (int q = 0; q < NQ; ++q)
+#pragma unroll
+    for (int k = 0; k < M; ++k) divr[q][k] = 0.0;
+  for (int f = 0; f < NF; ++f) {
+    double s[M];
+#pragma unroll
+    for (int k = 0; k < M; ++k) {
+      double t = 0.0;
+#pragma unroll
+      for (int n = 0; n < ND; ++n) t += tab[O::Vsf + f * ND + n] * U[n][k];
+      s[k] = t;
+    }
+    double F[D][M];
+    point_flux<PH, double>(s, a.pp, F, bad);
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+      const int jq = AFFINE ? 0 : q;
+#pragma unroll
+      for (int k = 0; k < M; ++k) {
+        double t = divr[q][k];
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+          double b = 0.0;
+#pragma unroll
+          for (int xi = 0; xi < D; ++xi) b += Jinv[jq][d][xi] * tab[O::dVfT + (f * D + xi) * NQ + q];
+          t += b * F[d][k];
+        }
+        divr[q][k] = t;
+      }
+    }
+  }
+
+  double lda[ND][M], ndis[ND][M];
+#pragma unroll
+  for (int n = 0; n < ND; ++n)
+#pragma unroll
+    for (int k = 0; k < M; ++k) lda[n][k] = ndis[n][k] = 0.0;
+  double fint[M];
+#pragma unroll
+  for (int k = 0; k < M; ++k) fint[k] = 0.0;
+  double vol = 0.0, lmax = 0.0;
+
+  double Kp[ND][M][M], Kx[ND][M][M];  // K+_n and (K_n for LDA / B, K-_n for N)
+  for (int q = 0; q < NQ; ++q) {
+    const int jq = AFFINE ? 0 : q;
+    const double wj = tab[O::w + q] * detJ[jq];
+    vol += wj;
+    double uq[M];
+#pragma unroll
+    for (int k = 0; k < M; ++k) {
+      double t = 0.0;
+#pragma unroll
+      for (int n = 0; n < ND; ++n) t += tab[O::V + q * ND + n] * U[n][k];
+      uq[k] = t;
+    }
+    // physical gradients of the basis at q
+    double G[ND][D];
+#pragma unroll
+    for (int n = 0; n < ND; ++n)
+#pragma unroll
+      for (int x = 0; x < D; ++x) {
+        double t = 0.0;
+#pragma unroll
+        for (int xi = 0; xi < D; ++xi) t += Jinv[jq][x][xi] * tab[O::dVT + (q * D + xi) * ND + n];
+        G[n][x] = t;
+      }
+    if (visc) {
+      double nuq = 0.0;
+#pragma unroll
+      for (int n = 0; n < ND; ++n) nuq += tab[O::V + q * ND + n] * nodal[n];
+      if (nuq != 0.0) {
+        double gu[D][M];
+#pragma unroll
+        for (int x = 0; x < D; ++x)
+#pragma unroll
+          for (int k = 0; k < M; ++k) {
+            double t = 0.0;
+#pragma unroll
+            for (int n = 0; n < ND; ++n) t += G[n][x] * U[n][k];
+            gu[x][k] = t;
+          }
+#pragma unroll
+        for (int n = 0; n < ND; ++n)
+#pragma unroll
+          for (int x = 0; x < D; ++x) {
+            const double cx = wj * nuq * G[n][x];
+#pragma unroll
+            for (int k = 0; k < M; ++k) lda[n][k] += cx * gu[x][k];
+          }
+      }
+    }
+    // K+ and K / K- of every node
+    if constexpr (M == 1) {
+      const double ax = (PH == PH_BURGERS2D) ? uq[0] : a.pp.p0;
+      const double ay = (PH == PH_BURGERS2D) ? 1.0 : a.pp.p1;
+#pragma unroll
+      for (int n = 0; n < ND; ++n) {
+        const double k = ax * G[n][0] + ay * G[n][1];
+        lmax = fmax(lmax, fabs(k));
+        Kp[n][0][0] = fmax(0.0, k);
+        Kx[n][0][0] = (SC == SC_N) ? fmin(0.0, k) : k;
+      }
+    } else {
+      const EulerState<double, D> st = euler_state<double, D>(uq, a.pp.p0);
+      bad = bad || st.bad;
+      const double gm1 = a.pp.p0 - 1.0;
+      double l0v[M], r0[M];
+      l0v[0] = 0.5 * gm1 * st.v2;
+      r0[0] = 1.0;
+#pragma unroll
+      for (int x = 0; x < D; ++x) {
+        l0v[1 + x] = -gm1 * st.V[x];
+        r0[1 + x] = st.V[x];
+      }
+      l0v[M - 1] = gm1;
+      r0[M - 1] = st.H;
+      const double i2a = 0.5 / st.a, i2a2 = i2a / st.a;
+#pragma unroll
+      for (int n = 0; n < ND; ++n) {
+        double nn = 0.0;
+#pragma unroll
+        for (int x = 0; x < D; ++x) nn += G[n][x] * G[n][x];
+        const double norm = sqrt(nn);
+        const bool tiny = (D == 2) ? (norm < 1.e-9) : (norm == 0.0);
+        const double inorm = tiny ? 0.0 : 1.0 / norm;
+        double nh[D], um = 0.0;
+#pragma unroll
+        for (int x = 0; x < D; ++x) {
+          nh[x] = G[n][x] * inorm;
+          um += st.V[x] * nh[x];
+        }
+        const double e0 = norm * um, ep = norm * (um + st.a), em = norm * (um - st.a);
+        lmax = fmax(lmax, fmax(fabs(e0), fmax(fabs(ep), fabs(em))));
+        // sign +1: K+, -1: K-
+        for (int sgn = 0; sgn < 2; ++sgn) {
+          const double p0 = sgn == 0 ? fmax(0.0, e0) : fmin(0.0, e0);
+          const double dpn = tiny ? 0.0 : (sgn == 0 ? fmax(0.0, ep) : fmin(0.0, ep)) - p0;
+          const double dmn = tiny ? 0.0 : (sgn == 0 ? fmax(0.0, em) : fmin(0.0, em)) - p0;
+          const double l0 = tiny ? 0.0 : p0;
+          const double sg = dpn + dmn, dl = dpn - dmn;
+#pragma unroll
+          for (int j = 0; j < M; ++j) {
+            // column j: the operator applied to the unit vector e_j
+            const double P = l0v[j];
+            double Q = (j == 0) ? -um : ((j <= D) ? nh[j - 1] : 0.0);
+            const double al = (sg * P + st.a * dl * Q) * i2a2;
+            const double be = 0.5 * sg * Q + dl * P * i2a;
+            double col[M];
+            col[0] = ((j == 0) ? l0 : 0.0) + al;
+#pragma unroll
+            for (int x = 0; x < D; ++x) col[1 + x] = ((j == 1 + x) ? l0 : 0.0) + al * st.V[x] + be * nh[x];
+            col[M - 1] = ((j == M - 1) ? l0 : 0.0) + al * st.H + be * um;
+#pragma unroll
+            for (int i = 0; i < M; ++i) {
+              if (sgn == 0) {
+                Kp[n][i][j] = col[i];
+              } else {
+                Kx[n][i][j] = (SC == SC_N) ? col[i] : Kp[n][i][j] + col[i];
+              }
+            }
+          }
+        }
+      }
+    }
+    // S = sum K+ (LDA, B) or sum K- (N), and its inverse
+    double S[M][M], Si[M][M];
+#pragma unroll
+    for (int i = 0; i < M; ++i)
+#pragma unroll
+      for (int j = 0; j < M; ++j) {
+        double t = 0.0;
+#pragma unroll
+        for (int n = 0; n < ND; ++n) t += (SC == SC_N) ? Kx[n][i][j] : Kp[n][i][j];
+        S[i][j] = t;
+      }
+    small_inverse<M>(S, Si);
+#pragma unroll
+    for (int k = 0; k < M; ++k) fint[k] += wj * divr[q][k];
+    for (int mm = 0; mm < ND; ++mm) {
+      double B[M][M];  // K+_m S^-1
+#pragma unroll
+      for (int i = 0; i < M; ++i)
+#pragma unroll
+        for (int j = 0; j < M; ++j) {
+          double t = 0.0;
+#pragma unroll
+          for (int k = 0; k < M; ++k) t += Kp[mm][i][k] * Si[k][j];
+          B[i][j] = t;
+        }
+      if (SC != SC_N) {
+#pragma unroll
+        for (int i = 0; i < M; ++i) {
+          double t = 0.0;
+#pragma unroll
+          for (int j = 0; j < M; ++j) t += B[i][j] * divr[q][j];
+          lda[mm][i] += wj * t;
+        }
+      }
+      for (int nn = 0; nn < ND; ++nn) {
+        double P[M][M];
+#pragma unroll
+        for (int i = 0; i < M; ++i)
+#pragma unroll
+          for (int j = 0; j < M; ++j) {
+            double t = 0.0;
+#pragma unroll
+            for (int k = 0; k < M; ++k) t += B[i][k] * Kx[nn][k][j];
+            P[i][j] = t;
+          }
+        double *row = jac + static_cast<size_t>(mm * M) * NDM + nn * M;
+        if (SC != SC_N) {
+#pragma unroll
+          for (int i = 0; i < M; ++i)
+#pragma unroll
+            for (int j = 0; j < M; ++j) row[i * NDM + j] += wj * P[i][j];
+        } else if (nn != mm) {
+#pragma unroll
+          for (int i = 0; i < M; ++i) {
+            double t = 0.0;
+#pragma unroll
+            for (int j = 0; j < M; ++j) t += P[i][j] * (U[mm][j] - U[nn][j]);
+            ndis[mm][i] += wj * t;
+#pragma unroll
+            for (int j = 0; j < M; ++j) row[i * NDM + j] += -wj * P[i][j];
+          }
+        } else {
+#pragma unroll
+          for (int i = 0; i < M; ++i)
+#pragma unroll
+            for (int j = 0; j < M; ++j) row[i * NDM + j] += wj * (Kp[mm][i][j] - P[i][j]);
+        }
+      }
+      if (SC == SC_B && !skip) {
+        for (int nn = 0; nn < ND; ++nn) {
+          double P[M][M];
+#pragma unroll
+          for (int i = 0; i < M; ++i)
+#pragma unroll
+            for (int j = 0; j < M; ++j) {
+              double t = 0.0;
+#pragma unroll
+              for (int k = 0; k < M; ++k) t += B[i][k] * Kp[nn][k][j];
+              P[i][j] = t;
+            }
+          double *row = jst + static_cast<size_t>(mm * M) * NDM + nn * M;
+          if (nn != mm) {
+#pragma unroll
+            for (int i = 0; i < M; ++i) {
+              double t = 0.0;
+#pragma unroll
+              for (int j = 0; j < M; ++j) t += P[i][j] * (U[mm][j] - U[nn][j]);
+              ndis[mm][i] += wj * t;
+#pragma unroll
+              for (int j = 0; j < M; ++j) row[i * NDM + j] += -wj * P[i][j];
+            }
+          } else {
+#pragma unroll
+            for (int i = 0; i < M; ++i)
+#pragma unroll
+              for (int j = 0; j < M; ++j) row[i * NDM + j] += wj * (Kp[mm][i][j] - P[i][j]);
+          }
+        }
+      }
+    }
+  }
+
+  // residuals, the blend of B, wave speed
+  double *phi = a.phi + static_cast<size_t>(c) * NDM;
+  if (SC == SC_N) {
+#pragma unroll
+    for (int n = 0; n < ND; ++n)
+#pragma unroll
+      for (int k = 0; k < M; ++k) phi[n * M + k] = ndis[n][k];
+  } else if (SC == SC_B) {
+    double th[ND];
+    if (ext_theta) {
+#pragma unroll
+      for (int n = 0; n < ND; ++n) th[n] = nodal[n];
+    } else {
+      double an = 0.0;
+#pragma unroll
+      for (int n = 0; n < ND; ++n) an += fabs(lda[n][0] + ndis[n][0]);
+      const double t0 = fabs(fint[0]) / (an + 1.e-6);
+#pragma unroll
+      for (int n = 0; n < ND; ++n) th[n] = t0;
+    }
+#pragma unroll
+    for (int n = 0; n < ND; ++n)
+#pragma unroll
+      for (int k = 0; k < M; ++k) phi[n * M + k] = lda[n][k] + th[n] * ndis[n][k];
+    if (!skip)
+      for (int i = 0; i < NDM; ++i) {
+        const double t = th[i / M];
+        for (int j = 0; j < NDM; ++j) jac[i * NDM + j] += t * jst[i * NDM + j];
+      }
+  } else {
+#pragma unroll
+    for (int n = 0; n < ND; ++n)
+#pragma unroll
+      for (int k = 0; k < M; ++k) phi[n * M + k] = lda[n][k];
+  }
+  a.wave[c] = vol * fmax(lmax, 1.e-6);
+  if (bad) atomicOr(a.status, 1);
+}
+
+}  // namespace pdrd

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include "rd_cell.hpp"
+#include "rd_jacobian.hpp"
 
 namespace pdrd {
 
@@ -11,6 +12,7 @@ using CellLauncher = hipError_t (*)(const CellArgs &, unsigned grid, unsigned bl
 using PatchLauncher = hipError_t (*)(const PatchArgs &, unsigned grid, int cells, hipStream_t);
 using PatchBatchLauncher = hipError_t (*)(const PatchArgs *args, const uint32_t *first, int count, unsigned grid,
                                           int cells, size_t lds_bytes, hipStream_t);
+using JacobianLauncher = hipError_t (*)(const JacArgs &, unsigned grid, hipStream_t);
 
 struct CellKey
 {
@@ -29,6 +31,8 @@ struct KernelSet
   // external blending coefficient); null for N and PSI, which take none
   CellLauncher cell_nodal = nullptr;
   PatchLauncher patch_nodal = nullptr;
+  // element residuals and Jacobians of the implicit form (null for PSI)
+  JacobianLauncher jacobian = nullptr;
   int patch_size = 0;             // largest (default) cells per patch workgroup
   size_t (*patch_lds_bytes)(int maxl, int cells) = nullptr;
   explicit operator bool() const { return cell != nullptr; }
@@ -75,6 +79,13 @@ hipError_t launch_patch_batch(const PatchArgs *args, const uint32_t *first, int 
   return hipGetLastError();
 }
 
+template <int D, int NG, int ND, int NF, int NQ, bool AFF, int PH, int SC>
+hipError_t launch_jacobian(const JacArgs &a, unsigned grid, hipStream_t s)
+{
+  hipLaunchKernelGGL((rd_jacobian_kernel<D, NG, ND, NF, NQ, AFF, PH, SC>), dim3(grid), dim3(64), 0, s, a);
+  return hipGetLastError();
+}
+
 template <int NQ, int ND, int M>
 size_t patch_lds_bytes(int maxl, int cells)
 {
@@ -94,6 +105,7 @@ KernelSet kernel_set()
     k.cell_nodal = &launch_cell<D, NG, ND, NF, NQ, AFF, PH, SC, 2>;
     k.patch_nodal = &launch_patch<D, NG, ND, NF, NQ, AFF, PH, SC, 2>;
   }
+  if constexpr (SC != SC_PSI) k.jacobian = &launch_jacobian<D, NG, ND, NF, NQ, AFF, PH, SC>;
   k.patch_size = PatchSize<PhysTraits<PH>::M, ND>::value;
   k.patch_lds_bytes = &patch_lds_bytes<NQ, ND, PhysTraits<PH>::M>;
   return k;

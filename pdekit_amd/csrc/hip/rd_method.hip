@@ -1448,6 +1448,63 @@ int pdrd_compute_artificial_viscosity(pdrd_method_t h, double kappa, double fact
   PDRD_CATCH
 }
 
+int pdrd_element_jacobians(pdrd_method_t h, int block, double *phi, double *jac, double *wave)
+{
+  PDRD_TRY
+  if (int rc = check_ready(h)) return rc;
+  if (block < 0 || block >= static_cast<int>(h->blocks.size())) return set_error(PDRD_EINVAL, "no such block");
+  Block &b = h->blocks[block];
+  if (!b.ks.jacobian) return set_error(PDRD_ENOTSUP, "the PSI scheme has no implicit form");
+  if (b.ne == 0) return PDRD_OK;
+  const size_t ndm = static_cast<size_t>(b.fe.nd) * h->m;
+  double *d_phi = nullptr, *d_jac = nullptr, *d_stab = nullptr, *d_wave = nullptr;
+  uint32_t *d_perm = nullptr;
+  auto release = [&]() {
+    (void)hipFree(d_phi);
+    (void)hipFree(d_jac);
+    (void)hipFree(d_stab);
+    (void)hipFree(d_wave);
+    (void)hipFree(d_perm);
+  };
+  if (hipMalloc(&d_phi, b.ne * ndm * sizeof(double)) != hipSuccess ||
+      hipMalloc(&d_jac, b.ne * ndm * ndm * sizeof(double)) != hipSuccess ||
+      hipMalloc(&d_stab, (h->scheme == pdrd::SC_B ? b.ne * ndm * ndm : 1) * sizeof(double)) != hipSuccess ||
+      hipMalloc(&d_wave, b.ne * sizeof(double)) != hipSuccess ||
+      hipMalloc(&d_perm, b.ne * sizeof(uint32_t)) != hipSuccess) {
+    release();
+    return set_error(PDRD_ENOMEM, "device memory for the element Jacobians");
+  }
+  hipError_t e = hipMemcpyAsync(d_perm, b.perm.data(), b.ne * sizeof(uint32_t), hipMemcpyHostToDevice, h->stream);
+  pdrd::JacArgs a{};
+  a.tab = b.d_tab;
+  a.gconn = b.d_gconn;
+  a.sconn = b.d_sconn;
+  a.perm = d_perm;
+  a.coords = h->d_coords;
+  a.u = h->d_u;
+  a.nodal = nodal_input(h);
+  a.phi = d_phi;
+  a.jac = d_jac;
+  a.jstab = d_stab;
+  a.wave = d_wave;
+  a.status = h->d_status;
+  a.ne = b.ne;
+  a.pp.p0 = h->params[0];
+  a.pp.p1 = h->params[1];
+  if (e == hipSuccess) e = b.ks.jacobian(a, grid_for(b.ne, 64), h->stream);
+  if (e == hipSuccess && phi)
+    e = hipMemcpyAsync(phi, d_phi, b.ne * ndm * sizeof(double), hipMemcpyDeviceToHost, h->stream);
+  if (e == hipSuccess && jac)
+    e = hipMemcpyAsync(jac, d_jac, b.ne * ndm * ndm * sizeof(double), hipMemcpyDeviceToHost, h->stream);
+  if (e == hipSuccess && wave)
+    e = hipMemcpyAsync(wave, d_wave, b.ne * sizeof(double), hipMemcpyDeviceToHost, h->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+  release();
+  if (e != hipSuccess) return hip_fail(e, "element Jacobians");
+  return PDRD_OK;
+  PDRD_CATCH
+}
+
 int pdrd_add_strong_dirichlet(pdrd_method_t h, uint32_t n, const uint32_t *nodes, const double *values)
 {
   if (int rc = check_ready(h)) return rc;

@@ -247,6 +247,18 @@ class PGRDMethodExplicit:
                 "pdrd_compute_artificial_viscosity")
         return self.vec_function("artificial_viscosity")
 
+    def element_jacobians(self, block: int = 0):
+        """compute_adv_res_and_jacobian of every cell of one block (its cells
+        in mesh order): phi[ne][nd][m], jac[ne][nd*m][nd*m], wave[ne]."""
+        blk = self.mesh.blocks[block]
+        ne, nd = blk.sol_conn.shape
+        phi = np.zeros((ne, nd, self.neq))
+        jac = np.zeros((ne, nd * self.neq, nd * self.neq))
+        wave = np.zeros(ne)
+        L.check(L.lib().pdrd_element_jacobians(self._h, int(block), L.ptr(phi), L.ptr(jac), L.ptr(wave)),
+                "pdrd_element_jacobians")
+        return phi, jac, wave
+
     def weak_bc_info(self, k: int) -> dict:
         a = np.zeros(3, dtype=np.int64)
         L.check(L.lib().pdrd_weak_bc_info(self._h, int(k), L.ptr(a, C.c_int64)), "pdrd_weak_bc_info")
