@@ -80,6 +80,7 @@ def lib():
             "orc_euler_kplus": [C.c_int, C.c_double, PD, PD, PD, PD],
             "orc_lf_blend": [PP, PD, C.POINTER(PD), C.POINTER(C.c_int), PD],
             "orc_cell_jacobians": [PP, C.c_int, PD, PD, PD, PD],
+            "orc_iterate_threads": [PP, C.c_double, C.c_int, C.c_uint32, PU, PD, PD, PD, PD, C.c_int],
             "orc_art_visc": [PP, PD, C.POINTER(OrcAvRule), C.POINTER(PD), C.POINTER(C.c_int), C.c_double,
                              C.c_double, PD],
         }.items():
@@ -283,6 +284,23 @@ def lower_degree_basis(shape, p, xi, w):
     G = M.T @ (np.asarray(w)[:, None] * M)
     Lc = np.linalg.cholesky(G)
     return np.ascontiguousarray(np.linalg.solve(Lc, M.T).T)
+
+
+def _iterate_threads(problem, u, cfl, n_iter, threads, dir_nodes=None, dir_values=None):
+    u = np.array(u, dtype=np.float64).reshape(problem.nn, problem.m)
+    res = np.zeros((problem.nn, problem.m))
+    dt = np.zeros(problem.nn)
+    if dir_nodes is None:
+        dir_nodes = np.zeros(0, dtype=np.uint32)
+        dir_values = np.zeros((0, problem.m))
+    dn = np.ascontiguousarray(dir_nodes, dtype=np.uint32)
+    dv = np.ascontiguousarray(np.broadcast_to(dir_values, (dn.size, problem.m)), dtype=np.float64)
+    rc = lib().orc_iterate_threads(C.byref(problem.p), float(cfl), int(n_iter), dn.size, _p(dn, C.c_uint32), _p(dv),
+                                   _p(u), _p(res), _p(dt), int(threads))
+    return u, res, dt, rc
+
+
+Problem.iterate_threads = _iterate_threads
 
 
 def invert(a):

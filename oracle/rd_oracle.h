@@ -132,6 +132,13 @@ typedef struct orc_av_rule
 int orc_art_visc(const orc_problem *p, const double *u, const orc_av_rule *rules, const double *const *p1w,
                  const int *nvert, double kappa, double factor, double *nu);
 
+/* orc_iterate with the cell loop split over `threads` threads, each summing
+ * into its own nodal arrays (added in thread order afterwards): the
+ * multi-core timing baseline. No norms. */
+int orc_iterate_threads(const orc_problem *p, double cfl, int n_iter, uint32_t nb_dirichlet,
+                        const uint32_t *dirichlet_nodes, const double *dirichlet_values, double *u, double *res,
+                        double *dt, int threads);
+
 /* norm_L2 per field: sqrt(sum_i r_i^2 / N) (MeshFunctionNorm.hpp). */
 void orc_norm_L2(const double *f, uint32_t n, int m, double *out);
 

@@ -244,3 +244,16 @@ def test_advection_converges(scheme):
     assert np.sqrt(np.mean(res[mask] ** 2)) < 1e-12
     if scheme == O.N:
         assert u.min() > -1e-12 and u.max() < 1 + 1e-12
+
+
+def test_threaded_iterations_match():
+    """The multi-core timing baseline computes the same iterations."""
+    mesh = rectangle(12, 10, split=4, order=2, jitter=0.15, seed=2)
+    x = mesh.node_coords[:, 0]
+    u = (1.5 - 2.0 * x)[:, None]
+    P = oracle_problem(mesh, O.BURGERS2D, O.B, ())
+    a, ra, _, _, rc = P.iterate(u, 0.5, 4)
+    b, rb, _, rc2 = P.iterate_threads(u, 0.5, 4, 3)
+    assert rc == rc2 == 0
+    assert np.abs(a - b).max() <= 1e-14 * np.abs(a).max()
+    assert np.abs(ra - rb).max() <= 1e-12 * np.abs(ra).max()
