@@ -167,6 +167,9 @@ def main():
                     help="plain FP64 cells, or FP32 interpolation / fluxes / K+ with FP64 solves and sums")
     ap.add_argument("--patch-cells", default="default", choices=["default", "auto", "64", "128", "256"],
                     help="cells per patch workgroup; auto: pdekit_amd.autotune times 64/128/256 first")
+    ap.add_argument("--chain", default="default", choices=["default", "0", "1"],
+                    help="patch assembly: shared-node sums of a sweep in the next sweep's gather (1) or a "
+                         "fix-up pass after every sweep (0); default: the library's choice")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-calibration-mb", type=int, default=0,
@@ -214,6 +217,8 @@ def main():
     meth.configure_mesh_data(wl.mesh).initialize_work_data(quad_order=wl.quad_order, patch_cells=cells)
     if args.precision == "mixed":
         meth.set_option(L.OPT_PRECISION, 1)
+    if args.chain != "default":
+        meth.set_option(L.OPT_CHAIN, int(args.chain))
     meth.set_vec_function("solution", wl.state)
     if wl.dirichlet_nodes.size:
         meth.add_boundary_condition("StrongDirichlet", "inflow", wl.dirichlet_nodes, wl.dirichlet_values)
@@ -286,6 +291,7 @@ def main():
                    "order": wl.mesh.order, "scheme": ["PGLDA", "PGN", "PGB", "PSI"][wl.scheme],
                    "scatter": SCATTER_NAMES[best[0]], "colours": info["nb_colours"],
                    "patch_cells": meth.get_option(L.OPT_PATCH_CELLS) if best[0] == 2 else None,
+                   "chain": meth.get_option(L.OPT_CHAIN) if best[0] == 2 else None,
                    "parallelism": "single GPU"},
         "roofline": roofline,
         # the FP64 vector pipe is the binding roof of this kernel: flops per

@@ -76,10 +76,15 @@ enum pdrd_option {
                              kernel's default: 256 scalar P1/P2, 128 otherwise), 64, 128 or
                              256 (capped at the default); read back after pdrd_initialize
                              it gives the largest in use */
-  PDRD_OPT_REFRESH = 5     /* mixed precision: every this many iterations (default 50) the
+  PDRD_OPT_REFRESH = 5,    /* mixed precision: every this many iterations (default 50) the
                              defect correction R64(u) - Rmixed(u) is recomputed (two sweeps)
                              and added to the mixed residual of the updates, so the iteration
                              converges to the FP64 steady state; 0 disables it */
+  PDRD_OPT_CHAIN = 6       /* patch assembly, pdrd_iterate: 1 = the shared nodes of a sweep
+                             are summed and stepped in the next sweep's gather (one launch
+                             per element type and sweep; bitwise the same results), 0 = a
+                             fix-up pass after every sweep; -1 (default) = 1 for the scalar
+                             physics, 0 for the Euler systems (as measured faster) */
 };
 
 /* ---- FE inputs (src/interpolation, src/mesh/point_set, shape_function) -- */

@@ -28,7 +28,7 @@ ADVECTION2D, BURGERS2D, EULER2D, EULER3D = 0, 1, 2, 3
 LDA, N, B, PSI = 0, 1, 2, 3
 SOLUTION, RESIDUAL, TIME_STEP, WAVE_SPEED, DUAL_VOLUME, BLEND_COEFF, ART_VISC = 0, 1, 2, 3, 4, 5, 6
 BLEND_LF = 0
-OPT_SCATTER, OPT_PRECISION, OPT_BLOCK, OPT_TIMING, OPT_PATCH_CELLS, OPT_REFRESH = 0, 1, 2, 3, 4, 5
+OPT_SCATTER, OPT_PRECISION, OPT_BLOCK, OPT_TIMING, OPT_PATCH_CELLS, OPT_REFRESH, OPT_CHAIN = 0, 1, 2, 3, 4, 5, 6
 WEAK_WALL, WEAK_SUBINLET, WEAK_SUBOUTLET, WEAK_FARFIELD, WEAK_SYMMETRY, WEAK_SUPERINLET, WEAK_DIRICHLET = range(7)
 
 # every symbol include/pdekit_rd.h declares

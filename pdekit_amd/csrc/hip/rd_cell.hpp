@@ -861,6 +861,15 @@ __global__ __launch_bounds__(256) void rd_cell_kernel(CellArgs a)
 //     u -= dt res (after the strong Dirichlet value),
 //   * nodes shared with other patches write their partial sums to their slot;
 //     patch_fixup_kernel sums the slots of every shared node in a fixed order.
+// Between two iterations the fix-up can be left out: the next sweep's
+// patches then finish their shared nodes themselves while gathering them
+// (prev_acc: the previous sweep's rows; base_in: the shared nodes' states at
+// its start), each holder summing the same rows in the same order, so all
+// holders agree bit for bit with each other and with the fix-up. The holder
+// owning the node's first row (its lowest-numbered patch) stores the new
+// state to u and, when the sweep after this one is deferred too, to
+// base_out. The rows and the states alternate between two buffers, since
+// other patches of a sweep still read the previous sweep's.
 // ---------------------------------------------------------------------------
 constexpr int LC_BITS = 10;  // local node index bits of a patch connectivity entry
 constexpr int LC_MASK = (1 << LC_BITS) - 1;
@@ -885,6 +894,11 @@ struct PatchArgs
   double *__restrict__ ws;                  // [N]
   double *__restrict__ slot_acc;            // [shared slots][M + 1] partial sums, grouped by node
   const uint32_t *__restrict__ slot_pos;    // [slots] row of a shared slot in slot_acc
+  const uint32_t *__restrict__ pn_sid;      // [slots] shared-node index of a shared slot
+  const uint2 *__restrict__ pn_rows;        // [slots] rows [x, y) of a shared slot's node
+  const double *__restrict__ prev_acc;      // rows of the previous sweep, not yet summed, or null
+  const double *__restrict__ base_in;       // [n_shared][M] states at the start of the previous sweep
+  double *__restrict__ base_out;            // [n_shared][M] states at the start of this sweep, or null
   double *__restrict__ norm_partial;        // [np][M + 1]
   const double *__restrict__ nodal;         // [N] blending coefficient (B) / artificial viscosity (LDA), PREC 2
   int *__restrict__ status;
@@ -954,35 +968,76 @@ __host__ __device__ constexpr size_t patch_lds_doubles_prec(int maxl, int P)
 
 // Finishes node g from its sums acc[M + 1] (residual sum, wave-speed sum);
 // u_old is the node's state at the start of the sweep.
+// Sum of the partial-sum rows [r0, r1) of one shared node, in row order;
+// the loads of four rows are issued before their additions.
+template <int M>
+__device__ __forceinline__ void sum_rows(const double *__restrict__ rows, uint32_t r0, uint32_t r1, double (&acc)[M + 1])
+{
+  constexpr int RB = 4;
+#pragma unroll
+  for (int k = 0; k <= M; ++k) acc[k] = 0.0;
+  for (uint32_t s = r0; s < r1; s += RB) {
+    double v[RB][M + 1];
+#pragma unroll
+    for (int j = 0; j < RB; ++j)
+      if (s + j < r1)
+#pragma unroll
+        for (int k = 0; k <= M; ++k) v[j][k] = rows[static_cast<size_t>(s + j) * (M + 1) + k];
+#pragma unroll
+    for (int j = 0; j < RB; ++j)
+      if (s + j < r1)
+#pragma unroll
+        for (int k = 0; k <= M; ++k) acc[k] += v[j][k];
+  }
+}
+
+// The pseudo-time step of node g: residual r = sum / V (+ the correction),
+// dt = CFL V / sum lambda, new state = (strong Dirichlet value or u_old) -
+// dt r. Both the in-kernel finish and the deferred one use it, so they round
+// alike.
+template <int M>
+__device__ __forceinline__ void node_step(const PatchArgs &a, uint32_t g, const double *acc, const double *u_old,
+                                          double (&r)[M], double (&un)[M], double &d, int &di)
+{
+  const double v = a.vol[g];
+  const double iv = 1.0 / v;
+  d = a.cfl * v / acc[M];
+  di = a.dir_index[g];
+#pragma unroll
+  for (int k = 0; k < M; ++k) {
+    const size_t j = static_cast<size_t>(g) * M + k;
+    r[k] = a.corr ? acc[k] * iv + a.corr[j] : acc[k] * iv;
+    const double u0 = (di >= 0) ? a.dir_vals[static_cast<size_t>(di) * M + k] : u_old[k];
+    un[k] = u0 - d * r[k];
+  }
+}
+
 template <int M>
 __device__ __forceinline__ void finish_node(const PatchArgs &a, uint32_t g, const double *acc, const double *u_old,
                                             double (&sq)[M + 1])
 {
-  const double v = a.vol[g];
-  const double iv = 1.0 / v;
-  const double w = acc[M];
   if (a.update) {
-    const double d = a.cfl * v / w;
-    const int di = a.dir_index[g];
+    double r[M], un[M], d;
+    int di;
+    node_step<M>(a, g, acc, u_old, r, un, d, di);
     const bool count = a.outputs && !(a.excl && di >= 0);
 #pragma unroll
     for (int k = 0; k < M; ++k) {
       const size_t j = static_cast<size_t>(g) * M + k;
-      const double r = a.corr ? acc[k] * iv + a.corr[j] : acc[k] * iv;
-      const double u0 = (di >= 0) ? a.dir_vals[static_cast<size_t>(di) * M + k] : u_old[k];
-      a.u[j] = u0 - d * r;
-      if (a.outputs) a.res[j] = r;
-      if (count) sq[k] += r * r;
+      a.u[j] = un[k];
+      if (a.outputs) a.res[j] = r[k];
+      if (count) sq[k] += r[k] * r[k];
     }
     if (count) sq[M] += 1.0;
     if (a.outputs) {
       a.dt[g] = d;
-      a.ws[g] = w;
+      a.ws[g] = acc[M];
     }
   } else {
+    const double iv = 1.0 / a.vol[g];
 #pragma unroll
     for (int k = 0; k < M; ++k) a.res[static_cast<size_t>(g) * M + k] = acc[k] * iv;
-    a.ws[g] = w;
+    a.ws[g] = acc[M];
   }
 }
 
@@ -1042,10 +1097,27 @@ __device__ __forceinline__ void patch_body(const PatchArgs &a, uint32_t pl, doub
     for (int n = 0; n < ND; ++n) lc[n] = a.lconn[static_cast<size_t>(n) * a.ne + e];
   }
   double *nodal_l = work + patch_work_doubles<NQ, ND, M>(PATCH);  // PREC 2 only
+  // (no global store before the cell work: the table loads stay scalar)
+  const int ninner = static_cast<int>(a.pn_inner[p]);
   for (int l = tid; l < nl; l += PATCH) {
-    const size_t g = a.pnodes[s0 + l];
+    const uint32_t g = a.pnodes[s0 + l];
+    double un[M];
+    if (a.prev_acc && l >= ninner) {
+      // shared node whose previous sweep is summed here
+      const uint32_t i = a.pn_sid[s0 + l];
+      const uint2 rr = a.pn_rows[s0 + l];
+      double acc[M + 1], uo[M], r[M], d;
+      int di;
 #pragma unroll
-    for (int k = 0; k < M; ++k) ul[l * M + k] = a.u[g * M + k];
+      for (int k = 0; k < M; ++k) uo[k] = a.base_in[static_cast<size_t>(i) * M + k];
+      sum_rows<M>(a.prev_acc, rr.x, rr.y, acc);
+      node_step<M>(a, g, acc, uo, r, un, d, di);
+    } else {
+#pragma unroll
+      for (int k = 0; k < M; ++k) un[k] = a.u[static_cast<size_t>(g) * M + k];
+    }
+#pragma unroll
+    for (int k = 0; k < M; ++k) ul[l * M + k] = un[k];
     coff[l] = a.pn_coff[s0 + l];
     if constexpr (PREC == 2) nodal_l[l] = a.nodal[g];
   }
@@ -1088,7 +1160,6 @@ __device__ __forceinline__ void patch_body(const PatchArgs &a, uint32_t pl, doub
   }
   __syncthreads();
 
-  const int ninner = static_cast<int>(a.pn_inner[p]);
   const int npair = ecount * ND;
   double sq[M + 1];
 #pragma unroll
@@ -1105,9 +1176,22 @@ __device__ __forceinline__ void patch_body(const PatchArgs &a, uint32_t pl, doub
     if (l < ninner) {
       finish_node<M>(a, a.pnodes[s0 + l], acc, ul + l * M, sq);
     } else {
-      double *slot = a.slot_acc + static_cast<size_t>(a.slot_pos[s0 + l]) * (M + 1);
+      const uint32_t sp = a.slot_pos[s0 + l];
+      double *slot = a.slot_acc + static_cast<size_t>(sp) * (M + 1);
 #pragma unroll
       for (int k = 0; k <= M; ++k) slot[k] = acc[k];
+      if (a.prev_acc || a.base_out) {
+        // the holder of the node's first row stores its start state
+        const uint32_t i = a.pn_sid[s0 + l];
+        if (sp == a.pn_rows[s0 + l].x) {
+#pragma unroll
+          for (int k = 0; k < M; ++k) {
+            const double v = ul[l * M + k];
+            if (a.prev_acc) a.u[static_cast<size_t>(a.pnodes[s0 + l]) * M + k] = v;
+            if (a.base_out) a.base_out[static_cast<size_t>(i) * M + k] = v;
+          }
+        }
+      }
     }
   }
   if (bad) atomicOr(a.status, 1);
@@ -1161,13 +1245,7 @@ __device__ __forceinline__ void fixup_body(const FixupArgs &f, uint32_t lb, uint
   for (int k = 0; k <= M; ++k) sq[k] = 0.0;
   for (uint32_t i = lb * blockDim.x + threadIdx.x; i < f.nb; i += nblk * blockDim.x) {
     double acc[M + 1];
-#pragma unroll
-    for (int k = 0; k <= M; ++k) acc[k] = 0.0;
-    for (uint32_t s = f.slot_off[i]; s < f.slot_off[i + 1]; ++s) {
-      const double *row = f.slot_acc + static_cast<size_t>(s) * (M + 1);
-#pragma unroll
-      for (int k = 0; k <= M; ++k) acc[k] += row[k];
-    }
+    sum_rows<M>(f.slot_acc, f.slot_off[i], f.slot_off[i + 1], acc);
     const uint32_t g = f.nodes[i];
     double u_old[M];
 #pragma unroll

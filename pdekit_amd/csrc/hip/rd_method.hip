@@ -120,8 +120,13 @@ struct pdrd_method
   uint32_t np_total = 0, n_shared = 0;
   uint32_t *d_pnode_off = nullptr, *d_pnodes = nullptr, *d_pn_inner = nullptr;
   uint16_t *d_pn_coff = nullptr;
-  uint32_t *d_shared_nodes = nullptr, *d_shared_off = nullptr, *d_slot_pos = nullptr;
+  uint32_t *d_shared_nodes = nullptr, *d_shared_off = nullptr, *d_slot_pos = nullptr, *d_pn_sid = nullptr;
   double *d_slot_acc = nullptr, *d_patch_partial = nullptr;
+  // second row buffer and the shared nodes' states [n_shared][m] of the two
+  // parities of a chain of sweeps whose shared-node sums are deferred
+  double *d_slot_acc_b = nullptr, *d_base[2] = {nullptr, nullptr};
+  uint2 *d_pn_rows = nullptr;  // [slots] rows of a shared slot's node
+  int chain = -1;              // PDRD_OPT_CHAIN (-1: the default of the method's kernels)
   size_t bytes = 0;
 
   hipStream_t stream = nullptr;
@@ -129,7 +134,8 @@ struct pdrd_method
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
   double last_cell_ms = 0.0, last_node_ms = 0.0;
 
-  // cached graph of one iteration without outputs
+  // cached graph of one iteration without outputs (atomic / colour
+  // assembly)
   hipGraphExec_t graph = nullptr;
   double graph_cfl = -1.0;
 };
@@ -385,7 +391,22 @@ unsigned fixup_blocks(const pdrd_method *h)
   return std::min<unsigned>(kNormMaxBlocks, std::max<unsigned>(1, grid_for(h->n_shared, 256)));
 }
 
-pdrd::PatchArgs patch_args(const pdrd_method *h, const Block *b, int update, int outputs, int excl, double cfl)
+// Place of one sweep in a chain of iterations whose shared-node sums are
+// deferred to the next sweep: pend = buffer holding the previous sweep's rows
+// and start states (-1: none, the state in u is complete), write = buffer of
+// this sweep's rows (and start states when record is set: the next sweep is
+// deferred too).
+struct Chain
+{
+  int pend = -1;
+  int write = 0;
+  bool record = false;
+};
+
+double *rows_buffer(const pdrd_method *h, int i) { return i ? h->d_slot_acc_b : h->d_slot_acc; }
+
+pdrd::PatchArgs patch_args(const pdrd_method *h, const Block *b, int update, int outputs, int excl, double cfl,
+                           Chain c = {})
 {
   pdrd::PatchArgs a{};
   if (b) {
@@ -410,8 +431,15 @@ pdrd::PatchArgs patch_args(const pdrd_method *h, const Block *b, int update, int
   a.res = h->d_res;
   a.dt = h->d_dt;
   a.ws = h->d_ws;
-  a.slot_acc = h->d_slot_acc;
+  a.slot_acc = rows_buffer(h, c.write);
   a.slot_pos = h->d_slot_pos;
+  a.pn_sid = h->d_pn_sid;
+  a.pn_rows = h->d_pn_rows;
+  if (c.pend >= 0) {
+    a.prev_acc = rows_buffer(h, c.pend);
+    a.base_in = h->d_base[c.pend];
+  }
+  if (c.record) a.base_out = h->d_base[c.write];
   a.norm_partial = h->d_patch_partial;
   a.nodal = nodal_input(h);
   a.status = h->d_status;
@@ -424,12 +452,12 @@ pdrd::PatchArgs patch_args(const pdrd_method *h, const Block *b, int update, int
   return a;
 }
 
-pdrd::FixupArgs fixup_args(const pdrd_method *h, int update, int outputs, int excl, double cfl)
+pdrd::FixupArgs fixup_args(const pdrd_method *h, int update, int outputs, int excl, double cfl, int buffer = 0)
 {
   pdrd::FixupArgs f{};
   f.nodes = h->d_shared_nodes;
   f.slot_off = h->d_shared_off;
-  f.slot_acc = h->d_slot_acc;
+  f.slot_acc = rows_buffer(h, buffer);
   f.pa = patch_args(h, nullptr, update, outputs, excl, cfl);
   f.nb = h->n_shared;
   f.norm_partial = h->d_patch_partial + static_cast<size_t>(h->np_total) * (h->m + 1);
@@ -438,10 +466,10 @@ pdrd::FixupArgs fixup_args(const pdrd_method *h, int update, int outputs, int ex
 
 // The patch kernels of every block (plain, mixed precision, or with the
 // nodal input of the scheme).
-int launch_patch_blocks(pdrd_method *h, int update, int outputs, int excl, double cfl)
+int launch_patch_blocks(pdrd_method *h, int update, int outputs, int excl, double cfl, Chain c = {})
 {
   for (Block &b : h->blocks) {
-    const pdrd::PatchArgs a = patch_args(h, &b, update, outputs, excl, cfl);
+    const pdrd::PatchArgs a = patch_args(h, &b, update, outputs, excl, cfl, c);
     const pdrd::PatchLauncher f = a.nodal ? b.ks.patch_nodal : (h->precision ? b.ks.patch_mixed : b.ks.patch);
     HIP_CHECK(f(a, b.np, b.cells, h->stream));
   }
@@ -449,9 +477,10 @@ int launch_patch_blocks(pdrd_method *h, int update, int outputs, int excl, doubl
 }
 
 // The weak boundary conditions' facet kernels: after the patch kernels of a
-// sweep (whose facet nodes are all shared, so their state is still the
-// sweep's start state) and before the shared-node sums.
-int launch_weak(pdrd_method *h, hipStream_t s)
+// sweep (whose facet nodes are all shared, so u holds their state at the
+// sweep's start) and before the shared-node sums; rows into row buffer
+// `buffer`.
+int launch_weak(pdrd_method *h, hipStream_t s, int buffer = 0)
 {
   for (const WeakDev &w : h->weak) {
     if (w.nf == 0) continue;
@@ -464,7 +493,7 @@ int launch_weak(pdrd_method *h, hipStream_t s)
     a.V = w.d_V;
     a.u = h->d_u;
     a.vol = h->d_vol;
-    a.slot_acc = h->d_slot_acc;
+    a.slot_acc = rows_buffer(h, buffer);
     a.status = h->d_status;
     a.nf = w.nf;
     a.nq = w.nq;
@@ -496,16 +525,12 @@ int launch_weak(pdrd_method *h, hipStream_t s)
   return PDRD_OK;
 }
 
-// Patch assembly of every block followed by the sums of the shared nodes.
-// update = 0: residual and wave speed of every node; update = 1: one
-// pseudo-time iteration (the strong Dirichlet values are applied in the
-// nodal step), with res / dt / ws / norm partials when outputs is set.
-int launch_patches(pdrd_method *h, int update, int outputs, int excl, double cfl)
+// The sums of the shared nodes (rows of row buffer `buffer`) and their nodal
+// step.
+int launch_fixup(pdrd_method *h, int update, int outputs, int excl, double cfl, int buffer = 0)
 {
-  if (int rc = launch_patch_blocks(h, update, outputs, excl, cfl)) return rc;
-  if (int rc = launch_weak(h, h->stream)) return rc;
   if (h->n_shared == 0) return PDRD_OK;
-  const pdrd::FixupArgs f = fixup_args(h, update, outputs, excl, cfl);
+  const pdrd::FixupArgs f = fixup_args(h, update, outputs, excl, cfl, buffer);
   const unsigned nb = fixup_blocks(h);
 #define FIX_CASE(MM)                                                                                    \
   case MM:                                                                                              \
@@ -520,6 +545,28 @@ int launch_patches(pdrd_method *h, int update, int outputs, int excl, double cfl
 #undef FIX_CASE
   HIP_CHECK(hipGetLastError());
   return PDRD_OK;
+}
+
+// Patch assembly of every block followed by the sums of the shared nodes.
+// update = 0: residual and wave speed of every node; update = 1: one
+// pseudo-time iteration (the strong Dirichlet values are applied in the
+// nodal step), with res / dt / ws / norm partials when outputs is set.
+int launch_patches(pdrd_method *h, int update, int outputs, int excl, double cfl)
+{
+  if (int rc = launch_patch_blocks(h, update, outputs, excl, cfl)) return rc;
+  if (int rc = launch_weak(h, h->stream)) return rc;
+  return launch_fixup(h, update, outputs, excl, cfl);
+}
+
+// One pseudo-time iteration of a chain (patch assembly): the shared nodes of
+// the previous sweep are finished in this sweep's gather when c.pend >= 0;
+// the shared-node sums of this sweep are summed here (close) or left to the
+// next sweep.
+int launch_chain_sweep(pdrd_method *h, int outputs, int excl, double cfl, Chain c, bool close)
+{
+  if (int rc = launch_patch_blocks(h, 1, outputs, excl, cfl, c)) return rc;
+  if (int rc = launch_weak(h, h->stream, c.write)) return rc;
+  return close ? launch_fixup(h, 1, outputs, excl, cfl, c.write) : PDRD_OK;
 }
 
 int launch_dirichlet(pdrd_method *h)
@@ -663,7 +710,7 @@ void free_device(pdrd_method *h)
     b.d_lconn = nullptr;
   }
   uint32_t **u32s[] = {&h->d_pnode_off, &h->d_pnodes, &h->d_pn_inner, &h->d_shared_nodes, &h->d_shared_off,
-                       &h->d_slot_pos};
+                       &h->d_slot_pos, &h->d_pn_sid};
   for (uint32_t **p : u32s) {
     (void)hipFree(*p);
     *p = nullptr;
@@ -680,6 +727,12 @@ void free_device(pdrd_method *h)
   (void)hipFree(h->d_pn_coff);
   (void)hipFree(h->d_slot_acc);
   (void)hipFree(h->d_patch_partial);
+  for (double **q : {&h->d_slot_acc_b, &h->d_base[0], &h->d_base[1]}) {
+    (void)hipFree(*q);
+    *q = nullptr;
+  }
+  (void)hipFree(h->d_pn_rows);
+  h->d_pn_rows = nullptr;
   (void)hipFree(h->d_corr);
   h->d_corr = nullptr;
   for (double **q : {&h->d_blend, &h->d_visc, &h->d_stab_tmp}) {
@@ -756,6 +809,75 @@ int iterate_norm(pdrd_method *h, double *norm)
   if (h->scatter_mode == 2)
     return reduce_partials(h, h->np_total + (h->n_shared ? fixup_blocks(h) : 0), norm, h->d_patch_partial);
   return reduce_partials(h, norm_blocks(h->nn), norm);
+}
+
+// Iterations of the patch assembly. Chained (PDRD_OPT_CHAIN), the
+// shared-node sums of one sweep are done in the next sweep's gather, so a
+// sweep is one launch per element type (plus the weak facet kernels); a
+// sweep closes the chain (runs the fix-up) when it is the call's last or when
+// a defect-correction refresh, which needs the complete state, comes before
+// the next one. Unchained, every sweep runs the fix-up. The sweeps are
+// launched directly: replaying them from captured HIP graphs measured slower
+// on MI355X (25.6 against 11.1 us per sweep of a 32 768-cell P1 mesh, the
+// launches being dearer than the kernels). With timing on, events bracket the
+// patch kernels (cell interval) and the facet kernels + fix-up (node
+// interval) of every sweep.
+// Default: chained for the scalar physics (fewer launches, little shared-node
+// work per patch), not for the Euler systems, whose patches hold more shared
+// nodes of more equations, so that the deferred steps cost more in their
+// gather than the fix-up pass does.
+bool chains(const pdrd_method *h) { return h->chain >= 0 ? h->chain == 1 : h->m == 1; }
+
+int iterate_patches(pdrd_method *h, double cfl, int n_iter, int excl, double *norm)
+{
+  const bool corr = corrects(h);
+  const bool chain = chains(h);
+  double cell = 0.0, node = 0.0;
+  int pend = -1;
+  if (!h->timing) HIP_CHECK(hipEventRecord(h->ev[3], h->stream));
+  for (int it = 0; it < n_iter; ++it) {
+    if (corr && h->since_refresh >= h->refresh)
+      if (int rc = refresh_correction(h)) return rc;
+    const bool last = it == n_iter - 1;
+    const bool close = !chain || last || (corr && h->since_refresh + 1 >= h->refresh);
+    const int out = last ? 1 : 0;
+    Chain c;
+    c.pend = pend;
+    c.write = (pend == 0) ? 1 : 0;
+    c.record = !close;
+    if (h->timing) {
+      HIP_CHECK(hipEventRecord(h->ev[0], h->stream));
+      if (int rc = launch_patch_blocks(h, 1, out, excl, cfl, c)) return rc;
+      HIP_CHECK(hipEventRecord(h->ev[1], h->stream));
+      if (int rc = launch_weak(h, h->stream, c.write)) return rc;
+      if (close)
+        if (int rc = launch_fixup(h, 1, out, excl, cfl, c.write)) return rc;
+      HIP_CHECK(hipEventRecord(h->ev[2], h->stream));
+      HIP_CHECK(hipEventSynchronize(h->ev[2]));
+      float a = 0.f, b = 0.f;
+      HIP_CHECK(hipEventElapsedTime(&a, h->ev[0], h->ev[1]));
+      HIP_CHECK(hipEventElapsedTime(&b, h->ev[1], h->ev[2]));
+      cell += a;
+      node += b;
+    } else {
+      if (int rc = launch_chain_sweep(h, out, excl, cfl, c, close)) return rc;
+    }
+    if (corr) ++h->since_refresh;
+    pend = close ? -1 : c.write;
+  }
+  if (h->timing) {
+    h->last_cell_ms = cell / n_iter;
+    h->last_node_ms = node / n_iter;
+  } else {
+    HIP_CHECK(hipEventRecord(h->ev[1], h->stream));
+    HIP_CHECK(hipEventSynchronize(h->ev[1]));
+    float ms = 0.f;
+    HIP_CHECK(hipEventElapsedTime(&ms, h->ev[3], h->ev[1]));
+    h->last_cell_ms = ms;
+    h->last_node_ms = 0.0;
+  }
+  if (norm) return iterate_norm(h, norm);
+  return PDRD_OK;
 }
 
 }  // namespace
@@ -1011,6 +1133,20 @@ int pdrd_initialize(pdrd_method_t h)
       std::vector<uint32_t> slot_pos(plan.pnodes.size(), 0);
       for (size_t k = 0; k < plan.shared_slots.size(); ++k)
         if (plan.shared_slots[k] != UINT32_MAX) slot_pos[plan.shared_slots[k]] = static_cast<uint32_t>(k);
+      // the shared-node index and the node's rows of every shared slot (0
+      // for the others)
+      std::vector<uint32_t> pn_sid(plan.pnodes.size(), 0);
+      std::vector<uint2> pn_rows(plan.pnodes.size(), uint2{0u, 0u});
+      {
+        std::vector<uint32_t> sid(h->nn, 0);
+        for (size_t i = 0; i < plan.shared_nodes.size(); ++i) sid[plan.shared_nodes[i]] = static_cast<uint32_t>(i);
+        for (size_t p = 0; p < plan.pn_inner.size(); ++p)
+          for (uint32_t k = plan.pnode_off[p] + plan.pn_inner[p]; k < plan.pnode_off[p + 1]; ++k) {
+            const uint32_t i = sid[plan.pnodes[k]];
+            pn_sid[k] = i;
+            pn_rows[k] = uint2{plan.shared_off[i], plan.shared_off[i + 1]};
+          }
+      }
       h->np_total = static_cast<uint32_t>(plan.pn_inner.size());
       h->n_shared = static_cast<uint32_t>(plan.shared_nodes.size());
       auto up32 = [&](uint32_t **d, const std::vector<uint32_t> &v) -> int {
@@ -1022,16 +1158,22 @@ int pdrd_initialize(pdrd_method_t h)
       };
       if ((rc = up32(&h->d_pnode_off, plan.pnode_off)) || (rc = up32(&h->d_pnodes, plan.pnodes)) ||
           (rc = up32(&h->d_pn_inner, plan.pn_inner)) || (rc = up32(&h->d_shared_nodes, plan.shared_nodes)) ||
-          (rc = up32(&h->d_shared_off, plan.shared_off)) || (rc = up32(&h->d_slot_pos, slot_pos))) {
+          (rc = up32(&h->d_shared_off, plan.shared_off)) || (rc = up32(&h->d_slot_pos, slot_pos)) ||
+          (rc = up32(&h->d_pn_sid, pn_sid))) {
         free_device(h);
         return rc;
       }
       if ((rc = dev_alloc(h, &h->d_pn_coff, plan.pn_coff.size()))) { free_device(h); return rc; }
       HIP_CHECK(hipMemcpy(h->d_pn_coff, plan.pn_coff.data(), plan.pn_coff.size() * sizeof(uint16_t), hipMemcpyHostToDevice));
-      if ((rc = dev_alloc(h, &h->d_slot_acc, static_cast<size_t>(plan.shared_slots.size()) * (h->m + 1)))) {
+      if ((rc = dev_alloc(h, &h->d_slot_acc, static_cast<size_t>(plan.shared_slots.size()) * (h->m + 1))) ||
+          (rc = dev_alloc(h, &h->d_slot_acc_b, static_cast<size_t>(plan.shared_slots.size()) * (h->m + 1))) ||
+          (rc = dev_alloc(h, &h->d_base[0], static_cast<size_t>(h->n_shared) * h->m)) ||
+          (rc = dev_alloc(h, &h->d_base[1], static_cast<size_t>(h->n_shared) * h->m)) ||
+          (rc = dev_alloc(h, &h->d_pn_rows, pn_rows.size()))) {
         free_device(h);
         return rc;
       }
+      HIP_CHECK(hipMemcpy(h->d_pn_rows, pn_rows.data(), pn_rows.size() * sizeof(uint2), hipMemcpyHostToDevice));
       if ((rc = dev_alloc(h, &h->d_patch_partial, static_cast<size_t>(h->np_total + kNormMaxBlocks) * (h->m + 1)))) {
         free_device(h);
         return rc;
@@ -1151,6 +1293,10 @@ int pdrd_set_option(pdrd_method_t h, int option, int value)
       if (value < 0) return set_error(PDRD_EINVAL, "refresh period must be >= 0");
       h->refresh = value;
       h->since_refresh = 1 << 30;
+      break;
+    case PDRD_OPT_CHAIN:
+      if (value < -1 || value > 1) return set_error(PDRD_EINVAL, "chain is -1 (default), 0 or 1");
+      h->chain = value;
       break;
     case PDRD_OPT_PATCH_CELLS:
       if (h->ready) return set_error(PDRD_EINVAL, "cells per patch are set before pdrd_initialize");
@@ -1611,39 +1757,17 @@ int pdrd_iterate(pdrd_method_t h, double cfl, int n_iter, int exclude_dirichlet,
   if (int rc = check_ready(h)) return rc;
   if (n_iter < 1) return set_error(PDRD_EINVAL, "n_iter must be >= 1");
   const int excl = exclude_dirichlet ? 1 : 0;
+  if (h->scatter_mode == 2) return iterate_patches(h, cfl, n_iter, excl, norm);
   if (h->timing) {
     // eager iterations with events around the cell loop and the nodal pass
     double cell = 0.0, node = 0.0;
     for (int it = 0; it < n_iter; ++it) {
       const int out = (it == n_iter - 1) ? 1 : 0;
-      if (corrects(h) && h->since_refresh >= h->refresh)
-        if (int rc = refresh_correction(h)) return rc;
-      if (corrects(h)) ++h->since_refresh;
       HIP_CHECK(hipEventRecord(h->ev[0], h->stream));
-      if (h->scatter_mode == 2) {
-        // the cell loop and the nodal step of patch-only nodes are one kernel;
-        // the second interval is the shared-node fix-up
-        if (int rc = launch_patch_blocks(h, 1, out, excl, cfl)) return rc;
-        HIP_CHECK(hipEventRecord(h->ev[1], h->stream));
-        if (int rc = launch_weak(h, h->stream)) return rc;
-        if (h->n_shared) {
-          // fix-up only: re-enqueue through launch_patches would repeat the cells
-          const pdrd::FixupArgs f = fixup_args(h, 1, out, excl, cfl);
-          const unsigned nbk = fixup_blocks(h);
-          if (h->m == 1)
-            hipLaunchKernelGGL(pdrd::patch_fixup_kernel<1>, dim3(nbk), dim3(256), 0, h->stream, f);
-          else if (h->m == 4)
-            hipLaunchKernelGGL(pdrd::patch_fixup_kernel<4>, dim3(nbk), dim3(256), 0, h->stream, f);
-          else
-            hipLaunchKernelGGL(pdrd::patch_fixup_kernel<5>, dim3(nbk), dim3(256), 0, h->stream, f);
-          HIP_CHECK(hipGetLastError());
-        }
-      } else {
-        if (int rc = launch_cells(h)) return rc;
-        HIP_CHECK(hipEventRecord(h->ev[1], h->stream));
-        if (int rc = launch_dirichlet(h)) return rc;
-        if (int rc = launch_node_pass(h, cfl, out, excl)) return rc;
-      }
+      if (int rc = launch_cells(h)) return rc;
+      HIP_CHECK(hipEventRecord(h->ev[1], h->stream));
+      if (int rc = launch_dirichlet(h)) return rc;
+      if (int rc = launch_node_pass(h, cfl, out, excl)) return rc;
       HIP_CHECK(hipEventRecord(h->ev[2], h->stream));
       HIP_CHECK(hipEventSynchronize(h->ev[2]));
       float a = 0.f, b = 0.f;
@@ -1735,6 +1859,7 @@ int pdrd_get_option(pdrd_method_t h, int option, int *value)
     case PDRD_OPT_BLOCK: *value = h->block_threads; return PDRD_OK;
     case PDRD_OPT_TIMING: *value = h->timing ? 1 : 0; return PDRD_OK;
     case PDRD_OPT_REFRESH: *value = h->refresh; return PDRD_OK;
+    case PDRD_OPT_CHAIN: *value = chains(h) ? 1 : 0; return PDRD_OK;
     case PDRD_OPT_PATCH_CELLS: {
       int c = 0;
       for (const Block &b : h->blocks) c = std::max(c, b.cells);
