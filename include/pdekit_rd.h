@@ -293,8 +293,11 @@ int pdrd_last_timing(pdrd_method_t h, double *cell_ms, double *node_ms);
  * problem at a time): n_iter iterations of each of `count` independent
  * methods, with the patches of every method of one element type / physics /
  * scheme in one launch per iteration and the shared-node sums of every method
- * with the same number of equations in another; the iterations are captured
- * once per call into a HIP graph on the batch's stream. Each method advances
+ * with the same number of equations in another, launched directly on the
+ * batch's stream (or, after pdrd_batch_set_graphs(b, 1), captured once per
+ * call into a HIP graph replayed for all but the last iteration: measured
+ * slower on MI355X, whose direct launches cost less than the replays; p99 of
+ * the serving stream 2.67 s direct against 2.87 s from graphs). Each method advances
  * exactly as pdrd_iterate(methods[i], cfl[i], n_iter, exclude_dirichlet)
  * would (same kernels, same order of sums, so the same bits). Methods must be
  * initialised on the batch's device with the patch assembly; norms receives
@@ -302,6 +305,7 @@ int pdrd_last_timing(pdrd_method_t h, double *cell_ms, double *node_ms);
 typedef struct pdrd_batch *pdrd_batch_t;
 int pdrd_batch_create(pdrd_batch_t *out);
 int pdrd_batch_destroy(pdrd_batch_t b);
+int pdrd_batch_set_graphs(pdrd_batch_t b, int on);
 int pdrd_batch_iterate(pdrd_batch_t b, int count, const pdrd_method_t *methods, const double *cfl, int n_iter,
                        int exclude_dirichlet, double *norms);
 

@@ -42,7 +42,7 @@ EXPORTED = (
     "pdrd_add_strong_dirichlet", "pdrd_assemble_rhs", "pdrd_assemble_lhs_and_rhs", "pdrd_solve",
     "pdrd_compute_residual_norm", "pdrd_residual_norm_L2", "pdrd_iterate", "pdrd_check_status",
     "pdrd_method_info", "pdrd_last_timing", "pdrd_last_error", "pdrd_version", "pdrd_get_option",
-    "pdrd_batch_create", "pdrd_batch_destroy", "pdrd_batch_iterate", "pdrd_add_weak_bc", "pdrd_weak_bc_info",
+    "pdrd_batch_create", "pdrd_batch_destroy", "pdrd_batch_set_graphs", "pdrd_batch_iterate", "pdrd_add_weak_bc", "pdrd_weak_bc_info",
     "pdrd_mesh_read_gmsh", "pdrd_mesh_write_gmsh", "pdrd_gmsh_append_node_data", "pdrd_gmsh_read_node_data",
     "pdrd_set_blending_coeff", "pdrd_set_artificial_viscosity", "pdrd_compute_blending_coeff",
     "pdrd_compute_artificial_viscosity", "pdrd_element_jacobians",
@@ -122,6 +122,7 @@ def lib() -> C.CDLL:
         "pdrd_weak_bc_info": [P, I, PI64],
         "pdrd_batch_create": [C.POINTER(P)],
         "pdrd_batch_destroy": [P],
+        "pdrd_batch_set_graphs": [P, C.c_int],
         "pdrd_batch_iterate": [P, I, C.POINTER(P), PD, I, I, PD],
         "pdrd_set_blending_coeff": [P, PD],
         "pdrd_set_artificial_viscosity": [P, PD],

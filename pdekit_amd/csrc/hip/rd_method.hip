@@ -8,6 +8,7 @@
 #include <cmath>
 #include <cstring>
 #include <map>
+#include <memory>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -1891,6 +1892,7 @@ struct pdrd_batch
   pdrd::FixupArgs *d_fargs = nullptr;
   uint32_t *d_first = nullptr;
   size_t pargs_cap = 0, fargs_cap = 0, first_cap = 0;
+  int graphs = 0;  // 1: replay the iterations of a call from a captured HIP graph
 };
 
 namespace {
@@ -1921,24 +1923,61 @@ struct BatchLaunch
   size_t lds = 0;
 };
 
-int enqueue_batch(pdrd_batch *b, const std::vector<BatchLaunch> &launches, size_t args_half, size_t fargs_half,
-                  int outputs, int count, const pdrd_method_t *methods)
+// Kinds of batched sweeps. A method whose iterations chain (and only outside
+// graph capture) takes the place in the chain of the kind; any other method
+// runs a complete sweep (fix-up included) of every kind, with outputs in the
+// last.
+enum BatchKind
 {
-  bool weak_done = false;
-  for (const BatchLaunch &l : launches) {
-    if (!l.patch && !weak_done) {
-      // the weak BCs of every method, after all patch launches
-      for (int i = 0; i < count; ++i)
-        if (int rc = launch_weak(methods[i], b->stream)) return rc;
-      weak_done = true;
-    }
+  BK_OPEN = 0,     // first of several sweeps
+  BK_MID_W1 = 1,   // deferred sweep writing buffer 1
+  BK_MID_W0 = 2,   // deferred sweep writing buffer 0
+  BK_LAST_W1 = 3,  // closing sweep writing buffer 1
+  BK_LAST_W0 = 4,  // closing sweep writing buffer 0
+  BK_SINGLE = 5,   // the only sweep of a call
+  BK_COUNT = 6
+};
+
+struct KindPlace
+{
+  Chain c;
+  bool close = true;
+  int outputs = 0;
+};
+
+KindPlace kind_place(int kind, bool chain)
+{
+  KindPlace k;
+  const bool last = kind == BK_LAST_W1 || kind == BK_LAST_W0 || kind == BK_SINGLE;
+  k.outputs = last ? 1 : 0;
+  if (!chain) return k;  // complete sweep into buffer 0
+  switch (kind) {
+    case BK_OPEN: k.c.write = 0; k.c.record = true; k.close = false; break;
+    case BK_MID_W1: k.c.pend = 0; k.c.write = 1; k.c.record = true; k.close = false; break;
+    case BK_MID_W0: k.c.pend = 1; k.c.write = 0; k.c.record = true; k.close = false; break;
+    case BK_LAST_W1: k.c.pend = 0; k.c.write = 1; break;
+    case BK_LAST_W0: k.c.pend = 1; k.c.write = 0; break;
+    default: break;
+  }
+  return k;
+}
+
+// One batched sweep of kind `kind`: the patch launches (arguments of the kind
+// at kind * args_per_kind), the weak facet kernels of every method, the
+// shared-node sums of the methods that close in this kind.
+int enqueue_batch(pdrd_batch *b, const std::vector<BatchLaunch> &patches, const std::vector<BatchLaunch> &fixups,
+                  size_t args_per_kind, int kind, int count, const pdrd_method_t *methods, const bool *chain)
+{
+  for (const BatchLaunch &l : patches) {
     if (l.grid == 0) continue;
-    if (l.patch) {
-      HIP_CHECK(l.patch(b->d_pargs + (outputs ? args_half : 0) + l.args0, b->d_first + l.first0, l.count, l.grid,
-                        l.cells, l.lds, b->stream));
-      continue;
-    }
-    const pdrd::FixupArgs *fa = b->d_fargs + (outputs ? fargs_half : 0) + l.args0;
+    HIP_CHECK(l.patch(b->d_pargs + kind * args_per_kind + l.args0, b->d_first + l.first0, l.count, l.grid, l.cells,
+                      l.lds, b->stream));
+  }
+  for (int i = 0; i < count; ++i)
+    if (int rc = launch_weak(methods[i], b->stream, kind_place(kind, chain[i]).c.write)) return rc;
+  for (const BatchLaunch &l : fixups) {
+    if (l.grid == 0) continue;
+    const pdrd::FixupArgs *fa = b->d_fargs + l.args0;
 #define FIXB_CASE(MM)                                                                                            \
   case MM:                                                                                                       \
     hipLaunchKernelGGL(pdrd::patch_fixup_batch_kernel<MM>, dim3(l.grid), dim3(256), 0, b->stream, fa,             \
@@ -1988,6 +2027,14 @@ int pdrd_batch_destroy(pdrd_batch_t b)
   return PDRD_OK;
 }
 
+int pdrd_batch_set_graphs(pdrd_batch_t b, int on)
+{
+  if (!b) return set_error(PDRD_EINVAL, "null batch");
+  if (on != 0 && on != 1) return set_error(PDRD_EINVAL, "graphs is 0 or 1");
+  b->graphs = on;
+  return PDRD_OK;
+}
+
 int pdrd_batch_iterate(pdrd_batch_t b, int count, const pdrd_method_t *methods, const double *cfl, int n_iter,
                        int exclude_dirichlet, double *norms)
 {
@@ -2010,81 +2057,108 @@ int pdrd_batch_iterate(pdrd_batch_t b, int count, const pdrd_method_t *methods, 
   }
   for (int i = 0; i < count; ++i) HIP_CHECK(hipStreamSynchronize(methods[i]->stream));
 
+  // sweeps of methods whose iterations chain defer their shared-node sums
+  // (direct launches only: the graph replays one kind of sweep)
+  std::unique_ptr<bool[]> chain_v(new bool[count]);
+  for (int i = 0; i < count; ++i) chain_v[i] = !b->graphs && chains(methods[i]);
+  const bool *chain = chain_v.get();
+
   // group the blocks of all methods by kernel, in order of first appearance
-  std::vector<BatchLaunch> launches;
+  std::vector<BatchLaunch> patches;
   std::vector<std::vector<std::pair<int, size_t>>> members;  // (method, block) per launch
   for (int i = 0; i < count; ++i)
     for (size_t k = 0; k < methods[i]->blocks.size(); ++k) {
       const Block &bl = methods[i]->blocks[k];
       const pdrd::PatchBatchLauncher fn = methods[i]->precision ? bl.ks.patch_batch_mixed : bl.ks.patch_batch;
       size_t g = 0;
-      while (g < launches.size() && (launches[g].patch != fn || launches[g].cells != bl.cells)) ++g;
-      if (g == launches.size()) {
+      while (g < patches.size() && (patches[g].patch != fn || patches[g].cells != bl.cells)) ++g;
+      if (g == patches.size()) {
         BatchLaunch l;
         l.patch = fn;
         l.cells = bl.cells;
-        launches.push_back(l);
+        patches.push_back(l);
         members.emplace_back();
       }
       members[g].emplace_back(i, k);
     }
-  const size_t npatch_launch = launches.size();
-  for (int mm : {1, 4, 5}) {
-    BatchLaunch l;
-    l.m = mm;
-    std::vector<std::pair<int, size_t>> mem;
-    for (int i = 0; i < count; ++i)
-      if (methods[i]->m == mm && methods[i]->n_shared) mem.emplace_back(i, 0);
-    if (mem.empty()) continue;
-    launches.push_back(l);
-    members.push_back(mem);
-  }
-  // argument arrays ([no outputs | outputs]) and prefix sums of workgroups
-  std::vector<pdrd::PatchArgs> pargs[2];
-  std::vector<pdrd::FixupArgs> fargs[2];
+  // patch arguments of every kind ([kind][launch members]) and prefix sums of
+  // workgroups (shared by the kinds)
+  std::vector<pdrd::PatchArgs> pargs;
   std::vector<uint32_t> first;
-  for (size_t g = 0; g < launches.size(); ++g) {
-    BatchLaunch &l = launches[g];
+  std::vector<pdrd::PatchArgs> pk[BK_COUNT];
+  for (size_t g = 0; g < patches.size(); ++g) {
+    BatchLaunch &l = patches[g];
     l.first0 = first.size();
+    l.args0 = pk[0].size();
     l.count = static_cast<int>(members[g].size());
     uint32_t acc = 0;
     first.push_back(0);
-    if (g < npatch_launch) {
-      l.args0 = pargs[0].size();
-      for (const auto &mk : members[g]) {
-        pdrd_method *h = methods[mk.first];
-        const Block &bl = h->blocks[mk.second];
-        for (int o = 0; o < 2; ++o) pargs[o].push_back(patch_args(h, &bl, 1, o, excl, cfl[mk.first]));
-        l.lds = std::max(l.lds, bl.ks.patch_lds_bytes(bl.maxl, bl.cells));
-        acc += bl.np;
-        first.push_back(acc);
+    for (const auto &mk : members[g]) {
+      pdrd_method *h = methods[mk.first];
+      const Block &bl = h->blocks[mk.second];
+      for (int k = 0; k < BK_COUNT; ++k) {
+        const KindPlace kp = kind_place(k, chain[mk.first]);
+        pk[k].push_back(patch_args(h, &bl, 1, kp.outputs, excl, cfl[mk.first], kp.c));
       }
-    } else {
-      l.args0 = fargs[0].size();
-      for (const auto &mk : members[g]) {
-        pdrd_method *h = methods[mk.first];
-        for (int o = 0; o < 2; ++o) fargs[o].push_back(fixup_args(h, 1, o, excl, cfl[mk.first]));
-        acc += fixup_blocks(h);
-        first.push_back(acc);
-      }
+      l.lds = std::max(l.lds, bl.ks.patch_lds_bytes(bl.maxl, bl.cells));
+      acc += bl.np;
+      first.push_back(acc);
     }
     l.grid = acc;
   }
-  const size_t ph = pargs[0].size(), fh = fargs[0].size();
-  if (int rc = grow(&b->d_pargs, &b->pargs_cap, 2 * ph)) return rc;
-  if (int rc = grow(&b->d_fargs, &b->fargs_cap, 2 * fh)) return rc;
+  const size_t ph = pk[0].size();
+  for (int k = 0; k < BK_COUNT; ++k) pargs.insert(pargs.end(), pk[k].begin(), pk[k].end());
+  // shared-node sums of every kind: one launch per number of equations over
+  // the methods that close in that kind
+  std::vector<BatchLaunch> fixups[BK_COUNT];
+  std::vector<pdrd::FixupArgs> fargs;
+  for (int k = 0; k < BK_COUNT; ++k)
+    for (int mm : {1, 4, 5}) {
+      BatchLaunch l;
+      l.m = mm;
+      l.first0 = first.size();
+      l.args0 = fargs.size();
+      uint32_t acc = 0;
+      first.push_back(0);
+      for (int i = 0; i < count; ++i) {
+        pdrd_method *h = methods[i];
+        const KindPlace kp = kind_place(k, chain[i]);
+        if (h->m != mm || h->n_shared == 0 || !kp.close) continue;
+        fargs.push_back(fixup_args(h, 1, kp.outputs, excl, cfl[i], kp.c.write));
+        acc += fixup_blocks(h);
+        first.push_back(acc);
+        ++l.count;
+      }
+      if (l.count == 0) {
+        first.pop_back();
+        continue;
+      }
+      l.grid = acc;
+      fixups[k].push_back(l);
+    }
+  if (int rc = grow(&b->d_pargs, &b->pargs_cap, pargs.size())) return rc;
+  if (int rc = grow(&b->d_fargs, &b->fargs_cap, fargs.size())) return rc;
   if (int rc = grow(&b->d_first, &b->first_cap, first.size())) return rc;
-  for (int o = 0; o < 2; ++o) {
-    if (ph) HIP_CHECK(hipMemcpy(b->d_pargs + o * ph, pargs[o].data(), ph * sizeof(pdrd::PatchArgs), hipMemcpyHostToDevice));
-    if (fh) HIP_CHECK(hipMemcpy(b->d_fargs + o * fh, fargs[o].data(), fh * sizeof(pdrd::FixupArgs), hipMemcpyHostToDevice));
-  }
+  if (!pargs.empty())
+    HIP_CHECK(hipMemcpy(b->d_pargs, pargs.data(), pargs.size() * sizeof(pdrd::PatchArgs), hipMemcpyHostToDevice));
+  if (!fargs.empty())
+    HIP_CHECK(hipMemcpy(b->d_fargs, fargs.data(), fargs.size() * sizeof(pdrd::FixupArgs), hipMemcpyHostToDevice));
   HIP_CHECK(hipMemcpy(b->d_first, first.data(), first.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
 
-  if (n_iter > 1) {
+  auto sweep = [&](int kind) { return enqueue_batch(b, patches, fixups[kind], ph, kind, count, methods, chain); };
+  if (n_iter == 1) {
+    if (int rc = sweep(BK_SINGLE)) return rc;
+  } else if (!b->graphs) {
+    if (int rc = sweep(BK_OPEN)) return rc;
+    for (int it = 1; it < n_iter - 1; ++it)
+      if (int rc = sweep((it & 1) ? BK_MID_W1 : BK_MID_W0)) return rc;
+    if (int rc = sweep(((n_iter - 1) & 1) ? BK_LAST_W1 : BK_LAST_W0)) return rc;
+  } else {
+    // no method chains here: every kind is a complete sweep, the last with outputs
     hipGraph_t g = nullptr;
     hipGraphExec_t ge = nullptr;
     HIP_CHECK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
-    const int rc = enqueue_batch(b, launches, ph, fh, 0, count, methods);
+    const int rc = sweep(BK_OPEN);
     hipError_t e = hipStreamEndCapture(b->stream, &g);
     if (rc) {
       if (g) (void)hipGraphDestroy(g);
@@ -2101,8 +2175,8 @@ int pdrd_batch_iterate(pdrd_batch_t b, int count, const pdrd_method_t *methods, 
     (void)hipStreamSynchronize(b->stream);
     (void)hipGraphExecDestroy(ge);
     if (e != hipSuccess) return hip_fail(e, "hipGraphLaunch");
+    if (int rc2 = sweep(BK_SINGLE)) return rc2;
   }
-  if (int rc = enqueue_batch(b, launches, ph, fh, 1, count, methods)) return rc;
   HIP_CHECK(hipStreamSynchronize(b->stream));
   if (norms) {
     size_t off = 0;

@@ -275,12 +275,15 @@ class Batch:
     """Iterations of several independent methods in shared launches
     (pdrd_batch_iterate): one launch per element type / physics / scheme and
     iteration for the patches of all methods, one for their shared nodes.
-    Every method advances exactly as its own iterate() would."""
+    Every method advances exactly as its own iterate() would. graphs: replay
+    the iterations of a call from a captured HIP graph instead of launching
+    them directly."""
 
-    def __init__(self):
+    def __init__(self, graphs: bool = False):
         h = C.c_void_p()
         L.check(L.lib().pdrd_batch_create(C.byref(h)), "pdrd_batch_create")
         self._h = h
+        L.check(L.lib().pdrd_batch_set_graphs(h, int(graphs)), "pdrd_batch_set_graphs")
 
     def iterate(self, methods, cfl, n_iter: int, exclude_dirichlet: bool = False):
         """methods: PGRDMethodExplicit list; cfl: one value or one per method.

@@ -107,7 +107,8 @@ def build_problem(req: Request) -> Problem:
 def footprint_bytes(p: Problem) -> int:
     """Device bytes of one resident request (a bound on what pdrd_initialize
     allocates: nodal fields and accumulators, coordinates, connectivity in
-    three layouts and the patch plan)."""
+    three layouts and the patch plan with its two buffers of shared-node
+    rows and states)."""
     m = p.mesh
     neq = p.u0.shape[1]
     N, E = m.nb_nodes, m.nb_cells
@@ -115,7 +116,8 @@ def footprint_bytes(p: Problem) -> int:
     ng = max(b.geo_conn.shape[1] for b in m.blocks)
     nodal = 8 * N * (3 * neq + 6) + 4 * N
     slots = 2 * N  # nodes plus their extra slots on patch borders, bounded
-    patch = slots * (8 + 8 * (neq + 1)) + E * (2 * nd + 4 * ng)
+    # per slot: node id, slot row, pair offset, shared index, row range, two rows
+    patch = slots * (4 + 4 + 2 + 4 + 8 + 2 * 8 * (neq + 1)) + 2 * 8 * neq * N + E * (2 * nd + 4 * ng)
     return int(nodal + 8 * m.coords.size + E * 4 * (2 * nd + 2 * ng) + patch + (1 << 20))
 
 

@@ -1,6 +1,7 @@
 """Batched iterations (pdrd_batch_iterate): several independent methods of
 different element types, orders, physics and schemes advanced in shared
-launches give bit for bit what each method's own pdrd_iterate gives."""
+launches (direct, or replayed from a captured HIP graph) give bit for bit
+what each method's own pdrd_iterate gives."""
 import numpy as np
 import pytest
 
@@ -29,13 +30,20 @@ def _make(name, scheme):
     return m, CFL[physics]
 
 
-@pytest.mark.parametrize("n_iter", [1, 7])
-def test_batch_matches_single_methods(n_iter):
-    print(f"batch of {len(MIX)} methods, {n_iter} iterations", flush=True)
+@pytest.mark.parametrize("chain", [None, 0, 1])
+@pytest.mark.parametrize("graphs", [False, True])
+@pytest.mark.parametrize("n_iter", [1, 2, 7, 8])
+def test_batch_matches_single_methods(n_iter, graphs, chain):
+    """chain: the methods' PDRD_OPT_CHAIN (None: each method's default, which
+    chains the scalar methods and not the Euler ones)."""
+    print(f"batch of {len(MIX)} methods, {n_iter} iterations, graphs {graphs}, chain {chain}", flush=True)
     alone = [_make(n, s) for n, s in MIX]
     mixed = [_make(n, s) for n, s in MIX]
+    if chain is not None:
+        for m, _ in alone + mixed:
+            m.set_option(L.OPT_CHAIN, chain)
     ref_norms = [m.iterate(c, n_iter) for m, c in alone]
-    b = Batch()
+    b = Batch(graphs=graphs)
     norms = b.iterate([m for m, _ in mixed], [c for _, c in mixed], n_iter)
     for (ma, _), (mb, _), na, nb, (name, _) in zip(alone, mixed, ref_norms, norms, MIX):
         for f in ("solution", "residuals", "time_step", "wave_speed"):

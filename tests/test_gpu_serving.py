@@ -26,3 +26,18 @@ def test_results_alone_and_in_the_mix_are_identical():
         assert a.converged and b.converged, (a.rid, a.norm, b.norm)
         assert a.iterations == b.iterations
         assert np.array_equal(a.solution, b.solution)
+
+
+def test_footprint_bounds_the_device_allocation():
+    """The admission footprint is an upper bound of what a request's method
+    allocates on the device."""
+    reqs = S.request_stream(6, rate=400.0, seed=11, adv_cells=(500, 20_000), burgers_cells=(500, 20_000))
+    ex = S.GpuExecutor()
+    for r in reqs:
+        p = S.build_problem(r)
+        m = ex.start(p)
+        used = m.info()["device_bytes"]
+        print(f"request {r.rid} {r.family} P{r.order} {p.mesh.nb_cells} cells: {used} of {S.footprint_bytes(p)} B",
+              flush=True)
+        assert 0 < used <= S.footprint_bytes(p)
+        ex.finish(m)
