@@ -926,14 +926,18 @@ struct PatchSize
 // Waves per SIMD the register allocation of a patch kernel must leave room
 // for: 5 for the scalar affine P1 / P2 cells (at most 96 VGPRs, measured
 // faster than 4 waves); 2 for the 2D Euler cells of up to 4 nodes (little or
-// no scratch, measured faster than 1 wave); the others keep the compiler's
-// choice (the 3D Euler cells run as fast at 1 wave, where capping them at 2
-// spills a few hundred bytes per thread to scratch memory; the 2D Euler P2 /
-// Q2 / P3 cells spill 250-1600 bytes per thread at 2 waves).
-template <int M, int ND, bool AFFINE>
+// no scratch, measured faster than 1 wave) and for the mixed-precision LDA
+// cells of the 3D Euler P1 tetrahedra (8 bytes of scratch; 1.4x faster than
+// at 1 wave); the others keep the compiler's choice (the FP64 3D Euler cells
+// run as fast at 1 wave, where capping them at 2 spills a few hundred bytes
+// per thread to scratch memory, as do the mixed N / B / PSI ones; the 2D
+// Euler P2 / Q2 / P3 cells spill 250-1600 bytes per thread at 2 waves).
+template <int M, int ND, bool AFFINE, int SC, int PREC>
 struct PatchMinWaves
 {
-  static constexpr int value = (M == 1 && ND <= 6 && AFFINE) ? 5 : ((M == 4 && ND <= 4) ? 2 : 1);
+  static constexpr int value =
+      (M == 1 && ND <= 6 && AFFINE) ? 5
+                                    : (((M == 4 && ND <= 4) || (M == 5 && ND <= 4 && SC == 0 && PREC == 1)) ? 2 : 1);
 };
 
 // Dynamic LDS of one patch workgroup of P cells, in doubles: node state
@@ -1200,7 +1204,7 @@ __device__ __forceinline__ void patch_body(const PatchArgs &a, uint32_t pl, doub
 
 template <int D, int NG, int ND, int NF, int NQ, bool AFFINE, int PH, int SC, int PREC>
 __global__ __launch_bounds__((PatchSize<PhysTraits<PH>::M, ND>::value),
-                             (PatchMinWaves<PhysTraits<PH>::M, ND, AFFINE>::value)) void rd_patch_kernel(PatchArgs a)
+                             (PatchMinWaves<PhysTraits<PH>::M, ND, AFFINE, SC, PREC>::value)) void rd_patch_kernel(PatchArgs a)
 {
   extern __shared__ double lds[];
   const uint32_t pl = xcd_block_index();
@@ -1214,7 +1218,7 @@ __global__ __launch_bounds__((PatchSize<PhysTraits<PH>::M, ND>::value),
 // the entries'.
 template <int D, int NG, int ND, int NF, int NQ, bool AFFINE, int PH, int SC, int PREC>
 __global__ __launch_bounds__((PatchSize<PhysTraits<PH>::M, ND>::value),
-                             (PatchMinWaves<PhysTraits<PH>::M, ND, AFFINE>::value)) void
+                             (PatchMinWaves<PhysTraits<PH>::M, ND, AFFINE, SC, PREC>::value)) void
 rd_patch_batch_kernel(const PatchArgs *__restrict__ args, const uint32_t *__restrict__ first, int count)
 {
   extern __shared__ double lds[];
