@@ -246,7 +246,7 @@ def main():
             best = (mode, cell_ms + node_ms)
     meth.set_option(L.OPT_SCATTER, best[0])
 
-    # warm-up (graph capture), then the timed region
+    # warm-up, then the timed region
     meth.set_option(L.OPT_TIMING, 0)
     meth.set_vec_function("solution", wl.state)
     meth.iterate(wl.cfl, max(1, args.warmup))

@@ -5,7 +5,7 @@ oracle, ≤ 1e-12 normwise relative:
   viscosity of the LDA scheme (random nodal fields, one cell in eight with
   theta below 1e-6 everywhere), one sweep in every assembly (patch, colour,
   atomic) on scalar P1 / P2 / P3, Euler 2D P1 + Q1 and P2 + Q2, Euler 3D P1,
-  and 10 iterations from a HIP graph captured before the input was set;
+  and 10 iterations of a method that iterated before the input was set;
 * the LF_Blend and smoothness-indicator providers computed on the device
   from the method's state (triangles P1-P3, quadrilaterals, tetrahedra).
 """
@@ -85,14 +85,14 @@ def test_nodal_input_sweep(name, scheme, scatter):
 
 @pytest.mark.parametrize("name", ["burgers_p2", "euler2d_mixed", "euler3d_p1"])
 @pytest.mark.parametrize("scheme", [L.B, L.LDA])
-def test_nodal_input_iterations_after_graph(name, scheme):
+def test_nodal_input_iterations_after_plain_ones(name, scheme):
     build, physics, params, state = CASES[name]
     mesh = build()
     u = state(mesh)
     m = PGRDMethodExplicit(physics, scheme, params=params if params else None)
     m.configure_mesh_data(mesh).initialize_work_data(quad_order=quad_orders(mesh))
     m.set_vec_function("solution", u)
-    m.iterate(0.3, 3)  # captures the plain iteration graph
+    m.iterate(0.3, 3)  # plain iterations first: the input must reach the later ones
     m.set_vec_function("solution", u)
     P = oracle_problem(mesh, physics, scheme, params)
     v = _nodal(mesh, 5) * (1.0 if scheme == L.B else 0.05)
