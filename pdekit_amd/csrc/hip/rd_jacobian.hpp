@@ -111,9 +111,22 @@ __global__ __launch_bounds__(64) void rd_jacobian_kernel(JacArgs a)
   const uint32_t c = a.perm[e];
   double *jac = a.jac + static_cast<size_t>(c) * NDM * NDM;
   double *jst = a.jstab + static_cast<size_t>(c) * NDM * NDM;
-  for (int i = 0; i < NDM * NDM; ++i) {
-    jac[i] = 0.0;
-    if (SC == SC_B) jst[i] = 0.0;
+  // Scalar cells of up to 6 nodes accumulate their Jacobian (and the B
+  // stabilisation block) in registers and store it once; the others
+  // accumulate in place in the caller's block.
+  constexpr bool REG = (M == 1) && (ND <= 6);
+  double Jr[REG ? NDM * NDM : 1], Jsr[(REG && SC == SC_B) ? NDM * NDM : 1];
+  if constexpr (REG) {
+#pragma unroll
+    for (int i = 0; i < NDM * NDM; ++i) Jr[i] = 0.0;
+    if constexpr (SC == SC_B)
+#pragma unroll
+      for (int i = 0; i < NDM * NDM; ++i) Jsr[i] = 0.0;
+  } else {
+    for (int i = 0; i < NDM * NDM; ++i) {
+      jac[i] = 0.0;
+      if (SC == SC_B) jst[i] = 0.0;
+    }
   }
   const bool ext_theta = (SC == SC_B) && a.nodal;
   const bool visc = (SC == SC_LDA) && a.nodal;
@@ -321,6 +334,39 @@ __global__ __launch_bounds__(64) void rd_jacobian_kernel(JacArgs a)
     small_inverse<M>(S, Si);
 #pragma unroll
     for (int k = 0; k < M; ++k) fint[k] += wj * divr[q][k];
+    if constexpr (REG) {
+      // the scalar form of the block loops below, fully unrolled
+      const double si = Si[0][0];
+#pragma unroll
+      for (int mm = 0; mm < ND; ++mm) {
+        const double bm = Kp[mm][0][0] * si;
+        if (SC != SC_N) lda[mm][0] += wj * (bm * divr[q][0]);
+#pragma unroll
+        for (int nn = 0; nn < ND; ++nn) {
+          const double p = bm * Kx[nn][0][0];
+          if (SC != SC_N) {
+            Jr[mm * NDM + nn] += wj * p;
+          } else if (nn != mm) {
+            ndis[mm][0] += wj * (p * (U[mm][0] - U[nn][0]));
+            Jr[mm * NDM + nn] += -wj * p;
+          } else {
+            Jr[mm * NDM + nn] += wj * (Kp[mm][0][0] - p);
+          }
+        }
+        if (SC == SC_B && !skip)
+#pragma unroll
+          for (int nn = 0; nn < ND; ++nn) {
+            const double p = bm * Kp[nn][0][0];
+            if (nn != mm) {
+              ndis[mm][0] += wj * (p * (U[mm][0] - U[nn][0]));
+              Jsr[mm * NDM + nn] += -wj * p;
+            } else {
+              Jsr[mm * NDM + nn] += wj * (Kp[mm][0][0] - p);
+            }
+          }
+      }
+      continue;
+    }
     for (int mm = 0; mm < ND; ++mm) {
       double B[M][M];  // K+_m S^-1
 #pragma unroll
@@ -433,17 +479,27 @@ __global__ __launch_bounds__(64) void rd_jacobian_kernel(JacArgs a)
     for (int n = 0; n < ND; ++n)
 #pragma unroll
       for (int k = 0; k < M; ++k) phi[n * M + k] = lda[n][k] + th[n] * ndis[n][k];
-    if (!skip)
+    if constexpr (REG) {
+      if (!skip)
+#pragma unroll
+        for (int i = 0; i < NDM; ++i)
+#pragma unroll
+          for (int j = 0; j < NDM; ++j) Jr[i * NDM + j] += th[i] * Jsr[i * NDM + j];
+    } else if (!skip) {
       for (int i = 0; i < NDM; ++i) {
         const double t = th[i / M];
         for (int j = 0; j < NDM; ++j) jac[i * NDM + j] += t * jst[i * NDM + j];
       }
+    }
   } else {
 #pragma unroll
     for (int n = 0; n < ND; ++n)
 #pragma unroll
       for (int k = 0; k < M; ++k) phi[n * M + k] = lda[n][k];
   }
+  if constexpr (REG)
+#pragma unroll
+    for (int i = 0; i < NDM * NDM; ++i) jac[i] = Jr[i];
   a.wave[c] = vol * fmax(lmax, 1.e-6);
   if (bad) atomicOr(a.status, 1);
 }
