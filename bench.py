@@ -170,6 +170,9 @@ def main():
     ap.add_argument("--chain", default="default", choices=["default", "0", "1"],
                     help="patch assembly: shared-node sums of a sweep in the next sweep's gather (1) or a "
                          "fix-up pass after every sweep (0); default: the library's choice")
+    ap.add_argument("--jacobians", action="store_true",
+                    help="time the element Jacobians of the implicit forms (pdrd_element_jacobians) on the "
+                         "workload's mesh and state instead of the explicit iterations")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-calibration-mb", type=int, default=0,
@@ -222,6 +225,9 @@ def main():
     meth.set_vec_function("solution", wl.state)
     if wl.dirichlet_nodes.size:
         meth.add_boundary_condition("StrongDirichlet", "inflow", wl.dirichlet_nodes, wl.dirichlet_values)
+    if args.jacobians:
+        jacobian_bench(args, wl, meth)
+        return
     setup_s = time.perf_counter() - t_setup
     print(f"[bench] {wl.name}: {E} elements, {wl.mesh.nb_nodes} nodes, setup {setup_s:.1f} s", file=sys.stderr,
           flush=True)
@@ -307,6 +313,51 @@ def main():
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(wl, args.cpu_seconds)
     print(json.dumps(out), flush=True)
+
+
+def jacobian_bench(args, wl, meth):
+    """Element Jacobians (pdrd_element_jacobians) of every block: the kernel's
+    duration from HIP events around its launch (pdrd_last_timing), mean of
+    `steps` calls per block after `warmup`; compulsory bytes = the kernel's
+    outputs (residual [nd m], Jacobian [(nd m)^2], wave speed per cell) and
+    inputs (connectivity, coordinates, u)."""
+    from pdekit_amd import _lib as L
+
+    m = wl.mesh
+    neq = wl.state.shape[1]
+    need = sum(b.sol_conn.shape[0] * (b.sol_conn.shape[1] * neq) ** 2 * 8 for b in m.blocks)
+    if need > 4 * 2**30:
+        print(f"bench.py --jacobians: the Jacobians of {wl.name} fill {need / 2**30:.1f} GiB of host memory "
+              "per call; use c1, c2 or c3", file=sys.stderr)
+        sys.exit(2)
+    total_ms, out_bytes = 0.0, 0
+    for blk, b in enumerate(m.blocks):
+        ne, nd = b.sol_conn.shape
+        for _ in range(max(1, args.warmup)):
+            meth.element_jacobians(blk)
+        ms = 0.0
+        for _ in range(args.steps):
+            meth.element_jacobians(blk)
+            ms += meth.last_timing()[0]
+        ms /= args.steps
+        total_ms += ms
+        ndm = nd * neq
+        out_bytes += ne * 8 * (ndm * ndm + ndm + 1) + 4 * b.sol_conn.size + 4 * b.geo_conn.size
+        print(f"[bench] jacobians block {blk}: {ne} cells, {ms * 1e3:.1f} us", file=sys.stderr, flush=True)
+    in_bytes = 8 * m.coords.size + 8 * neq * m.nb_nodes
+    alg = out_bytes + in_bytes
+    achieved = alg / (total_ms * 1e-3) / 1e9
+    print(json.dumps({
+        "metric": "element Jacobian evals/s", "value": round(m.nb_cells / (total_ms * 1e-3), 1),
+        "unit": "elements/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(total_ms, 4), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic",
+        "config": {"workload": wl.name, "scheme": ["PGLDA", "PGN", "PGB", "PSI"][wl.scheme],
+                   "elements": m.nb_cells, "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": "rd_jacobian_kernel",
+                     "algorithmic_bytes": int(alg), "avg_launch_us": round(total_ms * 1e3, 2)},
+    }), flush=True)
 
 
 if __name__ == "__main__":

@@ -286,7 +286,8 @@ int pdrd_method_info(pdrd_method_t h, int64_t *info6);
 /* Timing of the last pdrd_assemble_rhs (cell loop / nodal pass) or, with
  * PDRD_OPT_TIMING, the per-iteration means of the last pdrd_iterate, measured
  * with HIP events on the method's stream (ms). Without PDRD_OPT_TIMING,
- * pdrd_iterate reports its whole duration as cell_ms and 0 as node_ms. */
+ * pdrd_iterate reports its whole duration as cell_ms and 0 as node_ms. After
+ * pdrd_element_jacobians, cell_ms is the Jacobian kernel's duration. */
 int pdrd_last_timing(pdrd_method_t h, double *cell_ms, double *node_ms);
 
 /* Batches (no counterpart in the reference, whose driver loops run one

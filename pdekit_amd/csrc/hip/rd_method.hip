@@ -1638,7 +1638,9 @@ int pdrd_element_jacobians(pdrd_method_t h, int block, double *phi, double *jac,
   a.ne = b.ne;
   a.pp.p0 = h->params[0];
   a.pp.p1 = h->params[1];
+  if (e == hipSuccess) e = hipEventRecord(h->ev[0], h->stream);
   if (e == hipSuccess) e = b.ks.jacobian(a, grid_for(b.ne, 64), h->stream);
+  if (e == hipSuccess) e = hipEventRecord(h->ev[1], h->stream);
   if (e == hipSuccess && phi)
     e = hipMemcpyAsync(phi, d_phi, b.ne * ndm * sizeof(double), hipMemcpyDeviceToHost, h->stream);
   if (e == hipSuccess && jac)
@@ -1646,8 +1648,12 @@ int pdrd_element_jacobians(pdrd_method_t h, int block, double *phi, double *jac,
   if (e == hipSuccess && wave)
     e = hipMemcpyAsync(wave, d_wave, b.ne * sizeof(double), hipMemcpyDeviceToHost, h->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+  float ms = 0.f;
+  if (e == hipSuccess) e = hipEventElapsedTime(&ms, h->ev[0], h->ev[1]);
   release();
   if (e != hipSuccess) return hip_fail(e, "element Jacobians");
+  h->last_cell_ms = ms;  // the Jacobian kernel alone
+  h->last_node_ms = 0.0;
   return PDRD_OK;
   PDRD_CATCH
 }
