@@ -232,6 +232,17 @@ int pdrd_weak_bc_info(pdrd_method_t h, int k, int64_t *info3);
  * cells (not with PDRD_OPT_PRECISION = 1) and outside batches. */
 int pdrd_set_blending_coeff(pdrd_method_t h, const double *theta);
 int pdrd_set_artificial_viscosity(pdrd_method_t h, const double *nu);
+/* set_vec_function(SolverVecFn::sources, f) (PGRDMethodExplicit.hpp:350-356):
+ * a nodal source field [N][m] (host, copied; NULL switches it off). With it
+ * the cells compute the reaction residuals (compute_adv_reaction_residuals,
+ * PGRDMExplicitCellWorker.hpp:535-544): the residual at a quadrature point
+ * is div F - s_q with s_q the source interpolated like the solution
+ * (PGLDA.hpp:553-741, PGN.hpp:470-660, PGB.hpp:825-1050). Runs with the FP64
+ * patch assembly outside batches; a blending coefficient or artificial
+ * viscosity alongside it is refused (the reference's reaction LDA would drop
+ * the viscosity). pdrd_element_jacobians ignores it, as
+ * compute_adv_res_and_jacobian does. */
+int pdrd_set_sources(pdrd_method_t h, const double *sources);
 /* Providers, on the device from the current solution, each storing its
  * result as the method's input (and switching it on):
  * RDBlendingCoeff::calculate with "LF_Blend" (LFBlendingCoeffImpl.hpp:79-136

@@ -47,6 +47,7 @@ class OrcProblem(C.Structure):
         ("params", C.c_double * 4), ("blocks", C.POINTER(OrcBlock)),
         ("nweak", C.c_int), ("weak", C.POINTER(OrcWeak)),
         ("blend", C.POINTER(C.c_double)), ("art_visc", C.POINTER(C.c_double)),
+        ("sources", C.POINTER(C.c_double)),
     ]
 
 
@@ -151,6 +152,12 @@ class Problem:
         switches it off."""
         self._nu = None if nu is None else np.ascontiguousarray(nu, dtype=np.float64).reshape(self.nn)
         self.p.art_visc = None if nu is None else _p(self._nu)
+
+    def set_sources(self, src):
+        """Nodal source field [N][m] of the reaction residuals
+        (set_vec_function(sources)); None switches it off."""
+        self._src = None if src is None else np.ascontiguousarray(src, dtype=np.float64).reshape(self.nn, self.m)
+        self.p.sources = None if src is None else _p(self._src)
 
     def _p1(self):
         ws = [p1_weights(b.shape, self._tables[b.shape].sol_nodes) for b in self._mesh.blocks]

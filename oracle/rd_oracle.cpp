@@ -340,7 +340,9 @@ bool cell(const orc_problem *p, const orc_block &b, uint32_t c, const double *u,
   // cell (PGRDMExplicitCellWorker.hpp:507-533)
   double theta_n[MAXN], nu_n[MAXN];
   const bool ext_theta = p->blend && p->scheme == ORC_B;
-  const bool visc = p->art_visc && p->scheme == ORC_LDA;
+  // (the reaction residuals of a method with a source field apply no
+  // artificial viscosity, PGLDA.hpp:553-741)
+  const bool visc = p->art_visc && p->scheme == ORC_LDA && !(p->sources && !jac);
   for (int n = 0; n < nd; ++n) {
     const uint32_t v = b.sol_conn[static_cast<size_t>(c) * nd + n];
     for (int e = 0; e < m; ++e) U[n][e] = u[static_cast<size_t>(v) * m + e];
@@ -420,6 +422,16 @@ bool cell(const orc_problem *p, const orc_block &b, uint32_t c, const double *u,
         divF[q][e] += dphys;  // res_at_point = sum_dim grad_F[dim](q)
       }
   }
+  // source term of the reaction residuals: res_at_point -= s_q, s_q = V *
+  // source at the cell's nodes (PGLDA.hpp:712-713, PGN.hpp:621-622, PGB.hpp:987-988)
+  if (p->sources && !jac)
+    for (int q = 0; q < nq; ++q)
+      for (int e = 0; e < m; ++e) {
+        double s = 0.0;
+        for (int n = 0; n < nd; ++n)
+          s += b.V[q * nd + n] * p->sources[static_cast<size_t>(b.sol_conn[static_cast<size_t>(c) * nd + n]) * m + e];
+        divF[q][e] -= s;
+      }
 
   // Scheme (PGLDA.hpp:156-311, PGN.hpp:141-290, PGB.hpp:314-498)
   for (int n = 0; n < nd; ++n)

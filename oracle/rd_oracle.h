@@ -78,6 +78,13 @@ typedef struct orc_problem
    * artificial viscosity [N] of PGLDA's shock-capturing term, or NULL */
   const double *blend;
   const double *art_visc;
+  /* set_vec_function(SolverVecFn::sources, f): nodal source field [N][m] or
+   * NULL. With it the cell worker calls compute_adv_reaction_residuals
+   * (PGRDMExplicitCellWorker.hpp:535-544), whose residual at a quadrature
+   * point is div F - s_q with s_q = V . source (PGLDA.hpp:553-741, PGN.hpp:
+   * 470-660, PGB.hpp:825-1050) and which applies no artificial viscosity.
+   * The implicit forms (orc_cell_jacobians) take no source. */
+  const double *sources;
 } orc_problem;
 
 int orc_neq(int physics);

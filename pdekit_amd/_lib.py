@@ -44,7 +44,7 @@ EXPORTED = (
     "pdrd_method_info", "pdrd_last_timing", "pdrd_last_error", "pdrd_version", "pdrd_get_option",
     "pdrd_batch_create", "pdrd_batch_destroy", "pdrd_batch_set_graphs", "pdrd_batch_iterate", "pdrd_add_weak_bc", "pdrd_weak_bc_info",
     "pdrd_mesh_read_gmsh", "pdrd_mesh_write_gmsh", "pdrd_gmsh_append_node_data", "pdrd_gmsh_read_node_data",
-    "pdrd_set_blending_coeff", "pdrd_set_artificial_viscosity", "pdrd_compute_blending_coeff",
+    "pdrd_set_blending_coeff", "pdrd_set_artificial_viscosity", "pdrd_set_sources", "pdrd_compute_blending_coeff",
     "pdrd_compute_artificial_viscosity", "pdrd_element_jacobians",
 )
 
@@ -126,6 +126,7 @@ def lib() -> C.CDLL:
         "pdrd_batch_iterate": [P, I, C.POINTER(P), PD, I, I, PD],
         "pdrd_set_blending_coeff": [P, PD],
         "pdrd_set_artificial_viscosity": [P, PD],
+        "pdrd_set_sources": [P, PD],
         "pdrd_compute_blending_coeff": [P, I],
         "pdrd_compute_artificial_viscosity": [P, D, D],
         "pdrd_element_jacobians": [P, I, PD, PD, PD],

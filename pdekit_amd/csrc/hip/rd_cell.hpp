@@ -196,6 +196,16 @@ struct NodalNone
   __device__ __forceinline__ double operator()(int) const { return 0.0; }
 };
 
+// The nodal source field of the reaction residuals (PREC 3) from the
+// patch's LDS rows: source(n, k) of the cell's node n.
+template <int ND, int M>
+struct SourceLds
+{
+  const double *s;       // [local node][M]
+  const int (&row)[ND];  // local node of each cell node
+  __device__ __forceinline__ double operator()(int n, int k) const { return s[row[n] * M + k]; }
+};
+
 template <int ND>
 struct NodalLds
 {
@@ -226,6 +236,10 @@ struct NodalGlobal
 // nodal input `nodal` -- for the B scheme the external blending coefficient
 // (set_blending_coeff, PGB.hpp:141-148, 467-477), for the LDA scheme the
 // artificial viscosity of the shock-capturing term (PGLDA.hpp:245-256).
+// PREC 3: FP64 with a nodal source field (`nodal` gives source(n, k)): the
+// reaction residuals compute_adv_reaction_residuals, whose residual at a
+// quadrature point is div F - s_q, s_q = V . source (PGLDA.hpp:553-741,
+// PGN.hpp:470-660, PGB.hpp:825-1050).
 template <int D, int NG, int ND, int NF, int NQ, bool AFFINE, int PH, int SC, int PREC, class UA,
           class NA = NodalNone>
 __device__ __forceinline__ void cell_residual(const double *__restrict__ tab, const double (&X)[NG][D],
@@ -387,6 +401,17 @@ __device__ __forceinline__ void cell_residual(const double *__restrict__ tab, co
         }
       }
     }
+  }
+  if constexpr (PREC == 3) {
+#pragma unroll
+    for (int q = 0; q < NQ; ++q)
+#pragma unroll
+      for (int k = 0; k < M; ++k) {
+        double sq = 0.0;
+#pragma unroll
+        for (int n = 0; n < ND; ++n) sq += tab[O::V + q * ND + n] * nodal(n, k);
+        divr[q][k] -= sq;
+      }
   }
 #pragma unroll
   for (int q = 0; q < NQ; ++q)
@@ -963,11 +988,12 @@ __host__ __device__ constexpr size_t patch_lds_doubles(int maxl, int P)
 }
 
 // With a nodal input (PREC 2) its values follow the work region, one per
-// local node.
+// local node; with a source field (PREC 3), M per local node.
 template <int NQ, int ND, int M, int PREC>
 __host__ __device__ constexpr size_t patch_lds_doubles_prec(int maxl, int P)
 {
-  return patch_lds_doubles<NQ, ND, M>(maxl, P) + (PREC == 2 ? static_cast<size_t>(maxl) : 0);
+  return patch_lds_doubles<NQ, ND, M>(maxl, P) +
+         (PREC == 2 ? static_cast<size_t>(maxl) : (PREC == 3 ? static_cast<size_t>(maxl) * M : 0));
 }
 
 // Finishes node g from its sums acc[M + 1] (residual sum, wave-speed sum);
@@ -1100,7 +1126,7 @@ __device__ __forceinline__ void patch_body(const PatchArgs &a, uint32_t pl, doub
 #pragma unroll
     for (int n = 0; n < ND; ++n) lc[n] = a.lconn[static_cast<size_t>(n) * a.ne + e];
   }
-  double *nodal_l = work + patch_work_doubles<NQ, ND, M>(PATCH);  // PREC 2 only
+  double *nodal_l = work + patch_work_doubles<NQ, ND, M>(PATCH);  // PREC 2 and 3 only
   // (no global store before the cell work: the table loads stay scalar)
   const int ninner = static_cast<int>(a.pn_inner[p]);
   for (int l = tid; l < nl; l += PATCH) {
@@ -1124,6 +1150,9 @@ __device__ __forceinline__ void patch_body(const PatchArgs &a, uint32_t pl, doub
     for (int k = 0; k < M; ++k) ul[l * M + k] = un[k];
     coff[l] = a.pn_coff[s0 + l];
     if constexpr (PREC == 2) nodal_l[l] = a.nodal[g];
+    if constexpr (PREC == 3)
+#pragma unroll
+      for (int k = 0; k < M; ++k) nodal_l[l * M + k] = a.nodal[static_cast<size_t>(g) * M + k];
   }
   __syncthreads();
 
@@ -1139,7 +1168,13 @@ __device__ __forceinline__ void patch_body(const PatchArgs &a, uint32_t pl, doub
 #pragma unroll
       for (int k = 0; k < M; ++k) U[n][k] = ul[row[n] * M + k];
     }
-    if constexpr (PREC == 2 && M == 1)
+    if constexpr (PREC == 3 && M == 1)
+      cell_residual<D, NG, ND, NF, NQ, AFFINE, PH, SC, PREC>(a.tab, X, U, URegs<ND, M>{U}, a.pp, phi, wave, bad, work + tid,
+                                                       PATCH, SourceLds<ND, M>{nodal_l, row});
+    else if constexpr (PREC == 3)
+      cell_residual<D, NG, ND, NF, NQ, AFFINE, PH, SC, PREC>(a.tab, X, U, ULds<ND, M>{ul, row}, a.pp, phi, wave, bad,
+                                                       work + tid, PATCH, SourceLds<ND, M>{nodal_l, row});
+    else if constexpr (PREC == 2 && M == 1)
       cell_residual<D, NG, ND, NF, NQ, AFFINE, PH, SC, PREC>(a.tab, X, U, URegs<ND, M>{U}, a.pp, phi, wave, bad, work + tid,
                                                        PATCH, NodalLds<ND>{nodal_l, row});
     else if constexpr (PREC == 2)

@@ -31,10 +31,13 @@ struct KernelSet
   // external blending coefficient); null for N and PSI, which take none
   CellLauncher cell_nodal = nullptr;
   PatchLauncher patch_nodal = nullptr;
+  // FP64 patch assembly with a nodal source field (reaction residuals)
+  PatchLauncher patch_src = nullptr;
   // element residuals and Jacobians of the implicit form (null for PSI)
   JacobianLauncher jacobian = nullptr;
   int patch_size = 0;             // largest (default) cells per patch workgroup
   size_t (*patch_lds_bytes)(int maxl, int cells) = nullptr;
+  size_t (*patch_src_lds_bytes)(int maxl, int cells) = nullptr;
   explicit operator bool() const { return cell != nullptr; }
 };
 
@@ -92,6 +95,12 @@ size_t patch_lds_bytes(int maxl, int cells)
   return patch_lds_doubles<NQ, ND, M>(maxl, cells) * sizeof(double);
 }
 
+template <int NQ, int ND, int M>
+size_t patch_src_lds_bytes(int maxl, int cells)
+{
+  return patch_lds_doubles_prec<NQ, ND, M, 3>(maxl, cells) * sizeof(double);
+}
+
 template <int D, int NG, int ND, int NF, int NQ, bool AFF, int PH, int SC>
 KernelSet kernel_set()
 {
@@ -105,9 +114,11 @@ KernelSet kernel_set()
     k.cell_nodal = &launch_cell<D, NG, ND, NF, NQ, AFF, PH, SC, 2>;
     k.patch_nodal = &launch_patch<D, NG, ND, NF, NQ, AFF, PH, SC, 2>;
   }
+  k.patch_src = &launch_patch<D, NG, ND, NF, NQ, AFF, PH, SC, 3>;
   if constexpr (SC != SC_PSI) k.jacobian = &launch_jacobian<D, NG, ND, NF, NQ, AFF, PH, SC>;
   k.patch_size = PatchSize<PhysTraits<PH>::M, ND>::value;
   k.patch_lds_bytes = &patch_lds_bytes<NQ, ND, PhysTraits<PH>::M>;
+  k.patch_src_lds_bytes = &patch_src_lds_bytes<NQ, ND, PhysTraits<PH>::M>;
   return k;
 }
 

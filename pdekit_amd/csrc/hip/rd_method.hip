@@ -102,6 +102,10 @@ struct pdrd_method
   // for their providers
   double *d_blend = nullptr, *d_visc = nullptr, *d_stab_tmp = nullptr;
   bool use_blend = false, use_visc = false;
+  // nodal source field [N][m] (set_vec_function(sources)): the cells then
+  // compute the reaction residuals
+  double *d_src = nullptr;
+  bool use_src = false;
   int patch_cells = 0;  // requested cells per patch (0: each kernel's default)
   int block_threads = 256;
   int timing = 0;
@@ -353,6 +357,7 @@ const double *nodal_input(const pdrd_method *h)
 int launch_cells(pdrd_method *h)
 {
   if (h->precision) return set_error(PDRD_ENOTSUP, "the mixed-precision cells run with the patch assembly only");
+  if (h->use_src) return set_error(PDRD_ENOTSUP, "the source term runs with the patch assembly only");
   pdrd::CellArgs a{};
   a.nodal = nodal_input(h);
   a.coords = h->d_coords;
@@ -442,7 +447,7 @@ pdrd::PatchArgs patch_args(const pdrd_method *h, const Block *b, int update, int
   }
   if (c.record) a.base_out = h->d_base[c.write];
   a.norm_partial = h->d_patch_partial;
-  a.nodal = nodal_input(h);
+  a.nodal = h->use_src ? h->d_src : nodal_input(h);
   a.status = h->d_status;
   a.update = update;
   a.outputs = outputs;
@@ -471,7 +476,8 @@ int launch_patch_blocks(pdrd_method *h, int update, int outputs, int excl, doubl
 {
   for (Block &b : h->blocks) {
     const pdrd::PatchArgs a = patch_args(h, &b, update, outputs, excl, cfl, c);
-    const pdrd::PatchLauncher f = a.nodal ? b.ks.patch_nodal : (h->precision ? b.ks.patch_mixed : b.ks.patch);
+    const pdrd::PatchLauncher f =
+        h->use_src ? b.ks.patch_src : (a.nodal ? b.ks.patch_nodal : (h->precision ? b.ks.patch_mixed : b.ks.patch));
     HIP_CHECK(f(a, b.np, b.cells, h->stream));
   }
   return PDRD_OK;
@@ -736,11 +742,11 @@ void free_device(pdrd_method *h)
   h->d_pn_rows = nullptr;
   (void)hipFree(h->d_corr);
   h->d_corr = nullptr;
-  for (double **q : {&h->d_blend, &h->d_visc, &h->d_stab_tmp}) {
+  for (double **q : {&h->d_blend, &h->d_visc, &h->d_stab_tmp, &h->d_src}) {
     (void)hipFree(*q);
     *q = nullptr;
   }
-  h->use_blend = h->use_visc = false;
+  h->use_blend = h->use_visc = h->use_src = false;
   (void)hipFree(h->d_dir_index);
   h->d_pn_coff = nullptr;
   h->d_slot_acc = nullptr;
@@ -1272,12 +1278,14 @@ int pdrd_set_option(pdrd_method_t h, int option, int value)
         return set_error(PDRD_ENOTSUP, "patch assembly unavailable: " + h->patch_reason);
       if (value != 2 && !h->weak_specs.empty())
         return set_error(PDRD_ENOTSUP, "weak boundary conditions run with the patch assembly only");
+      if (value != 2 && h->use_src) return set_error(PDRD_ENOTSUP, "the source term runs with the patch assembly only");
       h->scatter_mode = value;
       break;
     case PDRD_OPT_PRECISION:
       if (value != 0 && value != 1) return set_error(PDRD_EINVAL, "precision is 0 (FP64) or 1 (mixed)");
       if (value == 1 && nodal_input(h))
         return set_error(PDRD_ENOTSUP, "the blending coefficient / artificial viscosity run with the FP64 cells");
+      if (value == 1 && h->use_src) return set_error(PDRD_ENOTSUP, "the source term runs with the FP64 cells");
       h->precision = value;
       h->since_refresh = 1 << 30;
       break;
@@ -1498,6 +1506,8 @@ extern "C" {
 int pdrd_set_blending_coeff(pdrd_method_t h, const double *theta)
 {
   if (int rc = check_ready(h)) return rc;
+  if (theta && h->use_src)
+    return set_error(PDRD_ENOTSUP, "a blending coefficient / artificial viscosity with a source field is not built");
   if (!theta) {
     if (h->use_blend) drop_graph(h);
     h->use_blend = false;
@@ -1517,6 +1527,8 @@ int pdrd_set_blending_coeff(pdrd_method_t h, const double *theta)
 int pdrd_set_artificial_viscosity(pdrd_method_t h, const double *nu)
 {
   if (int rc = check_ready(h)) return rc;
+  if (nu && h->use_src)
+    return set_error(PDRD_ENOTSUP, "a blending coefficient / artificial viscosity with a source field is not built");
   if (!nu) {
     if (h->use_visc) drop_graph(h);
     h->use_visc = false;
@@ -1533,10 +1545,37 @@ int pdrd_set_artificial_viscosity(pdrd_method_t h, const double *nu)
   return PDRD_OK;
 }
 
+int pdrd_set_sources(pdrd_method_t h, const double *sources)
+{
+  if (int rc = check_ready(h)) return rc;
+  if (!sources) {
+    h->use_src = false;
+    return PDRD_OK;
+  }
+  if (h->scatter_mode != 2) return set_error(PDRD_ENOTSUP, "the source term runs with the patch assembly only");
+  if (h->precision) return set_error(PDRD_ENOTSUP, "the source term runs with the FP64 cells");
+  if (nodal_input(h))
+    return set_error(PDRD_ENOTSUP, "a source field with a blending coefficient / artificial viscosity is not built");
+  for (const Block &b : h->blocks) {
+    if (!b.ks.patch_src) return set_error(PDRD_ENOTSUP, "no source-term kernel for this element type");
+    if (b.ks.patch_src_lds_bytes(b.maxl, b.cells) > 65536)
+      return set_error(PDRD_ENOTSUP, "the source values do not fit the patch LDS: set PDRD_OPT_PATCH_CELLS to 64");
+  }
+  const size_t nm = static_cast<size_t>(h->nn) * h->m;
+  if (!h->d_src)
+    if (int rc = dev_alloc(h, &h->d_src, nm)) return rc;
+  HIP_CHECK(hipMemcpyAsync(h->d_src, sources, nm * sizeof(double), hipMemcpyHostToDevice, h->stream));
+  HIP_CHECK(hipStreamSynchronize(h->stream));
+  h->use_src = true;
+  return PDRD_OK;
+}
+
 int pdrd_compute_blending_coeff(pdrd_method_t h, int kind)
 {
   PDRD_TRY
   if (int rc = check_ready(h)) return rc;
+  if (h->use_src)
+    return set_error(PDRD_ENOTSUP, "a blending coefficient / artificial viscosity with a source field is not built");
   if (kind != PDRD_BLEND_LF) return set_error(PDRD_EINVAL, "unknown blending coefficient");
   if (h->precision && h->scheme == pdrd::SC_B)
     return set_error(PDRD_ENOTSUP, "the blending coefficient runs with the FP64 cells");
@@ -1572,6 +1611,8 @@ int pdrd_compute_artificial_viscosity(pdrd_method_t h, double kappa, double fact
 {
   PDRD_TRY
   if (int rc = check_ready(h)) return rc;
+  if (h->use_src)
+    return set_error(PDRD_ENOTSUP, "a blending coefficient / artificial viscosity with a source field is not built");
   if (!(kappa > 0.0) || !(factor >= 0.0)) return set_error(PDRD_EINVAL, "kappa must be > 0 and factor >= 0");
   if (h->precision && h->scheme == pdrd::SC_LDA)
     return set_error(PDRD_ENOTSUP, "the artificial viscosity runs with the FP64 cells");
@@ -2058,6 +2099,7 @@ int pdrd_batch_iterate(pdrd_batch_t b, int count, const pdrd_method_t *methods, 
     if (nodal_input(h))
       return set_error(PDRD_ENOTSUP, "batched iterations of methods with a blending coefficient / artificial "
                                      "viscosity are not built");
+    if (h->use_src) return set_error(PDRD_ENOTSUP, "batched iterations of methods with a source field are not built");
     for (int j = 0; j < i; ++j)
       if (methods[j] == h) return set_error(PDRD_EINVAL, "a method appears twice in one batch");
   }

@@ -119,9 +119,23 @@ class PGRDMethodExplicit:
         return (n, self.neq) if which in (L.SOLUTION, L.RESIDUAL) else (n,)
 
     def set_vec_function(self, which, values):
+        """'solution', 'residuals', ... (a device field), or 'sources': the
+        nodal source field [N][m] of the reaction residuals (None removes it)."""
+        if which == "sources":
+            self.set_sources(values)
+            return
         w = FIELDS[which] if isinstance(which, str) else which
         a = np.ascontiguousarray(np.broadcast_to(np.asarray(values, dtype=np.float64), self._shape(w)))
         L.check(L.lib().pdrd_set_field(self._h, w, L.ptr(a)), "pdrd_set_field")
+
+    def set_sources(self, src):
+        """set_vec_function(sources, f): nodal source field [N][m]; None
+        switches it off."""
+        if src is None:
+            L.check(L.lib().pdrd_set_sources(self._h, None), "pdrd_set_sources")
+            return
+        a = np.ascontiguousarray(np.broadcast_to(np.asarray(src, dtype=np.float64), (self.mesh.nb_nodes, self.neq)))
+        L.check(L.lib().pdrd_set_sources(self._h, L.ptr(a)), "pdrd_set_sources")
 
     def vec_function(self, which) -> np.ndarray:
         w = FIELDS[which] if isinstance(which, str) else which

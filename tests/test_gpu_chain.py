@@ -180,3 +180,19 @@ def test_calls_resume_after_a_closed_chain():
     a = {f: m.vec_function(f).copy() for f in FIELDS}
     m.destroy()
     _same((a, norm), _run(make, cfl, 8, True))
+
+
+@pytest.mark.parametrize("n", [2, 3, 12])
+def test_chain_on_meshes_of_one_or_few_patches(n):
+    """make_unit_quad(n): 2, 8 and 242 triangles -- one patch without shared
+    nodes, then a few patches."""
+    def make(options):
+        mesh = unit_quad_triag(n)
+        m = PGRDMethodExplicit(L.ADVECTION2D, L.N, params=(0.8, 0.6))
+        m.configure_mesh_data(mesh).initialize_work_data(quad_order=4, **options)
+        bn = np.union1d(mesh.boundaries["Edge0"], mesh.boundaries["Edge3"]).astype(np.uint32)
+        m.add_boundary_condition("StrongDirichlet", "inflow", bn, np.ones((bn.size, 1)))
+        m.set_vec_function("solution", np.zeros((mesh.nb_nodes, 1)))
+        return m
+    _same(_run(make, 0.9, 6, True), _run(make, 0.9, 6, False))
+    _same(_run(make, 0.9, 6, True, {"patch_cells": 64}), _run(make, 0.9, 6, True, {"patch_cells": 64}, chain=0))
