@@ -45,7 +45,17 @@ enum pdrd_point_set { PDRD_EQUIDIST = 0, PDRD_WARPBLEND = 1 };
  * ADVECTION2D = RotationAdvection2D with a constant velocity (params[0..1]),
  * BURGERS2D = Burgers2D, EULER2D = Euler2DCons, EULER3D = Euler3DCons
  * (params[0] = gamma, params[1] = R). */
-enum pdrd_physics { PDRD_ADVECTION2D = 0, PDRD_BURGERS2D = 1, PDRD_EULER2D = 2, PDRD_EULER3D = 3 };
+/* PDRD_ADVECTION2D: RotationAdvection2D with a constant velocity (params ax,
+ * ay; ConstAdvection2D is (0, 1)); PDRD_ROTATION2D: RotationAdvection2D with
+ * AroundOrigin2D, the velocity (y, -x) at every point (RotationAdvection2D.hpp:
+ * 33-45, 201-224; no parameters; not with mixed precision or weak BCs). */
+enum pdrd_physics {
+  PDRD_ADVECTION2D = 0,
+  PDRD_BURGERS2D = 1,
+  PDRD_EULER2D = 2,
+  PDRD_EULER3D = 3,
+  PDRD_ROTATION2D = 4
+};
 /* SchemeTag: PGLDA, PGN, PGB; PSI = N residuals limited with the x+ limiter
  * of LFResidualLimiter.hpp:84-110. */
 enum pdrd_scheme { PDRD_LDA = 0, PDRD_N = 1, PDRD_B = 2, PDRD_PSI = 3 };

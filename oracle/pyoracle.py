@@ -25,6 +25,7 @@ class OrcBlock(C.Structure):
         ("dV", C.POINTER(C.c_double)), ("dVf", C.POINTER(C.c_double)), ("Vsf", C.POINTER(C.c_double)),
         ("Vgf", C.POINTER(C.c_double)), ("vol_w", C.POINTER(C.c_double)), ("vol_dVg", C.POINTER(C.c_double)),
         ("geo_conn", C.POINTER(C.c_uint32)), ("sol_conn", C.POINTER(C.c_uint32)),
+        ("Vg", C.POINTER(C.c_double)),
     ]
 
 
@@ -110,7 +111,7 @@ class Problem:
         for i, blk in enumerate(mesh.blocks):
             t, tv = tables[blk.shape], vol_tables[blk.shape]
             arrs = {k: np.ascontiguousarray(getattr(t, k), dtype=np.float64)
-                    for k in ("w", "dVg", "V", "dV", "dVf", "Vsf", "Vgf")}
+                    for k in ("w", "dVg", "V", "dV", "dVf", "Vsf", "Vgf", "Vg")}
             vw = np.ascontiguousarray(tv.w, dtype=np.float64)
             vdg = np.ascontiguousarray(tv.dVg, dtype=np.float64)
             g = np.ascontiguousarray(blk.geo_conn, dtype=np.uint32)

@@ -52,6 +52,7 @@ int neq_of(int physics)
 {
   switch (physics) {
     case ORC_ADVECTION2D:
+    case ORC_ROTATION2D:
     case ORC_BURGERS2D: return 1;
     case ORC_EULER2D: return 4;
     case ORC_EULER3D: return 5;
@@ -63,14 +64,24 @@ int neq_of(int physics)
 
 // compute_properties (Euler2DCons.hpp:159-218, Euler3DCons.hpp:159-222,
 // Burgers2D.hpp:154-173, RotationAdvection2D.hpp:201-224 with a constant
-// advection velocity)
-void properties(const orc_problem *p, const double *s, Props &pr)
+// advection velocity, or with AroundOrigin2D's (y, -x) at the point x,
+// RotationAdvection2D.hpp:33-45; without a point that is an error)
+void properties(const orc_problem *p, const double *s, Props &pr, const double *x = nullptr)
 {
   switch (p->physics) {
     case ORC_ADVECTION2D:
       pr.u = s[0];
       pr.V[0] = p->params[0];
       pr.V[1] = p->params[1];
+      return;
+    case ORC_ROTATION2D:
+      pr.u = s[0];
+      if (!x) {
+        pr.bad = true;
+        return;
+      }
+      pr.V[0] = x[1];
+      pr.V[1] = -x[0];
       return;
     case ORC_BURGERS2D:
       pr.u = s[0];
@@ -105,6 +116,7 @@ void flux(const orc_problem *p, const double *s, const Props &pr, double F[3][MA
 {
   switch (p->physics) {
     case ORC_ADVECTION2D:
+    case ORC_ROTATION2D:
       F[0][0] = pr.V[0] * s[0];
       F[1][0] = pr.V[1] * s[0];
       return;
@@ -140,7 +152,7 @@ void flux(const orc_problem *p, const double *s, const Props &pr, double F[3][MA
 // Euler3DCons.hpp:331-447; scalars: R = L = 1, D = V.g)
 void eigen_structure(int physics, const Props &pr, const double *g, Mat &R, Mat &L, double *D)
 {
-  if (physics == ORC_ADVECTION2D || physics == ORC_BURGERS2D) {
+  if (physics == ORC_ADVECTION2D || physics == ORC_ROTATION2D || physics == ORC_BURGERS2D) {
     R[0][0] = 1.;
     L[0][0] = 1.;
     D[0] = pr.V[0] * g[0] + pr.V[1] * g[1];
@@ -397,8 +409,12 @@ bool cell(const orc_problem *p, const orc_block &b, uint32_t c, const double *u,
       for (int n = 0; n < nd; ++n) s += b.Vsf[f * nd + n] * U[n][e];
       sf[e] = s;
     }
+    // the flux node's coordinates Gf = Vgf X (FluxSpaceMetric.hpp:562-603)
+    double xf[3] = {0, 0, 0};
+    for (int d = 0; d < D; ++d)
+      for (int g = 0; g < ng; ++g) xf[d] += b.Vgf[f * ng + g] * X[g][d];
     Props pr;
-    properties(p, sf, pr);
+    properties(p, sf, pr, xf);
     if (pr.bad) out.bad = true;
     double F[3][MAXM];
     flux(p, sf, pr, F);
@@ -467,8 +483,13 @@ bool cell(const orc_problem *p, const orc_block &b, uint32_t c, const double *u,
   for (int q = 0; q < nq; ++q) {
     const double wj = b.w[q] * detJ[q];
     elem_volume += wj;
+    // the quadrature point's coordinates Xq = Vg X (GeometryMetric.hpp:366-380)
+    double xq[3] = {0, 0, 0};
+    if (b.Vg)
+      for (int d = 0; d < D; ++d)
+        for (int g = 0; g < ng; ++g) xq[d] += b.Vg[q * ng + g] * X[g][d];
     Props pr;
-    properties(p, uq[q], pr);
+    properties(p, uq[q], pr, b.Vg ? xq : nullptr);
     if (pr.bad) out.bad = true;
     Mat sumKp;
     for (int i = 0; i < m; ++i)

@@ -17,7 +17,9 @@
 extern "C" {
 #endif
 
-enum { ORC_ADVECTION2D = 0, ORC_BURGERS2D = 1, ORC_EULER2D = 2, ORC_EULER3D = 3 };
+/* ORC_ROTATION2D: RotationAdvection2D<AroundOrigin2D>, the advection velocity
+ * (y, -x) at every point (RotationAdvection2D.hpp:33-45, 201-224). */
+enum { ORC_ADVECTION2D = 0, ORC_BURGERS2D = 1, ORC_EULER2D = 2, ORC_EULER3D = 3, ORC_ROTATION2D = 4 };
 enum { ORC_LDA = 0, ORC_N = 1, ORC_B = 2, ORC_PSI = 3 };
 
 /* One block of cells of a single element type. Tables row-major:
@@ -31,6 +33,7 @@ typedef struct orc_block
   const double *w, *dVg, *V, *dV, *dVf, *Vsf, *Vgf;
   const double *vol_w, *vol_dVg; /* [vol_nq], [dim][vol_nq][ng] */
   const uint32_t *geo_conn, *sol_conn;
+  const double *Vg; /* [nq][ng] geometry functions at the quadrature points */
 } orc_block;
 
 /* Weak boundary conditions: WeakWall, WeakSubInlet, WeakSubOutlet,

@@ -24,7 +24,7 @@ STATUS_NAMES = {
 # enumerations (mirror include/pdekit_rd.h)
 TRIAG, QUAD, TETRA = 0, 1, 2
 EQUIDIST, WARPBLEND = 0, 1
-ADVECTION2D, BURGERS2D, EULER2D, EULER3D = 0, 1, 2, 3
+ADVECTION2D, BURGERS2D, EULER2D, EULER3D, ROTATION2D = 0, 1, 2, 3, 4
 LDA, N, B, PSI = 0, 1, 2, 3
 SOLUTION, RESIDUAL, TIME_STEP, WAVE_SPEED, DUAL_VOLUME, BLEND_COEFF, ART_VISC = 0, 1, 2, 3, 4, 5, 6
 BLEND_LF = 0

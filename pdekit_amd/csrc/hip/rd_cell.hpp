@@ -360,7 +360,18 @@ __device__ __forceinline__ void cell_residual(const double *__restrict__ tab, co
       s[k] = t;
     }
     double F[D][M];
-    point_flux<PH, double>(s, pp, F, bad);
+    PhysParams ppf = pp;
+    if constexpr (PH == PH_ROTATION2D) {
+      // the velocity (y, -x) at the flux node's coordinates Vgf X
+      double xf[2] = {0.0, 0.0};
+#pragma unroll
+      for (int g = 0; g < NG; ++g)
+#pragma unroll
+        for (int d = 0; d < 2; ++d) xf[d] += tab[O::Vgf + f * NG + g] * X[g][d];
+      ppf.p0 = xf[1];
+      ppf.p1 = -xf[0];
+    }
+    point_flux<PH, double>(s, ppf, F, bad);
     double G[AFFINE ? D : 1][M];
     if constexpr (AFFINE) {
 #pragma unroll
@@ -491,6 +502,15 @@ __device__ __forceinline__ void cell_residual(const double *__restrict__ tab, co
       if constexpr (PH == PH_ADVECTION2D) {
         vel0 = TK(pp.p0);
         vel1 = TK(pp.p1);
+      } else if constexpr (PH == PH_ROTATION2D) {
+        // the velocity (y, -x) at the quadrature point's coordinates Vg X
+        double xq[2] = {0.0, 0.0};
+#pragma unroll
+        for (int g = 0; g < NG; ++g)
+#pragma unroll
+          for (int d = 0; d < 2; ++d) xq[d] += tab[O::Vg + q * NG + g] * X[g][d];
+        vel0 = TK(xq[1]);
+        vel1 = TK(-xq[0]);
       } else if constexpr (PREC == 1) {
         const float *vq = tabf + O::V + q * ND;
         float t = 0.0f;

@@ -169,7 +169,17 @@ __global__ __launch_bounds__(64) void rd_jacobian_kernel(JacArgs a)
       s[k] = t;
     }
     double F[D][M];
-    point_flux<PH, double>(s, a.pp, F, bad);
+    PhysParams ppf = a.pp;
+    if constexpr (PH == PH_ROTATION2D) {
+      double xf[2] = {0.0, 0.0};
+#pragma unroll
+      for (int g = 0; g < NG; ++g)
+#pragma unroll
+        for (int d = 0; d < 2; ++d) xf[d] += tab[O::Vgf + f * NG + g] * X[g][d];
+      ppf.p0 = xf[1];
+      ppf.p1 = -xf[0];
+    }
+    point_flux<PH, double>(s, ppf, F, bad);
 #pragma unroll
     for (int q = 0; q < NQ; ++q) {
       const int jq = AFFINE ? 0 : q;
@@ -249,8 +259,17 @@ __global__ __launch_bounds__(64) void rd_jacobian_kernel(JacArgs a)
     }
     // K+ and K / K- of every node
     if constexpr (M == 1) {
-      const double ax = (PH == PH_BURGERS2D) ? uq[0] : a.pp.p0;
-      const double ay = (PH == PH_BURGERS2D) ? 1.0 : a.pp.p1;
+      double ax = (PH == PH_BURGERS2D) ? uq[0] : a.pp.p0;
+      double ay = (PH == PH_BURGERS2D) ? 1.0 : a.pp.p1;
+      if constexpr (PH == PH_ROTATION2D) {
+        double xq[2] = {0.0, 0.0};
+#pragma unroll
+        for (int g = 0; g < NG; ++g)
+#pragma unroll
+          for (int d = 0; d < 2; ++d) xq[d] += tab[O::Vg + q * NG + g] * X[g][d];
+        ax = xq[1];
+        ay = -xq[0];
+      }
 #pragma unroll
       for (int n = 0; n < ND; ++n) {
         const double k = ax * G[n][0] + ay * G[n][1];

@@ -108,8 +108,10 @@ KernelSet kernel_set()
   k.cell = &launch_cell<D, NG, ND, NF, NQ, AFF, PH, SC>;
   k.patch = &launch_patch<D, NG, ND, NF, NQ, AFF, PH, SC, 0>;
   k.patch_batch = &launch_patch_batch<D, NG, ND, NF, NQ, AFF, PH, SC, 0>;
-  k.patch_mixed = &launch_patch<D, NG, ND, NF, NQ, AFF, PH, SC, 1>;
-  k.patch_batch_mixed = &launch_patch_batch<D, NG, ND, NF, NQ, AFF, PH, SC, 1>;
+  if constexpr (PH != PH_ROTATION2D) {  // (no mixed form of the rotating velocity)
+    k.patch_mixed = &launch_patch<D, NG, ND, NF, NQ, AFF, PH, SC, 1>;
+    k.patch_batch_mixed = &launch_patch_batch<D, NG, ND, NF, NQ, AFF, PH, SC, 1>;
+  }
   if constexpr (SC == SC_LDA || SC == SC_B) {
     k.cell_nodal = &launch_cell<D, NG, ND, NF, NQ, AFF, PH, SC, 2>;
     k.patch_nodal = &launch_patch<D, NG, ND, NF, NQ, AFF, PH, SC, 2>;
@@ -125,6 +127,7 @@ KernelSet kernel_set()
 // Lookup functions, one per compilation unit of instantiations.
 KernelSet find_cell_scalar_adv(const CellKey &k);
 KernelSet find_cell_scalar_burgers(const CellKey &k);
+KernelSet find_cell_scalar_rot(const CellKey &k);
 KernelSet find_cell_euler2d(const CellKey &k);
 KernelSet find_cell_euler3d(const CellKey &k);
 
@@ -133,6 +136,7 @@ inline KernelSet find_cell(const CellKey &k)
   switch (k.physics) {
     case PH_ADVECTION2D: return find_cell_scalar_adv(k);
     case PH_BURGERS2D: return find_cell_scalar_burgers(k);
+    case PH_ROTATION2D: return find_cell_scalar_rot(k);
     case PH_EULER2D: return find_cell_euler2d(k);
     case PH_EULER3D: return find_cell_euler3d(k);
   }

@@ -905,7 +905,7 @@ int pdrd_create(int dim, uint32_t nb_vertices, const double *vertex_coords, uint
   PDRD_TRY
   if (!out || !vertex_coords) return set_error(PDRD_EINVAL, "null argument");
   if (dim != 2 && dim != 3) return set_error(PDRD_EINVAL, "dim must be 2 or 3");
-  if (physics < 0 || physics > 3) return set_error(PDRD_EINVAL, "unknown physics");
+  if (physics < 0 || physics > 4) return set_error(PDRD_EINVAL, "unknown physics");
   if (scheme < 0 || scheme > 3) return set_error(PDRD_EINVAL, "unknown scheme");
   const int pdim = (physics == PDRD_EULER3D) ? 3 : 2;
   if (pdim != dim) return set_error(PDRD_EINVAL, "physics dimension does not match the mesh");
@@ -1286,6 +1286,8 @@ int pdrd_set_option(pdrd_method_t h, int option, int value)
       if (value == 1 && nodal_input(h))
         return set_error(PDRD_ENOTSUP, "the blending coefficient / artificial viscosity run with the FP64 cells");
       if (value == 1 && h->use_src) return set_error(PDRD_ENOTSUP, "the source term runs with the FP64 cells");
+      if (value == 1 && h->physics == PDRD_ROTATION2D)
+        return set_error(PDRD_ENOTSUP, "the rotating advection runs with the FP64 cells");
       h->precision = value;
       h->since_refresh = 1 << 30;
       break;
@@ -1381,6 +1383,8 @@ int pdrd_add_weak_bc(pdrd_method_t h, int type, uint32_t n, const uint32_t *node
   if (!h || (n && !nodes)) return set_error(PDRD_EINVAL, "null argument");
   if (h->ready) return set_error(PDRD_EINVAL, "weak boundary conditions are added before pdrd_initialize");
   if (type < PDRD_WEAK_WALL || type > PDRD_WEAK_DIRICHLET) return set_error(PDRD_EINVAL, "unknown weak BC type");
+  if (h->physics == pdrd::PH_ROTATION2D)
+    return set_error(PDRD_ENOTSUP, "weak boundary conditions of the rotating advection are not built");
   if (h->physics != pdrd::PH_EULER2D && h->physics != pdrd::PH_EULER3D && type != PDRD_WEAK_DIRICHLET)
     return set_error(PDRD_ENOTSUP, "only WeakDirichlet applies to scalar physics");
   // parameters: WeakSubInlet T_tot, p_tot; WeakSubOutlet p; WeakFarfield and

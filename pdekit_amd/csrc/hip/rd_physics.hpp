@@ -17,18 +17,21 @@
 
 namespace pdrd {
 
-enum { PH_ADVECTION2D = 0, PH_BURGERS2D = 1, PH_EULER2D = 2, PH_EULER3D = 3 };
+// PH_ROTATION2D: advection with the velocity (y, -x) of the point (the
+// kernels pass it in the PhysParams of each point).
+enum { PH_ADVECTION2D = 0, PH_BURGERS2D = 1, PH_EULER2D = 2, PH_EULER3D = 3, PH_ROTATION2D = 4 };
 enum { SC_LDA = 0, SC_N = 1, SC_B = 2, SC_PSI = 3 };
 
 template <int PH> struct PhysTraits;
 template <> struct PhysTraits<PH_ADVECTION2D> { static constexpr int M = 1, DIM = 2; };
+template <> struct PhysTraits<PH_ROTATION2D> { static constexpr int M = 1, DIM = 2; };
 template <> struct PhysTraits<PH_BURGERS2D> { static constexpr int M = 1, DIM = 2; };
 template <> struct PhysTraits<PH_EULER2D> { static constexpr int M = 4, DIM = 2; };
 template <> struct PhysTraits<PH_EULER3D> { static constexpr int M = 5, DIM = 3; };
 
 struct PhysParams
 {
-  double p0, p1;  // advection velocity (ax, ay) or (gamma, R)
+  double p0, p1;  // advection velocity (ax, ay) -- for PH_ROTATION2D the point's -- or (gamma, R)
 };
 
 // Thermodynamic state at a point (Euler). T is the arithmetic type of the
@@ -67,7 +70,7 @@ __device__ __forceinline__ EulerState<T, D> euler_state(const T *s, T gamma)
 template <int PH, typename T>
 __device__ __forceinline__ void point_flux(const T *s, const PhysParams &pp, T F[][PhysTraits<PH>::M], bool &bad)
 {
-  if constexpr (PH == PH_ADVECTION2D) {
+  if constexpr (PH == PH_ADVECTION2D || PH == PH_ROTATION2D) {
     F[0][0] = T(pp.p0) * s[0];
     F[1][0] = T(pp.p1) * s[0];
   } else if constexpr (PH == PH_BURGERS2D) {
@@ -132,7 +135,7 @@ template <int PH, typename T>
 __device__ __forceinline__ void flux_delta(const DeltaBase<PH, T> &b, const T *ds, const PhysParams &pp,
                                            T dF[][PhysTraits<PH>::M], bool &bad)
 {
-  if constexpr (PH == PH_ADVECTION2D) {
+  if constexpr (PH == PH_ADVECTION2D || PH == PH_ROTATION2D) {
     dF[0][0] = T(pp.p0) * ds[0];
     dF[1][0] = T(pp.p1) * ds[0];
   } else if constexpr (PH == PH_BURGERS2D) {

@@ -19,7 +19,7 @@ from .mesh import Mesh
 PHYSICS = {"Advection2D": L.ADVECTION2D, "Burgers2D": L.BURGERS2D, "Euler2DCons": L.EULER2D,
            "Euler3DCons": L.EULER3D}
 SCHEMES = {"PGLDA": L.LDA, "PGN": L.N, "PGB": L.B, "PSI": L.PSI}
-NEQ = {L.ADVECTION2D: 1, L.BURGERS2D: 1, L.EULER2D: 4, L.EULER3D: 5}
+NEQ = {L.ADVECTION2D: 1, L.BURGERS2D: 1, L.EULER2D: 4, L.EULER3D: 5, L.ROTATION2D: 1}
 FIELDS = {"solution": L.SOLUTION, "residuals": L.RESIDUAL, "time_step": L.TIME_STEP,
           "wave_speed": L.WAVE_SPEED, "dual_volume": L.DUAL_VOLUME, "blending_coeff": L.BLEND_COEFF,
           "artificial_viscosity": L.ART_VISC}
