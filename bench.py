@@ -310,6 +310,22 @@ def main():
         "sweep_hbm_frac": round(sweep_bytes / (t / args.steps) / 1e9 / HBM_PEAK_GBS, 4),
         "nodal_pass_us": round(node_ms * 1e3, 2),  # shared-node fix-up with the patch assembly
     }
+    # a caller that hands over host buffers every step (set the state, one
+    # sweep, read the state back through the C ABI): the PCIe-inclusive rate,
+    # reported beside the resident one, never as `value`
+    meth.set_option(L.OPT_TIMING, 0)
+    u_host = meth.vec_function("solution")
+    n_rt = 10
+    meth.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(n_rt):
+        meth.set_vec_function("solution", u_host)
+        meth.iterate(wl.cfl, 1)
+        u_host = meth.vec_function("solution")
+    t_rt = time.perf_counter() - t0
+    out["host_roundtrip"] = {"value": round(E * n_rt / t_rt, 1), "unit": "elements/s",
+                             "ms_per_step": round(t_rt / n_rt * 1e3, 4), "steps": n_rt,
+                             "bytes_per_step": int(2 * 8 * u_host.size)}
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(wl, args.cpu_seconds)
     print(json.dumps(out), flush=True)
