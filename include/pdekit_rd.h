@@ -230,6 +230,15 @@ int pdrd_add_weak_bc(pdrd_method_t h, int type, uint32_t n, const uint32_t *node
 /* After pdrd_initialize: facets, nodes per facet, facet quadrature points of
  * weak BC k (in the order added). */
 int pdrd_weak_bc_info(pdrd_method_t h, int k, int64_t *info3);
+/* WeakDirichlet with an expression of the point (set_expression,
+ * WeakDirichletBC.hpp:149-154, 216-229): the coordinates of weak BC k's
+ * facet points xq[nf][nq][dim] (after pdrd_initialize; nf, nq from
+ * pdrd_weak_bc_info), and the boundary state at each of them,
+ * state[nf][nq][m], which then replaces the constant one (NULL restores it).
+ * The caller evaluates its expression at the points; the reference's
+ * expressions may also read the state, which a precomputed table cannot. */
+int pdrd_weak_bc_points(pdrd_method_t h, int k, double *xq);
+int pdrd_set_weak_bc_state(pdrd_method_t h, int k, const double *state);
 
 /* Nodal inputs of the cell loop (PGRDMExplicitCellWorker.hpp:307-330, 501-533).
  * set_blending_coeff(f): PGB takes theta_i = f[node i] instead of its

@@ -38,6 +38,7 @@ class OrcWeak(C.Structure):
         ("nfn", C.c_int), ("nfv", C.c_int), ("nq", C.c_int),
         ("nodes", C.POINTER(C.c_uint32)), ("verts", C.POINTER(C.c_uint32)), ("inside", C.POINTER(C.c_double)),
         ("w", C.POINTER(C.c_double)), ("V", C.POINTER(C.c_double)),
+        ("qstate", C.POINTER(C.c_double)),
     ]
 
 
@@ -196,9 +197,10 @@ class Problem:
         assert rc == 0
         return out
 
-    def add_weak(self, bc_type, params, order, nodes, verts, inside, w, V):
+    def add_weak(self, bc_type, params, order, nodes, verts, inside, w, V, qstate=None):
         """A weak BC over facets: nodes [nf][nfn], verts [nf][nfv], inside
-        [nf][dim] (a point of each facet's cell), facet rule w[nq], V[nq][nfn]."""
+        [nf][dim] (a point of each facet's cell), facet rule w[nq], V[nq][nfn];
+        qstate [nf][nq][m]: WeakDirichlet's state at every facet point."""
         arrs = [np.ascontiguousarray(nodes, dtype=np.uint32), np.ascontiguousarray(verts, dtype=np.uint32),
                 np.ascontiguousarray(inside, dtype=np.float64), np.ascontiguousarray(w, dtype=np.float64),
                 np.ascontiguousarray(V, dtype=np.float64)]
@@ -211,6 +213,10 @@ class Problem:
         wk.nfv, wk.nq = arrs[1].shape[1], arrs[3].size
         wk.nodes, wk.verts = _p(arrs[0], C.c_uint32), _p(arrs[1], C.c_uint32)
         wk.inside, wk.w, wk.V = _p(arrs[2]), _p(arrs[3]), _p(arrs[4])
+        if qstate is not None:
+            qs = np.ascontiguousarray(qstate, dtype=np.float64)
+            self._keep.append(qs)
+            wk.qstate = _p(qs)
         self._weak.append(wk)
         arr = (OrcWeak * len(self._weak))(*self._weak)
         self._keep.append(arr)

@@ -935,7 +935,9 @@ int apply_weak(const orc_problem *p, const orc_weak &bc, const double *u, const 
         for (int e = 0; e < m; ++e) dF[e] = Gn[e] - Fn[e];
       } else {
         double g[MAXM];
-        if (bc.type == ORC_WEAK_FARFIELD || bc.type == ORC_WEAK_DIRICHLET) {
+        if (bc.type == ORC_WEAK_DIRICHLET && bc.qstate) {
+          for (int e = 0; e < m; ++e) g[e] = bc.qstate[(static_cast<size_t>(f) * bc.nq + q) * m + e];
+        } else if (bc.type == ORC_WEAK_FARFIELD || bc.type == ORC_WEAK_DIRICHLET) {
           for (int e = 0; e < m; ++e) g[e] = bc.params[e];
         } else if (bc.type == ORC_WEAK_SYMMETRY) {
           double vn = 0.0;

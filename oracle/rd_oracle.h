@@ -65,6 +65,10 @@ typedef struct orc_weak
   const uint32_t *verts; /* [nfacet][nfv] vertices */
   const double *inside;  /* [nfacet][dim] */
   const double *w, *V;
+  /* WeakDirichlet: the boundary state at every facet point [nfacet][nq][m]
+   * (an expression of the point, WeakDirichletBC.hpp:216-229), or NULL for
+   * the constant params */
+  const double *qstate;
 } orc_weak;
 
 typedef struct orc_problem

@@ -43,6 +43,7 @@ EXPORTED = (
     "pdrd_compute_residual_norm", "pdrd_residual_norm_L2", "pdrd_iterate", "pdrd_check_status",
     "pdrd_method_info", "pdrd_last_timing", "pdrd_last_error", "pdrd_version", "pdrd_get_option",
     "pdrd_batch_create", "pdrd_batch_destroy", "pdrd_batch_set_graphs", "pdrd_batch_iterate", "pdrd_add_weak_bc", "pdrd_weak_bc_info",
+    "pdrd_weak_bc_points", "pdrd_set_weak_bc_state",
     "pdrd_mesh_read_gmsh", "pdrd_mesh_write_gmsh", "pdrd_gmsh_append_node_data", "pdrd_gmsh_read_node_data",
     "pdrd_set_blending_coeff", "pdrd_set_artificial_viscosity", "pdrd_set_sources", "pdrd_compute_blending_coeff",
     "pdrd_compute_artificial_viscosity", "pdrd_element_jacobians",
@@ -120,6 +121,8 @@ def lib() -> C.CDLL:
         "pdrd_get_option": [P, I, PI],
         "pdrd_add_weak_bc": [P, I, U32, PU32, PD, I, I],
         "pdrd_weak_bc_info": [P, I, PI64],
+        "pdrd_weak_bc_points": [P, I, PD],
+        "pdrd_set_weak_bc_state": [P, I, PD],
         "pdrd_batch_create": [C.POINTER(P)],
         "pdrd_batch_destroy": [P],
         "pdrd_batch_set_graphs": [P, C.c_int],

@@ -70,6 +70,10 @@ struct WeakDev
   double relax = 1.0, params[5] = {0, 0, 0, 0, 0};
   uint32_t *d_nodes = nullptr, *d_rows = nullptr;
   double *d_normal = nullptr, *d_jac = nullptr, *d_w = nullptr, *d_V = nullptr;
+  // WeakDirichlet state at every facet point (set by the caller from an
+  // expression of the point), or null; the facet points' coordinates
+  double *d_qstate = nullptr;
+  std::vector<double> qpoints;  // [nf][nq][dim]
 };
 
 }  // namespace
@@ -506,6 +510,7 @@ int launch_weak(pdrd_method *h, hipStream_t s, int buffer = 0)
     a.nq = w.nq;
     a.type = w.type;
     a.relax = w.relax;
+    a.qstate = w.d_qstate;
     for (int i = 0; i < 5; ++i) a.params[i] = w.params[i];
     a.pp.p0 = h->params[0];
     a.pp.p1 = h->params[1];
@@ -729,6 +734,7 @@ void free_device(pdrd_method *h)
     (void)hipFree(w.d_jac);
     (void)hipFree(w.d_w);
     (void)hipFree(w.d_V);
+    (void)hipFree(w.d_qstate);
   }
   h->weak.clear();
   (void)hipFree(h->d_pn_coff);
@@ -1214,6 +1220,21 @@ int pdrd_initialize(pdrd_method_t h)
               return set_error(PDRD_EHIP, "hipMemcpy of the boundary facets");
             return PDRD_OK;
           };
+          // coordinates of the facet points: the vertices' P1 / linear functions
+          // at the same rule (straight facets)
+          {
+            std::vector<double> w1, V1;
+            if (wd.nf) pdrd::facet_tables(h->dim, fs.nfv, ws.quad_order + 1, w1, V1);
+            wd.qpoints.assign(static_cast<size_t>(wd.nf) * wd.nq * h->dim, 0.0);
+            for (uint32_t f = 0; f < wd.nf; ++f)
+              for (int q = 0; q < wd.nq; ++q)
+                for (int v = 0; v < fs.nfv; ++v) {
+                  const uint32_t vx = fs.verts[static_cast<size_t>(f) * fs.nfv + v];
+                  for (int d = 0; d < h->dim; ++d)
+                    wd.qpoints[(static_cast<size_t>(f) * wd.nq + q) * h->dim + d] +=
+                        V1[static_cast<size_t>(q) * fs.nfv + v] * h->coords[static_cast<size_t>(vx) * h->dim + d];
+                }
+          }
           h->weak.push_back(wd);
           WeakDev &wr = h->weak.back();
           if ((rc = up32(&wr.d_nodes, fs.nodes)) || (rc = up32(&wr.d_rows, rows)) ||
@@ -1417,6 +1438,36 @@ int pdrd_weak_bc_info(pdrd_method_t h, int k, int64_t *info3)
   info3[0] = h->weak[k].nf;
   info3[1] = h->weak[k].nfn;
   info3[2] = h->weak[k].nq;
+  return PDRD_OK;
+}
+
+int pdrd_weak_bc_points(pdrd_method_t h, int k, double *xq)
+{
+  if (!h || !xq) return set_error(PDRD_EINVAL, "null argument");
+  if (!h->ready) return set_error(PDRD_EINVAL, "not initialized");
+  if (k < 0 || k >= static_cast<int>(h->weak.size())) return set_error(PDRD_EINVAL, "no such weak BC");
+  const std::vector<double> &p = h->weak[k].qpoints;
+  std::copy(p.begin(), p.end(), xq);
+  return PDRD_OK;
+}
+
+int pdrd_set_weak_bc_state(pdrd_method_t h, int k, const double *state)
+{
+  if (!h) return set_error(PDRD_EINVAL, "null argument");
+  if (!h->ready) return set_error(PDRD_EINVAL, "not initialized");
+  if (k < 0 || k >= static_cast<int>(h->weak.size())) return set_error(PDRD_EINVAL, "no such weak BC");
+  WeakDev &w = h->weak[k];
+  if (w.type != PDRD_WEAK_DIRICHLET) return set_error(PDRD_ENOTSUP, "point states are for WeakDirichlet");
+  if (!state) {
+    (void)hipFree(w.d_qstate);
+    w.d_qstate = nullptr;
+    return PDRD_OK;
+  }
+  const size_t n = static_cast<size_t>(w.nf) * w.nq * h->m;
+  if (!w.d_qstate)
+    if (int rc = dev_alloc(h, &w.d_qstate, n)) return rc;
+  HIP_CHECK(hipMemcpyAsync(w.d_qstate, state, n * sizeof(double), hipMemcpyHostToDevice, h->stream));
+  HIP_CHECK(hipStreamSynchronize(h->stream));
   return PDRD_OK;
 }
 

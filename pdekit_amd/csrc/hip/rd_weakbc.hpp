@@ -59,6 +59,7 @@ struct WeakArgs
   double relax;                         // WeakWall relaxation (1 for P1)
   double params[5];                     // SubInlet: T_tot, p_tot; SubOutlet: p; Farfield, Dirichlet: state
   PhysParams pp;                        // gamma (Euler) or the advection velocity
+  const double *__restrict__ qstate;    // [nf][nq][M] WeakDirichlet state at every facet point, or null
 };
 
 // K-(n) x for the Euler equations at state st (unit normal, un = V.n).
@@ -88,9 +89,11 @@ __device__ __forceinline__ void euler_kminus(const EulerState<double, D> &st, co
 }
 
 // Correction dF at one facet point and the largest |eigenvalue| there.
+// gq: the boundary state of this point (WeakDirichlet with an expression of
+// the point, WeakDirichletBC.hpp:216-229), or null for the constant one.
 template <int PH>
 __device__ __forceinline__ double weak_point(const WeakArgs &a, const double *uq, const double *nrm, double *dF,
-                                             bool &bad)
+                                             bool &bad, const double *gq)
 {
   constexpr int D = PhysTraits<PH>::DIM, M = PhysTraits<PH>::M;
   if constexpr (M == 1) {
@@ -98,7 +101,7 @@ __device__ __forceinline__ double weak_point(const WeakArgs &a, const double *uq
     const double ax = (PH == PH_BURGERS2D) ? uq[0] : a.pp.p0;
     const double ay = (PH == PH_BURGERS2D) ? 1.0 : a.pp.p1;
     const double an = ax * nrm[0] + ay * nrm[1];
-    dF[0] = fmin(an, 0.0) * (a.params[0] - uq[0]);
+    dF[0] = fmin(an, 0.0) * ((gq ? gq[0] : a.params[0]) - uq[0]);
     return fabs(an);
   } else {
     const double gamma = a.pp.p0, gm1 = gamma - 1.0;
@@ -163,7 +166,7 @@ __device__ __forceinline__ double weak_point(const WeakArgs &a, const double *uq
       g[M - 1] = uq[M - 1];
     } else {
 #pragma unroll
-      for (int k = 0; k < M; ++k) g[k] = a.params[k];
+      for (int k = 0; k < M; ++k) g[k] = gq ? gq[k] : a.params[k];
     }
     double x[M];
 #pragma unroll
@@ -212,7 +215,8 @@ __global__ __launch_bounds__(128) void weak_bc_kernel(WeakArgs a)
       uq[k] = t;
     }
     double dF[M];
-    const double emax = weak_point<PH>(a, uq, nrm, dF, bad);
+    const double *gq = a.qstate ? a.qstate + (static_cast<size_t>(f) * a.nq + q) * M : nullptr;
+    const double emax = weak_point<PH>(a, uq, nrm, dF, bad, gq);
     coeff = fmax(coeff, fmax(floor, emax));
 #pragma unroll
     for (int k = 0; k < M; ++k) {

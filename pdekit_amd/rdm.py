@@ -278,6 +278,25 @@ class PGRDMethodExplicit:
         L.check(L.lib().pdrd_weak_bc_info(self._h, int(k), L.ptr(a, C.c_int64)), "pdrd_weak_bc_info")
         return dict(facets=int(a[0]), nodes_per_facet=int(a[1]), quadrature_points=int(a[2]))
 
+    def weak_bc_points(self, k: int) -> np.ndarray:
+        """Coordinates [nf][nq][dim] of the facet points of weak BC k."""
+        info = self.weak_bc_info(k)
+        xq = np.zeros((info["facets"], info["quadrature_points"], self.mesh.dim))
+        L.check(L.lib().pdrd_weak_bc_points(self._h, int(k), L.ptr(xq)), "pdrd_weak_bc_points")
+        return xq
+
+    def set_weak_bc_state(self, k: int, state):
+        """WeakDirichlet k's boundary state at every facet point, [nf][nq][m]
+        (e.g. an expression evaluated at weak_bc_points(k)); None restores the
+        constant state."""
+        if state is None:
+            L.check(L.lib().pdrd_set_weak_bc_state(self._h, int(k), None), "pdrd_set_weak_bc_state")
+            return
+        info = self.weak_bc_info(k)
+        a = np.ascontiguousarray(np.asarray(state, dtype=np.float64).reshape(
+            info["facets"], info["quadrature_points"], self.neq))
+        L.check(L.lib().pdrd_set_weak_bc_state(self._h, int(k), L.ptr(a)), "pdrd_set_weak_bc_state")
+
     def last_timing(self):
         c = C.c_double(0)
         n = C.c_double(0)
