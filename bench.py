@@ -193,7 +193,7 @@ def main():
         return
 
     if args.gpus != 1:
-        print(f"bench.py measures one GPU: this build has no multi-GPU path (--gpus {args.gpus} requested). "
+        print(f"bench.py measures one GPU: the cell loop runs on a single GPU and has no multi-GPU path (--gpus {args.gpus} requested). "
               "Run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
     if args.workload == "c5":
