@@ -12,7 +12,8 @@ from oracle import pyoracle as O
 from pdekit_amd import _lib as L
 from pdekit_amd.mesh import box_tetra, rectangle
 from pdekit_amd.rdm import PGRDMethodExplicit
-from tests.helpers import add_oracle_weak, boundary_facets, euler_state, oracle_problem, quad_orders, rel_err
+from tests.helpers import (add_oracle_weak, boundary_facets, euler_state, facet_rule, oracle_problem, quad_orders,
+                           rel_err)
 
 pytestmark = pytest.mark.gpu
 
@@ -182,3 +183,69 @@ def test_scalar_weak_dirichlet_matches_oracle(name, scheme):
     uo, res, _, _, rc = P.iterate(u, 0.3, 10)
     assert rc == 0
     assert rel_err(m.vec_function("solution"), uo) <= 1e-12
+
+
+@pytest.mark.parametrize("name", sorted(SCALAR))
+@pytest.mark.parametrize("scheme", [L.LDA, L.N])
+def test_scalar_weak_dirichlet_point_states_match_oracle(name, scheme):
+    """WeakDirichlet with a state at every facet point (the reference's
+    set_expression, WeakDirichletBC.hpp:225-228): the GPU's facet points are
+    the oracle's (same facets, same rule), g = 0.5 + 0.4 sin(2x + y) there
+    matches the oracle for one sweep and 10 iterations, and a point state
+    equal to the constant gives the constant's residual bit for bit."""
+    print(f"scalar WeakDirichlet point states {name} scheme {scheme}", flush=True)
+    make, phys, par = SCALAR[name]
+    mesh = make()
+    b = mesh.boundaries
+    x = mesh.node_coords
+    u = (0.5 + 0.3 * np.sin(3 * x[:, 0]) * np.cos(2 * x[:, 1]))[:, None].copy()
+    q = quad_orders(mesh)
+    qo = max(q.values())
+    sides = (("xmin", 1.0), ("ymin", -0.5), ("xmax", 0.25))
+
+    def gpu():
+        m = PGRDMethodExplicit(phys, scheme, params=par)
+        m.configure_mesh_data(mesh).initialize_work_data(quad_order=q)
+        for side, val in sides:
+            m.add_boundary_condition("WeakDirichlet", side, b[side], values=[val])
+        m.set_vec_function("solution", u)
+        return m
+
+    m = gpu()
+    P = oracle_problem(mesh, phys, scheme, par)
+    for k, (side, val) in enumerate(sides):
+        fn, fv, ins = boundary_facets(mesh, b[side])
+        w, V = facet_rule(2, fn.shape[1], qo + 1)
+        _, Vl = facet_rule(2, 2, qo + 1)
+        xq = np.einsum("qv,fvd->fqd", Vl, mesh.coords[fv])
+        assert np.abs(m.weak_bc_points(k) - xq).max() <= 1e-15
+        g = (0.5 + 0.4 * np.sin(2 * xq[..., 0] + xq[..., 1]))[..., None]
+        m.set_weak_bc_state(k, g)
+        P.add_weak(O.WEAK_DIRICHLET, (val,), mesh.order, fn, fv, ins, w, V, qstate=g)
+    m.assemble_rhs()
+    m.check_status()
+    ref, ref_ws, rc = P.assemble_rhs(u)
+    assert rc == 0
+    assert rel_err(m.vec_function("residuals"), ref) <= 1e-12
+    assert rel_err(m.vec_function("wave_speed"), ref_ws) <= 1e-12
+    m.iterate(0.3, 10)
+    uo, _, _, _, rc = P.iterate(u, 0.3, 10)
+    assert rc == 0
+    assert rel_err(m.vec_function("solution"), uo) <= 1e-12
+    # the constant at every point, then back to the constant: both bit for bit the constant's residual
+    m.set_vec_function("solution", u)
+    for k, (side, val) in enumerate(sides):
+        info = m.weak_bc_info(k)
+        m.set_weak_bc_state(k, np.full((info["facets"], info["quadrature_points"], 1), val))
+    m.assemble_rhs()
+    r_points = m.vec_function("residuals")
+    for k in range(len(sides)):
+        m.set_weak_bc_state(k, None)
+    m.assemble_rhs()
+    r_back = m.vec_function("residuals")
+    m.destroy()
+    m2 = gpu()
+    m2.assemble_rhs()
+    r_const = m2.vec_function("residuals")
+    m2.destroy()
+    assert np.array_equal(r_points, r_const) and np.array_equal(r_back, r_const)
