@@ -249,3 +249,51 @@ def test_scalar_weak_dirichlet_point_states_match_oracle(name, scheme):
     r_const = m2.vec_function("residuals")
     m2.destroy()
     assert np.array_equal(r_points, r_const) and np.array_equal(r_back, r_const)
+
+
+@pytest.mark.parametrize("name", sorted(MESHES))
+def test_euler_weak_dirichlet_point_states_match_oracle(name):
+    """The far-field set of Euler BCs with WeakDirichlet's state given at every
+    facet point (1.02 far * (1 + 0.03 sin(3x + 2y + z)) per component) on
+    edges (Gauss-Legendre) and faces (triangle rule), P1 and P2 facets: the
+    GPU's facet points are the oracle's, and one sweep and 10 iterations match
+    the oracle."""
+    print(f"Euler WeakDirichlet point states {name}", flush=True)
+    mesh = MESHES[name]()
+    phys = L.EULER2D if mesh.dim == 2 else L.EULER3D
+    u = euler_state(mesh, dim=mesh.dim)
+    m = PGRDMethodExplicit(phys, L.LDA, params=(GAMMA, 287.05))
+    q = quad_orders(mesh)
+    qo = max(q.values())
+    m.configure_mesh_data(mesh).initialize_work_data(quad_order=q)
+    P = oracle_problem(mesh, phys, L.LDA, (GAMMA, 287.05))
+    bcs = _bcs_far(mesh)
+    states = {}
+    for k, (kind, nodes, val) in enumerate(bcs):
+        m.add_boundary_condition(kind, kind, nodes, values=val)
+        if kind != "WeakDirichlet":
+            add_oracle_weak(P, mesh, kind, nodes, () if val is None else tuple(val), quad_order=qo)
+            continue
+        fn, fv, ins = boundary_facets(mesh, nodes)
+        w, V = facet_rule(mesh.dim, fn.shape[1], qo + 1)
+        _, Vl = facet_rule(mesh.dim, mesh.dim, qo + 1)
+        xq = np.einsum("qv,fvd->fqd", Vl, mesh.coords[fv])
+        s = np.sin(3 * xq[..., 0] + 2 * xq[..., 1] + (xq[..., 2] if mesh.dim == 3 else 0.0))
+        g = np.asarray(val)[None, None, :] * (1 + 0.03 * s)[..., None]
+        states[k] = (xq, g)
+        P.add_weak(O.WEAK_DIRICHLET, tuple(val), mesh.order, fn, fv, ins, w, V, qstate=g)
+    m.set_vec_function("solution", u)
+    for k, (xq, g) in states.items():
+        assert np.abs(m.weak_bc_points(k) - xq).max() <= 1e-14
+        m.set_weak_bc_state(k, g)
+    m.assemble_rhs()
+    m.check_status()
+    ref, ref_ws, rc = P.assemble_rhs(u)
+    assert rc == 0
+    assert rel_err(m.vec_function("residuals"), ref) <= 1e-12
+    assert rel_err(m.vec_function("wave_speed"), ref_ws) <= 1e-12
+    m.iterate(0.3, 10)
+    uo, _, _, _, rc = P.iterate(u, 0.3, 10)
+    assert rc == 0
+    assert rel_err(m.vec_function("solution"), uo) <= 1e-12
+    m.destroy()
