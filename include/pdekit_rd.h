@@ -340,6 +340,33 @@ int pdrd_batch_set_graphs(pdrd_batch_t b, int on);
 int pdrd_batch_iterate(pdrd_batch_t b, int count, const pdrd_method_t *methods, const double *cfl, int n_iter,
                        int exclude_dirichlet, double *norms);
 
+/* No counterpart in the reference: for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc). The cell loop of pdrd_assemble_rhs with FP64 atomics
+ * (PDRD_OPT_SCATTER = 0), without boundary conditions, over one block of
+ * cells held in caller-owned device memory:
+ *   tables        the packed FE tables of pdrd_fe_fill (fe->table_size doubles),
+ *   geo_conn      [ng][ne] and sol_conn [nd][ne] (structure of arrays),
+ *   vertex_coords [nb_vertices][dim], u [nb_nodes][m], dual_volume [nb_nodes]
+ *                 (the PDRD_DUAL_VOLUME field of a method on the same mesh);
+ * outputs residual [nb_nodes][m] = sum Phi / V and wave_speed [nb_nodes], and,
+ * when status is not null, the status word at *status (device int: bit 0
+ * non-positive pressure, bit 1 connectivity out of range -- such cells are
+ * skipped). params / nb_params as pdrd_create. Stream-ordered on `stream`
+ * (NULL: the null stream); returns once enqueued. Its scratch (FE tables in
+ * the device layout, nodal accumulators, status) is one static device array
+ * of PDRD_NOALLOC_SCRATCH_BYTES per device, set up at the first call on each
+ * device: nb_nodes * (m + 1) must not exceed PDRD_NOALLOC_MAX_ACC doubles
+ * (PDRD_EINVAL otherwise). Every call on a device shares that array, so calls
+ * that overlap on one device race; a call clears what it needs first. */
+#define PDRD_NOALLOC_SCRATCH_BYTES (32u << 20)
+#define PDRD_NOALLOC_MAX_ACC ((PDRD_NOALLOC_SCRATCH_BYTES / 8u) - 8192u - 32u)
+int pdrd_assemble_rhs_noalloc(int physics, int scheme, const double *params, int nb_params, const pdrd_fe_desc *fe,
+                              const double *tables, uint32_t ne, const uint32_t *geo_conn, const uint32_t *sol_conn,
+                              uint32_t nb_vertices, const double *vertex_coords, uint32_t nb_nodes, const double *u,
+                              const double *dual_volume, double *residual, double *wave_speed, int *status,
+                              void *stream);
+
 const char *pdrd_last_error(void);
 const char *pdrd_version(void);
 

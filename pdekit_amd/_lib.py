@@ -30,6 +30,8 @@ SOLUTION, RESIDUAL, TIME_STEP, WAVE_SPEED, DUAL_VOLUME, BLEND_COEFF, ART_VISC = 
 BLEND_LF = 0
 OPT_SCATTER, OPT_PRECISION, OPT_BLOCK, OPT_TIMING, OPT_PATCH_CELLS, OPT_REFRESH, OPT_CHAIN = 0, 1, 2, 3, 4, 5, 6
 WEAK_WALL, WEAK_SUBINLET, WEAK_SUBOUTLET, WEAK_FARFIELD, WEAK_SYMMETRY, WEAK_SUPERINLET, WEAK_DIRICHLET = range(7)
+NOALLOC_SCRATCH_BYTES = 32 << 20
+NOALLOC_MAX_ACC = NOALLOC_SCRATCH_BYTES // 8 - 8192 - 32  # doubles: nb_nodes * (m + 1) at most
 
 # every symbol include/pdekit_rd.h declares
 EXPORTED = (
@@ -46,7 +48,7 @@ EXPORTED = (
     "pdrd_weak_bc_points", "pdrd_set_weak_bc_state",
     "pdrd_mesh_read_gmsh", "pdrd_mesh_write_gmsh", "pdrd_gmsh_append_node_data", "pdrd_gmsh_read_node_data",
     "pdrd_set_blending_coeff", "pdrd_set_artificial_viscosity", "pdrd_set_sources", "pdrd_compute_blending_coeff",
-    "pdrd_compute_artificial_viscosity", "pdrd_element_jacobians",
+    "pdrd_compute_artificial_viscosity", "pdrd_element_jacobians", "pdrd_assemble_rhs_noalloc",
 )
 
 
@@ -133,6 +135,9 @@ def lib() -> C.CDLL:
         "pdrd_compute_blending_coeff": [P, I],
         "pdrd_compute_artificial_viscosity": [P, D, D],
         "pdrd_element_jacobians": [P, I, PD, PD, PD],
+        # device pointers for tables, connectivity, coords, u, V, outputs, status; stream
+        "pdrd_assemble_rhs_noalloc": [I, I, PD, I, C.POINTER(FeDesc), P, U32, P, P, U32, P, U32, P, P, P, P, P,
+                                      P],
         "pdrd_check_status": [P],
         "pdrd_method_info": [P, PI64],
         "pdrd_last_timing": [P, PD, PD],

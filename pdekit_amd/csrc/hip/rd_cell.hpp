@@ -44,6 +44,10 @@ struct CellArgs
   int atomic;      // 1: FP64 atomics, 0: plain read-modify-write (one colour)
   PhysParams pp;
   const double *__restrict__ nodal;  // [N] blending coefficient (B) or artificial viscosity (LDA), or null
+  // when non-zero, vertex ids >= nv_check or node ids >= nn_check are not
+  // followed: the cell is skipped and bit 2 of *status is set (connectivity
+  // the host has not validated)
+  uint32_t nv_check, nn_check;
 };
 
 template <int D, int NG, int ND, int NF, int NQ>
@@ -864,16 +868,27 @@ __global__ __launch_bounds__(256) void rd_cell_kernel(CellArgs a)
   if (local >= a.count) return;
   const uint32_t e = a.first + local;
   bool bad = false;
+  uint32_t nid[ND];
+#pragma unroll
+  for (int n = 0; n < ND; ++n) nid[n] = a.sconn[static_cast<size_t>(n) * a.ne + e];
+  if (a.nn_check) {
+    bool out = false;
+#pragma unroll
+    for (int g = 0; g < NG; ++g) out = out || a.gconn[static_cast<size_t>(g) * a.ne + e] >= a.nv_check;
+#pragma unroll
+    for (int n = 0; n < ND; ++n) out = out || nid[n] >= a.nn_check;
+    if (out) {
+      atomicOr(a.status, 2);
+      return;
+    }
+  }
   double X[NG][D];
   gather_coords<D, NG, NQ, M>(a, e, X);
-  uint32_t nid[ND];
   double U[ND][M];
 #pragma unroll
-  for (int n = 0; n < ND; ++n) {
-    nid[n] = a.sconn[static_cast<size_t>(n) * a.ne + e];
+  for (int n = 0; n < ND; ++n)
 #pragma unroll
     for (int k = 0; k < M; ++k) U[n][k] = a.u[static_cast<size_t>(nid[n]) * M + k];
-  }
   double phi[ND][M];
   double wave;
   if constexpr (PREC == 2)
