@@ -3,10 +3,13 @@
 
 One step = one explicit pseudo-time iteration of PGRDMethodExplicit over the
 whole mesh on the GPU: the fused cell kernel (gather, metrics, P+1 flux,
-K+, B-scheme distribution, scatter), the strong Dirichlet values and the
+K+, scheme distribution, scatter), the strong Dirichlet values and the
 nodal pass (residual / V, dt = CFL V / sum lambda, u -= dt res). The default
-workload is 2D Burgers on 1 048 576 jittered P2 triangles with the B scheme
-(pdekit_amd.workloads.c2). Inputs are resident in HBM before the timed region.
+workload is the 2D Euler channel over a bump with P2 triangles and Q2
+quadrilaterals (786 432 elements, 2 100 225 nodes, LDA scheme;
+pdekit_amd.workloads.c3p2); --workload c2 gives 2D Burgers on 1 048 576
+jittered P2 triangles with the B scheme. Inputs are resident in HBM before
+the timed region.
 
 Prints one JSON line: throughput in element residual evaluations per second,
 the roofline of the cell kernel (HIP events around every cell-loop launch in
@@ -51,6 +54,19 @@ def algorithmic_bytes(wl, scatter: int):
     return cell, sweep
 
 
+def linear_bytes(wl):
+    """Compulsory HBM bytes of one linear sweep of the mixed-precision form:
+    the FP32 element Jacobians ((nd m)^2 per cell), the connectivity (u32 per
+    cell node), and per node u_ref, R(u_ref) and u read, dt(u_ref) and V read,
+    u written."""
+    m = wl.mesh
+    neq = wl.state.shape[1]
+    jac = sum(4 * b.sol_conn.shape[0] * (b.sol_conn.shape[1] * neq) ** 2 for b in m.blocks)
+    conn = sum(4 * b.sol_conn.size for b in m.blocks)
+    N = m.nb_nodes
+    return jac + conn + 8 * N * (4 * neq + 2)
+
+
 def profile_entry(workload: str) -> dict:
     """Figures of the patch kernel measured with rocprofv3 --pmc, if a summary
     for this workload is committed under profiles/: HBM bytes per launch
@@ -69,10 +85,12 @@ def profile_entry(workload: str) -> dict:
 
 def cpu_baseline(wl, seconds: float):
     """The CPU oracle (FP64 restatement of the reference cell loop) on the same
-    mesh and state for a bounded number of iterations: split over the host
-    threads as PGRDMethodExplicit splits its cell loop (nb_threads std::threads
-    over cell ranges; at most 16, the box's CPU share), and single-threaded
-    (the reference's default nb_threads = 1)."""
+    mesh and state, timed per sweep with std::chrono::steady_clock around the
+    cell loop, boundary conditions, dt and update: split over the host threads
+    as PGRDMethodExplicit splits its cell loop (nb_threads std::threads over
+    cell ranges; at most 16, the box's CPU share), 3 warm-up sweeps then the
+    median of at least 20; and single-threaded (the reference's default
+    nb_threads = 1), the median of a bounded number of sweeps."""
     from oracle.pyoracle import Problem
     from pdekit_amd.fe import element_tables
 
@@ -82,25 +100,27 @@ def cpu_baseline(wl, seconds: float):
     P = Problem(m, tables, vol, wl.physics, wl.scheme, wl.params)
     threads = max(1, min(16, os.cpu_count() or 1))
 
-    def timed(run, budget):
-        t0 = time.perf_counter()
-        run(1)
-        t1 = time.perf_counter() - t0
-        n = max(1, int(budget / max(t1, 1e-3)))
-        t0 = time.perf_counter()
-        run(n)
-        return n, time.perf_counter() - t0
+    def sweeps(th, warm, timed):
+        t = np.zeros(warm + timed)
+        P.iterate_threads(wl.state, wl.cfl, warm + timed, th, wl.dirichlet_nodes, wl.dirichlet_values,
+                          sweep_times=t)
+        return t[warm:]
 
-    n1, t_1 = timed(lambda n: P.iterate(wl.state, wl.cfl, n, wl.dirichlet_nodes, wl.dirichlet_values),
-                    seconds / 2)
-    nt, t_t = timed(lambda n: P.iterate_threads(wl.state, wl.cfl, n, threads, wl.dirichlet_nodes,
-                                                wl.dirichlet_values), seconds / 2)
-    return {"value": wl.nb_elements * nt / t_t, "unit": "element residual evals/s", "cores": threads,
+    warm, timed = 3, 20
+    tm = sweeps(threads, warm, timed)
+    med = float(np.median(tm))
+    # single thread: as many sweeps as fit in the remaining budget (at least 3)
+    probe = sweeps(1, 0, 1)[0]
+    n1 = max(3, min(20, int(seconds / max(probe, 1e-3))))
+    t1 = sweeps(1, 0, n1)
+    med1 = float(np.median(t1))
+    return {"value": wl.nb_elements / med, "unit": "element residual evals/s", "cores": threads,
             "kind": "port",
-            "sample": f"{nt} full iterations of the same mesh and state ({wl.nb_elements} elements), "
-                      f"oracle/rd_oracle.cpp, {threads} threads over cell ranges, {t_t:.1f} s; "
-                      f"single thread: {n1} iterations in {t_1:.1f} s",
-            "single_thread_value": wl.nb_elements * n1 / t_1}
+            "sample": f"{wl.nb_elements} elements of the same mesh and state, oracle/rd_oracle.cpp, {threads} "
+                      f"threads over cell ranges: {warm} warm-up sweeps then the median of {timed} "
+                      f"({med:.3f} s per sweep); single thread: median of {n1} sweeps after one probe "
+                      f"({med1:.3f} s per sweep)",
+            "single_thread_value": wl.nb_elements / med1}
 
 
 def serving_bench(args):
@@ -157,14 +177,18 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--workload", default="c2", choices=["c1", "c2", "c3", "c4", "c3p2", "c4p2", "c5"],
-                    help="c1-c4, c3p2, c4p2: per-sweep throughput of one mesh; c5: the serving stream")
+    ap.add_argument("--workload", default="c3p2", choices=["c1", "c2", "c3", "c4", "c3p2", "c4p2", "c5"],
+                    help="c1-c4, c3p2, c4p2: per-sweep throughput of one mesh (default c3p2, the P2 Euler "
+                         "channel); c5: the serving stream")
     ap.add_argument("--requests", type=int, default=64, help="c5: requests in the stream")
     ap.add_argument("--size-scale", type=float, default=0.1,
                     help="c5: scale of the request sizes (1.0: advection 10k-1M, Burgers 50k-500k triangles)")
     ap.add_argument("--scatter", default="auto", choices=["auto", "atomic", "colour", "patch"])
     ap.add_argument("--precision", default="plain", choices=["plain", "mixed"],
-                    help="plain FP64 cells, or FP32 interpolation / fluxes / K+ with FP64 solves and sums")
+                    help="plain FP64 sweeps, or the mixed-precision form (linear sweeps with FP32 element "
+                         "Jacobians between FP64 sweeps); mixed also times the plain path for the speed-up")
+    ap.add_argument("--refresh", type=int, default=50,
+                    help="mixed: iterations between FP64 sweeps with fresh Jacobians (PDRD_OPT_REFRESH)")
     ap.add_argument("--patch-cells", default="default", choices=["default", "auto", "64", "128", "256"],
                     help="cells per patch workgroup; auto: pdekit_amd.autotune times 64/128/256 first")
     ap.add_argument("--chain", default="default", choices=["default", "0", "1"],
@@ -220,6 +244,7 @@ def main():
     meth.configure_mesh_data(wl.mesh).initialize_work_data(quad_order=wl.quad_order, patch_cells=cells)
     if args.precision == "mixed":
         meth.set_option(L.OPT_PRECISION, 1)
+        meth.set_option(L.OPT_REFRESH, args.refresh)
     if args.chain != "default":
         meth.set_option(L.OPT_CHAIN, int(args.chain))
     meth.set_vec_function("solution", wl.state)
@@ -265,19 +290,40 @@ def main():
     value = E * args.steps / t
 
     # per-launch duration of the cell loop in a second timed pass of the same
-    # length, HIP events on the method's stream around every launch
-    meth.set_option(L.OPT_TIMING, 1)
+    # length, HIP events on the method's stream around every launch (mixed:
+    # the linear sweeps' kernels only)
+    mixed = args.precision == "mixed"
+    meth.set_option(L.OPT_TIMING, 2 if mixed else 1)
     meth.iterate(wl.cfl, args.steps)
     cell_ms, node_ms = meth.last_timing()
     info = meth.info()
+    plain = None
+    if mixed:
+        # the plain FP64 path on the same method and state, same steps
+        meth.set_option(L.OPT_TIMING, 0)
+        meth.set_option(L.OPT_PRECISION, 0)
+        meth.set_vec_function("solution", wl.state)
+        meth.iterate(wl.cfl, max(1, args.warmup))
+        meth.synchronize()
+        t0 = time.perf_counter()
+        meth.iterate(wl.cfl, args.steps)
+        meth.synchronize()
+        tp = time.perf_counter() - t0
+        plain = {"value": round(E * args.steps / tp, 1), "ms_per_step": round(tp / args.steps * 1e3, 4),
+                 "speedup": round((E * args.steps / t) / (E * args.steps / tp), 3)}
+        print(f"[bench] mixed {t / args.steps * 1e3:.4f} ms/step, plain {tp / args.steps * 1e3:.4f} ms/step, "
+              f"linear sweep kernels {cell_ms * 1e3:.1f} + {node_ms * 1e3:.1f} us", file=sys.stderr, flush=True)
 
     cell_bytes, sweep_bytes = algorithmic_bytes(wl, best[0])
+    if mixed:
+        cell_bytes = linear_bytes(wl)
     achieved = cell_bytes / (cell_ms * 1e-3) / 1e9
     prof = profile_entry(wl.name) if (best[0] == 2 and args.precision == "plain") else {}
     traffic = prof.get("bytes_per_launch")
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                "kernel": "rd_patch_kernel (cell loop + nodal step)" if best[0] == 2 else "rd_cell_kernel (cell loop)",
+                "kernel": ("rd_patch_kernel, linear sweep (J delta + nodal step)" if mixed else
+                           "rd_patch_kernel (cell loop + nodal step)" if best[0] == 2 else "rd_cell_kernel (cell loop)"),
                 "algorithmic_bytes": int(cell_bytes),
                 "avg_launch_us": round(cell_ms * 1e3, 2)}
     out = {
@@ -294,7 +340,7 @@ def main():
         "dtype": "f64" if args.precision == "plain" else "f32+f64",
         "data": "synthetic",
         "config": {"workload": wl.name, "precision": args.precision, "elements": E, "nodes": wl.mesh.nb_nodes,
-                   "order": wl.mesh.order, "scheme": ["PGLDA", "PGN", "PGB", "PSI"][wl.scheme],
+                   "order": wl.mesh.order, "scheme": ["PGLDA", "PGN", "PGB", "PSI", "PSI/LDA blend"][wl.scheme],
                    "scatter": SCATTER_NAMES[best[0]], "colours": info["nb_colours"],
                    "patch_cells": meth.get_option(L.OPT_PATCH_CELLS) if best[0] == 2 else None,
                    "chain": meth.get_option(L.OPT_CHAIN) if best[0] == 2 else None,
@@ -307,9 +353,12 @@ def main():
                   "frac": round(prof["fp64_flops_per_launch"] / (cell_ms * 1e-3) / 1e12 / FP64_PEAK_TFLOPS, 4),
                   "flops_per_element": round(prof["fp64_flops_per_launch"] / E, 1)}
                  if "fp64_flops_per_launch" in prof else None),
-        "sweep_hbm_frac": round(sweep_bytes / (t / args.steps) / 1e9 / HBM_PEAK_GBS, 4),
+        "sweep_hbm_frac": None if mixed else round(sweep_bytes / (t / args.steps) / 1e9 / HBM_PEAK_GBS, 4),
         "nodal_pass_us": round(node_ms * 1e3, 2),  # shared-node fix-up with the patch assembly
     }
+    if mixed:
+        out["config"]["refresh"] = args.refresh
+        out["plain"] = plain
     # a caller that hands over host buffers every step (set the state, one
     # sweep, read the state back through the C ABI): the PCIe-inclusive rate,
     # reported beside the resident one, never as `value`
@@ -368,7 +417,7 @@ def jacobian_bench(args, wl, meth):
         "unit": "elements/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": round(total_ms, 4), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f64", "data": "synthetic",
-        "config": {"workload": wl.name, "scheme": ["PGLDA", "PGN", "PGB", "PSI"][wl.scheme],
+        "config": {"workload": wl.name, "scheme": ["PGLDA", "PGN", "PGB", "PSI", "PSI/LDA blend"][wl.scheme],
                    "elements": m.nb_cells, "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": "rd_jacobian_kernel",

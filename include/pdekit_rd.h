@@ -57,8 +57,10 @@ enum pdrd_physics {
   PDRD_ROTATION2D = 4
 };
 /* SchemeTag: PGLDA, PGN, PGB; PSI = N residuals limited with the x+ limiter
- * of LFResidualLimiter.hpp:84-110. */
-enum pdrd_scheme { PDRD_LDA = 0, PDRD_N = 1, PDRD_B = 2, PDRD_PSI = 3 };
+ * of LFResidualLimiter.hpp:84-110; BPSI = the PSI / LDA blend
+ * (1 - theta) LDA + theta PSI with PGB's cell theta (equation 0 of the N
+ * residuals, PGB.hpp:437-462). PSI and BPSI have no implicit form. */
+enum pdrd_scheme { PDRD_LDA = 0, PDRD_N = 1, PDRD_B = 2, PDRD_PSI = 3, PDRD_BPSI = 4 };
 /* Nodal fields held by a method (RDMethod::vec_function / scal_function). */
 enum pdrd_field {
   PDRD_SOLUTION = 0,   /* [N][m] */
@@ -75,21 +77,28 @@ enum pdrd_option {
                              2 = patch assembly (default): Morton-ordered patches of
                              cells summed in LDS, shared nodes summed by a fix-up pass;
                              1 and 2 are deterministic */
-  PDRD_OPT_PRECISION = 1, /* 0 = plain FP64, 1 = mixed (patch assembly only): FP32
-                             interpolation, flux differences, divergence and K+ parameters;
-                             FP64 sums of K+, solves, residuals, update and norms */
+  PDRD_OPT_PRECISION = 1, /* 0 = plain FP64, 1 = mixed (patch assembly only; not PSI, not
+                             the rotating velocity, not in batches): a linearised defect
+                             correction. An FP64 sweep sets the linearisation point u_ref
+                             with its FP64 residual and time step; the sweeps in between are
+                             linear, res = R(u_ref) + sum_e J_e (u - u_ref)_e / V with the
+                             element Jacobians J_e of the implicit form in FP32 and J delta
+                             in FP32, dt = dt(u_ref); nodal sums, update and norms in FP64.
+                             Its fixed point is the FP64 steady state. The last iteration of
+                             every pdrd_iterate call is an FP64 sweep (FP64 outputs and norm);
+                             pdrd_assemble_rhs / _lhs_and_rhs / solve stay FP64 */
   PDRD_OPT_BLOCK = 2,     /* threads per block of the cell kernels (64, 128, 192, 256) */
   PDRD_OPT_TIMING = 3,    /* 1: pdrd_iterate runs eagerly and times the cell loop and the
                              nodal pass of every iteration with HIP events on the method's
-                             stream; pdrd_last_timing then returns per-iteration means */
+                             stream; pdrd_last_timing then returns per-iteration means;
+                             2: the same over the linear sweeps of the mixed form only */
   PDRD_OPT_PATCH_CELLS = 4, /* cells per patch workgroup, set before pdrd_initialize: 0 (each
                              kernel's default: 256 scalar P1/P2, 128 otherwise), 64, 128 or
                              256 (capped at the default); read back after pdrd_initialize
                              it gives the largest in use */
-  PDRD_OPT_REFRESH = 5,    /* mixed precision: every this many iterations (default 50) the
-                             defect correction R64(u) - Rmixed(u) is recomputed (two sweeps)
-                             and added to the mixed residual of the updates, so the iteration
-                             converges to the FP64 steady state; 0 disables it */
+  PDRD_OPT_REFRESH = 5,    /* mixed precision: every this many iterations (default 50) an FP64
+                             sweep with fresh element Jacobians; 0 or 1: every sweep FP64 (the
+                             plain iteration, bit for bit) */
   PDRD_OPT_CHAIN = 6       /* patch assembly, pdrd_iterate: 1 = the shared nodes of a sweep
                              are summed and stepped in the next sweep's gather (one launch
                              per element type and sweep; bitwise the same results), 0 = a
@@ -234,11 +243,12 @@ int pdrd_weak_bc_info(pdrd_method_t h, int k, int64_t *info3);
  * WeakDirichletBC.hpp:149-154, 216-229): the coordinates of weak BC k's
  * facet points xq[nf][nq][dim] (after pdrd_initialize; nf, nq from
  * pdrd_weak_bc_info), and the boundary state at each of them,
- * state[nf][nq][m], which then replaces the constant one (NULL restores it).
- * The caller evaluates its expression at the points; the reference's
- * expressions may also read the state, which a precomputed table cannot. */
+ * state[nf][nq][m] (count = nf * nq * m doubles, checked), which then
+ * replaces the constant one (NULL restores it). The caller evaluates its
+ * expression at the points; the reference's expressions may also read the
+ * state, which a precomputed table cannot. */
 int pdrd_weak_bc_points(pdrd_method_t h, int k, double *xq);
-int pdrd_set_weak_bc_state(pdrd_method_t h, int k, const double *state);
+int pdrd_set_weak_bc_state(pdrd_method_t h, int k, const double *state, size_t count);
 
 /* Nodal inputs of the cell loop (PGRDMExplicitCellWorker.hpp:307-330, 501-533).
  * set_blending_coeff(f): PGB takes theta_i = f[node i] instead of its
@@ -247,8 +257,8 @@ int pdrd_set_weak_bc_state(pdrd_method_t h, int k, const double *state);
  * set_artificial_viscosity(nu): PGLDA adds w |J| nu_q grad(phi_i) . grad(u)
  * at every quadrature point, nu_q interpolated from the nodes like the state
  * (PGLDA.hpp:245-256). theta / nu: host [N], copied to the device; NULL
- * switches the input off. Other schemes ignore them; both run with the FP64
- * cells (not with PDRD_OPT_PRECISION = 1) and outside batches. */
+ * switches the input off. Other schemes ignore them; both run outside
+ * batches. */
 int pdrd_set_blending_coeff(pdrd_method_t h, const double *theta);
 int pdrd_set_artificial_viscosity(pdrd_method_t h, const double *nu);
 /* set_vec_function(SolverVecFn::sources, f) (PGRDMethodExplicit.hpp:350-356):
@@ -256,7 +266,7 @@ int pdrd_set_artificial_viscosity(pdrd_method_t h, const double *nu);
  * the cells compute the reaction residuals (compute_adv_reaction_residuals,
  * PGRDMExplicitCellWorker.hpp:535-544): the residual at a quadrature point
  * is div F - s_q with s_q the source interpolated like the solution
- * (PGLDA.hpp:553-741, PGN.hpp:470-660, PGB.hpp:825-1050). Runs with the FP64
+ * (PGLDA.hpp:553-741, PGN.hpp:470-660, PGB.hpp:825-1050). Runs with the
  * patch assembly outside batches; a blending coefficient or artificial
  * viscosity alongside it is refused (the reference's reaction LDA would drop
  * the viscosity). pdrd_element_jacobians ignores it, as
@@ -267,7 +277,8 @@ int pdrd_set_sources(pdrd_method_t h, const double *sources);
  * RDBlendingCoeff::calculate with "LF_Blend" (LFBlendingCoeffImpl.hpp:79-136
  * + MeshFunctionTools::interp_from_cells_to_nodes_p1_max, RDBlendingCoeff.hpp:
  * 150-165); ArtificialViscosity::compute_artificial_viscosity_nodal(kappa,
- * factor) (ArtificialViscosity.hpp:371-560; triangles and tetrahedra). */
+ * factor) (ArtificialViscosity.hpp:371-560; triangles, tetrahedra and, with the
+ * modes ModalExpansionQuad flags as leading, quadrilaterals). */
 enum pdrd_blend_kind { PDRD_BLEND_LF = 0 };
 int pdrd_compute_blending_coeff(pdrd_method_t h, int kind);
 int pdrd_compute_artificial_viscosity(pdrd_method_t h, double kappa, double factor);
@@ -301,9 +312,11 @@ int pdrd_compute_residual_norm(pdrd_method_t h, double *norm);
  * remaining nodes). */
 int pdrd_residual_norm_L2(pdrd_method_t h, int exclude_dirichlet, double *norm);
 /* n_iter times { assemble_lhs_and_rhs(cfl); solve() } on the device (the
- * driver loop of examples/testcases-rdm/Advection-Continuous-RDS-2D-Explicit.cpp:160-163),
- * replayed from one captured HIP graph; norm receives norm_L2 of the last
- * residual (exclude_dirichlet as above). */
+ * driver loop of examples/testcases-rdm/Advection-Continuous-RDS-2D-Explicit.cpp:160-163);
+ * norm receives norm_L2 of the last residual (exclude_dirichlet as above).
+ * The patch assembly launches its sweeps directly (chained per
+ * PDRD_OPT_CHAIN); the atomic / colour assemblies replay one captured HIP
+ * graph per iteration. */
 int pdrd_iterate(pdrd_method_t h, double cfl, int n_iter, int exclude_dirichlet, double *norm);
 
 /* Status of the last cell loop: PDRD_OK or PDRD_ENEGPRESSURE. */

@@ -14,7 +14,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "liboracle_rd.so")
 
 ADVECTION2D, BURGERS2D, EULER2D, EULER3D = 0, 1, 2, 3
-LDA, N, B, PSI = 0, 1, 2, 3
+LDA, N, B, PSI, BPSI = 0, 1, 2, 3, 4
 
 
 class OrcBlock(C.Structure):
@@ -57,7 +57,7 @@ class OrcAvRule(C.Structure):
     _fields_ = [
         ("p", C.c_int), ("nq", C.c_int), ("npm", C.c_int),
         ("w", C.POINTER(C.c_double)), ("V", C.POINTER(C.c_double)), ("dVg", C.POINTER(C.c_double)),
-        ("psi", C.POINTER(C.c_double)),
+        ("psi", C.POINTER(C.c_double)), ("hm", C.POINTER(C.c_double)),
     ]
 
 
@@ -83,7 +83,7 @@ def lib():
             "orc_euler_kplus": [C.c_int, C.c_double, PD, PD, PD, PD],
             "orc_lf_blend": [PP, PD, C.POINTER(PD), C.POINTER(C.c_int), PD],
             "orc_cell_jacobians": [PP, C.c_int, PD, PD, PD, PD],
-            "orc_iterate_threads": [PP, C.c_double, C.c_int, C.c_uint32, PU, PD, PD, PD, PD, C.c_int],
+            "orc_iterate_threads": [PP, C.c_double, C.c_int, C.c_uint32, PU, PD, PD, PD, PD, C.c_int, PD],
             "orc_art_visc": [PP, PD, C.POINTER(OrcAvRule), C.POINTER(PD), C.POINTER(C.c_int), C.c_double,
                              C.c_double, PD],
         }.items():
@@ -185,12 +185,18 @@ class Problem:
         rules = (OrcAvRule * len(self._mesh.blocks))()
         for i, b in enumerate(self._mesh.blocks):
             t = element_tables(b.shape, p, max(2 * p, 1))
-            psi = lower_degree_basis(b.shape, p, t.qp, t.w)
-            arrs = [np.ascontiguousarray(x, dtype=np.float64) for x in (t.w, t.V, t.dVg, psi)]
-            self._keep += arrs
             r = rules[i]
-            r.p, r.nq, r.npm = p, t.nq, psi.shape[1]
-            r.w, r.V, r.dVg, r.psi = (_p(x) for x in arrs)
+            if b.shape == 1:
+                hm = quad_zeroed_modes(p, t.qp, t.sol_nodes)
+                arrs = [np.ascontiguousarray(x, dtype=np.float64) for x in (t.w, t.V, t.dVg, hm)]
+                r.p, r.nq, r.npm = p, t.nq, 0
+                r.w, r.V, r.dVg, r.hm = (_p(x) for x in arrs)
+            else:
+                psi = lower_degree_basis(b.shape, p, t.qp, t.w)
+                arrs = [np.ascontiguousarray(x, dtype=np.float64) for x in (t.w, t.V, t.dVg, psi)]
+                r.p, r.nq, r.npm = p, t.nq, psi.shape[1]
+                r.w, r.V, r.dVg, r.psi = (_p(x) for x in arrs)
+            self._keep += arrs
         arr, nv = self._p1()
         out = np.zeros(self.nn)
         rc = lib().orc_art_visc(C.byref(self.p), _p(u), rules, arr, nv, float(kappa), float(factor), _p(out))
@@ -285,6 +291,34 @@ def p1_weights(shape, nodes):
                                           (1 + x[:, 1]) / 2, (1 + x[:, 2]) / 2], 1))
 
 
+def quad_modal_basis(p, xi):
+    """The quadrilateral modal basis of ModalExpansionQuad.cpp:36-105 at points
+    xi [n][2]: mode m (p + 1) + n is P_m(xi0) P_n(xi1) / sqrt(g_m g_n), P_k the
+    Legendre polynomials, g_k = 2 / (2k + 1): [n][(p + 1)^2]."""
+    xi = np.asarray(xi, dtype=np.float64)
+    out = []
+    for m in range(p + 1):
+        pm = np.polynomial.legendre.legval(xi[:, 0], np.eye(p + 1)[m])
+        for n in range(p + 1):
+            pn = np.polynomial.legendre.legval(xi[:, 1], np.eye(p + 1)[n])
+            out.append(pm * pn / np.sqrt((2.0 / (2 * m + 1)) * (2.0 / (2 * n + 1))))
+    return np.stack(out, 1)
+
+
+def quad_zeroed_modes(p, xi, nodes):
+    """hm[nq][nd]: u - u_{p-1} at the points for nodal values u, where
+    u_{p-1} leaves out the modes the reference flags as leading. Its
+    ModalExpansionQuad::is_leading_expansion_term (ModalExpansionQuad.cpp:
+    240-257) advances the flag index only for the 2p + 1 modes with m == p
+    or n == p, so the flags fall on mode ids 0 .. 2p: those modes are the
+    ones zeroed."""
+    Vn = quad_modal_basis(p, nodes)          # [nd][modes]: nodal values of the modes
+    coef = np.linalg.inv(Vn)                 # modal coefficients from nodal values
+    Vq = quad_modal_basis(p, xi)
+    zeroed = np.arange(2 * p + 1)
+    return np.ascontiguousarray(Vq[:, zeroed] @ coef[zeroed, :])
+
+
 def lower_degree_basis(shape, p, xi, w):
     """An orthonormal basis, under the rule (xi, w), of the polynomials of
     total degree < p on the reference simplex: [nq][npm] (Cholesky of the
@@ -300,7 +334,9 @@ def lower_degree_basis(shape, p, xi, w):
     return np.ascontiguousarray(np.linalg.solve(Lc, M.T).T)
 
 
-def _iterate_threads(problem, u, cfl, n_iter, threads, dir_nodes=None, dir_values=None):
+def _iterate_threads(problem, u, cfl, n_iter, threads, dir_nodes=None, dir_values=None, sweep_times=None):
+    """sweep_times: optional float64 array of n_iter entries that receives
+    the duration of every iteration in seconds."""
     u = np.array(u, dtype=np.float64).reshape(problem.nn, problem.m)
     res = np.zeros((problem.nn, problem.m))
     dt = np.zeros(problem.nn)
@@ -309,8 +345,11 @@ def _iterate_threads(problem, u, cfl, n_iter, threads, dir_nodes=None, dir_value
         dir_values = np.zeros((0, problem.m))
     dn = np.ascontiguousarray(dir_nodes, dtype=np.uint32)
     dv = np.ascontiguousarray(np.broadcast_to(dir_values, (dn.size, problem.m)), dtype=np.float64)
+    if sweep_times is not None:
+        assert sweep_times.dtype == np.float64 and sweep_times.size >= n_iter and sweep_times.flags.c_contiguous
     rc = lib().orc_iterate_threads(C.byref(problem.p), float(cfl), int(n_iter), dn.size, _p(dn, C.c_uint32), _p(dv),
-                                   _p(u), _p(res), _p(dt), int(threads))
+                                   _p(u), _p(res), _p(dt), int(threads),
+                                   None if sweep_times is None else _p(sweep_times))
     return u, res, dt, rc
 
 

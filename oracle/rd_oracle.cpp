@@ -10,6 +10,7 @@
 //   PGN                   src/solver/rdm/cellsplitters/PGN.hpp:96-290
 //   PGB (cell-wise theta) src/solver/rdm/cellsplitters/PGB.hpp:314-498
 //   PSI limiter           src/solver/rdm/cellsplitters/LFResidualLimiter.hpp:84-110
+//   PSI / LDA blend       PSI above blended with LDA by PGB's cell theta (PGB.hpp:437-477)
 //   physics               src/physics/euler/Euler2DCons.hpp:159-536, Euler3DCons.hpp:159-450,
 //                         src/physics/scalar/Burgers2D.hpp:154-275,
 //                         src/physics/scalar/RotationAdvection2D.hpp:201-336
@@ -29,12 +30,15 @@
 // Pinning status of each part is recorded by the tests that check it; the
 // PSI scheme has no reference counterpart on this path (the limiter formula
 // is the reference's LF limiter applied to the N-scheme residuals) and is
-// pinned only by its invariants (conservation, positivity).
+// pinned by the textbook P1 limiter and its invariants (conservation,
+// positivity); the PSI / LDA blend likewise, by its P1 closed form and
+// linearity preservation.
 #include "rd_oracle.h"
 
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <chrono>
 #include <thread>
 #include <vector>
 
@@ -477,7 +481,7 @@ bool cell(const orc_problem *p, const orc_block &b, uint32_t c, const double *u,
   // implicit forms: full K_n and K-_n, the Jacobian and its stabilisation part
   static thread_local Mat Kf[MAXN], Km[MAXN];
   const bool implicit = jac != nullptr;
-  if (implicit && p->scheme == ORC_PSI) return false;
+  if (implicit && (p->scheme == ORC_PSI || p->scheme == ORC_BPSI)) return false;
   const int ndm = nd * m;
   std::vector<double> jstab;
   if (implicit) {
@@ -730,6 +734,14 @@ bool cell(const orc_problem *p, const orc_block &b, uint32_t c, const double *u,
     // residuals, per equation, when |phi| > 1e-14 (LFResidualLimiter.hpp:84-110)
     for (int n = 0; n < nd; ++n)
       for (int e = 0; e < m; ++e) out.phi[n][e] = lda[n][e] + ndis[n][e];
+    // PSI / LDA blend: theta of PGB (equation 0 of the N residuals, +1e-6,
+    // PGB.hpp:437-462), then phi = (1 - theta) LDA + theta PSI
+    double theta = 0.0;
+    if (p->scheme == ORC_BPSI) {
+      double abs_phiN = 0.0;
+      for (int n = 0; n < nd; ++n) abs_phiN += std::fabs(out.phi[n][0]);
+      theta = std::fabs(flux_integral[0]) / (abs_phiN + 1.e-6);
+    }
     for (int e = 0; e < m; ++e) {
       const double phi = flux_integral[e];
       if (std::fabs(phi) > 1.e-14) {
@@ -741,6 +753,9 @@ bool cell(const orc_problem *p, const orc_block &b, uint32_t c, const double *u,
         for (int n = 0; n < nd; ++n) out.phi[n][e] = out.phi[n][e] / sum * phi;
       }
     }
+    if (p->scheme == ORC_BPSI)
+      for (int n = 0; n < nd; ++n)
+        for (int e = 0; e < m; ++e) out.phi[n][e] = (1.0 - theta) * lda[n][e] + theta * out.phi[n][e];
   }
   double wmax = ws[0];
   for (int n = 1; n < nd; ++n) wmax = std::max(wmax, ws[n]);
@@ -1150,17 +1165,28 @@ int orc_art_visc(const orc_problem *p, const double *u, const orc_av_rule *rules
         for (int n = 0; n < b.nd; ++n) s += r.V[q * b.nd + n] * u[static_cast<size_t>(cn[n]) * m];
         uq[q] = s;
       }
-      for (int k = 0; k < r.npm; ++k) {
-        double s = 0.0;
-        for (int q = 0; q < r.nq; ++q) s += r.w[q] * r.psi[q * r.npm + k] * uq[q];
-        coef[k] = s;
-      }
       double nom = 0.0, denom = 0.0;
-      for (int q = 0; q < r.nq; ++q) {
-        double pr = 0.0;
-        for (int k = 0; k < r.npm; ++k) pr += coef[k] * r.psi[q * r.npm + k];
-        nom += wj[q] * (uq[q] - pr) * (uq[q] - pr);
-        denom += wj[q] * uq[q] * uq[q];
+      if (r.hm) {
+        // quadrilaterals: u - u_{p-1} is the sum of the zeroed modal terms,
+        // a fixed table applied to the nodal values
+        for (int q = 0; q < r.nq; ++q) {
+          double d = 0.0;
+          for (int n = 0; n < b.nd; ++n) d += r.hm[q * b.nd + n] * u[static_cast<size_t>(cn[n]) * m];
+          nom += wj[q] * d * d;
+          denom += wj[q] * uq[q] * uq[q];
+        }
+      } else {
+        for (int k = 0; k < r.npm; ++k) {
+          double s = 0.0;
+          for (int q = 0; q < r.nq; ++q) s += r.w[q] * r.psi[q * r.npm + k] * uq[q];
+          coef[k] = s;
+        }
+        for (int q = 0; q < r.nq; ++q) {
+          double pr = 0.0;
+          for (int k = 0; k < r.npm; ++k) pr += coef[k] * r.psi[q * r.npm + k];
+          nom += wj[q] * (uq[q] - pr) * (uq[q] - pr);
+          denom += wj[q] * uq[q] * uq[q];
+        }
       }
       const double Se = (denom <= 1.e-10) ? 0.0 : nom / denom;
       const double se = std::log(Se) / std::log(10.);
@@ -1212,7 +1238,7 @@ int orc_iterate(const orc_problem *p, double cfl, int n_iter, uint32_t nb_dirich
 // The boundary conditions and the nodal step follow as in orc_iterate. A
 // timing baseline: its sums differ from orc_iterate's in rounding only.
 int orc_iterate_threads(const orc_problem *p, double cfl, int n_iter, uint32_t nb_dirichlet, const uint32_t *dnodes,
-                        const double *dvalues, double *u, double *res, double *dt, int threads)
+                        const double *dvalues, double *u, double *res, double *dt, int threads, double *sweep_s)
 {
   const int m = neq_of(p->physics);
   if (m < 0 || threads < 1) return -1;
@@ -1227,6 +1253,7 @@ int orc_iterate_threads(const orc_problem *p, double cfl, int n_iter, uint32_t n
   std::vector<int> trc(threads, 0);
   int rc = 0;
   for (int it = 0; it < n_iter; ++it) {
+    const auto t_start = std::chrono::steady_clock::now();
     std::vector<std::thread> pool;
     for (int t = 0; t < threads; ++t)
       pool.emplace_back([&, t]() {
@@ -1276,6 +1303,8 @@ int orc_iterate_threads(const orc_problem *p, double cfl, int n_iter, uint32_t n
     for (uint32_t i = 0; i < p->nn; ++i) dt[i] = cfl * vdual[i] / ws[i];
     for (uint32_t i = 0; i < p->nn; ++i)
       for (int e = 0; e < m; ++e) u[static_cast<size_t>(i) * m + e] -= dt[i] * res[static_cast<size_t>(i) * m + e];
+    if (sweep_s)
+      sweep_s[it] = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count();
   }
   return rc;
 }

@@ -20,7 +20,7 @@ extern "C" {
 /* ORC_ROTATION2D: RotationAdvection2D<AroundOrigin2D>, the advection velocity
  * (y, -x) at every point (RotationAdvection2D.hpp:33-45, 201-224). */
 enum { ORC_ADVECTION2D = 0, ORC_BURGERS2D = 1, ORC_EULER2D = 2, ORC_EULER3D = 3, ORC_ROTATION2D = 4 };
-enum { ORC_LDA = 0, ORC_N = 1, ORC_B = 2, ORC_PSI = 3 };
+enum { ORC_LDA = 0, ORC_N = 1, ORC_B = 2, ORC_PSI = 3, ORC_BPSI = 4 };
 
 /* One block of cells of a single element type. Tables row-major:
  * w[nq], dVg[dim][nq][ng], V[nq][nd], dV[dim][nq][nd], dVf[dim][nq][nf],
@@ -140,18 +140,24 @@ typedef struct orc_av_rule
 {
   int p, nq, npm;
   const double *w, *V, *dVg, *psi;
+  /* quadrilaterals: hm[nq][nd], the modal terms the reference zeroes in
+   * V_modal_p_m_1 evaluated at the points, applied to the nodal values
+   * (u - u_{p-1} at q = sum_n hm[q][n] u_n); null on simplices */
+  const double *hm;
 } orc_av_rule;
 /* ArtificialViscosity::compute_artificial_viscosity_nodal
- * (ArtificialViscosity.hpp:371-560) on simplices: nu[N]. */
+ * (ArtificialViscosity.hpp:371-560): nu[N]. */
 int orc_art_visc(const orc_problem *p, const double *u, const orc_av_rule *rules, const double *const *p1w,
                  const int *nvert, double kappa, double factor, double *nu);
 
 /* orc_iterate with the cell loop split over `threads` threads, each summing
  * into its own nodal arrays (added in thread order afterwards): the
- * multi-core timing baseline. No norms. */
+ * multi-core timing baseline. No norms. sweep_s (n_iter doubles, or null)
+ * receives the steady_clock duration of every iteration (cell loop,
+ * boundary conditions, dt and update). */
 int orc_iterate_threads(const orc_problem *p, double cfl, int n_iter, uint32_t nb_dirichlet,
                         const uint32_t *dirichlet_nodes, const double *dirichlet_values, double *u, double *res,
-                        double *dt, int threads);
+                        double *dt, int threads, double *sweep_s);
 
 /* norm_L2 per field: sqrt(sum_i r_i^2 / N) (MeshFunctionNorm.hpp). */
 void orc_norm_L2(const double *f, uint32_t n, int m, double *out);

@@ -25,7 +25,7 @@ STATUS_NAMES = {
 TRIAG, QUAD, TETRA = 0, 1, 2
 EQUIDIST, WARPBLEND = 0, 1
 ADVECTION2D, BURGERS2D, EULER2D, EULER3D, ROTATION2D = 0, 1, 2, 3, 4
-LDA, N, B, PSI = 0, 1, 2, 3
+LDA, N, B, PSI, BPSI = 0, 1, 2, 3, 4
 SOLUTION, RESIDUAL, TIME_STEP, WAVE_SPEED, DUAL_VOLUME, BLEND_COEFF, ART_VISC = 0, 1, 2, 3, 4, 5, 6
 BLEND_LF = 0
 OPT_SCATTER, OPT_PRECISION, OPT_BLOCK, OPT_TIMING, OPT_PATCH_CELLS, OPT_REFRESH, OPT_CHAIN = 0, 1, 2, 3, 4, 5, 6
@@ -124,7 +124,7 @@ def lib() -> C.CDLL:
         "pdrd_add_weak_bc": [P, I, U32, PU32, PD, I, I],
         "pdrd_weak_bc_info": [P, I, PI64],
         "pdrd_weak_bc_points": [P, I, PD],
-        "pdrd_set_weak_bc_state": [P, I, PD],
+        "pdrd_set_weak_bc_state": [P, I, PD, SZ],
         "pdrd_batch_create": [C.POINTER(P)],
         "pdrd_batch_destroy": [P],
         "pdrd_batch_set_graphs": [P, C.c_int],

@@ -6,7 +6,8 @@
 //   geometry metric (GeometryMetric.hpp:366-425, 3D 487-560)
 //   P+1 flux metric (FluxSpaceMetric.hpp:562-743)
 //   element scheme  (PGLDA.hpp:109-311, PGN.hpp:96-290, PGB.hpp:314-498,
-//                    PSI limiter LFResidualLimiter.hpp:84-110)
+//                    PSI limiter LFResidualLimiter.hpp:84-110, and the PSI /
+//                    LDA blend by PGB's cell theta, PGB.hpp:437-477)
 //   scatter res[v] += Phi_v, lambda[v] += |T| max lambda
 //           (PGRDMExplicitCellWorker.hpp:546-567)
 // The division by the dual volume is done once per node in the nodal pass.
@@ -72,10 +73,6 @@ struct TabOffsets
   static constexpr int dVT = dVfT + NF * D * NQ;     // [NQ][D][ND]
   static constexpr int dVgT = dVT + NQ * D * ND;     // [NQ][D][NG]
   static constexpr int dev_size = dVgT + NQ * D * NG;
-  // then a float copy of entries [0, dev_size) (same offsets), read by the
-  // mixed-precision cells
-  static constexpr int f32 = dev_size;
-  static constexpr int dev_size_all = dev_size + (dev_size + 1) / 2;
 };
 
 // Workgroups are dealt round-robin to the 8 XCDs; give each XCD one
@@ -231,12 +228,7 @@ struct NodalGlobal
 // doubles of LDS (stride `stride` between entries) that carries the flux
 // divergence from the flux-node loop to the quadrature loop.
 //
-// PREC 0: everything in FP64. PREC 1 (mixed): the interpolation of the state
-// to the flux nodes and quadrature points, the fluxes (as differences from a
-// nodal reference state, see flux_delta) and their divergence, and the
-// upwind parameters K+ are evaluated in FP32; the sums S = sum K+, the solves
-// with S, the element residuals, wave speeds, and everything after them
-// (nodal sums, time step, update, norms) stay FP64. PREC 2: FP64 with the
+// PREC 0: everything in FP64. PREC 2: FP64 with the
 // nodal input `nodal` -- for the B scheme the external blending coefficient
 // (set_blending_coeff, PGB.hpp:141-148, 467-477), for the LDA scheme the
 // artificial viscosity of the shock-capturing term (PGLDA.hpp:245-256).
@@ -255,20 +247,6 @@ __device__ __forceinline__ void cell_residual(const double *__restrict__ tab, co
   constexpr int M = PhysTraits<PH>::M;
   static_assert(PhysTraits<PH>::DIM == D, "physics / element dimension mismatch");
   using O = TabOffsets<D, NG, ND, NF, NQ>;
-  using TK = std::conditional_t<PREC == 1, float, double>;  // type of the FP32-able parts
-  const float *__restrict__ tabf = reinterpret_cast<const float *>(tab + O::f32);
-  // mixed: reference state and nodal differences
-  double ub[M];
-  float dU[PREC == 1 ? ND : 1][M];
-  if constexpr (PREC == 1) {
-#pragma unroll
-    for (int k = 0; k < M; ++k) ub[k] = U[0][k];
-#pragma unroll
-    for (int n = 0; n < ND; ++n)
-#pragma unroll
-      for (int k = 0; k < M; ++k) dU[n][k] = static_cast<float>(U[n][k] - ub[k]);
-  }
-
   // ---- geometry ----------------------------------------------------------
   constexpr int NJ = AFFINE ? 1 : NQ;
   double Jinv[NJ][D][D];
@@ -292,60 +270,7 @@ __device__ __forceinline__ void cell_residual(const double *__restrict__ tab, co
   // The flux-node loop stays rolled: each trip reads one row of Vsf and one
   // [D][NQ] slab of dVf with scalar loads, so the table never has to be held
   // in (and spilled from) scalar registers as a whole.
-  if constexpr (PREC == 1) {
-    DeltaBase<PH, float> base;
-    delta_base<PH, float>(ub, pp, base);
-    float Jf[AFFINE ? 1 : NQ][D][D];
-#pragma unroll
-    for (int q = 0; q < (AFFINE ? 1 : NQ); ++q)
-#pragma unroll
-      for (int d = 0; d < D; ++d)
-#pragma unroll
-        for (int xi = 0; xi < D; ++xi) Jf[q][d][xi] = static_cast<float>(Jinv[q][d][xi]);
-    float divf[NQ][M];
-#pragma unroll
-    for (int q = 0; q < NQ; ++q)
-#pragma unroll
-      for (int k = 0; k < M; ++k) divf[q][k] = 0.0f;
-#pragma unroll 1
-    for (int f = 0; f < NF; ++f) {
-      const float *vs = tabf + O::Vsf + f * ND;
-      const float *dvf = tabf + O::dVfT + f * D * NQ;
-      float ds[M];
-#pragma unroll
-      for (int k = 0; k < M; ++k) {
-        float t = 0.0f;
-#pragma unroll
-        for (int n = 1; n < ND; ++n) t += vs[n] * dU[n][k];  // dU[0] = 0
-        ds[k] = t;
-      }
-      float F[D][M];
-      flux_delta<PH, float>(base, ds, pp, F, bad);
-#pragma unroll
-      for (int q = 0; q < NQ; ++q) {
-        const int jq = AFFINE ? 0 : q;
-        float b[D];
-#pragma unroll
-        for (int d = 0; d < D; ++d) {
-          float t = 0.0f;
-#pragma unroll
-          for (int xi = 0; xi < D; ++xi) t += Jf[jq][d][xi] * dvf[xi * NQ + q];
-          b[d] = t;
-        }
-#pragma unroll
-        for (int k = 0; k < M; ++k) {
-          float t = divf[q][k];
-#pragma unroll
-          for (int d = 0; d < D; ++d) t += b[d] * F[d][k];
-          divf[q][k] = t;
-        }
-      }
-    }
-#pragma unroll
-    for (int q = 0; q < NQ; ++q)
-#pragma unroll
-      for (int k = 0; k < M; ++k) lds_div[(q * M + k) * stride] = static_cast<double>(divf[q][k]);
-  } else {
+  {
   double divr[NQ][M];
 #pragma unroll
   for (int q = 0; q < NQ; ++q)
@@ -502,10 +427,10 @@ __device__ __forceinline__ void cell_residual(const double *__restrict__ tab, co
     }
 
     if constexpr (M == 1) {
-      TK vel0, vel1;
+      double vel0, vel1;
       if constexpr (PH == PH_ADVECTION2D) {
-        vel0 = TK(pp.p0);
-        vel1 = TK(pp.p1);
+        vel0 = double(pp.p0);
+        vel1 = double(pp.p1);
       } else if constexpr (PH == PH_ROTATION2D) {
         // the velocity (y, -x) at the quadrature point's coordinates Vg X
         double xq[2] = {0.0, 0.0};
@@ -513,15 +438,8 @@ __device__ __forceinline__ void cell_residual(const double *__restrict__ tab, co
         for (int g = 0; g < NG; ++g)
 #pragma unroll
           for (int d = 0; d < 2; ++d) xq[d] += tab[O::Vg + q * NG + g] * X[g][d];
-        vel0 = TK(xq[1]);
-        vel1 = TK(-xq[0]);
-      } else if constexpr (PREC == 1) {
-        const float *vq = tabf + O::V + q * ND;
-        float t = 0.0f;
-#pragma unroll
-        for (int n = 1; n < ND; ++n) t += vq[n] * dU[n][0];
-        vel0 = static_cast<float>(ub[0]) + t;
-        vel1 = 1.0f;
+        vel0 = double(xq[1]);
+        vel1 = double(-xq[0]);
       } else {
         double uq = 0.0;
 #pragma unroll
@@ -530,16 +448,15 @@ __device__ __forceinline__ void cell_residual(const double *__restrict__ tab, co
         vel1 = 1.0;
       }
       // k_n = a . grad(phi_n) = sum_xi c_xi dphi_n/dxi with c_xi = sum_x a_x dxi/dx
-      const TK *dvk = PREC == 1 ? reinterpret_cast<const TK *>(tabf + O::dVT + q * D * ND)
-                                : reinterpret_cast<const TK *>(dvq);
-      TK c[D];
+      const double *dvk = dvq;
+      double c[D];
 #pragma unroll
-      for (int xi = 0; xi < D; ++xi) c[xi] = vel0 * TK(Jq[0][xi]) + vel1 * TK(Jq[1][xi]);
+      for (int xi = 0; xi < D; ++xi) c[xi] = vel0 * double(Jq[0][xi]) + vel1 * double(Jq[1][xi]);
       double kp[ND];
       double S = 0.0, SU = 0.0;
 #pragma unroll
       for (int n = 0; n < ND; ++n) {
-        TK knt = TK(0);
+        double knt = double(0);
 #pragma unroll
         for (int xi = 0; xi < D; ++xi) knt += c[xi] * dvk[xi * ND + n];
         const double kn = knt;
@@ -557,30 +474,18 @@ __device__ __forceinline__ void cell_residual(const double *__restrict__ tab, co
         if constexpr (NDIS) dis[n][0] += wj * (kp[n] * (Uq(n, 0) - ub));
       }
     } else {
-      TK uq[M];
-      if constexpr (PREC == 1) {
-        const float *vq = tabf + O::V + q * ND;
+      double uq[M];
 #pragma unroll
-        for (int k = 0; k < M; ++k) {
-          float t = 0.0f;
+      for (int k = 0; k < M; ++k) {
+        double t = 0.0;
 #pragma unroll
-          for (int n = 1; n < ND; ++n) t += vq[n] * dU[n][k];
-          uq[k] = static_cast<float>(ub[k]) + t;
-        }
-      } else {
-#pragma unroll
-        for (int k = 0; k < M; ++k) {
-          double t = 0.0;
-#pragma unroll
-          for (int n = 0; n < ND; ++n) t += tab[O::V + q * ND + n] * Uq(n, k);
-          uq[k] = t;
-        }
+        for (int n = 0; n < ND; ++n) t += tab[O::V + q * ND + n] * Uq(n, k);
+        uq[k] = t;
       }
-      const EulerState<TK, D> st = euler_state<TK, D>(uq, TK(pp.p0));
+      const EulerState<double, D> st = euler_state<double, D>(uq, double(pp.p0));
       bad = bad || st.bad;
       const double gm1 = pp.p0 - 1.0;
-      const TK *dvk = PREC == 1 ? reinterpret_cast<const TK *>(tabf + O::dVT + q * D * ND)
-                                : reinterpret_cast<const TK *>(dvq);
+      const double *dvk = dvq;
       // Compact K+ of every node. With the acoustic pairs written as
       //   r+- = (rho / 2a) (r0 +- a e_n),  l+- = (l0v +- a f_n) / (rho a),
       //   r0 = (1, V, H), e_n = (0, n, V.n), l0v = (gm1 |V|^2 / 2, -gm1 V, gm1),
@@ -593,39 +498,39 @@ __device__ __forceinline__ void cell_residual(const double *__restrict__ tab, co
       // T = sum sg n n^T.
       struct NodeK
       {
-        TK l0, sg, dl, um, nh[D], emax;
+        double l0, sg, dl, um, nh[D], emax;
       };
       // the K+ parameters of node n from the physical gradient of its basis
       // function (recomputed where keeping them for all nodes costs spills)
       auto node_k = [&](const int n) {
         NodeK k;
-        TK g[D];
+        double g[D];
 #pragma unroll
         for (int x = 0; x < D; ++x) {
-          TK t = TK(0);
+          double t = double(0);
 #pragma unroll
-          for (int xi = 0; xi < D; ++xi) t += TK(Jq[x][xi]) * dvk[xi * ND + n];
+          for (int xi = 0; xi < D; ++xi) t += double(Jq[x][xi]) * dvk[xi * ND + n];
           g[x] = t;
         }
-        TK nn = TK(0);
+        double nn = double(0);
 #pragma unroll
         for (int x = 0; x < D; ++x) nn += g[x] * g[x];
-        const TK norm = sqrt(nn);
-        const bool tiny = (D == 2) ? (norm < TK(1.e-9)) : (norm == TK(0));
-        const TK inorm = tiny ? TK(0) : TK(1) / norm;
-        TK u_n = TK(0);
+        const double norm = sqrt(nn);
+        const bool tiny = (D == 2) ? (norm < double(1.e-9)) : (norm == double(0));
+        const double inorm = tiny ? double(0) : double(1) / norm;
+        double u_n = double(0);
 #pragma unroll
         for (int x = 0; x < D; ++x) {
           k.nh[x] = g[x] * inorm;
           u_n += st.V[x] * k.nh[x];
         }
         k.um = u_n;
-        const TK e0 = norm * u_n, ep = norm * (u_n + st.a), em = norm * (u_n - st.a);
+        const double e0 = norm * u_n, ep = norm * (u_n + st.a), em = norm * (u_n - st.a);
         k.emax = fmax(fabs(e0), fmax(fabs(ep), fabs(em)));
-        const TK p0 = fmax(TK(0), e0);
-        const TK dpn = tiny ? TK(0) : fmax(TK(0), ep) - p0;
-        const TK dmn = tiny ? TK(0) : fmax(TK(0), em) - p0;
-        k.l0 = tiny ? TK(0) : p0;
+        const double p0 = fmax(double(0), e0);
+        const double dpn = tiny ? double(0) : fmax(double(0), ep) - p0;
+        const double dmn = tiny ? double(0) : fmax(double(0), em) - p0;
+        k.l0 = tiny ? double(0) : p0;
         k.sg = dpn + dmn;
         k.dl = dpn - dmn;
         return k;
@@ -644,15 +549,14 @@ __device__ __forceinline__ void cell_residual(const double *__restrict__ tab, co
       for (int n = 0; n < ND; ++n) {
         const NodeK k = node_k(n);
         if constexpr (KEEP) nk[n] = k;
-        lmax = fmax(lmax, static_cast<double>(k.emax));
+        lmax = fmax(lmax, k.emax);
         sl0 += k.l0;
         ssg += k.sg;
 #pragma unroll
         for (int x = 0; x < D; ++x) {
-          // FP64 products: S must be the sum of the K+ that kplus applies
-          w[x] += double(k.dl) * double(k.nh[x]);
+          w[x] += k.dl * k.nh[x];
 #pragma unroll
-          for (int y = 0; y < D; ++y) T[x][y] += double(k.sg) * double(k.nh[x]) * double(k.nh[y]);
+          for (int y = 0; y < D; ++y) T[x][y] += k.sg * k.nh[x] * k.nh[y];
         }
       }
       const double a = st.a;
@@ -826,6 +730,15 @@ __device__ __forceinline__ void cell_residual(const double *__restrict__ tab, co
 #pragma unroll
       for (int k = 0; k < M; ++k) phi[n][k] = lda[n][k] + theta * dis[n][k];
   } else {
+    // PSI: the x+ limiter of the N residuals, per equation; the PSI / LDA
+    // blend then takes (1 - theta) LDA + theta PSI with PGB's cell theta
+    double theta = 1.0;
+    if constexpr (SC == SC_BPSI) {
+      double an = 0.0;
+#pragma unroll
+      for (int n = 0; n < ND; ++n) an += fabs(lda[n][0] + dis[n][0]);
+      theta = fabs(fint[0]) / (an + 1.e-6);
+    }
 #pragma unroll
     for (int k = 0; k < M; ++k) {
       const double tot = fint[k];
@@ -839,7 +752,10 @@ __device__ __forceinline__ void cell_residual(const double *__restrict__ tab, co
         sx += xs[n];
       }
 #pragma unroll
-      for (int n = 0; n < ND; ++n) phi[n][k] = lim ? xs[n] / sx * tot : lda[n][k] + dis[n][k];
+      for (int n = 0; n < ND; ++n) {
+        const double psi = lim ? xs[n] / sx * tot : lda[n][k] + dis[n][k];
+        phi[n][k] = (SC == SC_BPSI) ? (1.0 - theta) * lda[n][k] + theta * psi : psi;
+      }
     }
   }
 
@@ -948,7 +864,14 @@ struct PatchArgs
   const double *__restrict__ vol;           // [N]
   const int *__restrict__ dir_index;        // [N] row in dir_vals or -1
   const double *__restrict__ dir_vals;      // [ndir][M]
-  const double *__restrict__ corr;          // [N][M] added to the residual in the update, or null
+  // linear sweeps of the mixed-precision form (PREC 4): the element
+  // Jacobians of the block in FP32, [ND M][ND M][ne] (entry-major, patch
+  // order), and the linearisation point: its state u_ref [N][M], its FP64
+  // residual lin_res [N][M] and time step lin_dt [N]
+  const float *__restrict__ jlin;
+  const double *__restrict__ u_ref;
+  const double *__restrict__ lin_res;
+  const double *__restrict__ lin_dt;
   double *__restrict__ res;                 // [N][M]
   double *__restrict__ dt;                  // [N]
   double *__restrict__ ws;                  // [N]
@@ -986,18 +909,18 @@ struct PatchSize
 // Waves per SIMD the register allocation of a patch kernel must leave room
 // for: 5 for the scalar affine P1 / P2 cells (at most 96 VGPRs, measured
 // faster than 4 waves); 2 for the 2D Euler cells of up to 4 nodes (little or
-// no scratch, measured faster than 1 wave) and for the mixed-precision LDA
-// cells of the 3D Euler P1 tetrahedra (8 bytes of scratch; 1.4x faster than
-// at 1 wave); the others keep the compiler's choice (the FP64 3D Euler cells
-// run as fast at 1 wave, where capping them at 2 spills a few hundred bytes
-// per thread to scratch memory, as do the mixed N / B / PSI ones; the 2D
-// Euler P2 / Q2 / P3 cells spill 250-1600 bytes per thread at 2 waves).
+// no scratch, measured faster than 1 wave); the others keep the compiler's
+// choice (the FP64 3D Euler cells run as fast at 1 wave, where capping them
+// at 2 spills a few hundred bytes per thread to scratch memory; the 2D Euler
+// P2 / Q2 / P3 cells spill 250-1600 bytes per thread at 2 waves). The linear
+// sweeps (PREC 4, J delta per cell) are bound by the Jacobians' bandwidth and
+// keep 4 waves (2 for blocks of more than 24 unknowns).
 template <int M, int ND, bool AFFINE, int SC, int PREC>
 struct PatchMinWaves
 {
   static constexpr int value =
-      (M == 1 && ND <= 6 && AFFINE) ? 5
-                                    : (((M == 4 && ND <= 4) || (M == 5 && ND <= 4 && SC == 0 && PREC == 1)) ? 2 : 1);
+      (PREC == 4) ? (ND * M <= 24 ? 4 : 2)
+                  : ((M == 1 && ND <= 6 && AFFINE) ? 5 : ((M == 4 && ND <= 4) ? 2 : 1));
 };
 
 // Dynamic LDS of one patch workgroup of P cells, in doubles: node state
@@ -1023,10 +946,24 @@ __host__ __device__ constexpr size_t patch_lds_doubles(int maxl, int P)
 }
 
 // With a nodal input (PREC 2) its values follow the work region, one per
-// local node; with a source field (PREC 3), M per local node.
+// local node; with a source field (PREC 3), M per local node. A linear sweep
+// (PREC 4) has no divergence slices: its work region holds the pair sums
+// and the norm reduction, followed by the state differences delta of the
+// patch's nodes, [maxl][M] floats.
+template <int NQ, int ND, int M>
+__host__ __device__ constexpr size_t patch_lin_work_doubles(int P)
+{
+  const size_t pairs = static_cast<size_t>(P) * ND * (M + 1);
+  const size_t red = static_cast<size_t>(P) * (M + 1);
+  return pairs > red ? pairs : red;
+}
+
 template <int NQ, int ND, int M, int PREC>
 __host__ __device__ constexpr size_t patch_lds_doubles_prec(int maxl, int P)
 {
+  if (PREC == 4)
+    return static_cast<size_t>(maxl) * M + patch_coff_doubles(maxl) + patch_lin_work_doubles<NQ, ND, M>(P) +
+           (static_cast<size_t>(maxl) * M + 1) / 2;
   return patch_lds_doubles<NQ, ND, M>(maxl, P) +
          (PREC == 2 ? static_cast<size_t>(maxl) : (PREC == 3 ? static_cast<size_t>(maxl) * M : 0));
 }
@@ -1056,22 +993,23 @@ __device__ __forceinline__ void sum_rows(const double *__restrict__ rows, uint32
   }
 }
 
-// The pseudo-time step of node g: residual r = sum / V (+ the correction),
-// dt = CFL V / sum lambda, new state = (strong Dirichlet value or u_old) -
-// dt r. Both the in-kernel finish and the deferred one use it, so they round
-// alike.
+// The pseudo-time step of node g: residual r = sum / V, dt = CFL V / sum
+// lambda, new state = (strong Dirichlet value or u_old) - dt r. In a linear
+// sweep (lin_res set) the sums are those of J delta and r = lin_res + sum / V,
+// dt = lin_dt: the residual linearised about u_ref with the time step there.
+// Both the in-kernel finish and the deferred one use it, so they round alike.
 template <int M>
 __device__ __forceinline__ void node_step(const PatchArgs &a, uint32_t g, const double *acc, const double *u_old,
                                           double (&r)[M], double (&un)[M], double &d, int &di)
 {
   const double v = a.vol[g];
   const double iv = 1.0 / v;
-  d = a.cfl * v / acc[M];
+  d = a.lin_res ? a.lin_dt[g] : a.cfl * v / acc[M];
   di = a.dir_index[g];
 #pragma unroll
   for (int k = 0; k < M; ++k) {
     const size_t j = static_cast<size_t>(g) * M + k;
-    r[k] = a.corr ? acc[k] * iv + a.corr[j] : acc[k] * iv;
+    r[k] = a.lin_res ? acc[k] * iv + a.lin_res[j] : acc[k] * iv;
     const double u0 = (di >= 0) ? a.dir_vals[static_cast<size_t>(di) * M + k] : u_old[k];
     un[k] = u0 - d * r[k];
   }
@@ -1152,16 +1090,19 @@ __device__ __forceinline__ void patch_body(const PatchArgs &a, uint32_t pl, doub
   uint16_t lc[ND];
   if (active) {
     const uint32_t e = e0 + tid;
+    if constexpr (PREC != 4) {
 #pragma unroll
-    for (int g = 0; g < NG; ++g) {
-      const size_t v = a.gconn[static_cast<size_t>(g) * a.ne + e];
+      for (int g = 0; g < NG; ++g) {
+        const size_t v = a.gconn[static_cast<size_t>(g) * a.ne + e];
 #pragma unroll
-      for (int d = 0; d < D; ++d) X[g][d] = a.coords[v * D + d];
+        for (int d = 0; d < D; ++d) X[g][d] = a.coords[v * D + d];
+      }
     }
 #pragma unroll
     for (int n = 0; n < ND; ++n) lc[n] = a.lconn[static_cast<size_t>(n) * a.ne + e];
   }
   double *nodal_l = work + patch_work_doubles<NQ, ND, M>(PATCH);  // PREC 2 and 3 only
+  float *delta_l = reinterpret_cast<float *>(work + patch_lin_work_doubles<NQ, ND, M>(PATCH));  // PREC 4 only
   // (no global store before the cell work: the table loads stay scalar)
   const int ninner = static_cast<int>(a.pn_inner[p]);
   for (int l = tid; l < nl; l += PATCH) {
@@ -1183,6 +1124,10 @@ __device__ __forceinline__ void patch_body(const PatchArgs &a, uint32_t pl, doub
     }
 #pragma unroll
     for (int k = 0; k < M; ++k) ul[l * M + k] = un[k];
+    if constexpr (PREC == 4)
+#pragma unroll
+      for (int k = 0; k < M; ++k)
+        delta_l[l * M + k] = static_cast<float>(un[k] - a.u_ref[static_cast<size_t>(g) * M + k]);
     coff[l] = a.pn_coff[s0 + l];
     if constexpr (PREC == 2) nodal_l[l] = a.nodal[g];
     if constexpr (PREC == 3)
@@ -1194,7 +1139,27 @@ __device__ __forceinline__ void patch_body(const PatchArgs &a, uint32_t pl, doub
   bool bad = false;
   double phi[ND][M];
   double wave = 0.0;
-  if (active) {
+  if constexpr (PREC == 4) {
+    // linear sweep: phi = J delta with the cell's FP32 Jacobian (entry-major
+    // over the block's cells, so that every load is coalesced)
+    if (active) {
+      constexpr int NDM = ND * M;
+      float dv[NDM];
+#pragma unroll
+      for (int n = 0; n < ND; ++n)
+#pragma unroll
+        for (int k = 0; k < M; ++k) dv[n * M + k] = delta_l[(lc[n] & LC_MASK) * M + k];
+      const float *__restrict__ jc = a.jlin + (e0 + tid);
+      const size_t stride = a.ne;
+#pragma unroll
+      for (int i = 0; i < NDM; ++i) {
+        float y = 0.0f;
+#pragma unroll
+        for (int j = 0; j < NDM; ++j) y += jc[static_cast<size_t>(i * NDM + j) * stride] * dv[j];
+        phi[i / M][i % M] = static_cast<double>(y);
+      }
+    }
+  } else if (active) {
     double U[ND][M];
     int row[ND];
 #pragma unroll

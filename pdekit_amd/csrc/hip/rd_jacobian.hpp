@@ -40,6 +40,37 @@ struct JacArgs
   int *__restrict__ status;
   uint32_t ne;
   PhysParams pp;
+  // FP32 output of the mixed-precision form instead of jac / jstab: the
+  // blocks entry-major, jlin[(i * ND M + j) * ne + lin_pos[e]] (jstab_lin: the
+  // B stabilisation scratch in the same layout); phi / wave may then be null
+  float *__restrict__ jlin;
+  float *__restrict__ jstab_lin;
+  const uint32_t *__restrict__ lin_pos;  // [ne] patch position of each block position
+};
+
+// One cell's Jacobian block: row-major FP64 at d, or entry-major FP32 at f
+// with stride `stride`.
+struct JacBlock
+{
+  double *d;
+  float *f;
+  size_t stride;
+  __device__ __forceinline__ void zero(int i) const
+  {
+    if (d) d[i] = 0.0;
+    else f[i * stride] = 0.0f;
+  }
+  __device__ __forceinline__ void add(int i, double v) const
+  {
+    if (d) d[i] += v;
+    else f[i * stride] += static_cast<float>(v);
+  }
+  __device__ __forceinline__ double get(int i) const { return d ? d[i] : static_cast<double>(f[i * stride]); }
+  __device__ __forceinline__ void set(int i, double v) const
+  {
+    if (d) d[i] = v;
+    else f[i * stride] = static_cast<float>(v);
+  }
 };
 
 // Gauss-Jordan inverse with partial pivoting of a small matrix.
@@ -109,8 +140,11 @@ __global__ __launch_bounds__(64) void rd_jacobian_kernel(JacArgs a)
     nodal[n] = a.nodal ? a.nodal[v] : 0.0;
   }
   const uint32_t c = a.perm[e];
-  double *jac = a.jac + static_cast<size_t>(c) * NDM * NDM;
-  double *jst = a.jstab + static_cast<size_t>(c) * NDM * NDM;
+  const bool lin = a.jlin != nullptr;
+  const uint32_t pos = lin ? a.lin_pos[e] : 0u;
+  const JacBlock jac{lin ? nullptr : a.jac + static_cast<size_t>(c) * NDM * NDM, lin ? a.jlin + pos : nullptr, a.ne};
+  const JacBlock jst{lin ? nullptr : (a.jstab ? a.jstab + static_cast<size_t>(c) * NDM * NDM : nullptr),
+                     lin ? (a.jstab_lin ? a.jstab_lin + pos : nullptr) : nullptr, a.ne};
   // Scalar cells of up to 6 nodes accumulate their Jacobian (and the B
   // stabilisation block) in registers and store it once; the others
   // accumulate in place in the caller's block.
@@ -124,8 +158,8 @@ __global__ __launch_bounds__(64) void rd_jacobian_kernel(JacArgs a)
       for (int i = 0; i < NDM * NDM; ++i) Jsr[i] = 0.0;
   } else {
     for (int i = 0; i < NDM * NDM; ++i) {
-      jac[i] = 0.0;
-      if (SC == SC_B) jst[i] = 0.0;
+      jac.zero(i);
+      if (SC == SC_B) jst.zero(i);
     }
   }
   const bool ext_theta = (SC == SC_B) && a.nodal;
@@ -417,12 +451,12 @@ __global__ __launch_bounds__(64) void rd_jacobian_kernel(JacArgs a)
             for (int k = 0; k < M; ++k) t += B[i][k] * Kx[nn][k][j];
             P[i][j] = t;
           }
-        double *row = jac + static_cast<size_t>(mm * M) * NDM + nn * M;
+        const int row = (mm * M) * NDM + nn * M;
         if (SC != SC_N) {
 #pragma unroll
           for (int i = 0; i < M; ++i)
 #pragma unroll
-            for (int j = 0; j < M; ++j) row[i * NDM + j] += wj * P[i][j];
+            for (int j = 0; j < M; ++j) jac.add(row + i * NDM + j, wj * P[i][j]);
         } else if (nn != mm) {
 #pragma unroll
           for (int i = 0; i < M; ++i) {
@@ -431,13 +465,13 @@ __global__ __launch_bounds__(64) void rd_jacobian_kernel(JacArgs a)
             for (int j = 0; j < M; ++j) t += P[i][j] * (U[mm][j] - U[nn][j]);
             ndis[mm][i] += wj * t;
 #pragma unroll
-            for (int j = 0; j < M; ++j) row[i * NDM + j] += -wj * P[i][j];
+            for (int j = 0; j < M; ++j) jac.add(row + i * NDM + j, -wj * P[i][j]);
           }
         } else {
 #pragma unroll
           for (int i = 0; i < M; ++i)
 #pragma unroll
-            for (int j = 0; j < M; ++j) row[i * NDM + j] += wj * (Kp[mm][i][j] - P[i][j]);
+            for (int j = 0; j < M; ++j) jac.add(row + i * NDM + j, wj * (Kp[mm][i][j] - P[i][j]));
         }
       }
       if (SC == SC_B && !skip) {
@@ -452,7 +486,7 @@ __global__ __launch_bounds__(64) void rd_jacobian_kernel(JacArgs a)
               for (int k = 0; k < M; ++k) t += B[i][k] * Kp[nn][k][j];
               P[i][j] = t;
             }
-          double *row = jst + static_cast<size_t>(mm * M) * NDM + nn * M;
+          const int row = (mm * M) * NDM + nn * M;
           if (nn != mm) {
 #pragma unroll
             for (int i = 0; i < M; ++i) {
@@ -461,13 +495,13 @@ __global__ __launch_bounds__(64) void rd_jacobian_kernel(JacArgs a)
               for (int j = 0; j < M; ++j) t += P[i][j] * (U[mm][j] - U[nn][j]);
               ndis[mm][i] += wj * t;
 #pragma unroll
-              for (int j = 0; j < M; ++j) row[i * NDM + j] += -wj * P[i][j];
+              for (int j = 0; j < M; ++j) jst.add(row + i * NDM + j, -wj * P[i][j]);
             }
           } else {
 #pragma unroll
             for (int i = 0; i < M; ++i)
 #pragma unroll
-              for (int j = 0; j < M; ++j) row[i * NDM + j] += wj * (Kp[mm][i][j] - P[i][j]);
+              for (int j = 0; j < M; ++j) jst.add(row + i * NDM + j, wj * (Kp[mm][i][j] - P[i][j]));
           }
         }
       }
@@ -475,12 +509,13 @@ __global__ __launch_bounds__(64) void rd_jacobian_kernel(JacArgs a)
   }
 
   // residuals, the blend of B, wave speed
-  double *phi = a.phi + static_cast<size_t>(c) * NDM;
+  double *phi = a.phi ? a.phi + static_cast<size_t>(c) * NDM : nullptr;
   if (SC == SC_N) {
+    if (phi)
 #pragma unroll
-    for (int n = 0; n < ND; ++n)
+      for (int n = 0; n < ND; ++n)
 #pragma unroll
-      for (int k = 0; k < M; ++k) phi[n * M + k] = ndis[n][k];
+        for (int k = 0; k < M; ++k) phi[n * M + k] = ndis[n][k];
   } else if (SC == SC_B) {
     double th[ND];
     if (ext_theta) {
@@ -494,10 +529,11 @@ __global__ __launch_bounds__(64) void rd_jacobian_kernel(JacArgs a)
 #pragma unroll
       for (int n = 0; n < ND; ++n) th[n] = t0;
     }
+    if (phi)
 #pragma unroll
-    for (int n = 0; n < ND; ++n)
+      for (int n = 0; n < ND; ++n)
 #pragma unroll
-      for (int k = 0; k < M; ++k) phi[n * M + k] = lda[n][k] + th[n] * ndis[n][k];
+        for (int k = 0; k < M; ++k) phi[n * M + k] = lda[n][k] + th[n] * ndis[n][k];
     if constexpr (REG) {
       if (!skip)
 #pragma unroll
@@ -507,19 +543,20 @@ __global__ __launch_bounds__(64) void rd_jacobian_kernel(JacArgs a)
     } else if (!skip) {
       for (int i = 0; i < NDM; ++i) {
         const double t = th[i / M];
-        for (int j = 0; j < NDM; ++j) jac[i * NDM + j] += t * jst[i * NDM + j];
+        for (int j = 0; j < NDM; ++j) jac.add(i * NDM + j, t * jst.get(i * NDM + j));
       }
     }
   } else {
+    if (phi)
 #pragma unroll
-    for (int n = 0; n < ND; ++n)
+      for (int n = 0; n < ND; ++n)
 #pragma unroll
-      for (int k = 0; k < M; ++k) phi[n * M + k] = lda[n][k];
+        for (int k = 0; k < M; ++k) phi[n * M + k] = lda[n][k];
   }
   if constexpr (REG)
 #pragma unroll
-    for (int i = 0; i < NDM * NDM; ++i) jac[i] = Jr[i];
-  a.wave[c] = vol * fmax(lmax, 1.e-6);
+    for (int i = 0; i < NDM * NDM; ++i) jac.set(i, Jr[i]);
+  if (a.wave) a.wave[c] = vol * fmax(lmax, 1.e-6);
   if (bad) atomicOr(a.status, 1);
 }
 

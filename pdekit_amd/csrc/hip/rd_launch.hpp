@@ -25,8 +25,7 @@ struct KernelSet
   CellLauncher cell = nullptr;    // element per thread, global scatter
   PatchLauncher patch = nullptr;  // patch assembly
   PatchBatchLauncher patch_batch = nullptr;  // patch assembly of several blocks / methods
-  PatchLauncher patch_mixed = nullptr;       // the same with FP32 interpolation, fluxes and K+
-  PatchBatchLauncher patch_batch_mixed = nullptr;
+  PatchLauncher patch_lin = nullptr;  // linear sweep of the mixed-precision form: J delta per cell
   // with the nodal input of the scheme (LDA: artificial viscosity, B:
   // external blending coefficient); null for N and PSI, which take none
   CellLauncher cell_nodal = nullptr;
@@ -38,6 +37,7 @@ struct KernelSet
   int patch_size = 0;             // largest (default) cells per patch workgroup
   size_t (*patch_lds_bytes)(int maxl, int cells) = nullptr;
   size_t (*patch_src_lds_bytes)(int maxl, int cells) = nullptr;
+  size_t (*patch_lin_lds_bytes)(int maxl, int cells) = nullptr;
   explicit operator bool() const { return cell != nullptr; }
 };
 
@@ -101,6 +101,12 @@ size_t patch_src_lds_bytes(int maxl, int cells)
   return patch_lds_doubles_prec<NQ, ND, M, 3>(maxl, cells) * sizeof(double);
 }
 
+template <int NQ, int ND, int M>
+size_t patch_lin_lds_bytes(int maxl, int cells)
+{
+  return patch_lds_doubles_prec<NQ, ND, M, 4>(maxl, cells) * sizeof(double);
+}
+
 template <int D, int NG, int ND, int NF, int NQ, bool AFF, int PH, int SC>
 KernelSet kernel_set()
 {
@@ -108,16 +114,19 @@ KernelSet kernel_set()
   k.cell = &launch_cell<D, NG, ND, NF, NQ, AFF, PH, SC>;
   k.patch = &launch_patch<D, NG, ND, NF, NQ, AFF, PH, SC, 0>;
   k.patch_batch = &launch_patch_batch<D, NG, ND, NF, NQ, AFF, PH, SC, 0>;
-  if constexpr (PH != PH_ROTATION2D) {  // (no mixed form of the rotating velocity)
-    k.patch_mixed = &launch_patch<D, NG, ND, NF, NQ, AFF, PH, SC, 1>;
-    k.patch_batch_mixed = &launch_patch_batch<D, NG, ND, NF, NQ, AFF, PH, SC, 1>;
+  // the linear sweep depends on the block size only: one instantiation per
+  // element type and equation count (that of the advection physics for the
+  // scalar ones)
+  if constexpr ((PH == PH_ADVECTION2D || PH == PH_EULER2D || PH == PH_EULER3D) && SC == SC_LDA) {
+    k.patch_lin = &launch_patch<D, NG, ND, NF, NQ, AFF, PH, SC, 4>;
+    k.patch_lin_lds_bytes = &patch_lin_lds_bytes<NQ, ND, PhysTraits<PH>::M>;
   }
   if constexpr (SC == SC_LDA || SC == SC_B) {
     k.cell_nodal = &launch_cell<D, NG, ND, NF, NQ, AFF, PH, SC, 2>;
     k.patch_nodal = &launch_patch<D, NG, ND, NF, NQ, AFF, PH, SC, 2>;
   }
   k.patch_src = &launch_patch<D, NG, ND, NF, NQ, AFF, PH, SC, 3>;
-  if constexpr (SC != SC_PSI) k.jacobian = &launch_jacobian<D, NG, ND, NF, NQ, AFF, PH, SC>;
+  if constexpr (SC != SC_PSI && SC != SC_BPSI) k.jacobian = &launch_jacobian<D, NG, ND, NF, NQ, AFF, PH, SC>;
   k.patch_size = PatchSize<PhysTraits<PH>::M, ND>::value;
   k.patch_lds_bytes = &patch_lds_bytes<NQ, ND, PhysTraits<PH>::M>;
   k.patch_src_lds_bytes = &patch_src_lds_bytes<NQ, ND, PhysTraits<PH>::M>;
@@ -156,6 +165,7 @@ KernelSet pick_cell(const CellKey &k)
         case SC_N: return kernel_set<D, NG, ND, NF, NQ, AFF, PH, SC_N>();
         case SC_B: return kernel_set<D, NG, ND, NF, NQ, AFF, PH, SC_B>();
         case SC_PSI: return kernel_set<D, NG, ND, NF, NQ, AFF, PH, SC_PSI>();
+        case SC_BPSI: return kernel_set<D, NG, ND, NF, NQ, AFF, PH, SC_BPSI>();
       }
     }
     return KernelSet{};

@@ -52,6 +52,15 @@ struct Block
   // smoothness-indicator rule w, V, dVg, psi packed
   double *d_p1w = nullptr, *d_av = nullptr;
   int av_nq = 0, av_npm = 0;
+  // mixed precision: patch position of every cell (caller index -> position),
+  // the element Jacobians in FP32 entry-major over the patch order
+  // [(nd m)^2][ne], the B stabilisation scratch of the Jacobian kernel in the
+  // same layout, the colour position -> patch position map, and the linear
+  // sweep's kernel (the scalar / Euler instantiation of this element type)
+  std::vector<uint32_t> ppos;
+  float *d_jlin = nullptr, *d_jstab_lin = nullptr;
+  uint32_t *d_lin_pos = nullptr;
+  pdrd::PatchLauncher lin = nullptr;
 };
 
 // A weak boundary condition as the caller gives it, and on the device.
@@ -95,11 +104,21 @@ struct pdrd_method
   int precision = 0;
   std::vector<WeakSpec> weak_specs;  // weak BCs added before pdrd_initialize
   std::vector<WeakDev> weak;
-  // defect correction of the mixed cells: every `refresh` iterations
-  // corr = R64(u) - Rmixed(u), added to the mixed residual in the update
+  // mixed precision (linearised defect correction). An FP64 sweep sets the
+  // linearisation point: u_ref = u, its FP64 residual in d_res and time step
+  // in d_dt; every `refresh` iterations it also recomputes the element
+  // Jacobians (FP32, per block). The sweeps in between are linear:
+  // res = R(u_ref) + sum_e J_e (u - u_ref)_e / V, dt = dt(u_ref). Its fixed
+  // point is the FP64 steady state; the FP32 rounding scales with u - u_ref.
   int refresh = 50;
-  int since_refresh = 1 << 30;
-  double *d_corr = nullptr;
+  int since_refresh = 1 << 30;  // iterations since the Jacobians were computed
+  bool lin_valid = false;       // u_ref, d_res, d_dt and the Jacobians form a linearisation
+  double *d_uref = nullptr;
+  // with weak BCs: R(u_ref) without its facet part (the linear sweeps run the
+  // facet kernels at the current state), and per shared node its number of
+  // facet rows
+  double *d_linres = nullptr;
+  uint32_t *d_shared_extra = nullptr;
   // nodal inputs of the cell loop: the external blending coefficient of the
   // B scheme (set_blending_coeff) and the artificial viscosity of the LDA
   // scheme (set_artificial_viscosity), device [N] each, and a scratch [N]
@@ -170,11 +189,24 @@ __global__ void finalize_kernel(uint32_t nn, int m, double *__restrict__ acc_res
   acc_ws[i] = 0.0;
 }
 
-// corr -= res (corr holds the FP64 residual, res the mixed one)
-__global__ void corr_kernel(size_t n, double *__restrict__ corr, const double *__restrict__ res)
+// Residual of the linear sweeps at the linearisation point without its
+// weak-facet part (the facet kernels run in the linear sweeps too, at the
+// current state): for every shared node with extra facet rows -- the last
+// `extra[i]` of its rows [off[i], off[i + 1]) -- linres = res - sum / V.
+// linres holds a copy of res on entry.
+__global__ void lin_res_kernel(uint32_t n, int m, const uint32_t *__restrict__ nodes, const uint32_t *__restrict__ off,
+                               const uint32_t *__restrict__ extra, const double *__restrict__ rows,
+                               const double *__restrict__ vol, double *__restrict__ linres)
 {
-  const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (i < n) corr[i] -= res[i];
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n || extra[i] == 0) return;
+  const uint32_t g = nodes[i];
+  const double iv = 1.0 / vol[g];
+  for (int k = 0; k < m; ++k) {
+    double sum = 0.0;
+    for (uint32_t r = off[i + 1] - extra[i]; r < off[i + 1]; ++r) sum += rows[static_cast<size_t>(r) * (m + 1) + k];
+    linres[static_cast<size_t>(g) * m + k] -= sum * iv;
+  }
 }
 
 __global__ void dirichlet_kernel(uint32_t n, int m, const uint32_t *__restrict__ nodes,
@@ -415,8 +447,9 @@ struct Chain
 
 double *rows_buffer(const pdrd_method *h, int i) { return i ? h->d_slot_acc_b : h->d_slot_acc; }
 
+// lin: a linear sweep of the mixed-precision form (J delta about u_ref).
 pdrd::PatchArgs patch_args(const pdrd_method *h, const Block *b, int update, int outputs, int excl, double cfl,
-                           Chain c = {})
+                           Chain c = {}, bool lin = false)
 {
   pdrd::PatchArgs a{};
   if (b) {
@@ -437,7 +470,12 @@ pdrd::PatchArgs patch_args(const pdrd_method *h, const Block *b, int update, int
   a.vol = h->d_vol;
   a.dir_index = h->d_dir_index;
   a.dir_vals = h->d_dir_vals;
-  a.corr = (update && h->precision == 1 && h->refresh > 0) ? h->d_corr : nullptr;
+  if (lin) {
+    a.jlin = b ? b->d_jlin : nullptr;
+    a.u_ref = h->d_uref;
+    a.lin_res = h->d_linres ? h->d_linres : h->d_res;
+    a.lin_dt = h->d_dt;
+  }
   a.res = h->d_res;
   a.dt = h->d_dt;
   a.ws = h->d_ws;
@@ -462,26 +500,28 @@ pdrd::PatchArgs patch_args(const pdrd_method *h, const Block *b, int update, int
   return a;
 }
 
-pdrd::FixupArgs fixup_args(const pdrd_method *h, int update, int outputs, int excl, double cfl, int buffer = 0)
+pdrd::FixupArgs fixup_args(const pdrd_method *h, int update, int outputs, int excl, double cfl, int buffer = 0,
+                           bool lin = false)
 {
   pdrd::FixupArgs f{};
   f.nodes = h->d_shared_nodes;
   f.slot_off = h->d_shared_off;
   f.slot_acc = rows_buffer(h, buffer);
-  f.pa = patch_args(h, nullptr, update, outputs, excl, cfl);
+  f.pa = patch_args(h, nullptr, update, outputs, excl, cfl, Chain{}, lin);
   f.nb = h->n_shared;
   f.norm_partial = h->d_patch_partial + static_cast<size_t>(h->np_total) * (h->m + 1);
   return f;
 }
 
-// The patch kernels of every block (plain, mixed precision, or with the
-// nodal input of the scheme).
-int launch_patch_blocks(pdrd_method *h, int update, int outputs, int excl, double cfl, Chain c = {})
+// The patch kernels of every block (FP64, with the nodal input of the
+// scheme or the source field, or the linear sweep of the mixed form).
+int launch_patch_blocks(pdrd_method *h, int update, int outputs, int excl, double cfl, Chain c = {},
+                        bool lin = false)
 {
   for (Block &b : h->blocks) {
-    const pdrd::PatchArgs a = patch_args(h, &b, update, outputs, excl, cfl, c);
+    const pdrd::PatchArgs a = patch_args(h, &b, update, outputs, excl, cfl, c, lin);
     const pdrd::PatchLauncher f =
-        h->use_src ? b.ks.patch_src : (a.nodal ? b.ks.patch_nodal : (h->precision ? b.ks.patch_mixed : b.ks.patch));
+        lin ? b.lin : (h->use_src ? b.ks.patch_src : (a.nodal ? b.ks.patch_nodal : b.ks.patch));
     HIP_CHECK(f(a, b.np, b.cells, h->stream));
   }
   return PDRD_OK;
@@ -539,10 +579,10 @@ int launch_weak(pdrd_method *h, hipStream_t s, int buffer = 0)
 
 // The sums of the shared nodes (rows of row buffer `buffer`) and their nodal
 // step.
-int launch_fixup(pdrd_method *h, int update, int outputs, int excl, double cfl, int buffer = 0)
+int launch_fixup(pdrd_method *h, int update, int outputs, int excl, double cfl, int buffer = 0, bool lin = false)
 {
   if (h->n_shared == 0) return PDRD_OK;
-  const pdrd::FixupArgs f = fixup_args(h, update, outputs, excl, cfl, buffer);
+  const pdrd::FixupArgs f = fixup_args(h, update, outputs, excl, cfl, buffer, lin);
   const unsigned nb = fixup_blocks(h);
 #define FIX_CASE(MM)                                                                                    \
   case MM:                                                                                              \
@@ -574,11 +614,25 @@ int launch_patches(pdrd_method *h, int update, int outputs, int excl, double cfl
 // the previous sweep are finished in this sweep's gather when c.pend >= 0;
 // the shared-node sums of this sweep are summed here (close) or left to the
 // next sweep.
-int launch_chain_sweep(pdrd_method *h, int outputs, int excl, double cfl, Chain c, bool close)
+int launch_chain_sweep(pdrd_method *h, int outputs, int excl, double cfl, Chain c, bool close, bool lin = false)
 {
-  if (int rc = launch_patch_blocks(h, 1, outputs, excl, cfl, c)) return rc;
+  if (int rc = launch_patch_blocks(h, 1, outputs, excl, cfl, c, lin)) return rc;
   if (int rc = launch_weak(h, h->stream, c.write)) return rc;
-  return close ? launch_fixup(h, 1, outputs, excl, cfl, c.write) : PDRD_OK;
+  return close ? launch_fixup(h, 1, outputs, excl, cfl, c.write, lin) : PDRD_OK;
+}
+
+// After an FP64 sweep of the mixed form with weak BCs: linres = its residual
+// without the facet rows (of row buffer `buffer`).
+int lin_residual(pdrd_method *h, int buffer)
+{
+  const size_t nm = static_cast<size_t>(h->nn) * h->m;
+  HIP_CHECK(hipMemcpyAsync(h->d_linres, h->d_res, nm * sizeof(double), hipMemcpyDeviceToDevice, h->stream));
+  if (h->n_shared == 0) return PDRD_OK;
+  hipLaunchKernelGGL(lin_res_kernel, dim3(grid_for(h->n_shared, 256)), dim3(256), 0, h->stream, h->n_shared, h->m,
+                     h->d_shared_nodes, h->d_shared_off, h->d_shared_extra, rows_buffer(h, buffer), h->d_vol,
+                     h->d_linres);
+  HIP_CHECK(hipGetLastError());
+  return PDRD_OK;
 }
 
 int launch_dirichlet(pdrd_method *h)
@@ -653,6 +707,15 @@ int check_ready(pdrd_method *h)
   if (!h) return set_error(PDRD_EINVAL, "null method");
   if (!h->ready) return set_error(PDRD_EINVAL, "method not initialized");
   return PDRD_OK;
+}
+
+// The mixed-precision form's linearisation no longer matches the method's
+// state or inputs: the next iteration starts with an FP64 sweep and fresh
+// Jacobians.
+void lin_invalidate(pdrd_method *h)
+{
+  h->lin_valid = false;
+  h->since_refresh = 1 << 30;
 }
 
 void drop_graph(pdrd_method *h)
@@ -746,8 +809,21 @@ void free_device(pdrd_method *h)
   }
   (void)hipFree(h->d_pn_rows);
   h->d_pn_rows = nullptr;
-  (void)hipFree(h->d_corr);
-  h->d_corr = nullptr;
+  for (double **q : {&h->d_uref, &h->d_linres}) {
+    (void)hipFree(*q);
+    *q = nullptr;
+  }
+  (void)hipFree(h->d_shared_extra);
+  h->d_shared_extra = nullptr;
+  h->lin_valid = false;
+  for (Block &b : h->blocks) {
+    for (float **q : {&b.d_jlin, &b.d_jstab_lin}) {
+      (void)hipFree(*q);
+      *q = nullptr;
+    }
+    (void)hipFree(b.d_lin_pos);
+    b.d_lin_pos = nullptr;
+  }
   for (double **q : {&h->d_blend, &h->d_visc, &h->d_stab_tmp, &h->d_src}) {
     (void)hipFree(*q);
     *q = nullptr;
@@ -791,31 +867,67 @@ int enqueue_iteration(pdrd_method *h, double cfl, int outputs, int excl)
   return launch_node_pass(h, cfl, outputs, excl);
 }
 
-// Defect correction of the mixed cells: corr = R64(u) - Rmixed(u) at the
-// current state (two residual sweeps), so that the mixed iteration's fixed
-// point is the FP64 steady state.
-int refresh_correction(pdrd_method *h)
+// Set-up of the mixed-precision form at its first use: u_ref, the FP32
+// Jacobian blocks of every block in patch order, the colour position ->
+// patch position map of the Jacobian kernel, the linear sweep kernel.
+int lin_setup(pdrd_method *h)
 {
-  const size_t nm = static_cast<size_t>(h->nn) * h->m;
-  if (!h->d_corr) {
-    if (int rc = dev_alloc(h, &h->d_corr, nm)) return rc;
-    drop_graph(h);  // captured iterations must see the correction
+  if (h->d_uref) return PDRD_OK;
+  if (h->scheme == PDRD_PSI || h->scheme == PDRD_BPSI)
+    return set_error(PDRD_ENOTSUP, "the mixed-precision form needs element Jacobians (none for PSI)");
+  if (h->physics == PDRD_ROTATION2D) return set_error(PDRD_ENOTSUP, "no mixed-precision form of the rotating velocity");
+  for (Block &b : h->blocks) {
+    const int lin_phys = (h->m == 1) ? pdrd::PH_ADVECTION2D : h->physics;
+    const pdrd::KernelSet lk =
+        pdrd::find_cell(pdrd::CellKey{b.fe.shape, b.fe.sol_order, b.fe.nq, lin_phys, pdrd::SC_LDA});
+    if (!lk.patch_lin || !b.ks.jacobian) return set_error(PDRD_ENOTSUP, "no linear sweep for this element type");
+    if (lk.patch_lin_lds_bytes(b.maxl, b.cells) > 65536)
+      return set_error(PDRD_ENOTSUP, "linear sweep: patch LDS above 64 KiB");
+    b.lin = lk.patch_lin;
   }
-  const int prec = h->precision;
-  h->precision = 0;
-  int rc = launch_patches(h, 0, 0, 0, 0.0);
-  h->precision = prec;
-  if (rc) return rc;
-  HIP_CHECK(hipMemcpyAsync(h->d_corr, h->d_res, nm * sizeof(double), hipMemcpyDeviceToDevice, h->stream));
-  if ((rc = launch_patches(h, 0, 0, 0, 0.0))) return rc;
-  hipLaunchKernelGGL(corr_kernel, dim3(static_cast<unsigned>((nm + 255) / 256)), dim3(256), 0, h->stream, nm,
-                     h->d_corr, h->d_res);
-  HIP_CHECK(hipGetLastError());
+  const size_t nm = static_cast<size_t>(h->nn) * h->m;
+  if (int rc = dev_alloc(h, &h->d_uref, nm)) return rc;
+  if (!h->weak.empty())
+    if (int rc = dev_alloc(h, &h->d_linres, nm)) return rc;
+  for (Block &b : h->blocks) {
+    const size_t ndm = static_cast<size_t>(b.fe.nd) * h->m;
+    if (int rc = dev_alloc(h, &b.d_jlin, ndm * ndm * b.ne)) return rc;
+    if (h->scheme == PDRD_B && !(h->m == 1 && b.fe.nd <= 6))
+      if (int rc = dev_alloc(h, &b.d_jstab_lin, ndm * ndm * b.ne)) return rc;
+    std::vector<uint32_t> pos(b.ne);
+    for (uint32_t e = 0; e < b.ne; ++e) pos[e] = b.ppos[b.perm[e]];
+    if (int rc = dev_alloc(h, &b.d_lin_pos, pos.size())) return rc;
+    HIP_CHECK(hipMemcpy(b.d_lin_pos, pos.data(), pos.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+  }
+  return PDRD_OK;
+}
+
+// The element Jacobians at the current state, FP32 into every block's d_jlin.
+int lin_jacobians(pdrd_method *h)
+{
+  for (Block &b : h->blocks) {
+    pdrd::JacArgs a{};
+    a.tab = b.d_tab;
+    a.gconn = b.d_gconn;
+    a.sconn = b.d_sconn;
+    a.perm = b.d_lin_pos;  // (the kernel's caller-slot index; unused with the FP32 output)
+    a.coords = h->d_coords;
+    a.u = h->d_u;
+    a.nodal = nodal_input(h);
+    a.status = h->d_status;
+    a.ne = b.ne;
+    a.pp.p0 = h->params[0];
+    a.pp.p1 = h->params[1];
+    a.jlin = b.d_jlin;
+    a.jstab_lin = b.d_jstab_lin;
+    a.lin_pos = b.d_lin_pos;
+    if (b.ne) HIP_CHECK(b.ks.jacobian(a, grid_for(b.ne, 64), h->stream));
+  }
   h->since_refresh = 0;
   return PDRD_OK;
 }
 
-bool corrects(const pdrd_method *h) { return h->precision == 1 && h->refresh > 0 && h->scatter_mode == 2; }
+bool mixed(const pdrd_method *h) { return h->precision == 1 && h->refresh > 1 && h->scatter_mode == 2; }
 
 int iterate_norm(pdrd_method *h, double *norm)
 {
@@ -827,9 +939,9 @@ int iterate_norm(pdrd_method *h, double *norm)
 // Iterations of the patch assembly. Chained (PDRD_OPT_CHAIN), the
 // shared-node sums of one sweep are done in the next sweep's gather, so a
 // sweep is one launch per element type (plus the weak facet kernels); a
-// sweep closes the chain (runs the fix-up) when it is the call's last or when
-// a defect-correction refresh, which needs the complete state, comes before
-// the next one. Unchained, every sweep runs the fix-up. The sweeps are
+// sweep closes the chain (runs the fix-up) when it is the call's last or,
+// in the mixed-precision form, when the next sweep is of the other kind (FP64
+// or linear, whose nodal steps differ). Unchained, every sweep runs the fix-up. The sweeps are
 // launched directly: replaying them from captured HIP graphs measured slower
 // on MI355X (25.6 against 11.1 us per sweep of a 32 768-cell P1 mesh, the
 // launches being dearer than the kernels). With timing on, events bracket the
@@ -843,44 +955,71 @@ bool chains(const pdrd_method *h) { return h->chain >= 0 ? h->chain == 1 : h->m 
 
 int iterate_patches(pdrd_method *h, double cfl, int n_iter, int excl, double *norm)
 {
-  const bool corr = corrects(h);
+  const bool mix = mixed(h);
+  if (mix)
+    if (int rc = lin_setup(h)) return rc;
   const bool chain = chains(h);
   double cell = 0.0, node = 0.0;
+  int timed = 0;
   int pend = -1;
+  // mixed form: iteration `it` is an FP64 sweep when there is no valid
+  // linearisation, when the Jacobians are due, and at the call's last
+  // iteration (whose outputs and norm are FP64); the others are linear
+  auto fp64_at = [&](int it, int since) {
+    return !mix || it == n_iter - 1 || since >= h->refresh || (it == 0 && !h->lin_valid);
+  };
   if (!h->timing) HIP_CHECK(hipEventRecord(h->ev[3], h->stream));
   for (int it = 0; it < n_iter; ++it) {
-    if (corr && h->since_refresh >= h->refresh)
-      if (int rc = refresh_correction(h)) return rc;
+    const bool f64 = fp64_at(it, h->since_refresh);
+    const bool lin = !f64;
+    if (mix && f64) {
+      // new linearisation point: Jacobians when due, then u_ref = u; the
+      // sweep below writes R(u_ref) and dt(u_ref) as its outputs
+      if (!h->lin_valid || h->since_refresh >= h->refresh)
+        if (int rc = lin_jacobians(h)) return rc;
+      HIP_CHECK(hipMemcpyAsync(h->d_uref, h->d_u, static_cast<size_t>(h->nn) * h->m * sizeof(double),
+                               hipMemcpyDeviceToDevice, h->stream));
+    }
     const bool last = it == n_iter - 1;
-    const bool close = !chain || last || (corr && h->since_refresh + 1 >= h->refresh);
-    const int out = last ? 1 : 0;
+    const bool next_f64 = !last && fp64_at(it + 1, h->since_refresh + 1);
+    // chains run within runs of sweeps of one kind
+    const bool close = !chain || last || (mix && (f64 || next_f64));
+    const int out = (last || (mix && f64)) ? 1 : 0;
     Chain c;
     c.pend = pend;
     c.write = (pend == 0) ? 1 : 0;
     c.record = !close;
     if (h->timing) {
       HIP_CHECK(hipEventRecord(h->ev[0], h->stream));
-      if (int rc = launch_patch_blocks(h, 1, out, excl, cfl, c)) return rc;
+      if (int rc = launch_patch_blocks(h, 1, out, excl, cfl, c, lin)) return rc;
       HIP_CHECK(hipEventRecord(h->ev[1], h->stream));
       if (int rc = launch_weak(h, h->stream, c.write)) return rc;
       if (close)
-        if (int rc = launch_fixup(h, 1, out, excl, cfl, c.write)) return rc;
+        if (int rc = launch_fixup(h, 1, out, excl, cfl, c.write, lin)) return rc;
       HIP_CHECK(hipEventRecord(h->ev[2], h->stream));
       HIP_CHECK(hipEventSynchronize(h->ev[2]));
       float a = 0.f, b = 0.f;
       HIP_CHECK(hipEventElapsedTime(&a, h->ev[0], h->ev[1]));
       HIP_CHECK(hipEventElapsedTime(&b, h->ev[1], h->ev[2]));
-      cell += a;
-      node += b;
+      if (h->timing == 1 || lin) {
+        cell += a;
+        node += b;
+        ++timed;
+      }
     } else {
-      if (int rc = launch_chain_sweep(h, out, excl, cfl, c, close)) return rc;
+      if (int rc = launch_chain_sweep(h, out, excl, cfl, c, close, lin)) return rc;
     }
-    if (corr) ++h->since_refresh;
+    if (mix && f64) {
+      h->lin_valid = true;
+      if (h->d_linres)
+        if (int rc = lin_residual(h, c.write)) return rc;
+    }
+    ++h->since_refresh;
     pend = close ? -1 : c.write;
   }
   if (h->timing) {
-    h->last_cell_ms = cell / n_iter;
-    h->last_node_ms = node / n_iter;
+    h->last_cell_ms = timed ? cell / timed : 0.0;
+    h->last_node_ms = timed ? node / timed : 0.0;
   } else {
     HIP_CHECK(hipEventRecord(h->ev[1], h->stream));
     HIP_CHECK(hipEventSynchronize(h->ev[1]));
@@ -912,7 +1051,7 @@ int pdrd_create(int dim, uint32_t nb_vertices, const double *vertex_coords, uint
   if (!out || !vertex_coords) return set_error(PDRD_EINVAL, "null argument");
   if (dim != 2 && dim != 3) return set_error(PDRD_EINVAL, "dim must be 2 or 3");
   if (physics < 0 || physics > 4) return set_error(PDRD_EINVAL, "unknown physics");
-  if (scheme < 0 || scheme > 3) return set_error(PDRD_EINVAL, "unknown scheme");
+  if (scheme < 0 || scheme > 4) return set_error(PDRD_EINVAL, "unknown scheme");
   const int pdim = (physics == PDRD_EULER3D) ? 3 : 2;
   if (pdim != dim) return set_error(PDRD_EINVAL, "physics dimension does not match the mesh");
   if (nb_nodes == 0 || nb_vertices == 0) return set_error(PDRD_EINVAL, "empty mesh");
@@ -994,12 +1133,6 @@ static std::vector<double> device_tables(const pdrd_fe_desc &fe, const std::vect
   for (int q = 0; q < NQ; ++q)
     for (int xi = 0; xi < D; ++xi)
       for (int g = 0; g < NG; ++g) out.push_back(t[o_dVg + (static_cast<size_t>(xi) * NQ + q) * NG + g]);
-  // float copy of everything so far, two floats per double slot
-  const size_t n = out.size();
-  std::vector<float> f(n + (n & 1), 0.0f);
-  for (size_t i = 0; i < n; ++i) f[i] = static_cast<float>(out[i]);
-  out.resize(n + f.size() / 2);
-  std::memcpy(out.data() + n, f.data(), f.size() * sizeof(float));
   return out;
 }
 
@@ -1137,6 +1270,8 @@ int pdrd_initialize(pdrd_method_t h)
         b.np = o.np;
         b.maxl = o.maxl;
         b.cells = pin[i].patch_size;
+        b.ppos.assign(b.ne, 0);
+        for (uint32_t e = 0; e < b.ne; ++e) b.ppos[o.order[e]] = e;
         if ((rc = dev_alloc(h, &b.d_pgconn, o.gconn.size()))) { free_device(h); return rc; }
         if ((rc = dev_alloc(h, &b.d_lconn, o.lconn.size()))) { free_device(h); return rc; }
         HIP_CHECK(hipMemcpy(b.d_pgconn, o.gconn.data(), o.gconn.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
@@ -1193,6 +1328,12 @@ int pdrd_initialize(pdrd_method_t h)
       }
       // weak BCs: the row of every (facet, node) pair after the node's patch rows
       if (!fsets.empty()) {
+        std::vector<uint32_t> sx(h->n_shared);
+        for (uint32_t i = 0; i < h->n_shared; ++i) sx[i] = extra[plan.shared_nodes[i]];
+        if ((rc = up32(&h->d_shared_extra, sx))) {
+          free_device(h);
+          return rc;
+        }
         std::vector<uint32_t> next(h->nn, UINT32_MAX);
         for (uint32_t i = 0; i < h->n_shared; ++i) {
           const uint32_t v = plan.shared_nodes[i];
@@ -1292,6 +1433,7 @@ int pdrd_destroy(pdrd_method_t h)
 int pdrd_set_option(pdrd_method_t h, int option, int value)
 {
   if (!h) return set_error(PDRD_EINVAL, "null method");
+  if (option == PDRD_OPT_SCATTER || option == PDRD_OPT_REFRESH || option == PDRD_OPT_CHAIN) lin_invalidate(h);
   switch (option) {
     case PDRD_OPT_SCATTER:
       if (value < 0 || value > 2) return set_error(PDRD_EINVAL, "scatter mode is 0, 1 or 2");
@@ -1304,13 +1446,12 @@ int pdrd_set_option(pdrd_method_t h, int option, int value)
       break;
     case PDRD_OPT_PRECISION:
       if (value != 0 && value != 1) return set_error(PDRD_EINVAL, "precision is 0 (FP64) or 1 (mixed)");
-      if (value == 1 && nodal_input(h))
-        return set_error(PDRD_ENOTSUP, "the blending coefficient / artificial viscosity run with the FP64 cells");
-      if (value == 1 && h->use_src) return set_error(PDRD_ENOTSUP, "the source term runs with the FP64 cells");
       if (value == 1 && h->physics == PDRD_ROTATION2D)
         return set_error(PDRD_ENOTSUP, "the rotating advection runs with the FP64 cells");
+      if (value == 1 && (h->scheme == PDRD_PSI || h->scheme == PDRD_BPSI))
+        return set_error(PDRD_ENOTSUP, "the mixed-precision form needs element Jacobians (none for PSI)");
       h->precision = value;
-      h->since_refresh = 1 << 30;
+      lin_invalidate(h);
       break;
     case PDRD_OPT_BLOCK:
       if (value < 64 || value > 256 || (value % 64) != 0)
@@ -1318,7 +1459,7 @@ int pdrd_set_option(pdrd_method_t h, int option, int value)
       h->block_threads = value;
       break;
     case PDRD_OPT_TIMING:
-      if (value != 0 && value != 1) return set_error(PDRD_EINVAL, "timing is 0 or 1");
+      if (value < 0 || value > 2) return set_error(PDRD_EINVAL, "timing is 0, 1 or 2");
       h->timing = value;
       break;
     case PDRD_OPT_REFRESH:
@@ -1367,6 +1508,7 @@ int pdrd_synchronize(pdrd_method_t h)
 int pdrd_set_field(pdrd_method_t h, int field, const double *host)
 {
   if (int rc = check_ready(h)) return rc;
+  lin_invalidate(h);
   if (!host) return set_error(PDRD_EINVAL, "null buffer");
   size_t n;
   double *d;
@@ -1451,10 +1593,11 @@ int pdrd_weak_bc_points(pdrd_method_t h, int k, double *xq)
   return PDRD_OK;
 }
 
-int pdrd_set_weak_bc_state(pdrd_method_t h, int k, const double *state)
+int pdrd_set_weak_bc_state(pdrd_method_t h, int k, const double *state, size_t count)
 {
   if (!h) return set_error(PDRD_EINVAL, "null argument");
   if (!h->ready) return set_error(PDRD_EINVAL, "not initialized");
+  lin_invalidate(h);
   if (k < 0 || k >= static_cast<int>(h->weak.size())) return set_error(PDRD_EINVAL, "no such weak BC");
   WeakDev &w = h->weak[k];
   if (w.type != PDRD_WEAK_DIRICHLET) return set_error(PDRD_ENOTSUP, "point states are for WeakDirichlet");
@@ -1464,6 +1607,9 @@ int pdrd_set_weak_bc_state(pdrd_method_t h, int k, const double *state)
     return PDRD_OK;
   }
   const size_t n = static_cast<size_t>(w.nf) * w.nq * h->m;
+  if (count != n)
+    return set_error(PDRD_EINVAL, "state holds " + std::to_string(count) + " values, the facet points need " +
+                                      std::to_string(n) + " ([facets][points][m])");
   if (!w.d_qstate)
     if (int rc = dev_alloc(h, &w.d_qstate, n)) return rc;
   HIP_CHECK(hipMemcpyAsync(w.d_qstate, state, n * sizeof(double), hipMemcpyHostToDevice, h->stream));
@@ -1488,11 +1634,13 @@ int stab_tables(pdrd_method *h, Block &b, bool smoothness)
     HIP_CHECK(hipMemcpy(b.d_p1w, p1w.data(), p1w.size() * sizeof(double), hipMemcpyHostToDevice));
   }
   if (smoothness && !b.d_av) {
-    if (shape == pdrd::Shape::Quad)
-      return set_error(PDRD_ENOTSUP, "the smoothness indicator is built for triangles and tetrahedra");
     const pdrd::ElementTables t = pdrd::make_element_tables(shape, 1, p, std::max(2 * p, 1), ps);
     int npm = 0;
-    const std::vector<double> psi = pdrd::lower_degree_basis(shape, p, t.qp, t.w, &npm);
+    // simplices: an orthonormal lower-degree basis; quadrilaterals: the table
+    // of the zeroed modal terms (npm = 0)
+    const std::vector<double> psi = (shape == pdrd::Shape::Quad)
+                                        ? pdrd::quad_zeroed_modes(p, t.qp, t.sol_nodes)
+                                        : pdrd::lower_degree_basis(shape, p, t.qp, t.w, &npm);
     if (t.nq > pdrd::STAB_MAX_NQ || npm > pdrd::STAB_MAX_NPM)
       return set_error(PDRD_ENOTSUP, "smoothness-indicator rule too large");
     std::vector<double> pack;
@@ -1530,6 +1678,7 @@ pdrd::StabArgs stab_args(const pdrd_method *h, const Block &b)
     a.V = a.w + a.nq;
     a.dVg = a.V + static_cast<size_t>(a.nq) * a.nd;
     a.psi = a.dVg + static_cast<size_t>(h->dim) * a.nq * a.ng;
+    if (b.fe.shape == PDRD_QUAD) a.hm = a.psi;  // the zeroed modal terms [nq][nd]
   }
   return a;
 }
@@ -1561,6 +1710,7 @@ extern "C" {
 int pdrd_set_blending_coeff(pdrd_method_t h, const double *theta)
 {
   if (int rc = check_ready(h)) return rc;
+  lin_invalidate(h);
   if (theta && h->use_src)
     return set_error(PDRD_ENOTSUP, "a blending coefficient / artificial viscosity with a source field is not built");
   if (!theta) {
@@ -1568,8 +1718,6 @@ int pdrd_set_blending_coeff(pdrd_method_t h, const double *theta)
     h->use_blend = false;
     return PDRD_OK;
   }
-  if (h->precision && h->scheme == pdrd::SC_B)
-    return set_error(PDRD_ENOTSUP, "the blending coefficient runs with the FP64 cells");
   if (!h->d_blend)
     if (int rc = dev_alloc(h, &h->d_blend, h->nn)) return rc;
   HIP_CHECK(hipMemcpyAsync(h->d_blend, theta, h->nn * sizeof(double), hipMemcpyHostToDevice, h->stream));
@@ -1582,6 +1730,7 @@ int pdrd_set_blending_coeff(pdrd_method_t h, const double *theta)
 int pdrd_set_artificial_viscosity(pdrd_method_t h, const double *nu)
 {
   if (int rc = check_ready(h)) return rc;
+  lin_invalidate(h);
   if (nu && h->use_src)
     return set_error(PDRD_ENOTSUP, "a blending coefficient / artificial viscosity with a source field is not built");
   if (!nu) {
@@ -1589,8 +1738,6 @@ int pdrd_set_artificial_viscosity(pdrd_method_t h, const double *nu)
     h->use_visc = false;
     return PDRD_OK;
   }
-  if (h->precision && h->scheme == pdrd::SC_LDA)
-    return set_error(PDRD_ENOTSUP, "the artificial viscosity runs with the FP64 cells");
   if (!h->d_visc)
     if (int rc = dev_alloc(h, &h->d_visc, h->nn)) return rc;
   HIP_CHECK(hipMemcpyAsync(h->d_visc, nu, h->nn * sizeof(double), hipMemcpyHostToDevice, h->stream));
@@ -1603,12 +1750,12 @@ int pdrd_set_artificial_viscosity(pdrd_method_t h, const double *nu)
 int pdrd_set_sources(pdrd_method_t h, const double *sources)
 {
   if (int rc = check_ready(h)) return rc;
+  lin_invalidate(h);
   if (!sources) {
     h->use_src = false;
     return PDRD_OK;
   }
   if (h->scatter_mode != 2) return set_error(PDRD_ENOTSUP, "the source term runs with the patch assembly only");
-  if (h->precision) return set_error(PDRD_ENOTSUP, "the source term runs with the FP64 cells");
   if (nodal_input(h))
     return set_error(PDRD_ENOTSUP, "a source field with a blending coefficient / artificial viscosity is not built");
   for (const Block &b : h->blocks) {
@@ -1629,11 +1776,10 @@ int pdrd_compute_blending_coeff(pdrd_method_t h, int kind)
 {
   PDRD_TRY
   if (int rc = check_ready(h)) return rc;
+  lin_invalidate(h);
   if (h->use_src)
     return set_error(PDRD_ENOTSUP, "a blending coefficient / artificial viscosity with a source field is not built");
   if (kind != PDRD_BLEND_LF) return set_error(PDRD_EINVAL, "unknown blending coefficient");
-  if (h->precision && h->scheme == pdrd::SC_B)
-    return set_error(PDRD_ENOTSUP, "the blending coefficient runs with the FP64 cells");
   for (Block &b : h->blocks)
     if (int rc = stab_tables(h, b, false)) return rc;
   if (!h->d_blend)
@@ -1666,11 +1812,10 @@ int pdrd_compute_artificial_viscosity(pdrd_method_t h, double kappa, double fact
 {
   PDRD_TRY
   if (int rc = check_ready(h)) return rc;
+  lin_invalidate(h);
   if (h->use_src)
     return set_error(PDRD_ENOTSUP, "a blending coefficient / artificial viscosity with a source field is not built");
   if (!(kappa > 0.0) || !(factor >= 0.0)) return set_error(PDRD_EINVAL, "kappa must be > 0 and factor >= 0");
-  if (h->precision && h->scheme == pdrd::SC_LDA)
-    return set_error(PDRD_ENOTSUP, "the artificial viscosity runs with the FP64 cells");
   for (Block &b : h->blocks)
     if (int rc = stab_tables(h, b, true)) return rc;
   if (!h->d_visc)
@@ -1757,6 +1902,7 @@ int pdrd_element_jacobians(pdrd_method_t h, int block, double *phi, double *jac,
 int pdrd_add_strong_dirichlet(pdrd_method_t h, uint32_t n, const uint32_t *nodes, const double *values)
 {
   if (int rc = check_ready(h)) return rc;
+  lin_invalidate(h);
   if (n == 0) return PDRD_OK;
   if (!nodes || !values) return set_error(PDRD_EINVAL, "null argument");
   for (uint32_t i = 0; i < n; ++i)
@@ -1799,6 +1945,7 @@ int pdrd_add_strong_dirichlet(pdrd_method_t h, uint32_t n, const uint32_t *nodes
 int pdrd_assemble_rhs(pdrd_method_t h)
 {
   if (int rc = check_ready(h)) return rc;
+  lin_invalidate(h);
   HIP_CHECK(hipEventRecord(h->ev[0], h->stream));
   if (h->scatter_mode == 2) {
     if (int rc = launch_patches(h, 0, 0, 0, 0.0)) return rc;
@@ -1834,6 +1981,7 @@ int pdrd_assemble_lhs_and_rhs(pdrd_method_t h, double cfl)
 int pdrd_solve(pdrd_method_t h)
 {
   if (int rc = check_ready(h)) return rc;
+  lin_invalidate(h);
   hipLaunchKernelGGL(update_kernel, dim3(grid_for(h->nn, 256)), dim3(256), 0, h->stream, h->nn, h->m, h->d_dt,
                      h->d_res, h->d_u);
   HIP_CHECK(hipGetLastError());
@@ -1884,9 +2032,7 @@ int pdrd_iterate(pdrd_method_t h, double cfl, int n_iter, int exclude_dirichlet,
     if (norm) return iterate_norm(h, norm);
     return PDRD_OK;
   }
-  const bool corr = corrects(h);
-  if (corr && h->since_refresh >= h->refresh)
-    if (int rc = refresh_correction(h)) return rc;
+  if (h->precision) return set_error(PDRD_ENOTSUP, "the mixed-precision form runs with the patch assembly only");
   HIP_CHECK(hipEventRecord(h->ev[0], h->stream));
   if (n_iter > 1) {
     if (!h->graph || h->graph_cfl != cfl) {
@@ -1905,17 +2051,9 @@ int pdrd_iterate(pdrd_method_t h, double cfl, int n_iter, int exclude_dirichlet,
       if (e != hipSuccess) return hip_fail(e, "hipGraphInstantiate");
       h->graph_cfl = cfl;
     }
-    for (int it = 0; it < n_iter - 1; ++it) {
-      if (corr && h->since_refresh >= h->refresh)
-        if (int rc = refresh_correction(h)) return rc;
-      HIP_CHECK(hipGraphLaunch(h->graph, h->stream));
-      if (corr) ++h->since_refresh;
-    }
+    for (int it = 0; it < n_iter - 1; ++it) HIP_CHECK(hipGraphLaunch(h->graph, h->stream));
   }
-  if (corr && h->since_refresh >= h->refresh)
-    if (int rc = refresh_correction(h)) return rc;
   if (int rc = enqueue_iteration(h, cfl, 1, excl)) return rc;
-  if (corr) ++h->since_refresh;
   HIP_CHECK(hipEventRecord(h->ev[1], h->stream));
   HIP_CHECK(hipEventSynchronize(h->ev[1]));
   float ms = 0.f;
@@ -1960,7 +2098,7 @@ int pdrd_get_option(pdrd_method_t h, int option, int *value)
     case PDRD_OPT_SCATTER: *value = h->scatter_mode; return PDRD_OK;
     case PDRD_OPT_PRECISION: *value = h->precision; return PDRD_OK;
     case PDRD_OPT_BLOCK: *value = h->block_threads; return PDRD_OK;
-    case PDRD_OPT_TIMING: *value = h->timing ? 1 : 0; return PDRD_OK;
+    case PDRD_OPT_TIMING: *value = h->timing; return PDRD_OK;
     case PDRD_OPT_REFRESH: *value = h->refresh; return PDRD_OK;
     case PDRD_OPT_CHAIN: *value = chains(h) ? 1 : 0; return PDRD_OK;
     case PDRD_OPT_PATCH_CELLS: {
@@ -2149,8 +2287,7 @@ int pdrd_batch_iterate(pdrd_batch_t b, int count, const pdrd_method_t *methods, 
     if (int rc = check_ready(h)) return rc;
     if (h->device != b->device) return set_error(PDRD_EINVAL, "batch methods must live on the batch's device");
     if (h->scatter_mode != 2) return set_error(PDRD_ENOTSUP, "batched iterations need the patch assembly");
-    if (corrects(h))
-      return set_error(PDRD_ENOTSUP, "batched iterations of mixed-precision methods need PDRD_OPT_REFRESH = 0");
+    if (mixed(h)) return set_error(PDRD_ENOTSUP, "batched iterations of mixed-precision methods are not built");
     if (nodal_input(h))
       return set_error(PDRD_ENOTSUP, "batched iterations of methods with a blending coefficient / artificial "
                                      "viscosity are not built");
@@ -2172,7 +2309,7 @@ int pdrd_batch_iterate(pdrd_batch_t b, int count, const pdrd_method_t *methods, 
   for (int i = 0; i < count; ++i)
     for (size_t k = 0; k < methods[i]->blocks.size(); ++k) {
       const Block &bl = methods[i]->blocks[k];
-      const pdrd::PatchBatchLauncher fn = methods[i]->precision ? bl.ks.patch_batch_mixed : bl.ks.patch_batch;
+      const pdrd::PatchBatchLauncher fn = bl.ks.patch_batch;
       size_t g = 0;
       while (g < patches.size() && (patches[g].patch != fn || patches[g].cells != bl.cells)) ++g;
       if (g == patches.size()) {

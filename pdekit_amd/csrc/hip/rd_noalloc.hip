@@ -122,7 +122,7 @@ int pdrd_assemble_rhs_noalloc(int physics, int scheme, const double *params, int
       (ne && (!geo_conn || !sol_conn)))
     return set_error(PDRD_EINVAL, "null argument");
   if (physics < 0 || physics > 4) return set_error(PDRD_EINVAL, "unknown physics");
-  if (scheme < 0 || scheme > 3) return set_error(PDRD_EINVAL, "unknown scheme");
+  if (scheme < 0 || scheme > 4) return set_error(PDRD_EINVAL, "unknown scheme");
   if (nb_params < 0 || (nb_params > 0 && !params)) return set_error(PDRD_EINVAL, "bad parameters");
   const int dim = (physics == PDRD_EULER3D) ? 3 : 2;
   if (fe->dim != dim) return set_error(PDRD_EINVAL, "element dimension does not match the physics");

@@ -5,9 +5,11 @@
 //     component), the nodal maximum of that over the node's cells, and the
 //     cell value 1 - the largest nodal value of the cell;
 //   * ArtificialViscosity::compute_artificial_viscosity (src/solver/art_visc/
-//     ArtificialViscosity.hpp:371-540) on simplices: S = ||u_h - P u_h||^2 /
-//     ||u_h||^2 with P the L2 projection onto the polynomials of degree < p
-//     (the modal expansion without its leading terms), s = log10 S,
+//     ArtificialViscosity.hpp:371-540): S = ||u_h - P u_h||^2 / ||u_h||^2 with
+//     P the modal expansion without its leading terms -- on simplices the L2
+//     projection onto the polynomials of degree < p, on quadrilaterals the
+//     tensor Legendre expansion without the modes the reference flags (mode
+//     ids 0 .. 2p, see quad_zeroed_modes) --, s = log10 S,
 //     eps0 = |K|^(1/dim) / p, s0 = 1 / p^4, and nu = 0, factor eps0, or
 //     factor eps0 (1 + sin(pi (s - s0) / (2 kappa))) / 2 below, above or
 //     inside s0 -+ kappa;
@@ -39,7 +41,8 @@ struct StabArgs
   double *__restrict__ out;            // [N] maxima / interpolated values
   // smoothness indicator: rule w[nq], nodal basis V[nq][nd], geometry
   // derivatives dVg[dim][nq][ng], orthonormal lower-degree basis psi[nq][npm]
-  const double *__restrict__ w, *__restrict__ V, *__restrict__ dVg, *__restrict__ psi;
+  // (simplices) or the zeroed modal terms hm[nq][nd] (quadrilaterals)
+  const double *__restrict__ w, *__restrict__ V, *__restrict__ dVg, *__restrict__ psi, *__restrict__ hm;
   uint32_t ne;
   int m, nd, nvert, ng, dim, nq, npm, p;
   double kappa, factor;
@@ -87,7 +90,7 @@ __global__ void lf_cell_kernel(StabArgs a)
   for (int v = 0; v < a.nvert; ++v) atomic_max_nonneg(a.out + a.sconn[static_cast<size_t>(v) * a.ne + e], fc);
 }
 
-// Smoothness-indicator viscosity of each simplex, maximised into its vertices.
+// Smoothness-indicator viscosity of each cell, maximised into its vertices.
 __global__ void av_cell_kernel(StabArgs a)
 {
   const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
@@ -123,17 +126,28 @@ __global__ void av_cell_kernel(StabArgs a)
     for (int n = 0; n < a.nd; ++n) s += a.V[q * a.nd + n] * un[n];
     uq[q] = s;
   }
-  for (int k = 0; k < a.npm; ++k) {
-    double s = 0.0;
-    for (int q = 0; q < a.nq; ++q) s += a.w[q] * a.psi[q * a.npm + k] * uq[q];
-    coef[k] = s;
-  }
   double nom = 0.0, denom = 0.0;
-  for (int q = 0; q < a.nq; ++q) {
-    double pr = 0.0;
-    for (int k = 0; k < a.npm; ++k) pr += coef[k] * a.psi[q * a.npm + k];
-    nom += wj[q] * (uq[q] - pr) * (uq[q] - pr);
-    denom += wj[q] * uq[q] * uq[q];
+  if (a.hm) {
+    // quadrilaterals: u - u_{p-1} from the nodal values through the table of
+    // the zeroed modal terms
+    for (int q = 0; q < a.nq; ++q) {
+      double d = 0.0;
+      for (int n = 0; n < a.nd; ++n) d += a.hm[q * a.nd + n] * un[n];
+      nom += wj[q] * d * d;
+      denom += wj[q] * uq[q] * uq[q];
+    }
+  } else {
+    for (int k = 0; k < a.npm; ++k) {
+      double s = 0.0;
+      for (int q = 0; q < a.nq; ++q) s += a.w[q] * a.psi[q * a.npm + k] * uq[q];
+      coef[k] = s;
+    }
+    for (int q = 0; q < a.nq; ++q) {
+      double pr = 0.0;
+      for (int k = 0; k < a.npm; ++k) pr += coef[k] * a.psi[q * a.npm + k];
+      nom += wj[q] * (uq[q] - pr) * (uq[q] - pr);
+      denom += wj[q] * uq[q] * uq[q];
+    }
   }
   const double Se = (denom <= 1.e-10) ? 0.0 : nom / denom;
   const double se = log(Se) / log(10.);

@@ -580,6 +580,44 @@ void pack(const ElementTables &t, double *out)
   }
 }
 
+std::vector<double> quad_zeroed_modes(int p, const std::vector<double> &qp, const std::vector<double> &nodes)
+{
+  // normalised tensor Legendre modes, mode m (p + 1) + n = P_m(x) P_n(y) / sqrt(g_m g_n)
+  const int nm = (p + 1) * (p + 1);
+  auto modes = [&](double x, double y, long double *out) {
+    long double Px[8], Py[8];
+    Px[0] = Py[0] = 1.0L;
+    Px[1] = x;
+    Py[1] = y;
+    for (int k = 1; k < p; ++k) {
+      Px[k + 1] = ((2 * k + 1) * x * Px[k] - k * Px[k - 1]) / (k + 1);
+      Py[k + 1] = ((2 * k + 1) * y * Py[k] - k * Py[k - 1]) / (k + 1);
+    }
+    for (int m = 0; m <= p; ++m)
+      for (int n = 0; n <= p; ++n)
+        out[m * (p + 1) + n] = Px[m] * Py[n] * std::sqrt((2.0L * m + 1) * (2.0L * n + 1)) / 2.0L;
+  };
+  if (p < 1 || p > 6) fail("quadrilateral modal basis of order 1-6");
+  const int nd = static_cast<int>(nodes.size() / 2);
+  if (nd != nm) fail("quadrilateral nodes do not match the modal basis");
+  // modal coefficients of nodal values: the inverse of the modes at the nodes
+  std::vector<long double> Vn(static_cast<size_t>(nd) * nm), inv;
+  for (int i = 0; i < nd; ++i) modes(nodes[2 * i], nodes[2 * i + 1], &Vn[static_cast<size_t>(i) * nm]);
+  invert_ld(Vn, nd, inv);
+  const int nq = static_cast<int>(qp.size() / 2);
+  std::vector<double> hm(static_cast<size_t>(nq) * nd);
+  std::vector<long double> vq(nm);
+  for (int q = 0; q < nq; ++q) {
+    modes(qp[2 * q], qp[2 * q + 1], vq.data());
+    for (int n = 0; n < nd; ++n) {
+      long double t = 0.0L;
+      for (int k = 0; k < 2 * p + 1; ++k) t += vq[k] * inv[static_cast<size_t>(k) * nd + n];
+      hm[static_cast<size_t>(q) * nd + n] = static_cast<double>(t);
+    }
+  }
+  return hm;
+}
+
 std::vector<double> lower_degree_basis(Shape s, int p, const std::vector<double> &qp, const std::vector<double> &w,
                                        int *npm)
 {

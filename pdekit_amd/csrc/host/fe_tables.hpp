@@ -102,6 +102,17 @@ ElementTables make_element_tables(Shape s, int geo_order, int sol_order, int qua
 std::vector<double> lower_degree_basis(Shape s, int p, const std::vector<double> &qp, const std::vector<double> &w,
                                        int *npm);
 
+// The smoothness indicator on quadrilaterals (ArtificialViscosity.hpp:
+// 371-540 with ModalExpansionQuad): hm[nq][nd] maps nodal values to
+// u - u_{p-1} at the points qp[nq][2], u_{p-1} being the modal expansion
+// (normalised tensor Legendre modes P_m(x) P_n(y) / sqrt(g_m g_n), g_k =
+// 2 / (2k + 1), mode id m (p + 1) + n) without the modes the reference flags
+// as leading. ModalExpansionQuad::is_leading_expansion_term (ModalExpansionQuad.
+// cpp:240-257) advances its flag index only for the 2p + 1 modes with m == p
+// or n == p, so its flags fall on mode ids 0 .. 2p; those are the modes left
+// out here, as there. nodes: the reference nodes of the solution [nd][2].
+std::vector<double> quad_zeroed_modes(int p, const std::vector<double> &qp, const std::vector<double> &nodes);
+
 // P1 / Q1 shape functions of the reference vertices at reference points
 // pts[n][dim]: [n][nvert].
 std::vector<double> vertex_functions(Shape s, const std::vector<double> &pts);

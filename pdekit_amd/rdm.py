@@ -18,7 +18,7 @@ from .mesh import Mesh
 
 PHYSICS = {"Advection2D": L.ADVECTION2D, "Burgers2D": L.BURGERS2D, "Euler2DCons": L.EULER2D,
            "Euler3DCons": L.EULER3D}
-SCHEMES = {"PGLDA": L.LDA, "PGN": L.N, "PGB": L.B, "PSI": L.PSI}
+SCHEMES = {"PGLDA": L.LDA, "PGN": L.N, "PGB": L.B, "PSI": L.PSI, "BPSI": L.BPSI}
 NEQ = {L.ADVECTION2D: 1, L.BURGERS2D: 1, L.EULER2D: 4, L.EULER3D: 5, L.ROTATION2D: 1}
 FIELDS = {"solution": L.SOLUTION, "residuals": L.RESIDUAL, "time_step": L.TIME_STEP,
           "wave_speed": L.WAVE_SPEED, "dual_volume": L.DUAL_VOLUME, "blending_coeff": L.BLEND_COEFF,
@@ -30,7 +30,7 @@ class PGRDMethodExplicit:
 
     physics: 'Advection2D' (constant velocity params=(ax, ay)), 'Burgers2D',
     'Euler2DCons', 'Euler3DCons' (params=(gamma, R)); scheme: 'PGLDA', 'PGN',
-    'PGB' or 'PSI'.
+    'PGB', 'PSI' or 'BPSI' (the PSI / LDA blend by PGB's cell theta).
     """
 
     def __init__(self, physics: str, scheme: str, params=None):
@@ -290,12 +290,15 @@ class PGRDMethodExplicit:
         (e.g. an expression evaluated at weak_bc_points(k)); None restores the
         constant state."""
         if state is None:
-            L.check(L.lib().pdrd_set_weak_bc_state(self._h, int(k), None), "pdrd_set_weak_bc_state")
+            L.check(L.lib().pdrd_set_weak_bc_state(self._h, int(k), None, 0), "pdrd_set_weak_bc_state")
             return
         info = self.weak_bc_info(k)
-        a = np.ascontiguousarray(np.asarray(state, dtype=np.float64).reshape(
-            info["facets"], info["quadrature_points"], self.neq))
-        L.check(L.lib().pdrd_set_weak_bc_state(self._h, int(k), L.ptr(a)), "pdrd_set_weak_bc_state")
+        shape = (info["facets"], info["quadrature_points"], self.neq)
+        a = np.asarray(state, dtype=np.float64)
+        if a.shape != shape and not (self.neq == 1 and a.shape == shape[:2]):
+            raise ValueError(f"weak BC {k} takes a state of shape {shape} ([facets][points][m]), got {a.shape}")
+        a = np.ascontiguousarray(a.reshape(shape))
+        L.check(L.lib().pdrd_set_weak_bc_state(self._h, int(k), L.ptr(a), a.size), "pdrd_set_weak_bc_state")
 
     def last_timing(self):
         c = C.c_double(0)

@@ -15,12 +15,13 @@ c3  2D Euler channel [-0.5,1.5] x [0,1] over y = 0.1 exp(-25 (x-0.5)^2):
     stream with a seeded 1% perturbation.
 c4  3D Euler box [0,1]^2 x [0,0.5] over z = 0.08 exp(-25((x-.5)^2+(y-.5)^2)):
     96^3 hexahedra split into 6 tetrahedra (5 308 416 P1 tetrahedra,
-    912 673 nodes), interior vertices jittered 10% with seed 4, M = 0.5.
+    912 673 nodes), interior vertices jittered 10% with seed 4, M = 0.5, the
+    PSI / LDA blend (PGB's cell theta).
 c3p2 the c3 channel with P2 triangles and Q2 quadrilaterals (786 432
     elements, 2 100 225 nodes), rule of order 4.
 c4p2 the c4 box with 48^3 hexahedra split into 6 P2 tetrahedra (663 552
     elements, 912 673 nodes), rule of order 2 as examples/testcases-rdm/
-    Bump-RDS-3D-Explicit.cpp:53 uses.
+    Bump-RDS-3D-Explicit.cpp:53 uses, the PSI / LDA blend.
 """
 from __future__ import annotations
 
@@ -93,9 +94,9 @@ def c3(nx: int = 1024, ny: int = 512, scheme: int = L.LDA) -> Workload:
                     {L.TRIAG: 4, L.QUAD: 4}, 0.5, _free_stream(mesh, 2))
 
 
-def c4(n: int = 96, scheme: int = L.LDA) -> Workload:
+def c4(n: int = 96, scheme: int = L.BPSI) -> Workload:
     mesh = box_tetra(n, n, n, extent=(1.0, 1.0, 0.5), jitter=0.1, seed=4, bump=(0.5, 0.08, 25.0, 0.5, 0.5))
-    return Workload(f"euler3d_bump_p1_{mesh.nb_cells}", mesh, L.EULER3D, scheme, (1.4, 287.05), {L.TETRA: 2},
+    return Workload(f"euler3d_bump_p1_bpsi_{mesh.nb_cells}", mesh, L.EULER3D, scheme, (1.4, 287.05), {L.TETRA: 2},
                     0.5, _free_stream(mesh, 3, seed=4))
 
 
@@ -106,10 +107,10 @@ def c3p2(nx: int = 1024, ny: int = 512, scheme: int = L.LDA) -> Workload:
                     {L.TRIAG: 4, L.QUAD: 4}, 0.5, _free_stream(mesh, 2))
 
 
-def c4p2(n: int = 48, scheme: int = L.LDA) -> Workload:
+def c4p2(n: int = 48, scheme: int = L.BPSI) -> Workload:
     mesh = box_tetra(n, n, n, extent=(1.0, 1.0, 0.5), order=2, jitter=0.1, seed=4,
                      bump=(0.5, 0.08, 25.0, 0.5, 0.5))
-    return Workload(f"euler3d_bump_p2_{mesh.nb_cells}", mesh, L.EULER3D, scheme, (1.4, 287.05), {L.TETRA: 2},
+    return Workload(f"euler3d_bump_p2_bpsi_{mesh.nb_cells}", mesh, L.EULER3D, scheme, (1.4, 287.05), {L.TETRA: 2},
                     0.5, _free_stream(mesh, 3, seed=4))
 
 

@@ -4,9 +4,10 @@ launch between sweeps, the row buffers alternating).
 The state, residual, time step and norm after iterate(n) must equal, bit for
 bit, those of n calls of iterate(1), each of which closes its sweep with the
 fix-up kernel -- for scalar and Euler physics, mixed element types, strong
-Dirichlet and weak boundary conditions, mixed precision with defect-correction
-refreshes inside the chain, the eager timing mode, and small patches (many
-shared nodes). The chained state is also checked against the oracle."""
+Dirichlet and weak boundary conditions, the eager timing mode, and small
+patches (many shared nodes); in the mixed-precision form (linear sweeps
+between FP64 ones) a chained call equals the same call with a fix-up after
+every sweep. The chained state is also checked against the oracle."""
 import numpy as np
 import pytest
 
@@ -150,12 +151,12 @@ def test_chained_iterations_with_small_patches(name):
 
 @pytest.mark.parametrize("refresh", [1, 3, 4])
 def test_chained_mixed_precision_with_refreshes(refresh):
-    """Refreshes of the defect correction fall inside the chain (every
-    `refresh` sweeps), so the chain is closed before each of them."""
+    """Mixed precision: chains run within runs of linear sweeps and are closed
+    before every FP64 sweep (every `refresh` iterations and the last)."""
     print(f"chain mixed refresh {refresh}", flush=True)
     make, cfl = CASES["burgers_p2_b"]
     _same(_run(make, cfl, 10, True, precision=1, refresh=refresh),
-          _run(make, cfl, 10, False, precision=1, refresh=refresh))
+          _run(make, cfl, 10, True, precision=1, refresh=refresh, chain=0))
 
 
 def test_chained_iterations_match_oracle():

@@ -14,7 +14,7 @@ from tests.helpers import CASES, oracle_problem, quad_orders, rel_err
 pytestmark = pytest.mark.gpu
 
 TOL = 1e-12
-SCHEMES = [L.LDA, L.N, L.B, L.PSI]
+SCHEMES = [L.LDA, L.N, L.B, L.PSI, L.BPSI]
 
 
 SCATTER = {"patch": 2, "atomic": 0, "colour": 1}

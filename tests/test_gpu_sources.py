@@ -99,8 +99,6 @@ def test_refused_combinations():
     with pytest.raises(L.PdrdError):
         m.set_option(L.OPT_SCATTER, 1)
     with pytest.raises(L.PdrdError):
-        m.set_option(L.OPT_PRECISION, 1)
-    with pytest.raises(L.PdrdError):
         m.set_blending_coeff(np.full(mesh.nb_nodes, 0.5))
     with pytest.raises(L.PdrdError):
         m.compute_blending_coeff()

@@ -3,8 +3,9 @@ algorithm must satisfy -- none of these checks retypes the oracle's formula:
 
 * P1 scalar schemes reduce to the textbook LDA / N distributions
   (beta_i = k_i+ / sum k+, N: k_i+ (u_i - u_in)), B = LDA + theta (N - LDA)
-  with theta = |Phi_T| / (sum |Phi_N| + 1e-6), and the PSI limiter of
-  LFResidualLimiter.hpp:84-110 applied to N; k_i = |T| a . grad(phi_i).
+  with theta = |Phi_T| / (sum |Phi_N| + 1e-6), the PSI limiter of
+  LFResidualLimiter.hpp:84-110 applied to N, and the PSI / LDA blend
+  (1 - theta) LDA + theta PSI; k_i = |T| a . grad(phi_i).
 * conservation: sum_i Phi_i = integral of div F for every scheme; with the
   P+1 flux element the divergence of a linear advection flux and of the
   Burgers flux of a P1 state is exact, so the total equals a closed form.
@@ -52,14 +53,18 @@ def _textbook(scheme, k, u):
         return lda
     if scheme == O.N:
         return n
+    th = np.abs(phiT) / (np.abs(n).sum(1) + 1e-6)
     if scheme == O.B:
-        th = np.abs(phiT) / (np.abs(n).sum(1) + 1e-6)
         return lda + th[:, None] * (n - lda)
     x = np.maximum(n / phiT[:, None], 0) + 1e-12
-    return x / x.sum(1)[:, None] * phiT[:, None]
+    psi = x / x.sum(1)[:, None] * phiT[:, None]
+    if scheme == O.PSI:
+        return psi
+    # the PSI / LDA blend with the same cell theta
+    return (1 - th[:, None]) * lda + th[:, None] * psi
 
 
-@pytest.mark.parametrize("scheme", [O.LDA, O.N, O.B, O.PSI])
+@pytest.mark.parametrize("scheme", [O.LDA, O.N, O.B, O.PSI, O.BPSI])
 def test_p1_advection_textbook_schemes(scheme):
     mesh = rectangle(7, 6, split=2, jitter=0.2, seed=11)
     rng = np.random.default_rng(1)
@@ -76,7 +81,7 @@ def test_p1_advection_textbook_schemes(scheme):
     assert np.allclose(wave, area * lam, rtol=1e-13, atol=0)
 
 
-@pytest.mark.parametrize("scheme", [O.LDA, O.N, O.B, O.PSI])
+@pytest.mark.parametrize("scheme", [O.LDA, O.N, O.B, O.PSI, O.BPSI])
 def test_burgers_p1_conservation_is_exact(scheme):
     """u linear in each cell: F = (u^2/2, u) is quadratic, interpolated exactly
     by the P2 flux element, so sum_i Phi_i = |T| (u_x mean(u) + u_y)."""
@@ -94,8 +99,9 @@ def test_burgers_p1_conservation_is_exact(scheme):
 
 @pytest.mark.parametrize("name", ["p2", "q1", "q2", "euler", "euler3d_p2"])
 def test_schemes_share_the_element_total(name):
-    """The N / B dissipation and the PSI limiter redistribute without changing
-    the element total, which equals the LDA total."""
+    """The N / B dissipation, the PSI limiter and the PSI / LDA blend
+    redistribute without changing the element total, which equals the LDA
+    total."""
     if name == "q2":
         mesh = rectangle(4, 3, split=1, order=2, jitter=0.15, seed=5)
         phys, par = O.BURGERS2D, ()
@@ -117,7 +123,7 @@ def test_schemes_share_the_element_total(name):
         phys, par = O.EULER2D, (1.4, 287.05)
         u = euler_state(mesh)
     tot = []
-    for sch in (O.LDA, O.N, O.B, O.PSI):
+    for sch in (O.LDA, O.N, O.B, O.PSI, O.BPSI):
         P = oracle_problem(mesh, phys, sch, par)
         phi, _, rc = P.cell_residuals(0, u)
         assert rc == 0
@@ -130,11 +136,12 @@ def test_schemes_share_the_element_total(name):
 @pytest.mark.parametrize("quads", [False, True])
 def test_linearity_preservation(quads):
     """P2 triangles, or Q2 quadrilaterals on bilinear cells: a linear exact
-    solution of a . grad u = 0 is represented exactly and LDA / B leave it."""
+    solution of a . grad u = 0 is represented exactly and LDA / B and the
+    PSI / LDA blend (theta = 0 where the element total vanishes) leave it."""
     mesh = rectangle(8, 7, split=1 if quads else 2, jitter=0.2, seed=8, order=2)
     x, y = mesh.node_coords.T
     u = 0.3 + (-A[1] * x + A[0] * y)  # a . grad u = 0
-    for sch in (O.LDA, O.B):
+    for sch in (O.LDA, O.B, O.BPSI):
         P = oracle_problem(mesh, O.ADVECTION2D, sch, A)
         res, _, rc = P.assemble_rhs(u)
         assert np.abs(res).max() < 1e-13

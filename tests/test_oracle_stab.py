@@ -14,8 +14,12 @@ reference (which does not build here):
   nu = 0; P1 cells against the closed form of ||u - mean||^2 / ||u||^2
   (mass matrix |K| (1 + delta_ij) / 12 on triangles, / 20 on tetrahedra); P2 /
   P3 cells against an independent least-squares projection in physical
-  coordinates with a higher-order rule; vertices take the largest value of
-  their cells and edge nodes the mean of their edge's vertices.
+  coordinates with a higher-order rule; quadrilaterals (ModalExpansionQuad,
+  normalised tensor Legendre modes) by single modes and pairs of modes on a
+  square cell -- which fixes the modes the reference zeroes -- and by an
+  independent modal least-squares fit on jittered Q2 cells; vertices take the
+  largest value of their cells and edge nodes the mean of their edge's
+  vertices.
 """
 import numpy as np
 import pytest
@@ -244,3 +248,71 @@ def test_art_visc_provider_p2_tetrahedra():
     # the rule of order 4 carries 10 significant digits
     assert vert.max() > 0
     assert np.abs(got[vids] - vert[vids]).max() <= 1e-8 * vert.max()
+
+
+def _legendre_mode(p, mode, xi0, xi1):
+    """psi_{m,n} = P_m(xi0) P_n(xi1) sqrt((2m + 1)(2n + 1)) / 2, closed-form
+    Legendre polynomials up to degree 2 (mode id m (p + 1) + n)."""
+    leg = (lambda x: np.ones_like(x), lambda x: x, lambda x: 0.5 * (3 * x * x - 1))
+    m, n = divmod(mode, p + 1)
+    return leg[m](xi0) * leg[n](xi1) * np.sqrt((2 * m + 1) * (2 * n + 1)) / 2
+
+
+@pytest.mark.parametrize("p", [1, 2])
+def test_art_visc_provider_quadrilateral_modes(p):
+    """One square cell [0, 1]^2 (reference coordinates xi = 2x - 1, so the
+    modal basis is orthonormal up to the constant Jacobian): with u a single
+    mode, S = 1 for a mode the reference flags as leading and S = 0 (nu = 0)
+    otherwise; with u = psi_a + psi_b, one flagged and one not, S = 1/2.
+    ModalExpansionQuad::is_leading_expansion_term advances its flag index only
+    for the 2p + 1 modes with m == p or n == p, so the flags land on mode ids
+    0 .. 2p, which this pins (ModalExpansionQuad.cpp:240-257)."""
+    mesh = rectangle(1, 1, split=1, order=p)
+    P = oracle_problem(mesh, O.BURGERS2D, O.LDA, ())
+    xi = 2 * mesh.node_coords - 1
+    flagged = set(range(2 * p + 1))
+    nmodes = (p + 1) ** 2
+    for a in range(nmodes):
+        u = _legendre_mode(p, a, xi[:, 0], xi[:, 1])[:, None]
+        got = P.art_visc(u, kappa=KAPPA)
+        expect = _nu_of(1.0 if a in flagged else 0.0, 1.0, 2, p)
+        assert np.abs(got - expect).max() <= 1e-13, (a, got, expect)
+    # pairs whose products the indicator's rule integrates exactly (Q1: the
+    # 2 x 2 Gauss rule, degree 3 per variable; Q2: the 8-point rule of total
+    # degree 4, with psi_6 = P_2(xi0) of total degree 2)
+    pairs = [(0, 3), (1, 3), (2, 3)] if p == 1 else [(0, 6), (1, 6), (3, 6)]
+    for a, b in pairs:
+        u = (_legendre_mode(p, a, xi[:, 0], xi[:, 1]) + _legendre_mode(p, b, xi[:, 0], xi[:, 1]))[:, None]
+        got = P.art_visc(u, kappa=KAPPA)
+        assert np.abs(got - _nu_of(0.5, 1.0, 2, p)).max() <= 1e-13, (a, b)
+
+
+def test_art_visc_provider_quadrilateral_projection():
+    """Jittered Q2 cells: S equals the weighted norm of the flagged modes of
+    the state's modal expansion over the weighted norm of the state, with the
+    modal coefficients obtained here by least squares of the nodal values onto
+    the closed-form modal basis (exact: u lies in the span), integrated with
+    the indicator's own rule (order 2p) and the cells' Jacobians."""
+    mesh = rectangle(3, 3, split=1, order=2, jitter=0.15, seed=7)
+    P = oracle_problem(mesh, O.BURGERS2D, O.LDA, ())
+    u = _state(mesh)
+    t = element_tables(L.QUAD, 2, 4)
+    nodes = element_tables(L.QUAD, 2, 4).sol_nodes
+    Vn = np.stack([_legendre_mode(2, k, nodes[:, 0], nodes[:, 1]) for k in range(9)], 1)
+    Vq = np.stack([_legendre_mode(2, k, t.qp[:, 0], t.qp[:, 1]) for k in range(9)], 1)
+    b = mesh.blocks[0]
+    X = mesh.coords[b.geo_conn]
+    vals = []
+    for c in range(b.sol_conn.shape[0]):
+        coef = np.linalg.lstsq(Vn, u[b.sol_conn[c], 0], rcond=None)[0]
+        J = np.einsum("dqg,gx->qdx", t.dVg, X[c])
+        wj = t.w * np.abs(np.linalg.det(J))
+        uq = Vq @ coef
+        hi = Vq[:, :5] @ coef[:5]
+        Se = (wj * hi ** 2).sum() / (wj * uq ** 2).sum()
+        vals.append(_nu_of(Se, wj.sum(), 2, 2))
+    expect = _cell_to_node(mesh, [np.array(vals)])
+    got = P.art_visc(u, kappa=KAPPA)
+    vert = np.unique(b.sol_conn[:, :4])
+    assert np.abs(got[vert] - expect[vert]).max() <= 1e-12 * expect.max()
+    assert expect.max() > 0
