@@ -125,15 +125,18 @@ def cpu_baseline(wl, seconds: float):
 
 def serving_bench(args):
     """The serving stream: seeded Poisson arrivals (20 requests/s, seed 5) of
-    steady advection P1-P3 and Burgers P2 solves to a residual norm of 1e-10,
-    served with continuous batching and, as the baseline, one at a time in
-    arrival order; p50 / p99 time-to-converge and throughput of both."""
+    steady advection P1-P3, Burgers P2 and Euler bump (P1 / Q1, Mach uniform
+    in [0.3, 0.6]) solves to a residual norm of 1e-10, served with continuous
+    batching and, as the baseline, one at a time in arrival order; p50 / p99
+    time-to-converge and throughput of both."""
     from pdekit_amd import serving as S
 
     sc = args.size_scale
+    e0, e1 = (int(x) for x in args.euler_cells.split(","))
     reqs = S.request_stream(args.requests, rate=20.0, seed=5,
                             adv_cells=(int(10_000 * sc), int(1_000_000 * sc)),
-                            burgers_cells=(int(50_000 * sc), int(500_000 * sc)), tol=1e-10)
+                            burgers_cells=(int(50_000 * sc), int(500_000 * sc)), tol=1e-10,
+                            euler_cells=(e0, e1))
     t0 = time.perf_counter()
     probs = {r.rid: S.build_problem(r) for r in reqs}
     print(f"[bench] c5: {len(reqs)} requests, {sum(p.mesh.nb_cells for p in probs.values())} cells in all, "
@@ -143,11 +146,11 @@ def serving_bench(args):
     kinds = {}
     for r in reqs:
         kinds.setdefault((r.family, r.order), r)
-    small = [S.Request(i, 0.0, k[0], k[1], 2_000, 1e-10) for i, k in enumerate(kinds)]
+    small = [S.Request(i, 0.0, k[0], k[1], 2_000, 1e-10, 0.5) for i, k in enumerate(kinds)]
     warm.run(small, {q.rid: S.build_problem(q) for q in small}, mode="batched")
     out = {}
     for mode in ("batched", "sequential"):
-        srv = S.Server(S.GpuExecutor(), chunk=50, max_iters=100_000)
+        srv = S.Server(S.GpuExecutor(), chunk=50, max_iters=args.max_iters)
         res, peak = srv.run(reqs, probs, mode=mode)
         out[mode] = S.summary(res)
         out[mode]["peak_footprint_bytes"] = int(peak)
@@ -163,7 +166,9 @@ def serving_bench(args):
         "dtype": "f64",
         "data": "synthetic",
         "config": {"workload": "serving_c5", "requests": len(reqs), "rate_per_s": 20.0, "seed": 5,
-                   "size_scale": sc, "tol": 1e-10, "chunk": 50, "parallelism": "single GPU"},
+                   "size_scale": sc, "euler_cells": [e0, e1], "tol": 1e-10, "chunk": 50,
+                   "max_iters": args.max_iters, "parallelism": "single GPU",
+                   "families": {f: sum(r.family == f for r in reqs) for f in S.FAMILIES}},
         "batched": b,
         "sequential": q,
         "p99_speedup_vs_sequential": round(q["p99_s"] / b["p99_s"], 3),
@@ -183,6 +188,10 @@ def main():
     ap.add_argument("--requests", type=int, default=64, help="c5: requests in the stream")
     ap.add_argument("--size-scale", type=float, default=0.1,
                     help="c5: scale of the request sizes (1.0: advection 10k-1M, Burgers 50k-500k triangles)")
+    ap.add_argument("--euler-cells", default="1000,4000",
+                    help="c5: size range of the Euler bump requests (elements; the explicit LDA iteration "
+                         "of the bump needs ~3e4 iterations to 1e-10 at 1.7k elements)")
+    ap.add_argument("--max-iters", type=int, default=150_000, help="c5: iteration cap of one request")
     ap.add_argument("--scatter", default="auto", choices=["auto", "atomic", "colour", "patch"])
     ap.add_argument("--precision", default="plain", choices=["plain", "mixed"],
                     help="plain FP64 sweeps, or the mixed-precision form (linear sweeps with FP32 element "

@@ -1140,10 +1140,13 @@ __device__ __forceinline__ void patch_body(const PatchArgs &a, uint32_t pl, doub
   double phi[ND][M];
   double wave = 0.0;
   if constexpr (PREC == 4) {
-    // linear sweep: phi = J delta with the cell's FP32 Jacobian (entry-major
-    // over the block's cells, so that every load is coalesced)
+    // linear sweep: J delta with the cell's FP32 Jacobian (entry-major over
+    // the block's cells, so that every load is coalesced), written straight
+    // to the (cell, node) pair slots (no divergence slices share the work
+    // region here); the node loop stays rolled, one block row per trip
     if (active) {
       constexpr int NDM = ND * M;
+      const int NPAIR_ = blockDim.x * ND;
       float dv[NDM];
 #pragma unroll
       for (int n = 0; n < ND; ++n)
@@ -1151,12 +1154,19 @@ __device__ __forceinline__ void patch_body(const PatchArgs &a, uint32_t pl, doub
         for (int k = 0; k < M; ++k) dv[n * M + k] = delta_l[(lc[n] & LC_MASK) * M + k];
       const float *__restrict__ jc = a.jlin + (e0 + tid);
       const size_t stride = a.ne;
+#pragma unroll 1
+      for (int n = 0; n < ND; ++n) {
+        const uint16_t l = a.lconn[static_cast<size_t>(n) * a.ne + e0 + tid];
+        const int slot = coff[l & LC_MASK] + (l >> LC_BITS);
 #pragma unroll
-      for (int i = 0; i < NDM; ++i) {
-        float y = 0.0f;
+        for (int k = 0; k < M; ++k) {
+          const float *__restrict__ row = jc + static_cast<size_t>((n * M + k) * NDM) * stride;
+          float y = 0.0f;
 #pragma unroll
-        for (int j = 0; j < NDM; ++j) y += jc[static_cast<size_t>(i * NDM + j) * stride] * dv[j];
-        phi[i / M][i % M] = static_cast<double>(y);
+          for (int j = 0; j < NDM; ++j) y += row[static_cast<size_t>(j) * stride] * dv[j];
+          work[k * NPAIR_ + slot] = static_cast<double>(y);
+        }
+        work[M * NPAIR_ + slot] = 0.0;
       }
     }
   } else if (active) {
@@ -1188,7 +1198,7 @@ __device__ __forceinline__ void patch_body(const PatchArgs &a, uint32_t pl, doub
                                                        work + tid, PATCH);
   }
   __syncthreads();  // the work region changes from divergence slices to pair sums
-  if (active) {
+  if (active && PREC != 4) {
 #pragma unroll
     for (int n = 0; n < ND; ++n) {
       const int slot = coff[lc[n] & LC_MASK] + (lc[n] >> LC_BITS);

@@ -2,8 +2,11 @@
 batching.
 
 Requests arrive on a seeded Poisson stream. Each request is a steady explicit
-RD solve (2D advection P1-P3 or 2D Burgers P2, strong Dirichlet inflow) run
-to a residual norm tolerance. The server
+RD solve run to a residual norm tolerance: 2D advection P1-P3 or 2D Burgers P2
+with strong Dirichlet inflow, or the 2D Euler channel over a bump (P1 / Q1
+elements as examples/testcases-rdm/Bump-RDS-2D-Explicit.cpp:56-70 uses, LDA,
+free stream at the request's Mach number, weak walls, subsonic inlet and
+outlet as there, :138-149, 235-243). The server
 
   * admits requests in arrival order as soon as their device footprint fits
     the HBM budget next to the requests already resident,
@@ -34,43 +37,56 @@ import numpy as np
 from . import _lib as L
 from .mesh import Mesh, rectangle, unit_quad_triag
 
-FAMILIES = ("advection", "burgers")
+FAMILIES = ("advection", "burgers", "euler")
+GAMMA, GAS_R = 1.4, 287.05
 
 
 @dataclass
 class Request:
     rid: int
     arrival: float   # seconds after the start of the stream
-    family: str      # 'advection' (P1-P3, N scheme) or 'burgers' (P2, N scheme)
+    family: str      # 'advection' (P1-P3, N scheme), 'burgers' (P2, N scheme) or 'euler' (P1 / Q1, LDA)
     order: int
-    cells: int       # target number of triangles
+    cells: int       # target number of elements
     tol: float = 1e-10
+    mach: float = 0.0  # euler: free-stream Mach number
 
 
 def request_stream(n: int, rate: float = 20.0, seed: int = 5, adv_cells=(10_000, 1_000_000),
-                   burgers_cells=(50_000, 500_000), tol: float = 1e-10) -> list[Request]:
+                   burgers_cells=(50_000, 500_000), tol: float = 1e-10, euler_cells=None,
+                   mach=(0.3, 0.6)) -> list[Request]:
     """Seeded Poisson arrivals (mean `rate` per second); family uniform over
-    advection / Burgers, advection order uniform in 1..3, sizes uniform in
-    the given ranges of triangles."""
+    advection / Burgers (and the Euler bump when euler_cells is given),
+    advection order uniform in 1..3, Mach number uniform in `mach`, sizes
+    uniform in the given ranges of elements. The Euler bump runs at P1 / Q1:
+    with P2 / Q2 the LDA iteration of this channel diverges (no shock
+    capturing; the reference's bump example is P1)."""
+    fams = FAMILIES if euler_cells is not None else FAMILIES[:2]
     rng = np.random.Generator(np.random.MT19937(seed))
     t = np.cumsum(rng.exponential(1.0 / rate, n))
     out = []
     for i in range(n):
-        fam = FAMILIES[int(rng.integers(0, len(FAMILIES)))]
+        fam = fams[int(rng.integers(0, len(fams)))]
+        m = 0.0
         if fam == "advection":
             order = int(rng.integers(1, 4))
             cells = int(rng.integers(adv_cells[0], adv_cells[1] + 1))
-        else:
+        elif fam == "burgers":
             order = 2
             cells = int(rng.integers(burgers_cells[0], burgers_cells[1] + 1))
-        out.append(Request(i, float(t[i]), fam, order, cells, tol))
+        else:
+            order = 1
+            cells = int(rng.integers(euler_cells[0], euler_cells[1] + 1))
+            m = float(rng.uniform(mach[0], mach[1]))
+        out.append(Request(i, float(t[i]), fam, order, cells, tol, m))
     return out
 
 
 @dataclass
 class Problem:
-    """Host data of one request: mesh, physics, scheme, initial state and the
-    strong Dirichlet inflow."""
+    """Host data of one request: mesh, physics, scheme, initial state, the
+    strong Dirichlet inflow (may be empty) and the weak boundary conditions
+    (type, nodes, parameters)."""
     mesh: Mesh
     physics: int
     scheme: int
@@ -79,9 +95,36 @@ class Problem:
     u0: np.ndarray
     bc_nodes: np.ndarray
     bc_values: np.ndarray
+    weak: list = field(default_factory=list)
+
+
+def bump_problem(cells: int, order: int, mach: float) -> Problem:
+    """The 2D channel [-0.5, 1.5] x [0, 1] over the bump y = 0.1 exp(-25 (x -
+    0.5)^2) with 2n x n cells, the lower half quadrilaterals, the upper half
+    split triangles (3 n^2 elements), free stream rho = 1, p = 1 / gamma at
+    `mach`; weak walls below and above, subsonic inlet (total temperature and
+    pressure of the free stream) and outlet (its pressure)."""
+    n = max(2, 2 * int(round(np.sqrt(cells / 3.0) / 2)))
+    mesh = rectangle(2 * n, n, extent=(-0.5, 1.5, 0.0, 1.0), split=2, quad_rows=n // 2, order=order,
+                     bump=(1.0, 0.1, 25.0, 0.5))
+    p = 1.0 / GAMMA
+    fac = 1.0 + 0.5 * (GAMMA - 1.0) * mach ** 2
+    u0 = np.zeros((mesh.nb_nodes, 4))
+    u0[:, 0] = 1.0
+    u0[:, 1] = mach
+    u0[:, 3] = p / (GAMMA - 1.0) + 0.5 * mach ** 2
+    b = mesh.boundaries
+    weak = [("WeakWall", b["ymin"], {}), ("WeakWall", b["ymax"], {}),
+            ("WeakSubInlet", b["xmin"], {"total_temperature": p / GAS_R * fac,
+                                         "total_pressure": p * fac ** (GAMMA / (GAMMA - 1.0))}),
+            ("WeakSubOutlet", b["xmax"], {"total_pressure": p})]
+    empty = np.zeros(0, dtype=np.uint32)
+    return Problem(mesh, L.EULER2D, L.LDA, (GAMMA, GAS_R), 0.5, u0, empty, np.zeros((0, 4)), weak)
 
 
 def build_problem(req: Request) -> Problem:
+    if req.family == "euler":
+        return bump_problem(req.cells, req.order, req.mach)
     if req.family == "advection":
         # make_unit_quad(N) on [0,1]^2: 2 (N-1)^2 triangles
         N = max(3, int(round(np.sqrt(req.cells / 2.0))) + 1)
@@ -118,7 +161,11 @@ def footprint_bytes(p: Problem) -> int:
     slots = 2 * N  # nodes plus their extra slots on patch borders, bounded
     # per slot: node id, slot row, pair offset, shared index, row range, two rows
     patch = slots * (4 + 4 + 2 + 4 + 8 + 2 * 8 * (neq + 1)) + 2 * 8 * neq * N + E * (2 * nd + 4 * ng)
-    return int(nodal + 8 * m.coords.size + E * 4 * (2 * nd + 2 * ng) + patch + (1 << 20))
+    # weak facets: per boundary node and facet, two extra rows, its row and
+    # node ids, the facet normal / Jacobian / rule
+    nb = sum(int(np.asarray(w[1]).size) for w in p.weak)
+    weak = nb * (2 * 2 * 8 * (neq + 1) + 2 * 4 * 2 + 8 * 8)
+    return int(nodal + 8 * m.coords.size + E * 4 * (2 * nd + 2 * ng) + patch + weak + (1 << 20))
 
 
 @dataclass
@@ -152,8 +199,11 @@ class GpuExecutor:
         from .rdm import PGRDMethodExplicit
         m = PGRDMethodExplicit(p.physics, p.scheme, params=p.params if p.params else None)
         m.configure_mesh_data(p.mesh).initialize_work_data(quad_order=6 if p.mesh.order == 3 else 4)
+        for kind, nodes, par in p.weak:  # before the method's first device use
+            m.add_boundary_condition(kind, kind, nodes, params=par)
         m.set_vec_function("solution", p.u0)
-        m.add_boundary_condition("StrongDirichlet", "inflow", p.bc_nodes, p.bc_values)
+        if p.bc_nodes.size:
+            m.add_boundary_condition("StrongDirichlet", "inflow", p.bc_nodes, p.bc_values)
         m.cfl = p.cfl
         return m
 
