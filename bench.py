@@ -123,6 +123,9 @@ def cpu_baseline(wl, seconds: float):
             "single_thread_value": wl.nb_elements / med1}
 
 
+SCHEMES = {"lda": 0, "n": 1, "b": 2, "psi": 3, "bpsi": 4}
+
+
 def serving_bench(args):
     """The serving stream: seeded Poisson arrivals (20 requests/s, seed 5) of
     steady advection P1-P3, Burgers P2 and Euler bump (P1 / Q1, Mach uniform
@@ -149,12 +152,27 @@ def serving_bench(args):
     small = [S.Request(i, 0.0, k[0], k[1], 2_000, 1e-10, 0.5) for i, k in enumerate(kinds)]
     warm.run(small, {q.rid: S.build_problem(q) for q in small}, mode="batched")
     out = {}
-    for mode in ("batched", "sequential"):
+    modes = args.modes.split(",")
+    for mode in modes:
         srv = S.Server(S.GpuExecutor(), chunk=50, max_iters=args.max_iters)
-        res, peak = srv.run(reqs, probs, mode=mode)
+        done = [0, time.perf_counter()]
+
+        def progress(res, mode=mode, done=done):
+            done[0] += 1
+            if done[0] % 64 == 0 or time.perf_counter() - done[1] > 30.0:
+                done[1] = time.perf_counter()
+                print(f"[bench] c5 {mode}: {done[0]} of {len(reqs)} requests done (request {res.rid}, "
+                      f"{res.family}, {res.iterations} iterations)", file=sys.stderr, flush=True)
+
+        res, peak = srv.run(reqs, probs, mode=mode, on_finish=progress)
         out[mode] = S.summary(res)
         out[mode]["peak_footprint_bytes"] = int(peak)
         print(f"[bench] c5 {mode}: {out[mode]}", file=sys.stderr, flush=True)
+    if len(modes) < 2:
+        print(json.dumps({"metric": "p99 time-to-converge (serving stream)", "modes": out,
+                          "config": {"workload": "serving_c5", "requests": len(reqs), "size_scale": sc,
+                                     "euler_cells": [e0, e1]}}), flush=True)
+        return
     b, q = out["batched"], out["sequential"]
     line = {
         "metric": "p99 time-to-converge (serving stream)",
@@ -192,7 +210,11 @@ def main():
                     help="c5: size range of the Euler bump requests (elements; the explicit LDA iteration "
                          "of the bump needs ~3e4 iterations to 1e-10 at 1.7k elements)")
     ap.add_argument("--max-iters", type=int, default=150_000, help="c5: iteration cap of one request")
+    ap.add_argument("--modes", default="batched,sequential",
+                    help="c5: serving modes to run (the JSON line compares them only when both run)")
     ap.add_argument("--scatter", default="auto", choices=["auto", "atomic", "colour", "patch"])
+    ap.add_argument("--scheme", default="default", choices=["default"] + list(SCHEMES),
+                    help="element scheme in place of the workload's own (the workload name then carries it)")
     ap.add_argument("--precision", default="plain", choices=["plain", "mixed"],
                     help="plain FP64 sweeps, or the mixed-precision form (linear sweeps with FP32 element "
                          "Jacobians between FP64 sweeps); mixed also times the plain path for the speed-up")
@@ -238,7 +260,8 @@ def main():
     from pdekit_amd.workloads import BUILDERS
 
     t_setup = time.perf_counter()
-    wl = BUILDERS[args.workload]()
+    wl = BUILDERS[args.workload]() if args.scheme == "default" else \
+        BUILDERS[args.workload](scheme=SCHEMES[args.scheme])
     E = wl.nb_elements
     cells = None
     if args.patch_cells == "auto":
@@ -429,7 +452,8 @@ def jacobian_bench(args, wl, meth):
         "config": {"workload": wl.name, "scheme": ["PGLDA", "PGN", "PGB", "PSI", "PSI/LDA blend"][wl.scheme],
                    "elements": m.nb_cells, "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": "rd_jacobian_kernel",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "kernel": "rd_jacobian_rows_kernel" if neq > 1 else "rd_jacobian_kernel",
                      "algorithmic_bytes": int(alg), "avg_launch_us": round(total_ms * 1e3, 2)},
     }), flush=True)
 

@@ -234,10 +234,12 @@ class Server:
         self.clock = clock
         self.sleep = sleep
 
-    def run(self, requests: list[Request], problems: dict, mode: str = "batched", keep_solutions: bool = False):
+    def run(self, requests: list[Request], problems: dict, mode: str = "batched", keep_solutions: bool = False,
+            on_finish=None):
         """problems: rid -> Problem (host data prepared by the clients before
-        the stream starts). Returns the Results in request order and the
-        largest device footprint held at once."""
+        the stream starts); on_finish(result), if given, is called as each
+        request leaves the device. Returns the Results in request order and
+        the largest device footprint held at once."""
         reqs = sorted(requests, key=lambda r: r.arrival)
         results = {r.rid: Result(r.rid, r.family, r.order, r.cells, r.arrival) for r in reqs}
         pending = list(reqs)
@@ -282,6 +284,8 @@ class Server:
                     if keep_solutions:
                         res.solution = u
                     used -= a[2]
+                    if on_finish is not None:
+                        on_finish(res)
                 else:
                     keep.append(a)
             active = keep

@@ -33,6 +33,9 @@ from . import _lib as L
 from .mesh import Mesh, box_tetra, rectangle, unit_quad_triag
 
 
+_TAG = {L.LDA: "lda", L.N: "n", L.B: "b", L.PSI: "psi", L.BPSI: "bpsi"}
+
+
 @dataclass
 class Workload:
     name: str
@@ -83,7 +86,7 @@ def c2(n: int = 512, scheme: int = L.B, seed: int = 2) -> Workload:
     nodes = np.concatenate([ymin, xmin, xmax]).astype(np.uint32)
     vals = np.concatenate([1.5 - 2.0 * mesh.node_coords[ymin, 0], np.full(xmin.size, 1.5),
                            np.full(xmax.size, -0.5)])[:, None]
-    return Workload(f"burgers2d_p2_b_{mesh.nb_cells}", mesh, L.BURGERS2D, scheme, (), {L.TRIAG: 4}, 0.5, u,
+    return Workload(f"burgers2d_p2_{_TAG[scheme]}_{mesh.nb_cells}", mesh, L.BURGERS2D, scheme, (), {L.TRIAG: 4}, 0.5, u,
                     nodes, vals)
 
 
@@ -96,7 +99,7 @@ def c3(nx: int = 1024, ny: int = 512, scheme: int = L.LDA) -> Workload:
 
 def c4(n: int = 96, scheme: int = L.BPSI) -> Workload:
     mesh = box_tetra(n, n, n, extent=(1.0, 1.0, 0.5), jitter=0.1, seed=4, bump=(0.5, 0.08, 25.0, 0.5, 0.5))
-    return Workload(f"euler3d_bump_p1_bpsi_{mesh.nb_cells}", mesh, L.EULER3D, scheme, (1.4, 287.05), {L.TETRA: 2},
+    return Workload(f"euler3d_bump_p1_{_TAG[scheme]}_{mesh.nb_cells}", mesh, L.EULER3D, scheme, (1.4, 287.05), {L.TETRA: 2},
                     0.5, _free_stream(mesh, 3, seed=4))
 
 
@@ -110,7 +113,7 @@ def c3p2(nx: int = 1024, ny: int = 512, scheme: int = L.LDA) -> Workload:
 def c4p2(n: int = 48, scheme: int = L.BPSI) -> Workload:
     mesh = box_tetra(n, n, n, extent=(1.0, 1.0, 0.5), order=2, jitter=0.1, seed=4,
                      bump=(0.5, 0.08, 25.0, 0.5, 0.5))
-    return Workload(f"euler3d_bump_p2_bpsi_{mesh.nb_cells}", mesh, L.EULER3D, scheme, (1.4, 287.05), {L.TETRA: 2},
+    return Workload(f"euler3d_bump_p2_{_TAG[scheme]}_{mesh.nb_cells}", mesh, L.EULER3D, scheme, (1.4, 287.05), {L.TETRA: 2},
                     0.5, _free_stream(mesh, 3, seed=4))
 
 
