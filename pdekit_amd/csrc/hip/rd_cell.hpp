@@ -98,7 +98,7 @@ __device__ __forceinline__ double det_inv(const double J[D][D], double Ji[D][D])
 {
   if constexpr (D == 2) {
     const double det = J[0][0] * J[1][1] - J[0][1] * J[1][0];
-    const double id = 1.0 / det;
+    const double id = recip(det);
     Ji[0][0] = id * J[1][1];
     Ji[1][1] = id * J[0][0];
     Ji[0][1] = -id * J[0][1];
@@ -109,7 +109,7 @@ __device__ __forceinline__ double det_inv(const double J[D][D], double Ji[D][D])
     const double c01 = J[1][2] * J[2][0] - J[1][0] * J[2][2];
     const double c02 = J[1][0] * J[2][1] - J[1][1] * J[2][0];
     const double det = J[0][0] * c00 + J[0][1] * c01 + J[0][2] * c02;
-    const double id = 1.0 / det;
+    const double id = recip(det);
     Ji[0][0] = c00 * id;
     Ji[1][0] = c01 * id;
     Ji[2][0] = c02 * id;
@@ -147,7 +147,7 @@ __device__ __forceinline__ void solve_pp(double A[M][M], double B[M][R])
         B[r][j] = sw ? t : B[r][j];
       }
     }
-    const double ip = 1.0 / A[c][c];
+    const double ip = recip(A[c][c]);
     ipv[c] = ip;
 #pragma unroll
     for (int r = c + 1; r < M; ++r) {
@@ -515,9 +515,11 @@ __device__ __forceinline__ void cell_residual(const double *__restrict__ tab, co
         double nn = double(0);
 #pragma unroll
         for (int x = 0; x < D; ++x) nn += g[x] * g[x];
-        const double norm = sqrt(nn);
+        double norm, inorm;
+        sqrt_rsqrt(nn, norm, inorm);
+        norm = (nn > double(0)) ? norm : double(0);  // sqrt_rsqrt(0) is NaN
         const bool tiny = (D == 2) ? (norm < double(1.e-9)) : (norm == double(0));
-        const double inorm = tiny ? double(0) : double(1) / norm;
+        inorm = tiny ? double(0) : inorm;
         double u_n = double(0);
 #pragma unroll
         for (int x = 0; x < D; ++x) {
@@ -560,7 +562,7 @@ __device__ __forceinline__ void cell_residual(const double *__restrict__ tab, co
         }
       }
       const double a = st.a;
-      const double i2a = 0.5 / a, i2a2 = i2a / a;
+      const double i2a = 0.5 * st.inv_a, i2a2 = i2a * st.inv_a;
       double l0v[M];
       l0v[0] = 0.5 * gm1 * st.v2;
 #pragma unroll
@@ -1272,6 +1274,75 @@ rd_patch_batch_kernel(const PatchArgs *__restrict__ args, const uint32_t *__rest
   int i = 0;
   while (i + 1 < count && first[i + 1] <= g) ++i;
   patch_body<D, NG, ND, NF, NQ, AFFINE, PH, SC, PREC>(args[i], g - first[i], lds);
+}
+
+// All sweeps of one pdrd_iterate call in one cooperative launch (scalar
+// physics, one element type, chained sweeps): every workgroup runs patches
+// blockIdx.x, blockIdx.x + gridDim.x, ... of a sweep, then waits at a grid
+// barrier for the others before the next sweep reads their rows. The row
+// buffers and shared-node states alternate as between the launches of
+// chained sweeps (sweep it writes rows[it & 1] and reads rows[(it - 1) & 1]);
+// the last sweep's shared nodes are summed by the fix-up kernel afterwards.
+struct PersistArgs
+{
+  double *rows[2];
+  double *base[2];
+  uint32_t *sync;  // arrival counter, zero at the launch
+  int n_iter;
+  int outputs;     // outputs of the last sweep
+  uint32_t polls;  // polls of the counter before a barrier gives up
+};
+
+// Grid barrier: every workgroup adds one arrival and polls until the count
+// reaches `target`, at most `polls` times (then it gives up: false). The
+// fences make the sweep's global writes visible to every XCD before the
+// arrival and drop stale cache lines after the wait.
+__device__ __forceinline__ bool grid_barrier(uint32_t *cnt, uint32_t target, uint32_t polls, int *flag)
+{
+  __threadfence();
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    __hip_atomic_fetch_add(cnt, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    int ok = 1;
+    uint32_t i = 0;
+    while (__hip_atomic_load(cnt, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < target) {
+      if (++i > polls) {
+        ok = 0;
+        break;
+      }
+      __builtin_amdgcn_s_sleep(2);
+    }
+    *flag = ok;
+  }
+  __syncthreads();
+  __threadfence();
+  return *flag != 0;
+}
+
+template <int D, int NG, int ND, int NF, int NQ, bool AFFINE, int PH, int SC>
+__global__ __launch_bounds__((PatchSize<PhysTraits<PH>::M, ND>::value),
+                             (PatchMinWaves<PhysTraits<PH>::M, ND, AFFINE, SC, 0>::value)) void
+rd_patch_persistent_kernel(PatchArgs a, PersistArgs p)
+{
+  extern __shared__ double lds[];
+  __shared__ int flag;
+  for (int it = 0; it < p.n_iter; ++it) {
+    const int w = it & 1;
+    const bool last = it == p.n_iter - 1;
+    a.slot_acc = p.rows[w];
+    a.prev_acc = it ? p.rows[w ^ 1] : nullptr;
+    a.base_in = it ? p.base[w ^ 1] : nullptr;
+    a.base_out = last ? nullptr : p.base[w];
+    a.outputs = last ? p.outputs : 0;
+    for (uint32_t pl = blockIdx.x; pl < a.np; pl += gridDim.x) {
+      patch_body<D, NG, ND, NF, NQ, AFFINE, PH, SC, 0>(a, pl, lds);
+      __syncthreads();  // the next patch reuses the LDS
+    }
+    if (!last && !grid_barrier(p.sync, static_cast<uint32_t>(it + 1) * gridDim.x, p.polls, &flag)) {
+      if (threadIdx.x == 0) atomicOr(a.status, 4);
+      return;
+    }
+  }
 }
 
 struct FixupArgs

@@ -36,12 +36,45 @@ struct PhysParams
   double p0, p1;  // advection velocity (ax, ay) -- for PH_ROTATION2D the point's -- or (gamma, R)
 };
 
-// Thermodynamic state at a point (Euler). T is the arithmetic type of the
-// interpolated state.
+// 1/x for |x| and 1/|x| in the normal range: the hardware reciprocal
+// estimate (relative error below 2^-22) refined by two Newton steps
+// y <- y + y (1 - x y), each of which squares the relative error, so the
+// result is within an ulp or two of the IEEE quotient at a third of the
+// instructions of the IEEE division sequence. x = 0 gives NaN.
+__device__ __forceinline__ double recip(double x)
+{
+  double y = __builtin_amdgcn_rcp(x);
+  double e = fma(-x, y, 1.0);
+  y = fma(y, e, y);
+  e = fma(-x, y, 1.0);
+  return fma(y, e, y);
+}
+
+// sqrt(x) and 1/sqrt(x) for x > 0 in the normal range, from one hardware
+// reciprocal square root estimate y: Goldschmidt's coupled iteration on
+// g = x y -> sqrt(x) and h = y / 2 -> 1 / (2 sqrt(x)) (r = 1/2 - g h,
+// g <- g + g r, h <- h + h r; two steps take the relative error from 2^-22
+// below 2^-60), then one correction of g from its residual x - g^2. Replaces
+// a square root and a division. x = 0 gives NaN for both.
+__device__ __forceinline__ void sqrt_rsqrt(double x, double &s, double &rs)
+{
+  const double y = __builtin_amdgcn_rsq(x);
+  double g = x * y, h = 0.5 * y;
+  double r = fma(-g, h, 0.5);
+  g = fma(g, r, g);
+  h = fma(h, r, h);
+  r = fma(-g, h, 0.5);
+  g = fma(g, r, g);
+  h = fma(h, r, h);
+  s = fma(fma(-g, g, x), h, g);
+  rs = 2.0 * h;
+}
+
+// Thermodynamic state at a point (Euler); inv_a = 1 / a.
 template <typename T, int D>
 struct EulerState
 {
-  T rho, inv_rho, V[D], v2, P, H, a;
+  T rho, inv_rho, V[D], v2, P, H, a, inv_a;
   bool bad;
 };
 
@@ -51,7 +84,7 @@ __device__ __forceinline__ EulerState<T, D> euler_state(const T *s, T gamma)
   EulerState<T, D> st;
   const T gm1 = gamma - T(1);
   st.rho = s[0];
-  st.inv_rho = T(1) / st.rho;
+  st.inv_rho = recip(st.rho);
   T v2 = T(0);
 #pragma unroll
   for (int d = 0; d < D; ++d) {
@@ -64,7 +97,7 @@ __device__ __forceinline__ EulerState<T, D> euler_state(const T *s, T gamma)
   st.bad = !(st.P > T(0));
   const T RT = st.P * st.inv_rho;
   st.H = rhoE * st.inv_rho + RT;
-  st.a = sqrt(gamma * RT);
+  sqrt_rsqrt(gamma * RT, st.a, st.inv_a);
   return st;
 }
 
