@@ -1294,28 +1294,30 @@ struct PersistArgs
 };
 
 // Grid barrier: every workgroup adds one arrival and polls until the count
-// reaches `target`, at most `polls` times (then it gives up: false). The
-// fences make the sweep's global writes visible to every XCD before the
-// arrival and drop stale cache lines after the wait.
+// reaches `target`, at most `polls` times (then it gives up: false). One
+// thread per workgroup synchronises: the workgroup barrier completes the
+// workgroup's stores, the release of the arrival writes its XCD's L2 back so
+// that every XCD sees the sweep, the polls are relaxed (they must not drop
+// the caches that other workgroups still compute from), and one acquire
+// fence after the wait drops the stale lines.
 __device__ __forceinline__ bool grid_barrier(uint32_t *cnt, uint32_t target, uint32_t polls, int *flag)
 {
-  __threadfence();
   __syncthreads();
   if (threadIdx.x == 0) {
     __hip_atomic_fetch_add(cnt, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
     int ok = 1;
     uint32_t i = 0;
-    while (__hip_atomic_load(cnt, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < target) {
+    while (__hip_atomic_load(cnt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
       if (++i > polls) {
         ok = 0;
         break;
       }
       __builtin_amdgcn_s_sleep(2);
     }
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
     *flag = ok;
   }
   __syncthreads();
-  __threadfence();
   return *flag != 0;
 }
 
