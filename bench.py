@@ -153,12 +153,8 @@ def serving_bench(args):
     warm.run(small, {q.rid: S.build_problem(q) for q in small}, mode="batched")
     out = {}
     modes = args.modes.split(",")
-    if args.persistent:
-        # each mode with direct launches and with the persistent kernel for
-        # the requests that run alone
-        modes = [f"{m}{sfx}" for m in modes for sfx in ("", "+persistent")]
     for mode in modes:
-        srv = S.Server(S.GpuExecutor(persistent=mode.endswith("+persistent")), chunk=50, max_iters=args.max_iters)
+        srv = S.Server(S.GpuExecutor(), chunk=50, max_iters=args.max_iters)
         done = [0, time.perf_counter()]
 
         def progress(res, mode=mode, done=done):
@@ -168,11 +164,11 @@ def serving_bench(args):
                 print(f"[bench] c5 {mode}: {done[0]} of {len(reqs)} requests done (request {res.rid}, "
                       f"{res.family}, {res.iterations} iterations)", file=sys.stderr, flush=True)
 
-        res, peak = srv.run(reqs, probs, mode=mode.split("+")[0], on_finish=progress)
+        res, peak = srv.run(reqs, probs, mode=mode, on_finish=progress)
         out[mode] = S.summary(res)
         out[mode]["peak_footprint_bytes"] = int(peak)
         print(f"[bench] c5 {mode}: {out[mode]}", file=sys.stderr, flush=True)
-    if len(modes) < 2 or args.persistent:
+    if len(modes) < 2:
         print(json.dumps({"metric": "p99 time-to-converge (serving stream)", "modes": out,
                           "config": {"workload": "serving_c5", "requests": len(reqs), "size_scale": sc,
                                      "euler_cells": [e0, e1]}}), flush=True)
@@ -229,9 +225,6 @@ def main():
     ap.add_argument("--chain", default="default", choices=["default", "0", "1"],
                     help="patch assembly: shared-node sums of a sweep in the next sweep's gather (1) or a "
                          "fix-up pass after every sweep (0); default: the library's choice")
-    ap.add_argument("--persistent", action="store_true",
-                    help="time the chained sweeps of one iterate call launched one by one against one "
-                         "cooperative launch with grid barriers (scalar 2D workloads: c1, c2)")
     ap.add_argument("--jacobians", action="store_true",
                     help="time the element Jacobians of the implicit forms (pdrd_element_jacobians) on the "
                          "workload's mesh and state instead of the explicit iterations")
@@ -291,9 +284,6 @@ def main():
         meth.add_boundary_condition("StrongDirichlet", "inflow", wl.dirichlet_nodes, wl.dirichlet_values)
     if args.jacobians:
         jacobian_bench(args, wl, meth)
-        return
-    if args.persistent:
-        persistent_bench(args, wl, meth)
         return
     setup_s = time.perf_counter() - t_setup
     print(f"[bench] {wl.name}: {E} elements, {wl.mesh.nb_nodes} nodes, setup {setup_s:.1f} s", file=sys.stderr,
@@ -465,48 +455,6 @@ def jacobian_bench(args, wl, meth):
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
                      "kernel": "rd_jacobian_rows_kernel" if neq > 1 else "rd_jacobian_kernel",
                      "algorithmic_bytes": int(alg), "avg_launch_us": round(total_ms * 1e3, 2)},
-    }), flush=True)
-
-
-def persistent_bench(args, wl, meth):
-    """Direct launches (one chained sweep per launch) against the persistent
-    kernel (every sweep of the call in one cooperative launch, grid barriers
-    between sweeps): wall time of `steps` sweeps in one iterate call, median
-    of 7 calls after `warmup`, same method, state and CFL; the two must give
-    the same bits."""
-    from pdekit_amd import _lib as L
-
-    E = wl.nb_elements
-    out = {}
-    sols = {}
-    for mode in (0, 1):
-        meth.set_option(L.OPT_PERSISTENT, mode)
-        meth.set_vec_function("solution", wl.state)
-        meth.iterate(wl.cfl, max(2, args.warmup))
-        ts = []
-        for _ in range(7):
-            meth.set_vec_function("solution", wl.state)
-            meth.synchronize()
-            t0 = time.perf_counter()
-            meth.iterate(wl.cfl, args.steps)
-            meth.synchronize()
-            ts.append(time.perf_counter() - t0)
-        sols[mode] = meth.vec_function("solution")
-        t = float(np.median(ts))
-        name = "persistent" if mode else "direct"
-        out[name] = {"us_per_sweep": round(t / args.steps * 1e6, 3), "evals_per_s": round(E * args.steps / t, 1)}
-        if mode:
-            out[name]["workgroups"] = meth.get_option(L.OPT_PERSIST_GRID)
-        print(f"[bench] {name}: {out[name]}", file=sys.stderr, flush=True)
-    meth.check_status()
-    print(json.dumps({
-        "metric": "element residual evals/s", "value": out["persistent"]["evals_per_s"], "unit": "elements/s",
-        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "higher_is_better": True, "dtype": "f64",
-        "data": "synthetic", "config": {"workload": wl.name, "elements": E, "mode": "persistent vs direct",
-                                        "parallelism": "single GPU"},
-        "direct": out["direct"], "persistent": out["persistent"],
-        "speedup": round(out["direct"]["us_per_sweep"] / out["persistent"]["us_per_sweep"], 3),
-        "bitwise_equal": bool(np.array_equal(sols[0], sols[1])),
     }), flush=True)
 
 

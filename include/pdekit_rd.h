@@ -99,21 +99,11 @@ enum pdrd_option {
   PDRD_OPT_REFRESH = 5,    /* mixed precision: every this many iterations (default 50) an FP64
                              sweep with fresh element Jacobians; 0 or 1: every sweep FP64 (the
                              plain iteration, bit for bit) */
-  PDRD_OPT_CHAIN = 6,      /* patch assembly, pdrd_iterate: 1 = the shared nodes of a sweep
+  PDRD_OPT_CHAIN = 6       /* patch assembly, pdrd_iterate: 1 = the shared nodes of a sweep
                              are summed and stepped in the next sweep's gather (one launch
                              per element type and sweep; bitwise the same results), 0 = a
                              fix-up pass after every sweep; -1 (default) = 1 for the scalar
                              physics, 0 for the Euler systems (as measured faster) */
-  PDRD_OPT_PERSISTENT = 7, /* pdrd_iterate: 1 = the chained sweeps of a call run in one
-                             cooperative launch, the workgroups meeting at a grid barrier
-                             between sweeps (bitwise the chained results); scalar 2D physics,
-                             one element type, FP64, no weak BCs, nodal inputs, sources or
-                             timing (PDRD_ENOTSUP otherwise); 0 (default) = one launch per
-                             sweep. A barrier that polls PDRD_OPT_BARRIER_POLLS times without
-                             completing gives up and the call returns PDRD_EHIP */
-  PDRD_OPT_BARRIER_POLLS = 8, /* polls of a grid barrier before it gives up (default 2^20, about
-                             a second) */
-  PDRD_OPT_PERSIST_GRID = 9   /* read only: workgroups of the last persistent launch */
 };
 
 /* ---- FE inputs (src/interpolation, src/mesh/point_set, shape_function) -- */

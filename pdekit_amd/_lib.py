@@ -29,7 +29,6 @@ LDA, N, B, PSI, BPSI = 0, 1, 2, 3, 4
 SOLUTION, RESIDUAL, TIME_STEP, WAVE_SPEED, DUAL_VOLUME, BLEND_COEFF, ART_VISC = 0, 1, 2, 3, 4, 5, 6
 BLEND_LF = 0
 OPT_SCATTER, OPT_PRECISION, OPT_BLOCK, OPT_TIMING, OPT_PATCH_CELLS, OPT_REFRESH, OPT_CHAIN = 0, 1, 2, 3, 4, 5, 6
-OPT_PERSISTENT, OPT_BARRIER_POLLS, OPT_PERSIST_GRID = 7, 8, 9
 WEAK_WALL, WEAK_SUBINLET, WEAK_SUBOUTLET, WEAK_FARFIELD, WEAK_SYMMETRY, WEAK_SUPERINLET, WEAK_DIRICHLET = range(7)
 NOALLOC_SCRATCH_BYTES = 32 << 20
 NOALLOC_MAX_ACC = NOALLOC_SCRATCH_BYTES // 8 - 8192 - 32  # doubles: nb_nodes * (m + 1) at most

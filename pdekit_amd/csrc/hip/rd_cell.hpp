@@ -1276,77 +1276,6 @@ rd_patch_batch_kernel(const PatchArgs *__restrict__ args, const uint32_t *__rest
   patch_body<D, NG, ND, NF, NQ, AFFINE, PH, SC, PREC>(args[i], g - first[i], lds);
 }
 
-// All sweeps of one pdrd_iterate call in one cooperative launch (scalar
-// physics, one element type, chained sweeps): every workgroup runs patches
-// blockIdx.x, blockIdx.x + gridDim.x, ... of a sweep, then waits at a grid
-// barrier for the others before the next sweep reads their rows. The row
-// buffers and shared-node states alternate as between the launches of
-// chained sweeps (sweep it writes rows[it & 1] and reads rows[(it - 1) & 1]);
-// the last sweep's shared nodes are summed by the fix-up kernel afterwards.
-struct PersistArgs
-{
-  double *rows[2];
-  double *base[2];
-  uint32_t *sync;  // arrival counter, zero at the launch
-  int n_iter;
-  int outputs;     // outputs of the last sweep
-  uint32_t polls;  // polls of the counter before a barrier gives up
-};
-
-// Grid barrier: every workgroup adds one arrival and polls until the count
-// reaches `target`, at most `polls` times (then it gives up: false). One
-// thread per workgroup synchronises: the workgroup barrier completes the
-// workgroup's stores, the release of the arrival writes its XCD's L2 back so
-// that every XCD sees the sweep, the polls are relaxed (they must not drop
-// the caches that other workgroups still compute from), and one acquire
-// fence after the wait drops the stale lines.
-__device__ __forceinline__ bool grid_barrier(uint32_t *cnt, uint32_t target, uint32_t polls, int *flag)
-{
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    __hip_atomic_fetch_add(cnt, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-    int ok = 1;
-    uint32_t i = 0;
-    while (__hip_atomic_load(cnt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
-      if (++i > polls) {
-        ok = 0;
-        break;
-      }
-      __builtin_amdgcn_s_sleep(2);
-    }
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-    *flag = ok;
-  }
-  __syncthreads();
-  return *flag != 0;
-}
-
-template <int D, int NG, int ND, int NF, int NQ, bool AFFINE, int PH, int SC>
-__global__ __launch_bounds__((PatchSize<PhysTraits<PH>::M, ND>::value),
-                             (PatchMinWaves<PhysTraits<PH>::M, ND, AFFINE, SC, 0>::value)) void
-rd_patch_persistent_kernel(PatchArgs a, PersistArgs p)
-{
-  extern __shared__ double lds[];
-  __shared__ int flag;
-  for (int it = 0; it < p.n_iter; ++it) {
-    const int w = it & 1;
-    const bool last = it == p.n_iter - 1;
-    a.slot_acc = p.rows[w];
-    a.prev_acc = it ? p.rows[w ^ 1] : nullptr;
-    a.base_in = it ? p.base[w ^ 1] : nullptr;
-    a.base_out = last ? nullptr : p.base[w];
-    a.outputs = last ? p.outputs : 0;
-    for (uint32_t pl = blockIdx.x; pl < a.np; pl += gridDim.x) {
-      patch_body<D, NG, ND, NF, NQ, AFFINE, PH, SC, 0>(a, pl, lds);
-      __syncthreads();  // the next patch reuses the LDS
-    }
-    if (!last && !grid_barrier(p.sync, static_cast<uint32_t>(it + 1) * gridDim.x, p.polls, &flag)) {
-      if (threadIdx.x == 0) atomicOr(a.status, 4);
-      return;
-    }
-  }
-}
-
 struct FixupArgs
 {
   const uint32_t *__restrict__ nodes;     // [nb] shared nodes

@@ -3,8 +3,6 @@
 
 #include <hip/hip_runtime.h>
 
-#include <algorithm>
-
 #include "rd_cell.hpp"
 #include "rd_jacobian.hpp"
 
@@ -15,8 +13,6 @@ using PatchLauncher = hipError_t (*)(const PatchArgs &, unsigned grid, int cells
 using PatchBatchLauncher = hipError_t (*)(const PatchArgs *args, const uint32_t *first, int count, unsigned grid,
                                           int cells, size_t lds_bytes, hipStream_t);
 using JacobianLauncher = hipError_t (*)(const JacArgs &, unsigned grid, hipStream_t);
-using PersistLauncher = hipError_t (*)(const PatchArgs &, const PersistArgs &, int cells, hipStream_t,
-                                       unsigned *grid_out);
 
 struct CellKey
 {
@@ -38,9 +34,6 @@ struct KernelSet
   PatchLauncher patch_src = nullptr;
   // element residuals and Jacobians of the implicit form (null for PSI)
   JacobianLauncher jacobian = nullptr;
-  // all chained sweeps of a call in one cooperative launch (scalar 2D
-  // physics only; null otherwise)
-  PersistLauncher persist = nullptr;
   int patch_size = 0;             // largest (default) cells per patch workgroup
   size_t (*patch_lds_bytes)(int maxl, int cells) = nullptr;
   size_t (*patch_src_lds_bytes)(int maxl, int cells) = nullptr;
@@ -87,28 +80,6 @@ hipError_t launch_patch_batch(const PatchArgs *args, const uint32_t *first, int 
   hipLaunchKernelGGL((rd_patch_batch_kernel<D, NG, ND, NF, NQ, AFF, PH, SC, PREC>), dim3(grid), dim3(cells), lds, s,
                      args, first, count);
   return hipGetLastError();
-}
-
-// Cooperative launch of the persistent patch kernel: as many workgroups as
-// fit on the device at once (at most one per patch), which the cooperative
-// launch guarantees to be resident together.
-template <int D, int NG, int ND, int NF, int NQ, bool AFF, int PH, int SC>
-hipError_t launch_persist(const PatchArgs &a, const PersistArgs &p, int cells, hipStream_t s, unsigned *grid_out)
-{
-  constexpr int M = PhysTraits<PH>::M;
-  const void *f = reinterpret_cast<const void *>(&rd_patch_persistent_kernel<D, NG, ND, NF, NQ, AFF, PH, SC>);
-  const size_t lds = patch_lds_doubles_prec<NQ, ND, M, 0>(a.maxl, cells) * sizeof(double);
-  int dev = 0, cus = 0, per_cu = 0;
-  hipError_t e = hipGetDevice(&dev);
-  if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-  if (e == hipSuccess) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, f, cells, lds);
-  if (e != hipSuccess) return e;
-  const unsigned grid = std::min<unsigned>(a.np, static_cast<unsigned>(std::max(1, per_cu * cus)));
-  if (grid_out) *grid_out = grid;
-  PatchArgs aa = a;
-  PersistArgs pp = p;
-  void *args[] = {&aa, &pp};
-  return hipLaunchCooperativeKernel(f, dim3(grid), dim3(cells), args, static_cast<unsigned>(lds), s);
 }
 
 template <int D, int NG, int ND, int NF, int NQ, bool AFF, int PH, int SC>
@@ -160,7 +131,6 @@ KernelSet kernel_set()
   }
   k.patch_src = &launch_patch<D, NG, ND, NF, NQ, AFF, PH, SC, 3>;
   if constexpr (SC != SC_PSI && SC != SC_BPSI) k.jacobian = &launch_jacobian<D, NG, ND, NF, NQ, AFF, PH, SC>;
-  if constexpr (PH == PH_ADVECTION2D || PH == PH_BURGERS2D) k.persist = &launch_persist<D, NG, ND, NF, NQ, AFF, PH, SC>;
   k.patch_size = PatchSize<PhysTraits<PH>::M, ND>::value;
   k.patch_lds_bytes = &patch_lds_bytes<NQ, ND, PhysTraits<PH>::M>;
   k.patch_src_lds_bytes = &patch_src_lds_bytes<NQ, ND, PhysTraits<PH>::M>;

@@ -155,12 +155,6 @@ struct pdrd_method
   double *d_slot_acc_b = nullptr, *d_base[2] = {nullptr, nullptr};
   uint2 *d_pn_rows = nullptr;  // [slots] rows of a shared slot's node
   int chain = -1;              // PDRD_OPT_CHAIN (-1: the default of the method's kernels)
-  // PDRD_OPT_PERSISTENT: the sweeps of a call in one cooperative launch;
-  // d_sync is its grid barrier's arrival counter (allocated on first use)
-  int persistent = 0;
-  uint32_t persist_polls = 1u << 20;
-  uint32_t *d_sync = nullptr;
-  unsigned persist_grid = 0;
   size_t bytes = 0;
 
   hipStream_t stream = nullptr;
@@ -194,8 +188,6 @@ __global__ void finalize_kernel(uint32_t nn, int m, double *__restrict__ acc_res
   ws[i] = acc_ws[i];
   acc_ws[i] = 0.0;
 }
-
-__global__ void zero_u32_kernel(uint32_t *p) { *p = 0u; }
 
 // Residual of the linear sweeps at the linearisation point without its
 // weak-facet part (the facet kernels run in the linear sweeps too, at the
@@ -849,8 +841,6 @@ void free_device(pdrd_method *h)
     *p = nullptr;
   }
   (void)hipFree(h->d_status);
-  (void)hipFree(h->d_sync);
-  h->d_sync = nullptr;
   (void)hipFree(h->d_dir_nodes);
   (void)hipFree(h->d_is_dir);
   h->d_status = nullptr;
@@ -963,62 +953,8 @@ int iterate_norm(pdrd_method *h, double *norm)
 // gather than the fix-up pass does.
 bool chains(const pdrd_method *h) { return h->chain >= 0 ? h->chain == 1 : h->m == 1; }
 
-// PDRD_OPT_PERSISTENT: the chained sweeps of the call in one cooperative
-// launch of the persistent patch kernel (grid barriers between the sweeps),
-// then the fix-up of the last sweep's shared nodes. The results are those of
-// the chained launches, bit for bit. A barrier that polls more than
-// persist_polls times gives up: the kernel sets status bit 2 and every
-// workgroup returns, and the call reports the time-out.
-int iterate_persistent(pdrd_method *h, double cfl, int n_iter, int excl, double *norm)
-{
-  const char *why = nullptr;
-  if (h->blocks.size() != 1) why = "a mesh of one element type";
-  else if (!h->blocks[0].ks.persist) why = "the scalar 2D physics";
-  else if (!chains(h)) why = "chained sweeps (PDRD_OPT_CHAIN)";
-  else if (mixed(h)) why = "the FP64 iteration";
-  else if (!h->weak.empty()) why = "no weak boundary conditions";
-  else if (h->use_src || nodal_input(h)) why = "no nodal input or source field";
-  else if (h->timing) why = "PDRD_OPT_TIMING off";
-  if (why) return set_error(PDRD_ENOTSUP, std::string("the persistent iteration needs ") + why);
-  if (n_iter > 0) {
-    if (!h->d_sync) HIP_CHECK(hipMalloc(&h->d_sync, 256));
-    HIP_CHECK(hipEventRecord(h->ev[3], h->stream));
-    hipLaunchKernelGGL(zero_u32_kernel, dim3(1), dim3(1), 0, h->stream, h->d_sync);
-    HIP_CHECK(hipGetLastError());
-    Block &b = h->blocks[0];
-    const pdrd::PatchArgs a = patch_args(h, &b, 1, 0, excl, cfl, Chain{}, false);
-    pdrd::PersistArgs p{};
-    p.rows[0] = rows_buffer(h, 0);
-    p.rows[1] = rows_buffer(h, 1);
-    p.base[0] = h->d_base[0];
-    p.base[1] = h->d_base[1];
-    p.sync = h->d_sync;
-    p.n_iter = n_iter;
-    p.outputs = 1;
-    p.polls = h->persist_polls;
-    HIP_CHECK(b.ks.persist(a, p, b.cells, h->stream, &h->persist_grid));
-    if (int rc = launch_fixup(h, 1, 1, excl, cfl, (n_iter - 1) & 1, false)) return rc;
-    HIP_CHECK(hipEventRecord(h->ev[1], h->stream));
-    HIP_CHECK(hipEventSynchronize(h->ev[1]));
-    float ms = 0.f;
-    HIP_CHECK(hipEventElapsedTime(&ms, h->ev[3], h->ev[1]));
-    h->last_cell_ms = ms;
-    h->last_node_ms = 0.0;
-    int st = 0;
-    HIP_CHECK(hipMemcpy(&st, h->d_status, sizeof(int), hipMemcpyDeviceToHost));
-    if (st & 4) {
-      const int zero = 0;
-      HIP_CHECK(hipMemcpy(h->d_status, &zero, sizeof(int), hipMemcpyHostToDevice));
-      return set_error(PDRD_EHIP, "persistent iteration: a grid barrier timed out (the sweeps are incomplete)");
-    }
-  }
-  if (norm) return iterate_norm(h, norm);
-  return PDRD_OK;
-}
-
 int iterate_patches(pdrd_method *h, double cfl, int n_iter, int excl, double *norm)
 {
-  if (h->persistent) return iterate_persistent(h, cfl, n_iter, excl, norm);
   const bool mix = mixed(h);
   if (mix)
     if (int rc = lin_setup(h)) return rc;
@@ -1534,14 +1470,6 @@ int pdrd_set_option(pdrd_method_t h, int option, int value)
     case PDRD_OPT_CHAIN:
       if (value < -1 || value > 1) return set_error(PDRD_EINVAL, "chain is -1 (default), 0 or 1");
       h->chain = value;
-      break;
-    case PDRD_OPT_PERSISTENT:
-      if (value != 0 && value != 1) return set_error(PDRD_EINVAL, "persistent is 0 or 1");
-      h->persistent = value;
-      break;
-    case PDRD_OPT_BARRIER_POLLS:
-      if (value < 0) return set_error(PDRD_EINVAL, "barrier polls must be >= 0");
-      h->persist_polls = static_cast<uint32_t>(value);
       break;
     case PDRD_OPT_PATCH_CELLS:
       if (h->ready) return set_error(PDRD_EINVAL, "cells per patch are set before pdrd_initialize");
@@ -2173,9 +2101,6 @@ int pdrd_get_option(pdrd_method_t h, int option, int *value)
     case PDRD_OPT_TIMING: *value = h->timing; return PDRD_OK;
     case PDRD_OPT_REFRESH: *value = h->refresh; return PDRD_OK;
     case PDRD_OPT_CHAIN: *value = chains(h) ? 1 : 0; return PDRD_OK;
-    case PDRD_OPT_PERSISTENT: *value = h->persistent; return PDRD_OK;
-    case PDRD_OPT_BARRIER_POLLS: *value = static_cast<int>(h->persist_polls); return PDRD_OK;
-    case PDRD_OPT_PERSIST_GRID: *value = static_cast<int>(h->persist_grid); return PDRD_OK;
     case PDRD_OPT_PATCH_CELLS: {
       int c = 0;
       for (const Block &b : h->blocks) c = std::max(c, b.cells);

@@ -189,14 +189,11 @@ class Result:
 
 class GpuExecutor:
     """Requests as PGRDMethodExplicit handles on the current GPU, advanced by
-    pdrd_batch_iterate. persistent: a request advanced alone whose method
-    allows it (scalar physics, no weak boundary conditions) runs the sweeps
-    of a chunk in one cooperative launch (PDRD_OPT_PERSISTENT)."""
+    pdrd_batch_iterate."""
 
-    def __init__(self, persistent: bool = False):
+    def __init__(self):
         from .rdm import Batch
         self.batch = Batch()
-        self.persistent = persistent
 
     def start(self, p: Problem):
         from .rdm import PGRDMethodExplicit
@@ -207,8 +204,6 @@ class GpuExecutor:
         m.set_vec_function("solution", p.u0)
         if p.bc_nodes.size:
             m.add_boundary_condition("StrongDirichlet", "inflow", p.bc_nodes, p.bc_values)
-        if self.persistent and p.physics in (L.ADVECTION2D, L.BURGERS2D) and not p.weak:
-            m.set_option(L.OPT_PERSISTENT, 1)
         m.cfl = p.cfl
         return m
 
