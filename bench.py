@@ -453,7 +453,7 @@ def jacobian_bench(args, wl, meth):
                    "elements": m.nb_cells, "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                     "kernel": "rd_jacobian_rows_kernel" if neq > 1 else "rd_jacobian_kernel",
+                     "kernel": "rd_jacobian_qp_kernel" if neq > 1 else "rd_jacobian_kernel",
                      "algorithmic_bytes": int(alg), "avg_launch_us": round(total_ms * 1e3, 2)},
     }), flush=True)
 

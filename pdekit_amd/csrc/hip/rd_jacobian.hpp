@@ -8,13 +8,15 @@
 //     theta_m J_stab, J_stab,mn = -w K+_m S+^-1 K+_n, J_stab,mm =
 //     w (K+_m - K+_m S+^-1 K+_m);
 // with S+- = sum_n K+-_n, K_n = A(u_q) . grad phi_n, theta the nodal input or
-// the cell-wise theta of equation 0. One thread per cell, FP64. K+- of the
-// Euler equations come from the compact form l0 x + alpha r0 + beta e_n (see
-// cell_residual) applied to the unit vectors; K = K+ + K-. The flux
-// divergence at the quadrature points is the P+1 flux element's, as in
-// cell_residual. Element blocks are written row-major (row = node * m +
-// equation) to the caller's cell slot perm[e]; the Jacobian is accumulated
-// in place, the stabilisation part of B in a scratch block of the same size.
+// the cell-wise theta of equation 0, all in FP64. Scalar cells: one thread
+// per cell (rd_jacobian_kernel; cells of up to 6 nodes accumulate in
+// registers, larger ones in place in the caller's block, the stabilisation
+// part of B in a scratch block of the same size). Euler cells: one lane per
+// (cell, quadrature point) with K+- in their compact form
+// (rd_jacobian_qp_kernel below). The flux divergence at the quadrature
+// points is the P+1 flux element's, as in cell_residual. Element blocks are
+// written row-major (row = node * m + equation) to the caller's cell slot
+// perm[e], or entry-major in FP32 for the mixed-precision form.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -561,23 +563,24 @@ __global__ __launch_bounds__(64) void rd_jacobian_kernel(JacArgs a)
 }
 
 // ---------------------------------------------------------------------------
-// Euler cells: one wave per block row of the element Jacobian. A workgroup
-// holds 64 cells and ND waves; wave m computes, for its 64 cells, the block
-// row of node m, J_mn = sum_q w B_m X_n with B_m = K+_m S^-1 and
-//   LDA: X_n = K_n;  N: -K-_n, plus K+_m on the diagonal block;
-//   B: the LDA block and the stabilisation block -B_m K+_n (plus K+_m on the
-//      diagonal), combined with theta_m once the residuals are summed.
-// Every K is applied in its compact form X = l I + r0 alpha^T + e beta^T
-// (alpha = A1 l0v + A2 f, beta = B1 f + B2 l0v; e = (0, n, V.n),
-// f = (-V.n, n, 0) for the unit normal n of the node, r0 = (1, V, H),
-// l0v = (gm1 |V|^2 / 2, -gm1 V, gm1)), so B_m X_n = l B_m + (B_m r0) alpha^T +
-// (B_m e) beta^T costs about 3 M^2 FMAs and no K is held as a matrix. The
-// per-cell work that is not per row (flux divergence, S and its inverse) is
-// repeated by every wave, so the waves never wait on each other. The rows
-// are accumulated in registers, NCH block columns per pass over the
-// quadrature points, and stored once. The residuals use the same sums:
-// lda_m = w K+_m S+^-1 r, and the N / B parts w K+_m (u_m - S^-1 sum_n X_n u_n)
-// (= sum_{n != m} w B_m X_n (u_m - u_n), since sum_n X_n = S).
+// Euler cells, one lane per (cell, quadrature point): LPC lanes per cell (NQ
+// rounded up to 4, 8 or 16; the extra lanes take the last point with weight
+// zero), so the per-point work -- the K+- parameters of the nodes, S and its
+// inverse -- is done once. Lane q forms its point's share of the residuals
+// and of every block J_mn = w B_m X_n (B_m = K+_m S^-1; X_n = K_n for LDA,
+// -K-_n plus K+_m on the diagonal for N, K_n - theta_m K+_n plus theta_m K+_m
+// on the diagonal for B), all K applied in the compact form
+// l I + r0 alpha^T + e beta^T. The sums over the points run through a
+// wave-private LDS slab: every lane writes its values, then each lane adds
+// the group's values of the entries it owns (a fixed order: deterministic).
+// The lanes of a cell are in one wave, so no workgroup barrier is needed, but
+// the wave waits for its outstanding LDS accesses (s_waitcnt lgkmcnt(0))
+// between the writes and the reads of an exchange and before the slab is
+// written again: without those waits back-to-back exchanges read values of
+// the next one (measured: residuals 20 % off). The flux divergence is split
+// over the lanes by flux node and summed the same way.
+
+// K+- parameters of one node at a quadrature point (compact form)
 template <int D>
 struct NodeQ
 {
@@ -587,26 +590,73 @@ struct NodeQ
   double kl, kn;        // K = K+ + K-: l = V.g, sigma = 0, delta = 2 |g| a (0 where g is tiny)
 };
 
+template <int NQ>
+struct QpLanes
+{
+  static constexpr int value = NQ <= 4 ? 4 : (NQ <= 8 ? 8 : 16);
+};
+
+// Stride of the exchange slab rows (doubles): 64 lanes plus one, so that the
+// lanes of a group, reading one column of consecutive rows, fall in different
+// banks.
+constexpr int QP_XS = 65;
+
+template <int PH>
+struct QpSlab
+{
+  static constexpr int M = PhysTraits<PH>::M;
+  static constexpr int value = (M * M + 3) / 4 * 4 > 16 ? 32 : 16;  // rows per wave: the largest exchange
+};
+
+// out[r] = sum over the lanes l of the group of v_l[q R + r] (q: the lane's
+// index in its group, gl: the group's first lane in the wave).
+template <int LPC, int R>
+__device__ __forceinline__ void group_scatter(double *xw, int lane, int gl, int q, const double (&v)[LPC * R],
+                                              double (&out)[R])
+{
+  // the slab's previous reads (by other lanes) complete before it is
+  // overwritten, and the writes before it is read; the memory clobber keeps
+  // the compiler from moving LDS accesses across the waits
+  asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+#pragma unroll
+  for (int j = 0; j < LPC * R; ++j) xw[j * QP_XS + lane] = v[j];
+  asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    double s = 0.0;
+#pragma unroll
+    for (int l = 0; l < LPC; ++l) s += xw[(q * R + r) * QP_XS + gl + l];
+    out[r] = s;
+  }
+}
+
 template <int D, int NG, int ND, int NF, int NQ, bool AFFINE, int PH, int SC>
-__global__ __launch_bounds__(64 * ND) void rd_jacobian_rows_kernel(JacArgs a)
+__global__ __launch_bounds__(128) void rd_jacobian_qp_kernel(JacArgs a)
 {
   constexpr int M = PhysTraits<PH>::M;
   static_assert(M == D + 2, "Euler cells only");
   constexpr int NDM = ND * M;
-  constexpr int NACC = (SC == SC_B) ? 2 : 1;
-  constexpr int CAP = 64 / (M * M * NACC);
-  constexpr int NCH = CAP < 1 ? 1 : (CAP > ND ? ND : CAP);
-  constexpr int NPASS = (ND + NCH - 1) / NCH;
+  constexpr int LPC = QpLanes<NQ>::value;
+  constexpr int CPB = 128 / LPC;
+  constexpr int RJ = (M * M + LPC - 1) / LPC;
+  constexpr int NT = ND + 2;  // theta inputs: t0 of every node, fint0, vol
+  constexpr int RT = (NT + LPC - 1) / LPC;
+  constexpr int RP = (M + LPC - 1) / LPC;
+  static_assert(LPC * RJ <= QpSlab<PH>::value && LPC * RT <= QpSlab<PH>::value && LPC * RP <= QpSlab<PH>::value &&
+                    LPC <= QpSlab<PH>::value,
+                "exchange slab too small");
   using O = TabOffsets<D, NG, ND, NF, NQ>;
-  // the flux divergence of the workgroup's 64 cells, [NQ][M][64]
-  extern __shared__ double divl[];
-  const int m = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x >> 6));
+  extern __shared__ double xl[];
   const int lane = threadIdx.x & 63;
-  const uint32_t e0 = blockIdx.x * 64u + lane;
-  // lanes past the end work on the last cell and store nothing, so that
-  // every lane reaches the barriers below
+  double *xw = xl + (threadIdx.x >> 6) * QP_XS * QpSlab<PH>::value;
+  const int q = threadIdx.x % LPC;
+  const int gl = lane - q;
+  const uint32_t e0 = blockIdx.x * CPB + threadIdx.x / LPC;
+  // lanes past the end work on the last cell and store nothing
   const bool active = e0 < a.ne;
   const uint32_t e = active ? e0 : a.ne - 1;
+  const bool qa = q < NQ;
+  const int qq = qa ? q : NQ - 1;
   const double *__restrict__ tab = a.tab;
   bool bad = false;
   double X[NG][D];
@@ -634,16 +684,7 @@ __global__ __launch_bounds__(64 * ND) void rd_jacobian_rows_kernel(JacArgs a)
 #pragma unroll
   for (int n = 0; n < ND; ++n) tmax = fmax(tmax, nodal[n]);
   const bool skip = ext_theta && tmax < 1.e-6;
-  double Um[M];
-#pragma unroll
-  for (int k = 0; k < M; ++k) {
-    double t = U[0][k];
-#pragma unroll
-    for (int n = 1; n < ND; ++n) t = (n == m) ? U[n][k] : t;
-    Um[k] = t;
-  }
-  // geometry at quadrature point q
-  auto geometry = [&](const int q, double (&Ji)[D][D]) {
+  auto geometry = [&](const int qp, double (&Ji)[D][D]) {
     double J[D][D];
 #pragma unroll
     for (int xi = 0; xi < D; ++xi)
@@ -651,25 +692,25 @@ __global__ __launch_bounds__(64 * ND) void rd_jacobian_rows_kernel(JacArgs a)
       for (int x = 0; x < D; ++x) {
         double s = 0.0;
 #pragma unroll
-        for (int g = 0; g < NG; ++g) s += tab[O::dVgT + (q * D + xi) * NG + g] * X[g][x];
+        for (int g = 0; g < NG; ++g) s += tab[O::dVgT + (qp * D + xi) * NG + g] * X[g][x];
         J[xi][x] = s;
       }
     return det_inv<D>(J, Ji);
   };
-  double Ja[D][D];
-  const double deta = AFFINE ? geometry(0, Ja) : 0.0;
 
+  // flux divergence: lane q evaluates the flux nodes q, q + LPC, ... and adds
+  // their share at every point; then lane q sums the point q of the group
+  double r[M];
   {
-    // flux divergence of the P+1 flux element at the quadrature points: wave
-    // m takes the flux nodes m, m + ND, ...; the partial sums are added into
-    // LDS in wave order (deterministic)
-    double divr[NQ][M];
+    double divr[LPC][M];
 #pragma unroll
-    for (int q = 0; q < NQ; ++q)
+    for (int t = 0; t < LPC; ++t)
 #pragma unroll
-      for (int k = 0; k < M; ++k) divr[q][k] = 0.0;
+      for (int k = 0; k < M; ++k) divr[t][k] = 0.0;
+    double Ja[D][D];
+    if constexpr (AFFINE) geometry(0, Ja);
 #pragma unroll 1
-    for (int f = m; f < NF; f += ND) {
+    for (int f = q; f < NF; f += LPC) {
       double s[M];
 #pragma unroll
       for (int k = 0; k < M; ++k) {
@@ -681,7 +722,7 @@ __global__ __launch_bounds__(64 * ND) void rd_jacobian_rows_kernel(JacArgs a)
       double F[D][M];
       point_flux<PH, double>(s, a.pp, F, bad);
 #pragma unroll
-      for (int q = 0; q < NQ; ++q) {
+      for (int t = 0; t < NQ; ++t) {
         double Jq[D][D];
         if constexpr (AFFINE) {
 #pragma unroll
@@ -689,415 +730,397 @@ __global__ __launch_bounds__(64 * ND) void rd_jacobian_rows_kernel(JacArgs a)
 #pragma unroll
             for (int y = 0; y < D; ++y) Jq[x][y] = Ja[x][y];
         } else {
-          geometry(q, Jq);
+          geometry(t, Jq);
         }
 #pragma unroll
-        for (int k = 0; k < M; ++k) {
-          double t = divr[q][k];
+        for (int d = 0; d < D; ++d) {
+          double b = 0.0;
 #pragma unroll
-          for (int d = 0; d < D; ++d) {
-            double b = 0.0;
+          for (int xi = 0; xi < D; ++xi) b += Jq[d][xi] * tab[O::dVfT + (f * D + xi) * NQ + t];
 #pragma unroll
-            for (int xi = 0; xi < D; ++xi) b += Jq[d][xi] * tab[O::dVfT + (f * D + xi) * NQ + q];
-            t += b * F[d][k];
-          }
-          divr[q][k] = t;
+          for (int k = 0; k < M; ++k) divr[t][k] += b * F[d][k];
         }
       }
     }
-#pragma unroll 1
-    for (int w = 0; w < ND; ++w) {
-      if (w == m) {
 #pragma unroll
-        for (int q = 0; q < NQ; ++q)
+    for (int k = 0; k < M; ++k) {
+      double v[LPC], o[1];
 #pragma unroll
-          for (int k = 0; k < M; ++k) {
-            double *p = divl + (q * M + k) * 64 + lane;
-            *p = (w == 0) ? divr[q][k] : *p + divr[q][k];
-          }
-      }
-      __syncthreads();
+      for (int t = 0; t < LPC; ++t) v[t] = divr[t][k];
+      group_scatter<LPC, 1>(xw, lane, gl, q, v, o);
+      r[k] = o[0];
     }
   }
 
+  // the point's state, node parameters, S and its inverse
+  double Jinv[D][D];
+  const double detq = geometry(qq, Jinv);
+  const double wj = qa ? tab[O::w + qq] * detq : 0.0;
+  double uq[M];
+#pragma unroll
+  for (int k = 0; k < M; ++k) {
+    double t = 0.0;
+#pragma unroll
+    for (int n = 0; n < ND; ++n) t += tab[O::V + qq * ND + n] * U[n][k];
+    uq[k] = t;
+  }
+  const EulerState<double, D> st = euler_state<double, D>(uq, a.pp.p0);
+  bad = bad || st.bad;
   const double gm1 = a.pp.p0 - 1.0;
-  double lda_m[M], dis_m[M], t0[ND], fint0 = 0.0, vol = 0.0, lmax = 0.0;
+  const double av = st.a;
+  const double i2a = 0.5 * st.inv_a, i2a2 = i2a * st.inv_a;
+  double r0[M], l0v[M];
+  r0[0] = 1.0;
+  l0v[0] = 0.5 * gm1 * st.v2;
 #pragma unroll
-  for (int k = 0; k < M; ++k) lda_m[k] = dis_m[k] = 0.0;
+  for (int x = 0; x < D; ++x) {
+    r0[1 + x] = st.V[x];
+    l0v[1 + x] = -gm1 * st.V[x];
+  }
+  r0[M - 1] = st.H;
+  l0v[M - 1] = gm1;
+  NodeQ<D> nq[ND];
+  double G[ND][D];
+  double lmax = 0.0;
 #pragma unroll
-  for (int n = 0; n < ND; ++n) t0[n] = 0.0;
-  double theta = 0.0;
+  for (int n = 0; n < ND; ++n) {
+    double nn = 0.0;
+#pragma unroll
+    for (int x = 0; x < D; ++x) {
+      double t = 0.0;
+#pragma unroll
+      for (int xi = 0; xi < D; ++xi) t += Jinv[x][xi] * tab[O::dVT + (qq * D + xi) * ND + n];
+      G[n][x] = t;
+      nn += t * t;
+    }
+    double norm, inorm;
+    sqrt_rsqrt(nn, norm, inorm);
+    norm = (nn > 0.0) ? norm : 0.0;
+    const bool tiny = (D == 2) ? (norm < 1.e-9) : (norm == 0.0);
+    inorm = tiny ? 0.0 : inorm;
+    double um = 0.0;
+#pragma unroll
+    for (int x = 0; x < D; ++x) {
+      nq[n].nh[x] = G[n][x] * inorm;
+      um += st.V[x] * nq[n].nh[x];
+    }
+    nq[n].um = um;
+    const double e0p = norm * um, ep = norm * (um + av), em = norm * (um - av);
+    lmax = fmax(lmax, fmax(fabs(e0p), fmax(fabs(ep), fabs(em))));
+    const double pp0 = fmax(0.0, e0p), mp0 = fmin(0.0, e0p);
+    const double dpp = tiny ? 0.0 : fmax(0.0, ep) - pp0, dmp = tiny ? 0.0 : fmax(0.0, em) - pp0;
+    const double dpm = tiny ? 0.0 : fmin(0.0, ep) - mp0, dmm = tiny ? 0.0 : fmin(0.0, em) - mp0;
+    nq[n].lp = tiny ? 0.0 : pp0;
+    nq[n].sgp = dpp + dmp;
+    nq[n].dlp = dpp - dmp;
+    nq[n].lm = tiny ? 0.0 : mp0;
+    nq[n].sgm = dpm + dmm;
+    nq[n].dlm = dpm - dmm;
+    nq[n].kl = tiny ? 0.0 : e0p;
+    nq[n].kn = tiny ? 0.0 : norm;
+  }
+  // compact coefficients (l, A1, A2, B1, B2) of K+ (sel 0), K- (1), K (2)
+  auto coef = [&](const NodeQ<D> &k, int sel, double (&cf)[5]) {
+    if (sel == 2) {
+      cf[0] = k.kl;
+      cf[1] = 0.0;
+      cf[2] = k.kn;
+      cf[3] = 0.0;
+      cf[4] = k.kn;
+      return;
+    }
+    const double l = sel == 0 ? k.lp : k.lm;
+    const double sg = sel == 0 ? k.sgp : k.sgm;
+    const double dl = sel == 0 ? k.dlp : k.dlm;
+    cf[0] = l;
+    cf[1] = sg * i2a2;
+    cf[2] = av * dl * i2a2;
+    cf[3] = 0.5 * sg;
+    cf[4] = dl * i2a;
+  };
+  auto evec = [&](const NodeQ<D> &k, double (&ev)[M], double (&fv)[M]) {
+    ev[0] = 0.0;
+    fv[0] = -k.um;
+#pragma unroll
+    for (int x = 0; x < D; ++x) ev[1 + x] = fv[1 + x] = k.nh[x];
+    ev[M - 1] = k.um;
+    fv[M - 1] = 0.0;
+  };
+  auto apply = [&](const NodeQ<D> &k, int sel, const double (&y)[M], double (&out)[M]) {
+    double cf[5], ev[M], fv[M];
+    coef(k, sel, cf);
+    evec(k, ev, fv);
+    double P = 0.0, Q = 0.0;
+#pragma unroll
+    for (int j = 0; j < M; ++j) {
+      P += l0v[j] * y[j];
+      Q += fv[j] * y[j];
+    }
+    const double al = cf[1] * P + cf[2] * Q, be = cf[3] * Q + cf[4] * P;
+#pragma unroll
+    for (int i = 0; i < M; ++i) out[i] = cf[0] * y[i] + al * r0[i] + be * ev[i];
+  };
+  const int ssel = (SC == SC_N) ? 1 : 0;
+  double Si[M][M];
+  {
+    double S[M][M];
+#pragma unroll
+    for (int i = 0; i < M; ++i)
+#pragma unroll
+      for (int j = 0; j < M; ++j) {
+        S[i][j] = 0.0;
+        Si[i][j] = (i == j) ? 1.0 : 0.0;
+      }
+#pragma unroll
+    for (int n = 0; n < ND; ++n) {
+      double cf[5], ev[M], fv[M];
+      coef(nq[n], ssel, cf);
+      evec(nq[n], ev, fv);
+#pragma unroll
+      for (int j = 0; j < M; ++j) {
+        const double al = cf[1] * l0v[j] + cf[2] * fv[j];
+        const double be = cf[3] * fv[j] + cf[4] * l0v[j];
+#pragma unroll
+        for (int i = 0; i < M; ++i) S[i][j] += ((i == j) ? cf[0] : 0.0) + r0[i] * al + ev[i] * be;
+      }
+    }
+    solve_pp<M, M>(S, Si);
+  }
 
-#pragma unroll 1
-  for (int pass = 0; pass < NPASS; ++pass) {
-    const int n0 = pass * NCH;
-    double acc[NACC][NCH][M][M];
+  // the point's residual shares: LDA part w K+_n S+^-1 r, N / B part
+  // w K+_n (u_n - S^-1 sum X_n u_n)
+  constexpr bool HL = SC != SC_N, HD = SC != SC_LDA;
+  double lda[HL ? ND : 1][M], dis[HD ? ND : 1][M];
+  if constexpr (HL) {
+    double x[M];
 #pragma unroll
-    for (int t = 0; t < NACC; ++t)
+    for (int i = 0; i < M; ++i) {
+      double t = 0.0;
 #pragma unroll
-      for (int j = 0; j < NCH; ++j)
+      for (int k = 0; k < M; ++k) t += Si[i][k] * r[k];
+      x[i] = t;
+    }
 #pragma unroll
-        for (int r = 0; r < M; ++r)
+    for (int n = 0; n < ND; ++n) {
+      double y[M];
+      apply(nq[n], 0, x, y);
 #pragma unroll
-          for (int cc = 0; cc < M; ++cc) acc[t][j][r][cc] = 0.0;
-
-#pragma unroll 1
-    for (int q = 0; q < NQ; ++q) {
-      double Jinv[D][D];
-      double detq;
-      if constexpr (AFFINE) {
+      for (int k = 0; k < M; ++k) lda[n][k] = wj * y[k];
+    }
+    if (visc) {
+      double nuq = 0.0;
 #pragma unroll
-        for (int x = 0; x < D; ++x)
+      for (int n = 0; n < ND; ++n) nuq += tab[O::V + qq * ND + n] * nodal[n];
+      double gu[D][M];
 #pragma unroll
-          for (int y = 0; y < D; ++y) Jinv[x][y] = Ja[x][y];
-        detq = deta;
-      } else {
-        detq = geometry(q, Jinv);
-      }
-      const double wj = tab[O::w + q] * detq;
-      double uq[M];
+      for (int x = 0; x < D; ++x)
 #pragma unroll
-      for (int k = 0; k < M; ++k) {
-        double t = 0.0;
-#pragma unroll
-        for (int n = 0; n < ND; ++n) t += tab[O::V + q * ND + n] * U[n][k];
-        uq[k] = t;
-      }
-      const EulerState<double, D> st = euler_state<double, D>(uq, a.pp.p0);
-      bad = bad || st.bad;
-      const double av = st.a;
-      const double i2a = 0.5 * st.inv_a, i2a2 = i2a * st.inv_a;
-      double r0[M], l0v[M];
-      r0[0] = 1.0;
-      l0v[0] = 0.5 * gm1 * st.v2;
-#pragma unroll
-      for (int x = 0; x < D; ++x) {
-        r0[1 + x] = st.V[x];
-        l0v[1 + x] = -gm1 * st.V[x];
-      }
-      r0[M - 1] = st.H;
-      l0v[M - 1] = gm1;
-      // node parameters at q
-      NodeQ<D> nq[ND];
-      double G[ND][D];
-#pragma unroll
-      for (int n = 0; n < ND; ++n) {
-        double nn = 0.0;
-#pragma unroll
-        for (int x = 0; x < D; ++x) {
+        for (int k = 0; k < M; ++k) {
           double t = 0.0;
 #pragma unroll
-          for (int xi = 0; xi < D; ++xi) t += Jinv[x][xi] * tab[O::dVT + (q * D + xi) * ND + n];
-          G[n][x] = t;
-          nn += t * t;
+          for (int n = 0; n < ND; ++n) t += G[n][x] * U[n][k];
+          gu[x][k] = t;
         }
-        double norm, inorm;
-        sqrt_rsqrt(nn, norm, inorm);
-        norm = (nn > 0.0) ? norm : 0.0;
-        const bool tiny = (D == 2) ? (norm < 1.e-9) : (norm == 0.0);
-        inorm = tiny ? 0.0 : inorm;
-        double um = 0.0;
 #pragma unroll
-        for (int x = 0; x < D; ++x) {
-          nq[n].nh[x] = G[n][x] * inorm;
-          um += st.V[x] * nq[n].nh[x];
+      for (int n = 0; n < ND; ++n)
+#pragma unroll
+        for (int k = 0; k < M; ++k) {
+          double t = 0.0;
+#pragma unroll
+          for (int x = 0; x < D; ++x) t += G[n][x] * gu[x][k];
+          lda[n][k] += wj * nuq * t;
         }
-        nq[n].um = um;
-        const double e0 = norm * um, ep = norm * (um + av), em = norm * (um - av);
-        lmax = fmax(lmax, fmax(fabs(e0), fmax(fabs(ep), fabs(em))));
-        const double pp0 = fmax(0.0, e0), mp0 = fmin(0.0, e0);
-        const double dpp = tiny ? 0.0 : fmax(0.0, ep) - pp0, dmp = tiny ? 0.0 : fmax(0.0, em) - pp0;
-        const double dpm = tiny ? 0.0 : fmin(0.0, ep) - mp0, dmm = tiny ? 0.0 : fmin(0.0, em) - mp0;
-        nq[n].lp = tiny ? 0.0 : pp0;
-        nq[n].sgp = dpp + dmp;
-        nq[n].dlp = dpp - dmp;
-        nq[n].lm = tiny ? 0.0 : mp0;
-        nq[n].sgm = dpm + dmm;
-        nq[n].dlm = dpm - dmm;
-        nq[n].kl = tiny ? 0.0 : e0;
-        nq[n].kn = tiny ? 0.0 : norm;
+    }
+  }
+  if constexpr (HD) {
+    double su[M], ub[M], y[M];
+#pragma unroll
+    for (int k = 0; k < M; ++k) su[k] = 0.0;
+#pragma unroll
+    for (int n = 0; n < ND; ++n) {
+      apply(nq[n], ssel, U[n], y);
+#pragma unroll
+      for (int k = 0; k < M; ++k) su[k] += y[k];
+    }
+#pragma unroll
+    for (int i = 0; i < M; ++i) {
+      double t = 0.0;
+#pragma unroll
+      for (int k = 0; k < M; ++k) t += Si[i][k] * su[k];
+      ub[i] = t;
+    }
+#pragma unroll
+    for (int n = 0; n < ND; ++n) {
+      double du[M];
+#pragma unroll
+      for (int k = 0; k < M; ++k) du[k] = U[n][k] - ub[k];
+      apply(nq[n], 0, du, y);
+#pragma unroll
+      for (int k = 0; k < M; ++k) dis[n][k] = wj * y[k];
+    }
+  }
+
+  // sums over the points: theta inputs (t0 of every node, fint0, vol) to
+  // every lane, the largest eigenvalue, then theta
+  double tot[NT];
+  {
+    double v[LPC * RT], o[RT];
+#pragma unroll
+    for (int t = 0; t < LPC * RT; ++t) v[t] = 0.0;
+    if constexpr (SC == SC_B)
+#pragma unroll
+      for (int n = 0; n < ND; ++n) v[n] = lda[n][0] + dis[n][0];
+    v[ND] = wj * r[0];
+    v[ND + 1] = wj;
+    group_scatter<LPC, RT>(xw, lane, gl, q, v, o);
+#pragma unroll
+    for (int rr = 0; rr < RT; ++rr) xw[(q * RT + rr) * QP_XS + gl] = o[rr];
+    asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+#pragma unroll
+    for (int t = 0; t < NT; ++t) tot[t] = xw[t * QP_XS + gl];
+    asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+    xw[lane] = lmax;
+    asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+    double lm = 0.0;
+#pragma unroll
+    for (int l = 0; l < LPC; ++l) lm = fmax(lm, xw[gl + l]);
+    lmax = lm;
+  }
+  double th[ND];
+#pragma unroll
+  for (int n = 0; n < ND; ++n) th[n] = 0.0;
+  if constexpr (SC == SC_B) {
+    if (ext_theta) {
+#pragma unroll
+      for (int n = 0; n < ND; ++n) th[n] = skip ? 0.0 : nodal[n];
+    } else {
+      double an = 0.0;
+#pragma unroll
+      for (int n = 0; n < ND; ++n) an += fabs(tot[n]);
+      const double t0 = fabs(tot[ND]) / (an + 1.e-6);
+#pragma unroll
+      for (int n = 0; n < ND; ++n) th[n] = t0;
+    }
+  }
+
+  // residuals: lane q stores the entries q RP ... of every node
+  if (a.phi) {
+    double *phi = a.phi + static_cast<size_t>(c) * NDM;
+#pragma unroll
+    for (int n = 0; n < ND; ++n) {
+      double v[LPC * RP], o[RP];
+#pragma unroll
+      for (int t = 0; t < LPC * RP; ++t) {
+        double val = 0.0;
+        if (t < M) {
+          if constexpr (SC == SC_LDA) val = lda[n][t];
+          else if constexpr (SC == SC_N) val = dis[n][t];
+          else val = lda[n][t] + th[n] * dis[n][t];
+        }
+        v[t] = val;
       }
-      // compact coefficients (l, A1, A2, B1, B2) of K+ (sel 0), K- (1), K (2)
-      auto coef = [&](const NodeQ<D> &k, int sel, double (&cf)[5]) {
-        if (sel == 2) {
-          // K = (V.g) I + |g| (r0 f^T + e l0v^T)
-          cf[0] = k.kl;
-          cf[1] = 0.0;
-          cf[2] = k.kn;
-          cf[3] = 0.0;
-          cf[4] = k.kn;
-          return;
+      group_scatter<LPC, RP>(xw, lane, gl, q, v, o);
+#pragma unroll
+      for (int rr = 0; rr < RP; ++rr) {
+        const int k = q * RP + rr;
+        if (active && k < M) phi[n * M + k] = o[rr];
+      }
+    }
+  }
+  if (active && q == 0 && a.wave) a.wave[c] = tot[ND + 1] * fmax(lmax, 1.e-6);
+
+  // Jacobian blocks, row block by row block
+#pragma unroll 1
+  for (int m = 0; m < ND; ++m) {
+    NodeQ<D> km = nq[0];
+    double thm = th[0];
+#pragma unroll
+    for (int n = 1; n < ND; ++n)
+      if (n == m) {
+        km = nq[n];
+        thm = th[n];
+      }
+    double cfm[5], evm[M], fvm[M];
+    coef(km, 0, cfm);
+    evec(km, evm, fvm);
+    // B_m = K+_m Si = l Si + r0 (alpha^T Si) + e (beta^T Si)
+    double Bm[M][M];
+    {
+      double pa[M], pb[M];
+#pragma unroll
+      for (int j = 0; j < M; ++j) {
+        double sl = 0.0, sf = 0.0;
+#pragma unroll
+        for (int k = 0; k < M; ++k) {
+          sl += l0v[k] * Si[k][j];
+          sf += fvm[k] * Si[k][j];
         }
-        const double l = sel == 0 ? k.lp : k.lm;
-        const double sg = sel == 0 ? k.sgp : k.sgm;
-        const double dl = sel == 0 ? k.dlp : k.dlm;
-        cf[0] = l;
-        cf[1] = sg * i2a2;
-        cf[2] = av * dl * i2a2;
-        cf[3] = 0.5 * sg;
-        cf[4] = dl * i2a;
-      };
-      auto evec = [&](const NodeQ<D> &k, double (&ev)[M], double (&fv)[M]) {
-        ev[0] = 0.0;
-        fv[0] = -k.um;
-#pragma unroll
-        for (int x = 0; x < D; ++x) ev[1 + x] = fv[1 + x] = k.nh[x];
-        ev[M - 1] = k.um;
-        fv[M - 1] = 0.0;
-      };
-      // X y
-      auto apply = [&](const NodeQ<D> &k, int sel, const double (&y)[M], double (&out)[M]) {
-        double cf[5], ev[M], fv[M];
-        coef(k, sel, cf);
-        evec(k, ev, fv);
-        double P = 0.0, Q = 0.0;
-#pragma unroll
-        for (int j = 0; j < M; ++j) {
-          P += l0v[j] * y[j];
-          Q += fv[j] * y[j];
-        }
-        const double al = cf[1] * P + cf[2] * Q, be = cf[3] * Q + cf[4] * P;
-#pragma unroll
-        for (int i = 0; i < M; ++i) out[i] = cf[0] * y[i] + al * r0[i] + be * ev[i];
-      };
-      const int ssel = (SC == SC_N) ? 1 : 0;
-      // S = sum_n X_n and its inverse
-      double S[M][M], Si[M][M];
+        pa[j] = cfm[1] * sl + cfm[2] * sf;
+        pb[j] = cfm[3] * sf + cfm[4] * sl;
+      }
 #pragma unroll
       for (int i = 0; i < M; ++i)
 #pragma unroll
-        for (int j = 0; j < M; ++j) {
-          S[i][j] = 0.0;
-          Si[i][j] = (i == j) ? 1.0 : 0.0;
-        }
+        for (int j = 0; j < M; ++j) Bm[i][j] = cfm[0] * Si[i][j] + r0[i] * pa[j] + evm[i] * pb[j];
+    }
+    double cB[M];
 #pragma unroll
-      for (int n = 0; n < ND; ++n) {
-        double cf[5], ev[M], fv[M];
-        coef(nq[n], ssel, cf);
-        evec(nq[n], ev, fv);
+    for (int i = 0; i < M; ++i) {
+      double t = 0.0;
 #pragma unroll
-        for (int j = 0; j < M; ++j) {
-          const double al = cf[1] * l0v[j] + cf[2] * fv[j];
-          const double be = cf[3] * fv[j] + cf[4] * l0v[j];
+      for (int k = 0; k < M; ++k) t += Bm[i][k] * r0[k];
+      cB[i] = t;
+    }
 #pragma unroll
-          for (int i = 0; i < M; ++i) S[i][j] += ((i == j) ? cf[0] : 0.0) + r0[i] * al + ev[i] * be;
-        }
+    for (int n = 0; n < ND; ++n) {
+      double ev[M], fv[M], cf[5];
+      evec(nq[n], ev, fv);
+      if constexpr (SC == SC_LDA) {
+        coef(nq[n], 2, cf);
+      } else if constexpr (SC == SC_N) {
+        coef(nq[n], 1, cf);
+#pragma unroll
+        for (int t = 0; t < 5; ++t) cf[t] = -cf[t];
+      } else {
+        double cp[5];
+        coef(nq[n], 2, cf);
+        coef(nq[n], 0, cp);
+#pragma unroll
+        for (int t = 0; t < 5; ++t) cf[t] -= thm * cp[t];
       }
-      solve_pp<M, M>(S, Si);
-      // node m's parameters
-      NodeQ<D> km = nq[0];
-#pragma unroll
-      for (int n = 1; n < ND; ++n)
-        if (n == m) km = nq[n];
-      // B_m = K+_m Si = l Si + r0 (alpha^T Si) + e (beta^T Si)
-      double Bm[M][M];
-      {
-        double cf[5], ev[M], fv[M];
-        coef(km, 0, cf);
-        evec(km, ev, fv);
-        double pa[M], pb[M];
-#pragma unroll
-        for (int j = 0; j < M; ++j) {
-          double sl = 0.0, sf = 0.0;
-#pragma unroll
-          for (int k = 0; k < M; ++k) {
-            sl += l0v[k] * Si[k][j];
-            sf += fv[k] * Si[k][j];
-          }
-          pa[j] = cf[1] * sl + cf[2] * sf;
-          pb[j] = cf[3] * sf + cf[4] * sl;
-        }
-#pragma unroll
-        for (int i = 0; i < M; ++i)
-#pragma unroll
-          for (int j = 0; j < M; ++j) Bm[i][j] = cf[0] * Si[i][j] + r0[i] * pa[j] + ev[i] * pb[j];
-      }
-      double cB[M];  // B_m r0
+      double dB[M];
 #pragma unroll
       for (int i = 0; i < M; ++i) {
         double t = 0.0;
 #pragma unroll
-        for (int k = 0; k < M; ++k) t += Bm[i][k] * r0[k];
-        cB[i] = t;
+        for (int k = 1; k < M; ++k) t += Bm[i][k] * ev[k];
+        dB[i] = t;
       }
-      // acc[t][j] += w B_m X_n for the pass's block columns
+      // the diagonal block's K+_m term: N weight 1, B weight theta_m
+      const double dw = (n == m) ? ((SC == SC_N) ? 1.0 : ((SC == SC_B) ? thm : 0.0)) : 0.0;
+      double v[LPC * RJ], o[RJ];
 #pragma unroll
-      for (int j = 0; j < NCH; ++j) {
-        const int n = n0 + j;
-        if (n < ND) {
-          NodeQ<D> kn = nq[0];
+      for (int t = M * M; t < LPC * RJ; ++t) v[t] = 0.0;
 #pragma unroll
-          for (int n2 = 1; n2 < ND; ++n2)
-            if (n2 == n) kn = nq[n2];
-          double ev[M], fv[M];
-          evec(kn, ev, fv);
-          double dB[M];  // B_m e_n
+      for (int i = 0; i < M; ++i) {
+        const double ga = cf[1] * cB[i] + cf[4] * dB[i];
+        const double de = cf[2] * cB[i] + cf[3] * dB[i];
 #pragma unroll
-          for (int i = 0; i < M; ++i) {
-            double t = 0.0;
-#pragma unroll
-            for (int k = 1; k < M; ++k) t += Bm[i][k] * ev[k];
-            dB[i] = t;
+        for (int j = 0; j < M; ++j) {
+          double val = cf[0] * Bm[i][j] + ga * l0v[j] + de * fv[j];
+          if constexpr (SC != SC_LDA) {
+            const double al = cfm[1] * l0v[j] + cfm[2] * fvm[j];
+            const double be = cfm[3] * fvm[j] + cfm[4] * l0v[j];
+            val += dw * (((i == j) ? cfm[0] : 0.0) + r0[i] * al + evm[i] * be);
           }
-#pragma unroll
-          for (int t = 0; t < NACC; ++t) {
-            // LDA: K_n; N: K-_n (negated); B: K_n (t = 0), K+_n (t = 1, negated)
-            const int sel = (SC == SC_N) ? 1 : (t == 0 ? 2 : 0);
-            const double sgn = (SC == SC_N || t == 1) ? -wj : wj;
-            double cf[5];
-            coef(kn, sel, cf);
-#pragma unroll
-            for (int i = 0; i < M; ++i) {
-              const double ga = sgn * (cf[1] * cB[i] + cf[4] * dB[i]);
-              const double de = sgn * (cf[2] * cB[i] + cf[3] * dB[i]);
-              const double li = sgn * cf[0];
-#pragma unroll
-              for (int cc = 0; cc < M; ++cc) acc[t][j][i][cc] += li * Bm[i][cc] + ga * l0v[cc] + de * fv[cc];
-            }
-          }
-          if ((SC == SC_N || SC == SC_B) && n == m) {
-            // + w K+_m on the diagonal block (N: the Jacobian; B: its stabilisation part)
-            double cf[5], evm[M], fvm[M];
-            coef(km, 0, cf);
-            evec(km, evm, fvm);
-#pragma unroll
-            for (int cc = 0; cc < M; ++cc) {
-              const double al = cf[1] * l0v[cc] + cf[2] * fvm[cc];
-              const double be = cf[3] * fvm[cc] + cf[4] * l0v[cc];
-#pragma unroll
-              for (int i = 0; i < M; ++i)
-                acc[NACC - 1][j][i][cc] += wj * (((i == cc) ? cf[0] : 0.0) + r0[i] * al + evm[i] * be);
-            }
-          }
+          v[i * M + j] = wj * val;
         }
       }
-      if (pass == 0) {
-        // residuals and wave speed
-        double r[M];
+      group_scatter<LPC, RJ>(xw, lane, gl, q, v, o);
 #pragma unroll
-        for (int k = 0; k < M; ++k) r[k] = divl[(q * M + k) * 64 + lane];
-        vol += wj;
-        fint0 += wj * r[0];
-        if constexpr (SC != SC_N) {
-          double x[M], y[M];
-#pragma unroll
-          for (int i = 0; i < M; ++i) {
-            double t = 0.0;
-#pragma unroll
-            for (int k = 0; k < M; ++k) t += Si[i][k] * r[k];
-            x[i] = t;
-          }
-          apply(km, 0, x, y);
-#pragma unroll
-          for (int k = 0; k < M; ++k) lda_m[k] += wj * y[k];
-          if constexpr (SC == SC_B) {
-#pragma unroll
-            for (int n = 0; n < ND; ++n) {
-              apply(nq[n], 0, x, y);
-              t0[n] += wj * y[0];
-            }
-          }
-        }
-        if constexpr (SC == SC_N || SC == SC_B) {
-          // ubar = Si sum_n X_n u_n
-          double su[M], ub[M], y[M];
-#pragma unroll
-          for (int k = 0; k < M; ++k) su[k] = 0.0;
-#pragma unroll
-          for (int n = 0; n < ND; ++n) {
-            apply(nq[n], ssel, U[n], y);
-#pragma unroll
-            for (int k = 0; k < M; ++k) su[k] += y[k];
-          }
-#pragma unroll
-          for (int i = 0; i < M; ++i) {
-            double t = 0.0;
-#pragma unroll
-            for (int k = 0; k < M; ++k) t += Si[i][k] * su[k];
-            ub[i] = t;
-          }
-          double du[M];
-#pragma unroll
-          for (int k = 0; k < M; ++k) du[k] = Um[k] - ub[k];
-          apply(km, 0, du, y);
-#pragma unroll
-          for (int k = 0; k < M; ++k) dis_m[k] += wj * y[k];
-          if constexpr (SC == SC_B) {
-#pragma unroll
-            for (int n = 0; n < ND; ++n) {
-#pragma unroll
-              for (int k = 0; k < M; ++k) du[k] = U[n][k] - ub[k];
-              apply(nq[n], 0, du, y);
-              t0[n] += wj * y[0];
-            }
-          }
-        }
-        if (visc) {
-          double nuq = 0.0;
-#pragma unroll
-          for (int n = 0; n < ND; ++n) nuq += tab[O::V + q * ND + n] * nodal[n];
-          if (nuq != 0.0) {
-            double Gm[D];
-#pragma unroll
-            for (int x = 0; x < D; ++x) {
-              double t = G[0][x];
-#pragma unroll
-              for (int n = 1; n < ND; ++n) t = (n == m) ? G[n][x] : t;
-              Gm[x] = t;
-            }
-#pragma unroll
-            for (int k = 0; k < M; ++k) {
-              double t = 0.0;
-#pragma unroll
-              for (int n = 0; n < ND; ++n) {
-                double gg = 0.0;
-#pragma unroll
-                for (int x = 0; x < D; ++x) gg += Gm[x] * G[n][x];
-                t += gg * U[n][k];
-              }
-              lda_m[k] += wj * nuq * t;
-            }
-          }
-        }
-      }
-    }
-
-    if (pass == 0) {
-      // theta of B, the residual row of node m, the wave speed
-      if constexpr (SC == SC_B) {
-        if (ext_theta) {
-          double t = nodal[0];
-#pragma unroll
-          for (int n = 1; n < ND; ++n) t = (n == m) ? nodal[n] : t;
-          theta = t;
-        } else {
-          double an = 0.0;
-#pragma unroll
-          for (int n = 0; n < ND; ++n) an += fabs(t0[n]);
-          theta = fabs(fint0) / (an + 1.e-6);
-        }
-      }
-      if (active && a.phi) {
-        double *phi = a.phi + static_cast<size_t>(c) * NDM + m * M;
-#pragma unroll
-        for (int k = 0; k < M; ++k)
-          phi[k] = (SC == SC_N) ? dis_m[k] : ((SC == SC_B && !skip) ? lda_m[k] + theta * dis_m[k] : lda_m[k]);
-      }
-      if (active && m == 0 && a.wave) a.wave[c] = vol * fmax(lmax, 1.e-6);
-    }
-    // store the pass's block columns of row block m
-#pragma unroll
-    for (int j = 0; j < NCH; ++j) {
-      const int n = n0 + j;
-      if (active && n < ND) {
-#pragma unroll
-        for (int i = 0; i < M; ++i)
-#pragma unroll
-          for (int cc = 0; cc < M; ++cc) {
-            double v = acc[0][j][i][cc];
-            if constexpr (SC == SC_B) v += skip ? 0.0 : theta * acc[NACC - 1][j][i][cc];
-            jac.set((m * M + i) * NDM + n * M + cc, v);
-          }
+      for (int rr = 0; rr < RJ; ++rr) {
+        const int t = q * RJ + rr;
+        if (active && t < M * M) jac.set((m * M + t / M) * NDM + n * M + t % M, o[rr]);
       }
     }
   }

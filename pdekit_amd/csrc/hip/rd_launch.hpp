@@ -85,9 +85,14 @@ hipError_t launch_patch_batch(const PatchArgs *args, const uint32_t *first, int 
 template <int D, int NG, int ND, int NF, int NQ, bool AFF, int PH, int SC>
 hipError_t launch_jacobian(const JacArgs &a, unsigned grid, hipStream_t s)
 {
-  if constexpr (PhysTraits<PH>::M > 1)
-    hipLaunchKernelGGL((rd_jacobian_rows_kernel<D, NG, ND, NF, NQ, AFF, PH, SC>), dim3(grid), dim3(64 * ND),
-                       static_cast<size_t>(NQ) * PhysTraits<PH>::M * 64 * sizeof(double), s, a);
+  if constexpr (PhysTraits<PH>::M > 1) {
+    // the Euler cells: LPC lanes per cell, 128 threads per workgroup (grid
+    // from the cell count: `grid` counts workgroups of 64 cells)
+    constexpr int CPB = 128 / QpLanes<NQ>::value;
+    (void)grid;
+    hipLaunchKernelGGL((rd_jacobian_qp_kernel<D, NG, ND, NF, NQ, AFF, PH, SC>), dim3((a.ne + CPB - 1) / CPB), dim3(128),
+                       static_cast<size_t>(2) * QP_XS * QpSlab<PH>::value * sizeof(double), s, a);
+  }
   else
     hipLaunchKernelGGL((rd_jacobian_kernel<D, NG, ND, NF, NQ, AFF, PH, SC>), dim3(grid), dim3(64), 0, s, a);
   return hipGetLastError();
