@@ -78,7 +78,7 @@ enum pdrd_option {
                              cells summed in LDS, shared nodes summed by a fix-up pass;
                              1 and 2 are deterministic */
   PDRD_OPT_PRECISION = 1, /* 0 = plain FP64, 1 = mixed (patch assembly only; not PSI, not
-                             the rotating velocity, not in batches): a linearised defect
+                             in batches): a linearised defect
                              correction. An FP64 sweep sets the linearisation point u_ref
                              with its FP64 residual and time step; the sweeps in between are
                              linear, res = R(u_ref) + sum_e J_e (u - u_ref)_e / V with the
@@ -343,9 +343,12 @@ int pdrd_last_timing(pdrd_method_t h, double *cell_ms, double *node_ms);
  * slower on MI355X, whose direct launches cost less than the replays; p99 of
  * the serving stream 2.67 s direct against 2.87 s from graphs). Each method advances
  * exactly as pdrd_iterate(methods[i], cfl[i], n_iter, exclude_dirichlet)
- * would (same kernels, same order of sums, so the same bits). Methods must be
- * initialised on the batch's device with the patch assembly; norms receives
- * the norms of the methods one after the other (neq doubles each). */
+ * would (same kernels, same order of sums, so the same bits); methods with a
+ * nodal input (blending coefficient, artificial viscosity) or a source field
+ * join the launches of their own kernel. Methods must be initialised on the
+ * batch's device with the patch assembly, in FP64 (PDRD_OPT_PRECISION 0);
+ * norms receives the norms of the methods one after the other (neq doubles
+ * each). */
 typedef struct pdrd_batch *pdrd_batch_t;
 int pdrd_batch_create(pdrd_batch_t *out);
 int pdrd_batch_destroy(pdrd_batch_t b);

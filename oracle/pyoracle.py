@@ -13,7 +13,7 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "liboracle_rd.so")
 
-ADVECTION2D, BURGERS2D, EULER2D, EULER3D = 0, 1, 2, 3
+ADVECTION2D, BURGERS2D, EULER2D, EULER3D, ROTATION2D = 0, 1, 2, 3, 4
 LDA, N, B, PSI, BPSI = 0, 1, 2, 3, 4
 
 
@@ -39,6 +39,7 @@ class OrcWeak(C.Structure):
         ("nodes", C.POINTER(C.c_uint32)), ("verts", C.POINTER(C.c_uint32)), ("inside", C.POINTER(C.c_double)),
         ("w", C.POINTER(C.c_double)), ("V", C.POINTER(C.c_double)),
         ("qstate", C.POINTER(C.c_double)),
+        ("qpoints", C.POINTER(C.c_double)),
     ]
 
 
@@ -203,10 +204,11 @@ class Problem:
         assert rc == 0
         return out
 
-    def add_weak(self, bc_type, params, order, nodes, verts, inside, w, V, qstate=None):
+    def add_weak(self, bc_type, params, order, nodes, verts, inside, w, V, qstate=None, qpoints=None):
         """A weak BC over facets: nodes [nf][nfn], verts [nf][nfv], inside
         [nf][dim] (a point of each facet's cell), facet rule w[nq], V[nq][nfn];
-        qstate [nf][nq][m]: WeakDirichlet's state at every facet point."""
+        qstate [nf][nq][m]: WeakDirichlet's state at every facet point;
+        qpoints [nf][nq][dim]: the facet points (the rotating velocity)."""
         arrs = [np.ascontiguousarray(nodes, dtype=np.uint32), np.ascontiguousarray(verts, dtype=np.uint32),
                 np.ascontiguousarray(inside, dtype=np.float64), np.ascontiguousarray(w, dtype=np.float64),
                 np.ascontiguousarray(V, dtype=np.float64)]
@@ -223,6 +225,10 @@ class Problem:
             qs = np.ascontiguousarray(qstate, dtype=np.float64)
             self._keep.append(qs)
             wk.qstate = _p(qs)
+        if qpoints is not None:
+            qp = np.ascontiguousarray(qpoints, dtype=np.float64)
+            self._keep.append(qp)
+            wk.qpoints = _p(qp)
         self._weak.append(wk)
         arr = (OrcWeak * len(self._weak))(*self._weak)
         self._keep.append(arr)

@@ -921,7 +921,7 @@ int apply_weak(const orc_problem *p, const orc_weak &bc, const double *u, const 
       for (int n = 0; n < bc.nfn; ++n)
         for (int e = 0; e < m; ++e) uq[e] += bc.V[q * bc.nfn + n] * u[static_cast<size_t>(fn[n]) * m + e];
       Props pr;
-      properties(p, uq, pr);
+      properties(p, uq, pr, bc.qpoints ? bc.qpoints + (static_cast<size_t>(f) * bc.nq + q) * dim : nullptr);
       if (pr.bad) rc = -5;
       Mat R, L;
       double Dv[MAXM];

@@ -69,6 +69,9 @@ typedef struct orc_weak
    * (an expression of the point, WeakDirichletBC.hpp:216-229), or NULL for
    * the constant params */
   const double *qstate;
+  /* the coordinates of every facet point [nfacet][nq][dim], or NULL; needed
+   * by the rotating advection, whose velocity is the point's (y, -x) */
+  const double *qpoints;
 } orc_weak;
 
 typedef struct orc_problem

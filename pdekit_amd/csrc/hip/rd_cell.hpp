@@ -207,6 +207,18 @@ struct SourceLds
   __device__ __forceinline__ double operator()(int n, int k) const { return s[row[n] * M + k]; }
 };
 
+// PREC 5 (the B scheme with a source field and an external blending
+// coefficient): source(n, k) as SourceLds, theta(n) from a second LDS column.
+template <int ND, int M>
+struct SourceThetaLds
+{
+  const double *s;       // [local node][M]
+  const double *th;      // [local node]
+  const int (&row)[ND];
+  __device__ __forceinline__ double operator()(int n, int k) const { return s[row[n] * M + k]; }
+  __device__ __forceinline__ double operator()(int n) const { return th[row[n]]; }
+};
+
 template <int ND>
 struct NodalLds
 {
@@ -342,7 +354,7 @@ __device__ __forceinline__ void cell_residual(const double *__restrict__ tab, co
       }
     }
   }
-  if constexpr (PREC == 3) {
+  if constexpr (PREC == 3 || PREC == 5) {
 #pragma unroll
     for (int q = 0; q < NQ; ++q)
 #pragma unroll
@@ -707,7 +719,7 @@ __device__ __forceinline__ void cell_residual(const double *__restrict__ tab, co
     for (int n = 0; n < ND; ++n)
 #pragma unroll
       for (int k = 0; k < M; ++k) phi[n][k] = lda[n][k] + dis[n][k];
-  } else if constexpr (SC == SC_B && PREC == 2) {
+  } else if constexpr (SC == SC_B && (PREC == 2 || PREC == 5)) {
     // external nodal coefficient; below 1e-6 at every node the cell keeps
     // its LDA residuals
     double th[ND];
@@ -885,7 +897,9 @@ struct PatchArgs
   const double *__restrict__ base_in;       // [n_shared][M] states at the start of the previous sweep
   double *__restrict__ base_out;            // [n_shared][M] states at the start of this sweep, or null
   double *__restrict__ norm_partial;        // [np][M + 1]
-  const double *__restrict__ nodal;         // [N] blending coefficient (B) / artificial viscosity (LDA), PREC 2
+  const double *__restrict__ nodal;         // [N] blending coefficient (B) / artificial viscosity (LDA), PREC 2;
+                                            // [N][M] source field, PREC 3 and 5
+  const double *__restrict__ theta;         // [N] PREC 5: the B scheme's external blending coefficient
   int *__restrict__ status;
   uint32_t ne, np, first_patch;             // first_patch: offset of this block's patches
   int maxl;                                 // LDS rows of node state
@@ -967,7 +981,8 @@ __host__ __device__ constexpr size_t patch_lds_doubles_prec(int maxl, int P)
     return static_cast<size_t>(maxl) * M + patch_coff_doubles(maxl) + patch_lin_work_doubles<NQ, ND, M>(P) +
            (static_cast<size_t>(maxl) * M + 1) / 2;
   return patch_lds_doubles<NQ, ND, M>(maxl, P) +
-         (PREC == 2 ? static_cast<size_t>(maxl) : (PREC == 3 ? static_cast<size_t>(maxl) * M : 0));
+         (PREC == 2 ? static_cast<size_t>(maxl)
+                    : (PREC == 3 ? static_cast<size_t>(maxl) * M : (PREC == 5 ? static_cast<size_t>(maxl) * (M + 1) : 0)));
 }
 
 // Finishes node g from its sums acc[M + 1] (residual sum, wave-speed sum);
@@ -1132,9 +1147,10 @@ __device__ __forceinline__ void patch_body(const PatchArgs &a, uint32_t pl, doub
         delta_l[l * M + k] = static_cast<float>(un[k] - a.u_ref[static_cast<size_t>(g) * M + k]);
     coff[l] = a.pn_coff[s0 + l];
     if constexpr (PREC == 2) nodal_l[l] = a.nodal[g];
-    if constexpr (PREC == 3)
+    if constexpr (PREC == 3 || PREC == 5)
 #pragma unroll
       for (int k = 0; k < M; ++k) nodal_l[l * M + k] = a.nodal[static_cast<size_t>(g) * M + k];
+    if constexpr (PREC == 5) nodal_l[static_cast<size_t>(a.maxl) * M + l] = a.theta[g];
   }
   __syncthreads();
 
@@ -1180,7 +1196,15 @@ __device__ __forceinline__ void patch_body(const PatchArgs &a, uint32_t pl, doub
 #pragma unroll
       for (int k = 0; k < M; ++k) U[n][k] = ul[row[n] * M + k];
     }
-    if constexpr (PREC == 3 && M == 1)
+    if constexpr (PREC == 5 && M == 1)
+      cell_residual<D, NG, ND, NF, NQ, AFFINE, PH, SC, PREC>(
+          a.tab, X, U, URegs<ND, M>{U}, a.pp, phi, wave, bad, work + tid, PATCH,
+          SourceThetaLds<ND, M>{nodal_l, nodal_l + static_cast<size_t>(a.maxl) * M, row});
+    else if constexpr (PREC == 5)
+      cell_residual<D, NG, ND, NF, NQ, AFFINE, PH, SC, PREC>(
+          a.tab, X, U, ULds<ND, M>{ul, row}, a.pp, phi, wave, bad, work + tid, PATCH,
+          SourceThetaLds<ND, M>{nodal_l, nodal_l + static_cast<size_t>(a.maxl) * M, row});
+    else if constexpr (PREC == 3 && M == 1)
       cell_residual<D, NG, ND, NF, NQ, AFFINE, PH, SC, PREC>(a.tab, X, U, URegs<ND, M>{U}, a.pp, phi, wave, bad, work + tid,
                                                        PATCH, SourceLds<ND, M>{nodal_l, row});
     else if constexpr (PREC == 3)

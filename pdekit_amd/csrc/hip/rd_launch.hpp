@@ -25,18 +25,26 @@ struct KernelSet
   CellLauncher cell = nullptr;    // element per thread, global scatter
   PatchLauncher patch = nullptr;  // patch assembly
   PatchBatchLauncher patch_batch = nullptr;  // patch assembly of several blocks / methods
+  // the same with a nodal input (LDA / B) or a source field
+  PatchBatchLauncher patch_nodal_batch = nullptr;
+  PatchBatchLauncher patch_src_batch = nullptr;
   PatchLauncher patch_lin = nullptr;  // linear sweep of the mixed-precision form: J delta per cell
   // with the nodal input of the scheme (LDA: artificial viscosity, B:
   // external blending coefficient); null for N and PSI, which take none
   CellLauncher cell_nodal = nullptr;
   PatchLauncher patch_nodal = nullptr;
-  // FP64 patch assembly with a nodal source field (reaction residuals)
+  // FP64 patch assembly with a nodal source field (reaction residuals), and
+  // for the B scheme with an external blending coefficient beside it
   PatchLauncher patch_src = nullptr;
+  PatchLauncher patch_src_theta = nullptr;
+  PatchBatchLauncher patch_src_theta_batch = nullptr;
   // element residuals and Jacobians of the implicit form (null for PSI)
   JacobianLauncher jacobian = nullptr;
   int patch_size = 0;             // largest (default) cells per patch workgroup
   size_t (*patch_lds_bytes)(int maxl, int cells) = nullptr;
   size_t (*patch_src_lds_bytes)(int maxl, int cells) = nullptr;
+  size_t (*patch_nodal_lds_bytes)(int maxl, int cells) = nullptr;
+  size_t (*patch_src_theta_lds_bytes)(int maxl, int cells) = nullptr;
   size_t (*patch_lin_lds_bytes)(int maxl, int cells) = nullptr;
   explicit operator bool() const { return cell != nullptr; }
 };
@@ -111,6 +119,18 @@ size_t patch_src_lds_bytes(int maxl, int cells)
 }
 
 template <int NQ, int ND, int M>
+size_t patch_nodal_lds_bytes(int maxl, int cells)
+{
+  return patch_lds_doubles_prec<NQ, ND, M, 2>(maxl, cells) * sizeof(double);
+}
+
+template <int NQ, int ND, int M>
+size_t patch_src_theta_lds_bytes(int maxl, int cells)
+{
+  return patch_lds_doubles_prec<NQ, ND, M, 5>(maxl, cells) * sizeof(double);
+}
+
+template <int NQ, int ND, int M>
 size_t patch_lin_lds_bytes(int maxl, int cells)
 {
   return patch_lds_doubles_prec<NQ, ND, M, 4>(maxl, cells) * sizeof(double);
@@ -133,8 +153,16 @@ KernelSet kernel_set()
   if constexpr (SC == SC_LDA || SC == SC_B) {
     k.cell_nodal = &launch_cell<D, NG, ND, NF, NQ, AFF, PH, SC, 2>;
     k.patch_nodal = &launch_patch<D, NG, ND, NF, NQ, AFF, PH, SC, 2>;
+    k.patch_nodal_batch = &launch_patch_batch<D, NG, ND, NF, NQ, AFF, PH, SC, 2>;
+    k.patch_nodal_lds_bytes = &patch_nodal_lds_bytes<NQ, ND, PhysTraits<PH>::M>;
   }
   k.patch_src = &launch_patch<D, NG, ND, NF, NQ, AFF, PH, SC, 3>;
+  k.patch_src_batch = &launch_patch_batch<D, NG, ND, NF, NQ, AFF, PH, SC, 3>;
+  if constexpr (SC == SC_B) {
+    k.patch_src_theta = &launch_patch<D, NG, ND, NF, NQ, AFF, PH, SC, 5>;
+    k.patch_src_theta_batch = &launch_patch_batch<D, NG, ND, NF, NQ, AFF, PH, SC, 5>;
+    k.patch_src_theta_lds_bytes = &patch_src_theta_lds_bytes<NQ, ND, PhysTraits<PH>::M>;
+  }
   if constexpr (SC != SC_PSI && SC != SC_BPSI) k.jacobian = &launch_jacobian<D, NG, ND, NF, NQ, AFF, PH, SC>;
   k.patch_size = PatchSize<PhysTraits<PH>::M, ND>::value;
   k.patch_lds_bytes = &patch_lds_bytes<NQ, ND, PhysTraits<PH>::M>;

@@ -83,6 +83,7 @@ struct WeakDev
   // expression of the point), or null; the facet points' coordinates
   double *d_qstate = nullptr;
   std::vector<double> qpoints;  // [nf][nq][dim]
+  double *d_qvel = nullptr;     // rotating advection: (y, -x) at every facet point
 };
 
 }  // namespace
@@ -388,6 +389,41 @@ const double *nodal_input(const pdrd_method *h)
   return nullptr;
 }
 
+// With a source field: the B scheme's external blending coefficient, which
+// its reaction residuals use (PGB.hpp:825-1050, the skip test at :878), or
+// null. The LDA scheme's reaction residuals apply no artificial viscosity
+// (PGLDA.hpp:553-741), so a viscosity set beside a source field is not used.
+const double *source_theta(const pdrd_method *h)
+{
+  return (h->use_src && h->scheme == pdrd::SC_B && h->use_blend) ? h->d_blend : nullptr;
+}
+
+// The patch kernel and its LDS bytes for a block with this method's inputs.
+pdrd::PatchLauncher input_kernel(const pdrd_method *h, const Block &b)
+{
+  if (h->use_src) return source_theta(h) ? b.ks.patch_src_theta : b.ks.patch_src;
+  return nodal_input(h) ? b.ks.patch_nodal : b.ks.patch;
+}
+
+size_t input_lds_bytes(const pdrd_method *h, const Block &b)
+{
+  if (h->use_src)
+    return source_theta(h) ? b.ks.patch_src_theta_lds_bytes(b.maxl, b.cells) : b.ks.patch_src_lds_bytes(b.maxl, b.cells);
+  return nodal_input(h) ? b.ks.patch_nodal_lds_bytes(b.maxl, b.cells) : b.ks.patch_lds_bytes(b.maxl, b.cells);
+}
+
+// Checks that every block has the kernel of the method's inputs and that its
+// LDS fits a workgroup.
+int check_inputs(const pdrd_method *h)
+{
+  for (const Block &b : h->blocks) {
+    if (!input_kernel(h, b)) return set_error(PDRD_ENOTSUP, "no kernel for this scheme with these nodal inputs");
+    if (input_lds_bytes(h, b) > 65536)
+      return set_error(PDRD_ENOTSUP, "the nodal inputs do not fit the patch LDS: set PDRD_OPT_PATCH_CELLS to 64");
+  }
+  return PDRD_OK;
+}
+
 // Cell loop of the atomic / colour modes over every block (and colour),
 // accumulating into acc_res / acc_ws.
 int launch_cells(pdrd_method *h)
@@ -490,6 +526,7 @@ pdrd::PatchArgs patch_args(const pdrd_method *h, const Block *b, int update, int
   if (c.record) a.base_out = h->d_base[c.write];
   a.norm_partial = h->d_patch_partial;
   a.nodal = h->use_src ? h->d_src : nodal_input(h);
+  a.theta = source_theta(h);
   a.status = h->d_status;
   a.update = update;
   a.outputs = outputs;
@@ -521,7 +558,7 @@ int launch_patch_blocks(pdrd_method *h, int update, int outputs, int excl, doubl
   for (Block &b : h->blocks) {
     const pdrd::PatchArgs a = patch_args(h, &b, update, outputs, excl, cfl, c, lin);
     const pdrd::PatchLauncher f =
-        lin ? b.lin : (h->use_src ? b.ks.patch_src : (a.nodal ? b.ks.patch_nodal : b.ks.patch));
+        lin ? b.lin : input_kernel(h, b);
     HIP_CHECK(f(a, b.np, b.cells, h->stream));
   }
   return PDRD_OK;
@@ -551,6 +588,7 @@ int launch_weak(pdrd_method *h, hipStream_t s, int buffer = 0)
     a.type = w.type;
     a.relax = w.relax;
     a.qstate = w.d_qstate;
+    a.qvel = w.d_qvel;
     for (int i = 0; i < 5; ++i) a.params[i] = w.params[i];
     a.pp.p0 = h->params[0];
     a.pp.p1 = h->params[1];
@@ -565,6 +603,8 @@ int launch_weak(pdrd_method *h, hipStream_t s, int buffer = 0)
       WEAK_CASE(pdrd::PH_ADVECTION2D, 3)
       WEAK_CASE(pdrd::PH_BURGERS2D, 2)
       WEAK_CASE(pdrd::PH_BURGERS2D, 3)
+      WEAK_CASE(pdrd::PH_ROTATION2D, 2)
+      WEAK_CASE(pdrd::PH_ROTATION2D, 3)
       WEAK_CASE(pdrd::PH_EULER2D, 2)
       WEAK_CASE(pdrd::PH_EULER2D, 3)
       WEAK_CASE(pdrd::PH_EULER3D, 3)
@@ -798,6 +838,7 @@ void free_device(pdrd_method *h)
     (void)hipFree(w.d_w);
     (void)hipFree(w.d_V);
     (void)hipFree(w.d_qstate);
+    (void)hipFree(w.d_qvel);
   }
   h->weak.clear();
   (void)hipFree(h->d_pn_coff);
@@ -875,7 +916,6 @@ int lin_setup(pdrd_method *h)
   if (h->d_uref) return PDRD_OK;
   if (h->scheme == PDRD_PSI || h->scheme == PDRD_BPSI)
     return set_error(PDRD_ENOTSUP, "the mixed-precision form needs element Jacobians (none for PSI)");
-  if (h->physics == PDRD_ROTATION2D) return set_error(PDRD_ENOTSUP, "no mixed-precision form of the rotating velocity");
   for (Block &b : h->blocks) {
     const int lin_phys = (h->m == 1) ? pdrd::PH_ADVECTION2D : h->physics;
     const pdrd::KernelSet lk =
@@ -1384,6 +1424,18 @@ int pdrd_initialize(pdrd_method_t h)
             free_device(h);
             return rc;
           }
+          if (h->physics == pdrd::PH_ROTATION2D) {
+            // AroundOrigin2D: the velocity (y, -x) of every facet point
+            std::vector<double> vel(static_cast<size_t>(wr.nf) * wr.nq * 2);
+            for (size_t i = 0; i < vel.size() / 2; ++i) {
+              vel[2 * i] = wr.qpoints[2 * i + 1];
+              vel[2 * i + 1] = -wr.qpoints[2 * i];
+            }
+            if ((rc = upd(&wr.d_qvel, vel))) {
+              free_device(h);
+              return rc;
+            }
+          }
         }
       }
     } else {
@@ -1446,8 +1498,6 @@ int pdrd_set_option(pdrd_method_t h, int option, int value)
       break;
     case PDRD_OPT_PRECISION:
       if (value != 0 && value != 1) return set_error(PDRD_EINVAL, "precision is 0 (FP64) or 1 (mixed)");
-      if (value == 1 && h->physics == PDRD_ROTATION2D)
-        return set_error(PDRD_ENOTSUP, "the rotating advection runs with the FP64 cells");
       if (value == 1 && (h->scheme == PDRD_PSI || h->scheme == PDRD_BPSI))
         return set_error(PDRD_ENOTSUP, "the mixed-precision form needs element Jacobians (none for PSI)");
       h->precision = value;
@@ -1546,8 +1596,6 @@ int pdrd_add_weak_bc(pdrd_method_t h, int type, uint32_t n, const uint32_t *node
   if (!h || (n && !nodes)) return set_error(PDRD_EINVAL, "null argument");
   if (h->ready) return set_error(PDRD_EINVAL, "weak boundary conditions are added before pdrd_initialize");
   if (type < PDRD_WEAK_WALL || type > PDRD_WEAK_DIRICHLET) return set_error(PDRD_EINVAL, "unknown weak BC type");
-  if (h->physics == pdrd::PH_ROTATION2D)
-    return set_error(PDRD_ENOTSUP, "weak boundary conditions of the rotating advection are not built");
   if (h->physics != pdrd::PH_EULER2D && h->physics != pdrd::PH_EULER3D && type != PDRD_WEAK_DIRICHLET)
     return set_error(PDRD_ENOTSUP, "only WeakDirichlet applies to scalar physics");
   // parameters: WeakSubInlet T_tot, p_tot; WeakSubOutlet p; WeakFarfield and
@@ -1711,8 +1759,6 @@ int pdrd_set_blending_coeff(pdrd_method_t h, const double *theta)
 {
   if (int rc = check_ready(h)) return rc;
   lin_invalidate(h);
-  if (theta && h->use_src)
-    return set_error(PDRD_ENOTSUP, "a blending coefficient / artificial viscosity with a source field is not built");
   if (!theta) {
     if (h->use_blend) drop_graph(h);
     h->use_blend = false;
@@ -1724,6 +1770,10 @@ int pdrd_set_blending_coeff(pdrd_method_t h, const double *theta)
   HIP_CHECK(hipStreamSynchronize(h->stream));
   if (!h->use_blend) drop_graph(h);
   h->use_blend = true;
+  if (int rc = check_inputs(h)) {
+    h->use_blend = false;
+    return rc;
+  }
   return PDRD_OK;
 }
 
@@ -1731,8 +1781,6 @@ int pdrd_set_artificial_viscosity(pdrd_method_t h, const double *nu)
 {
   if (int rc = check_ready(h)) return rc;
   lin_invalidate(h);
-  if (nu && h->use_src)
-    return set_error(PDRD_ENOTSUP, "a blending coefficient / artificial viscosity with a source field is not built");
   if (!nu) {
     if (h->use_visc) drop_graph(h);
     h->use_visc = false;
@@ -1756,13 +1804,10 @@ int pdrd_set_sources(pdrd_method_t h, const double *sources)
     return PDRD_OK;
   }
   if (h->scatter_mode != 2) return set_error(PDRD_ENOTSUP, "the source term runs with the patch assembly only");
-  if (nodal_input(h))
-    return set_error(PDRD_ENOTSUP, "a source field with a blending coefficient / artificial viscosity is not built");
-  for (const Block &b : h->blocks) {
-    if (!b.ks.patch_src) return set_error(PDRD_ENOTSUP, "no source-term kernel for this element type");
-    if (b.ks.patch_src_lds_bytes(b.maxl, b.cells) > 65536)
-      return set_error(PDRD_ENOTSUP, "the source values do not fit the patch LDS: set PDRD_OPT_PATCH_CELLS to 64");
-  }
+  h->use_src = true;
+  const int chk = check_inputs(h);
+  h->use_src = false;
+  if (chk) return chk;
   const size_t nm = static_cast<size_t>(h->nn) * h->m;
   if (!h->d_src)
     if (int rc = dev_alloc(h, &h->d_src, nm)) return rc;
@@ -1777,8 +1822,6 @@ int pdrd_compute_blending_coeff(pdrd_method_t h, int kind)
   PDRD_TRY
   if (int rc = check_ready(h)) return rc;
   lin_invalidate(h);
-  if (h->use_src)
-    return set_error(PDRD_ENOTSUP, "a blending coefficient / artificial viscosity with a source field is not built");
   if (kind != PDRD_BLEND_LF) return set_error(PDRD_EINVAL, "unknown blending coefficient");
   for (Block &b : h->blocks)
     if (int rc = stab_tables(h, b, false)) return rc;
@@ -1804,6 +1847,10 @@ int pdrd_compute_blending_coeff(pdrd_method_t h, int kind)
   if (int rc = stab_interp(h, h->d_blend)) return rc;
   if (!h->use_blend) drop_graph(h);
   h->use_blend = true;
+  if (int rc = check_inputs(h)) {
+    h->use_blend = false;
+    return rc;
+  }
   return PDRD_OK;
   PDRD_CATCH
 }
@@ -1813,8 +1860,6 @@ int pdrd_compute_artificial_viscosity(pdrd_method_t h, double kappa, double fact
   PDRD_TRY
   if (int rc = check_ready(h)) return rc;
   lin_invalidate(h);
-  if (h->use_src)
-    return set_error(PDRD_ENOTSUP, "a blending coefficient / artificial viscosity with a source field is not built");
   if (!(kappa > 0.0) || !(factor >= 0.0)) return set_error(PDRD_EINVAL, "kappa must be > 0 and factor >= 0");
   for (Block &b : h->blocks)
     if (int rc = stab_tables(h, b, true)) return rc;
@@ -2288,10 +2333,6 @@ int pdrd_batch_iterate(pdrd_batch_t b, int count, const pdrd_method_t *methods, 
     if (h->device != b->device) return set_error(PDRD_EINVAL, "batch methods must live on the batch's device");
     if (h->scatter_mode != 2) return set_error(PDRD_ENOTSUP, "batched iterations need the patch assembly");
     if (mixed(h)) return set_error(PDRD_ENOTSUP, "batched iterations of mixed-precision methods are not built");
-    if (nodal_input(h))
-      return set_error(PDRD_ENOTSUP, "batched iterations of methods with a blending coefficient / artificial "
-                                     "viscosity are not built");
-    if (h->use_src) return set_error(PDRD_ENOTSUP, "batched iterations of methods with a source field are not built");
     for (int j = 0; j < i; ++j)
       if (methods[j] == h) return set_error(PDRD_EINVAL, "a method appears twice in one batch");
   }
@@ -2309,7 +2350,12 @@ int pdrd_batch_iterate(pdrd_batch_t b, int count, const pdrd_method_t *methods, 
   for (int i = 0; i < count; ++i)
     for (size_t k = 0; k < methods[i]->blocks.size(); ++k) {
       const Block &bl = methods[i]->blocks[k];
-      const pdrd::PatchBatchLauncher fn = bl.ks.patch_batch;
+      // the kernel of the method's inputs: plain, with a nodal input, or with a source field
+      const pdrd_method *hm = methods[i];
+      const pdrd::PatchBatchLauncher fn =
+          hm->use_src ? (source_theta(hm) ? bl.ks.patch_src_theta_batch : bl.ks.patch_src_batch)
+                      : (nodal_input(hm) ? bl.ks.patch_nodal_batch : bl.ks.patch_batch);
+      if (!fn) return set_error(PDRD_ENOTSUP, "no batched kernel for this method's inputs");
       size_t g = 0;
       while (g < patches.size() && (patches[g].patch != fn || patches[g].cells != bl.cells)) ++g;
       if (g == patches.size()) {
@@ -2340,7 +2386,7 @@ int pdrd_batch_iterate(pdrd_batch_t b, int count, const pdrd_method_t *methods, 
         const KindPlace kp = kind_place(k, chain[mk.first]);
         pk[k].push_back(patch_args(h, &bl, 1, kp.outputs, excl, cfl[mk.first], kp.c));
       }
-      l.lds = std::max(l.lds, bl.ks.patch_lds_bytes(bl.maxl, bl.cells));
+      l.lds = std::max(l.lds, input_lds_bytes(h, bl));
       acc += bl.np;
       first.push_back(acc);
     }

@@ -60,6 +60,7 @@ struct WeakArgs
   double params[5];                     // SubInlet: T_tot, p_tot; SubOutlet: p; Farfield, Dirichlet: state
   PhysParams pp;                        // gamma (Euler) or the advection velocity
   const double *__restrict__ qstate;    // [nf][nq][M] WeakDirichlet state at every facet point, or null
+  const double *__restrict__ qvel;      // [nf][nq][2] rotating advection: the velocity (y, -x) of every facet point
 };
 
 // K-(n) x for the Euler equations at state st (unit normal, un = V.n).
@@ -93,13 +94,14 @@ __device__ __forceinline__ void euler_kminus(const EulerState<double, D> &st, co
 // the point, WeakDirichletBC.hpp:216-229), or null for the constant one.
 template <int PH>
 __device__ __forceinline__ double weak_point(const WeakArgs &a, const double *uq, const double *nrm, double *dF,
-                                             bool &bad, const double *gq)
+                                             bool &bad, const double *gq, const double *qv)
 {
   constexpr int D = PhysTraits<PH>::DIM, M = PhysTraits<PH>::M;
   if constexpr (M == 1) {
-    // WeakDirichlet of a scalar: dF = min(a.n, 0) (g - u)
-    const double ax = (PH == PH_BURGERS2D) ? uq[0] : a.pp.p0;
-    const double ay = (PH == PH_BURGERS2D) ? 1.0 : a.pp.p1;
+    // WeakDirichlet of a scalar: dF = min(a.n, 0) (g - u); the rotating
+    // advection's velocity is the point's (y, -x)
+    const double ax = (PH == PH_BURGERS2D) ? uq[0] : ((PH == PH_ROTATION2D) ? qv[0] : a.pp.p0);
+    const double ay = (PH == PH_BURGERS2D) ? 1.0 : ((PH == PH_ROTATION2D) ? qv[1] : a.pp.p1);
     const double an = ax * nrm[0] + ay * nrm[1];
     dF[0] = fmin(an, 0.0) * ((gq ? gq[0] : a.params[0]) - uq[0]);
     return fabs(an);
@@ -216,7 +218,8 @@ __global__ __launch_bounds__(128) void weak_bc_kernel(WeakArgs a)
     }
     double dF[M];
     const double *gq = a.qstate ? a.qstate + (static_cast<size_t>(f) * a.nq + q) * M : nullptr;
-    const double emax = weak_point<PH>(a, uq, nrm, dF, bad, gq);
+    const double *qv = (PH == PH_ROTATION2D) ? a.qvel + (static_cast<size_t>(f) * a.nq + q) * 2 : nullptr;
+    const double emax = weak_point<PH>(a, uq, nrm, dF, bad, gq, qv);
     coeff = fmax(coeff, fmax(floor, emax));
 #pragma unroll
     for (int k = 0; k < M; ++k) {

@@ -162,8 +162,11 @@ def add_oracle_weak(P, mesh, bc_type, nodes, params=(), quad_order=4):
     from oracle import pyoracle as O
     fn, fv, ins = boundary_facets(mesh, nodes)
     w, V = facet_rule(mesh.dim, fn.shape[1], quad_order + 1)
+    # the facet points: the vertices' linear functions at the same rule
+    _, V1 = facet_rule(mesh.dim, fv.shape[1], quad_order + 1)
+    qpoints = np.einsum("qv,fvd->fqd", V1, mesh.coords[fv])
     kinds = {"WeakWall": O.WEAK_WALL, "WeakSubInlet": O.WEAK_SUBINLET, "WeakSubOutlet": O.WEAK_SUBOUTLET,
              "WeakFarfield": O.WEAK_FARFIELD, "WeakSymmetry": O.WEAK_SYMMETRY, "WeakSuperInlet": O.WEAK_SUPERINLET,
              "WeakDirichlet": O.WEAK_DIRICHLET}
-    P.add_weak(kinds[bc_type], params, mesh.order, fn, fv, ins, w, V)
+    P.add_weak(kinds[bc_type], params, mesh.order, fn, fv, ins, w, V, qpoints=qpoints)
     return fn

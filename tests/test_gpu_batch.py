@@ -71,3 +71,41 @@ def test_batch_rejects_bad_input():
     m2.set_vec_function("solution", np.zeros((mesh.nb_nodes, 1)))
     with pytest.raises(L.PdrdError):
         b.iterate([m, m2], [c, c], 2)
+
+
+def _with_inputs(kind):
+    """Methods whose cells take a nodal input or a source field."""
+    rng = np.random.default_rng(9)
+    if kind == "visc":
+        m, c = _make("euler2d_mixed", L.LDA)
+        m.set_artificial_viscosity(rng.uniform(0.0, 0.05, m.mesh.nb_nodes))
+    elif kind == "theta":
+        m, c = _make("burgers_p2", L.B)
+        m.set_blending_coeff(rng.uniform(0.0, 1.0, m.mesh.nb_nodes))
+    elif kind == "src_adv":
+        m, c = _make("adv_p2", L.LDA)
+        m.set_sources(rng.uniform(-0.5, 0.5, (m.mesh.nb_nodes, 1)))
+    elif kind == "src_euler":
+        m, c = _make("euler2d_p2", L.N)
+        m.set_sources(rng.uniform(-0.01, 0.01, (m.mesh.nb_nodes, 4)))
+    else:
+        m, c = _make("adv_p1", L.N)
+    return m, c
+
+
+@pytest.mark.parametrize("n_iter", [1, 7])
+def test_batch_with_nodal_inputs_and_sources(n_iter):
+    """Methods with an artificial viscosity, an external blending coefficient
+    or a source field, beside a plain one, in one batch: the same bits as their
+    own pdrd_iterate."""
+    kinds = ["visc", "theta", "src_adv", "src_euler", "plain"]
+    print(f"batch with nodal inputs and sources, {n_iter} iterations", flush=True)
+    alone = [_with_inputs(k) for k in kinds]
+    batched = [_with_inputs(k) for k in kinds]
+    ref = [m.iterate(c, n_iter) for m, c in alone]
+    norms = Batch().iterate([m for m, _ in batched], [c for _, c in batched], n_iter)
+    for (ma, _), (mb, _), na, nb, k in zip(alone, batched, ref, norms, kinds):
+        for f in ("solution", "residuals", "time_step", "wave_speed"):
+            a, c = ma.vec_function(f), mb.vec_function(f)
+            assert np.array_equal(a, c), (k, f, np.abs(a - c).max())
+        assert np.array_equal(na, nb), k

@@ -7,7 +7,7 @@
   batch;
 * the mixed-precision form with weak BCs and every sweep FP64 is the plain
   iteration bit for bit;
-* nodal inputs are refused in batches.
+* a method with a nodal input batches with one with weak BCs.
 """
 import os
 
@@ -95,9 +95,18 @@ def test_mixed_precision_with_weak_bcs():
     assert np.array_equal(plain.vec_function("residuals"), one.vec_function("residuals"))
 
 
-def test_nodal_inputs_are_refused_where_not_built():
+def test_nodal_inputs_in_a_batch_with_weak_bcs():
+    """A method with an external blending coefficient and one with weak BCs in
+    one batch: the same bits as their own pdrd_iterate."""
     mesh = MESHES[0]()
-    m = _method(mesh, L.B, weak=False)
-    m.set_blending_coeff(np.full(mesh.nb_nodes, 0.5))
-    with pytest.raises(L.PdrdError):
-        Batch().iterate([m], [0.3], 2)
+    theta = np.random.default_rng(3).uniform(0.0, 1.0, mesh.nb_nodes)
+    alone = [_method(mesh, L.B, weak=False), _method(mesh)]
+    batched = [_method(mesh, L.B, weak=False), _method(mesh)]
+    for m in (alone[0], batched[0]):
+        m.set_blending_coeff(theta)
+    for m in alone:
+        m.iterate(0.3, 6)
+    Batch().iterate(batched, [0.3, 0.3], 6)
+    for a, b in zip(alone, batched):
+        assert np.array_equal(a.vec_function("solution"), b.vec_function("solution"))
+        assert np.array_equal(a.vec_function("residuals"), b.vec_function("residuals"))

@@ -2,15 +2,17 @@
 (y, -x) at every point) on the GPU against the oracle, <= 1e-12 normwise
 relative: one sweep on P1 / P2 / P3 triangles and Q1 / Q2 / mixed meshes in
 every scheme and assembly, 20 iterations with strong Dirichlet inflow values
-(chained and unchained), the element Jacobians, a radial steady state, and
-the refused combinations (mixed precision, weak boundary conditions)."""
+(chained and unchained), the element Jacobians, a radial steady state,
+WeakDirichlet (the velocity of every facet point) for one sweep and 10
+iterations, and the mixed-precision form's steady state within 1e-8 of the
+FP64 one."""
 import numpy as np
 import pytest
 
 from pdekit_amd import _lib as L
 from pdekit_amd.mesh import rectangle, unit_quad_triag
 from pdekit_amd.rdm import PGRDMethodExplicit
-from tests.helpers import advection_state, oracle_problem, quad_orders, rel_err
+from tests.helpers import add_oracle_weak, advection_state, oracle_problem, quad_orders, rel_err
 
 pytestmark = pytest.mark.gpu
 
@@ -96,11 +98,58 @@ def test_radial_state_is_steady_on_the_gpu():
     m.destroy()
 
 
-def test_rotation_refusals():
-    mesh = MESHES["p1"]()
-    m = _method(mesh, L.LDA)
-    with pytest.raises(L.PdrdError):
-        m.set_option(L.OPT_PRECISION, 1)
-    with pytest.raises(L.PdrdError):
-        m.add_boundary_condition("WeakDirichlet", "wd", mesh.boundaries["Edge0"], values=[1.0])
+@pytest.mark.parametrize("name", ["p1", "p2", "q1_mixed"])
+@pytest.mark.parametrize("scheme", [L.LDA, L.N])
+def test_rotation_weak_dirichlet_matches_oracle(name, scheme):
+    print(f"rotation WeakDirichlet {name} scheme {scheme}", flush=True)
+    mesh = MESHES[name]()
+    b = mesh.boundaries
+    sides = [s for s in ("xmin", "xmax", "ymin", "ymax") if s in b] or ["Edge0", "Edge1", "Edge2", "Edge3"]
+    u = advection_state(mesh)
+    q = quad_orders(mesh)
+    m = _method(mesh, scheme)
+    P = oracle_problem(mesh, ROT, scheme, (), q)
+    for k, side in enumerate(sides):
+        val = 0.3 + 0.2 * k
+        m.add_boundary_condition("WeakDirichlet", side, b[side], values=[val])
+        add_oracle_weak(P, mesh, "WeakDirichlet", b[side], (val,), quad_order=max(q.values()))
+    m.set_vec_function("solution", u)
+    m.assemble_rhs()
+    m.check_status()
+    ref, ref_ws, rc = P.assemble_rhs(u)
+    assert rc == 0
+    assert rel_err(m.vec_function("residuals"), ref) <= 1e-12
+    assert rel_err(m.vec_function("wave_speed"), ref_ws) <= 1e-12
+    m.iterate(0.5, 10)
+    uo, _, _, _, rc = P.iterate(u, 0.5, 10)
+    assert rc == 0
+    assert rel_err(m.vec_function("solution"), uo) <= 1e-12
     m.destroy()
+
+
+def test_rotation_mixed_steady_state_matches_plain():
+    """Steady inflow problem on make_unit_quad(33) (strong Dirichlet on x = 0
+    and y = 1, N scheme, CFL 0.9): the FP64 iteration and the mixed form each
+    run to a residual norm <= 1e-12; the states agree within 1e-8."""
+    mesh = unit_quad_triag(33)
+    bn = np.union1d(mesh.boundaries["Edge2"], mesh.boundaries["Edge3"]).astype(np.uint32)
+    g = np.exp(-20 * (mesh.node_coords[bn, 0] + mesh.node_coords[bn, 1] - 0.8) ** 2)[:, None]
+    out = {}
+    for prec in (0, 1):
+        m = _method(mesh, L.N)
+        m.set_option(L.OPT_PRECISION, prec)
+        m.set_vec_function("solution", np.zeros((mesh.nb_nodes, 1)))
+        m.add_boundary_condition("StrongDirichlet", "inflow", bn, g)
+        it = 0
+        while True:
+            nrm = m.iterate(0.9, 100, exclude_dirichlet=True)
+            it += 100
+            if nrm.max() <= 1e-12 or it >= 40_000:
+                break
+        out[prec] = (m.vec_function("solution"), float(nrm.max()), it)
+        m.destroy()
+    (up, np_, ip), (um, nm, im) = out[0], out[1]
+    d = float(np.abs(um - up).max() / np.abs(up).max())
+    print(f"rotation steady: plain {np_:.2e} after {ip}, mixed {nm:.2e} after {im}, difference {d:.3e}", flush=True)
+    assert np_ <= 1e-12 and nm <= 1e-12
+    assert d <= 1e-8

@@ -4,13 +4,15 @@ residuals, div F - s_q at every quadrature point) against the oracle,
 seeded random source field, 10 iterations with strong Dirichlet inflow
 values (chained and unchained sweeps bit for bit alike), a linear
 manufactured steady state of advection whose residual vanishes, the source
-switched off again, and the refused combinations."""
+switched off again, the source beside the B scheme's external blending
+coefficient and beside the LDA scheme's artificial viscosity (which its
+reaction residuals do not apply), and the refused combinations."""
 import numpy as np
 import pytest
 
 from pdekit_amd import _lib as L
 from pdekit_amd.mesh import unit_quad_triag
-from pdekit_amd.rdm import Batch, PGRDMethodExplicit
+from pdekit_amd.rdm import PGRDMethodExplicit
 from tests.helpers import CASES, oracle_problem, quad_orders, rel_err
 
 pytestmark = pytest.mark.gpu
@@ -93,21 +95,79 @@ def test_manufactured_steady_state_on_the_gpu():
     m.destroy()
 
 
+@pytest.mark.parametrize("name", ["burgers_p2", "euler2d_mixed", "adv_p2", "euler3d_p1"])
+def test_source_with_blending_coeff_matches_oracle(name):
+    """The B scheme's reaction residuals with an external blending coefficient
+    (PGB.hpp:825-1050, skip below 1e-6 at :878): one sweep against the oracle,
+    with a coefficient that is partly below the skip level."""
+    print(f"sources + theta {name}", flush=True)
+    mesh, u, m, P, src = _setup(name, L.B)
+    theta = np.random.default_rng(5).uniform(0.0, 1.0, mesh.nb_nodes)
+    theta[::3] = 0.0
+    m.set_vec_function("sources", src)
+    m.set_blending_coeff(theta)
+    m.assemble_rhs()
+    m.check_status()
+    P.set_sources(src)
+    P.set_blending_coeff(theta)
+    ref, ref_ws, rc = P.assemble_rhs(u)
+    assert rc == 0
+    assert rel_err(m.vec_function("residuals"), ref) <= 1e-12
+    assert rel_err(m.vec_function("wave_speed"), ref_ws) <= 1e-12
+    m.destroy()
+
+
+def test_source_with_blending_coeff_iterations_match_oracle():
+    mesh = unit_quad_triag(33)
+    x, y = mesh.node_coords[:, 0], mesh.node_coords[:, 1]
+    src = (0.5 + np.sin(3 * x) * np.cos(2 * y))[:, None]
+    theta = 0.5 + 0.5 * np.sin(5 * x + y)
+    bn = np.union1d(mesh.boundaries["Edge0"], mesh.boundaries["Edge3"]).astype(np.uint32)
+    g = np.where(mesh.node_coords[bn, 1] < 0.5, 1.0, 0.0)[:, None]
+    m = PGRDMethodExplicit(L.ADVECTION2D, L.B, params=(0.8, 0.6))
+    m.configure_mesh_data(mesh).initialize_work_data(quad_order=4)
+    m.set_vec_function("solution", np.zeros((mesh.nb_nodes, 1)))
+    m.add_boundary_condition("StrongDirichlet", "inflow", bn, g)
+    m.set_blending_coeff(theta)
+    m.set_vec_function("sources", src)
+    m.iterate(0.9, 10)
+    P = oracle_problem(mesh, L.ADVECTION2D, L.B, (0.8, 0.6))
+    P.set_sources(src)
+    P.set_blending_coeff(theta)
+    uo, res, dt, norms, rc = P.iterate(np.zeros(mesh.nb_nodes), 0.9, 10, bn, g)
+    assert rc == 0
+    assert rel_err(m.vec_function("solution"), uo) <= 1e-12
+    m.destroy()
+
+
+@pytest.mark.parametrize("name", ["adv_p2", "euler2d_mixed"])
+def test_source_ignores_artificial_viscosity(name):
+    """The LDA scheme's reaction residuals apply no artificial viscosity
+    (PGLDA.hpp:553-741): with a source field the viscosity changes nothing,
+    bit for bit, and the result is the oracle's."""
+    mesh, u, m, P, src = _setup(name, L.LDA)
+    nu = np.random.default_rng(6).uniform(0.0, 0.05, mesh.nb_nodes)
+    m.set_vec_function("sources", src)
+    m.assemble_rhs()
+    r_src = m.vec_function("residuals")
+    m.set_artificial_viscosity(nu)
+    m.assemble_rhs()
+    assert np.array_equal(m.vec_function("residuals"), r_src)
+    P.set_sources(src)
+    P.set_artificial_viscosity(nu)
+    ref, _, rc = P.assemble_rhs(u)
+    assert rc == 0
+    assert rel_err(r_src, ref) <= 1e-12
+    m.destroy()
+
+
 def test_refused_combinations():
     mesh, u, m, P, src = _setup("burgers_p2", L.B)
     m.set_vec_function("sources", src)
     with pytest.raises(L.PdrdError):
         m.set_option(L.OPT_SCATTER, 1)
-    with pytest.raises(L.PdrdError):
-        m.set_blending_coeff(np.full(mesh.nb_nodes, 0.5))
-    with pytest.raises(L.PdrdError):
-        m.compute_blending_coeff()
-    m2 = _setup("burgers_p2", L.N)[2]
-    with pytest.raises(L.PdrdError):
-        Batch().iterate([m, m2], [0.5, 0.5], 2)
     m.set_sources(None)
     m.set_option(L.OPT_SCATTER, 1)
     with pytest.raises(L.PdrdError):
         m.set_vec_function("sources", src)
     m.destroy()
-    m2.destroy()

@@ -180,3 +180,34 @@ def test_scalar_weak_dirichlet():
     fn_out, _, _ = boundary_facets(mesh, b["xmax"])
     assert abs(d[np.unique(fn_in)].sum() - (-0.8) * (1.0 - 0.25)) < 1e-13
     assert np.abs(d[np.setdiff1d(np.unique(fn_out), np.unique(fn_in))]).max() < 1e-15
+
+
+def test_rotation_weak_dirichlet():
+    """The rotating advection, velocity (y, -x) of the point, on [0, 1]^2: the
+    edges x = 0 (n = (-1, 0), a.n = -y) and y = 1 (n = (0, 1), a.n = -x) are
+    inflow, so with uniform u and g the correction integrates to
+    -(g - u) * 1/2 over each; the edges x = 1 (a.n = y) and y = 0 (a.n = x)
+    are outflow and add nothing."""
+    mesh = rectangle(6, 5, split=2, jitter=0.1, seed=2)
+    b = mesh.boundaries
+    P0 = oracle_problem(mesh, O.ROTATION2D, O.N, ())
+    P1 = oracle_problem(mesh, O.ROTATION2D, O.N, ())
+    for side in ("xmin", "ymax", "xmax", "ymin"):
+        add_oracle_weak(P1, mesh, "WeakDirichlet", b[side], (1.0,))
+    U = np.full((mesh.nb_nodes, 1), 0.25)
+    r0, w0, rc0 = P0.assemble_rhs(U)
+    r1, w1, rc1 = P1.assemble_rhs(U)
+    assert rc0 == 0 and rc1 == 0
+    vol = P0.dual_volume()
+    d = (r1 - r0)[:, 0] * vol
+    corner_in = np.intersect1d(b["xmin"], b["ymax"])
+    x_in = np.unique(boundary_facets(mesh, b["xmin"])[0])
+    y_in = np.unique(boundary_facets(mesh, b["ymax"])[0])
+    assert abs(d[np.union1d(x_in, y_in)].sum() - 2 * (-0.5) * (1.0 - 0.25)) < 1e-13
+    # each inflow edge alone (the corner node is shared; its share of either edge is separated by adding one edge)
+    P2 = oracle_problem(mesh, O.ROTATION2D, O.N, ())
+    add_oracle_weak(P2, mesh, "WeakDirichlet", b["xmin"], (1.0,))
+    r2, _, _ = P2.assemble_rhs(U)
+    assert abs(((r2 - r0)[:, 0] * vol).sum() - (-0.5) * (1.0 - 0.25)) < 1e-13
+    out = np.setdiff1d(np.union1d(b["xmax"], b["ymin"]), np.union1d(x_in, y_in))
+    assert corner_in.size == 1 and np.abs(d[out]).max() < 1e-15
