@@ -248,7 +248,10 @@ struct NodalGlobal
 // reaction residuals compute_adv_reaction_residuals, whose residual at a
 // quadrature point is div F - s_q, s_q = V . source (PGLDA.hpp:553-741,
 // PGN.hpp:470-660, PGB.hpp:825-1050).
-template <int D, int NG, int ND, int NF, int NQ, bool AFFINE, int PH, int SC, int PREC, class UA,
+// DLDS: the caller's LDS column at lds_div extends past the NQ M divergence
+// slices to ND (M + 1) entries (the patch kernel's work region), so the 3D P2
+// cells may keep part of their dissipation accumulators there.
+template <int D, int NG, int ND, int NF, int NQ, bool AFFINE, int PH, int SC, int PREC, bool DLDS = false, class UA,
           class NA = NodalNone>
 __device__ __forceinline__ void cell_residual(const double *__restrict__ tab, const double (&X)[NG][D],
                                               const double (&U)[ND][PhysTraits<PH>::M], const UA &Uq,
@@ -374,7 +377,26 @@ __device__ __forceinline__ void cell_residual(const double *__restrict__ tab, co
   // ---- element scheme ------------------------------------------------------
   constexpr bool NDIS = (SC != SC_LDA);
   double lda[ND][M];
-  double dis[NDIS ? ND : 1][M];
+  // dissipation accumulators: of the 3D P2 cells in the patch kernel, the
+  // first DKL components in the free part of the thread's LDS column (past the
+  // divergence slices; read back before the work region is reused), the
+  // others in registers -- registers the N / B / PSI cells would otherwise
+  // spill to scratch memory
+  constexpr int DKL_ = (ND * (M + 1) - NQ * M) / ND;
+  constexpr int DKL = (DLDS && NDIS && D == 3 && ND == 10 && DKL_ > 0) ? (DKL_ < M ? DKL_ : M) : 0;
+  constexpr int DKR = (M - DKL > 0) ? M - DKL : 1;
+  double dis[NDIS ? ND : 1][DKR];
+  double *const disl = lds_div + static_cast<size_t>(NQ) * M * stride;
+  auto dis_get = [&](const int n, const int k) -> double {
+    if (k < DKL) return disl[(n * DKL + k) * stride];
+    return dis[n][k >= DKL ? k - DKL : 0];
+  };
+  auto dis_add = [&](const int n, const int k, const double v) {
+    if (k < DKL)
+      disl[(n * DKL + k) * stride] += v;
+    else
+      dis[n][k >= DKL ? k - DKL : 0] += v;
+  };
   // max over nodes and quadrature points of |eigenvalue|; the floor 1e-6 of
   // every (node, point) term is applied once at the end (max(1e-6, max x) =
   // max over x of max(1e-6, x))
@@ -388,7 +410,12 @@ __device__ __forceinline__ void cell_residual(const double *__restrict__ tab, co
 #pragma unroll
   for (int n = 0; n < (NDIS ? ND : 1); ++n)
 #pragma unroll
-    for (int k = 0; k < M; ++k) dis[n][k] = 0.0;
+    for (int k = 0; k < M; ++k) {
+      if (k < DKL)
+        disl[(n * DKL + k) * stride] = 0.0;
+      else
+        dis[n][k >= DKL ? k - DKL : 0] = 0.0;
+    }
 #pragma unroll
   for (int k = 0; k < M; ++k) fint[k] = 0.0;
   double vol = 0.0;
@@ -483,7 +510,7 @@ __device__ __forceinline__ void cell_residual(const double *__restrict__ tab, co
 #pragma unroll
       for (int n = 0; n < ND; ++n) {
         lda[n][0] += kp[n] * wr;
-        if constexpr (NDIS) dis[n][0] += wj * (kp[n] * (Uq(n, 0) - ub));
+        if constexpr (NDIS) dis_add(n, 0, wj * (kp[n] * (Uq(n, 0) - ub)));
       }
     } else {
       double uq[M];
@@ -595,10 +622,21 @@ __device__ __forceinline__ void cell_residual(const double *__restrict__ tab, co
         out[M - 1] = k.l0 * v[M - 1] + al * st.H + be * k.um;
       };
       auto node_at = [&](const int n) -> NodeK {
-        if constexpr (KEEP)
+        if constexpr (KEEP) {
           return nk[n];
-        else
+        } else if constexpr (SC == SC_LDA) {
+          // an opaque copy of the index: the parameters are recomputed here,
+          // not kept alive from the loop over S (which common-subexpression
+          // elimination would otherwise do); measured faster for LDA (fewer
+          // registers: 2 waves per SIMD on P1, no scratch on P2) and slower
+          // for the dissipative schemes, whose second application per node
+          // reuses the parameters
+          int nn = n;
+          asm volatile("" : "+s"(nn));
+          return node_k(nn);
+        } else {
           return node_k(n);
+        }
       };
       // S = sl0 I + ssg/2a^2 r0 l0v^T + 1/2 sum sg e f^T + 1/2a (r0 (F w)^T + (E w) l0v^T)
       double r0[M], ew[M], fw[M];
@@ -678,7 +716,7 @@ __device__ __forceinline__ void cell_residual(const double *__restrict__ tab, co
           for (int k = 0; k < M; ++k) du[k] = wj * (Uq(n, k) - B[k][R - 1]);
           kplus(kn, du, t);
 #pragma unroll
-          for (int k = 0; k < M; ++k) dis[n][k] += t[k];
+          for (int k = 0; k < M; ++k) dis_add(n, k, t[k]);
         }
       }
     }
@@ -718,7 +756,7 @@ __device__ __forceinline__ void cell_residual(const double *__restrict__ tab, co
 #pragma unroll
     for (int n = 0; n < ND; ++n)
 #pragma unroll
-      for (int k = 0; k < M; ++k) phi[n][k] = lda[n][k] + dis[n][k];
+      for (int k = 0; k < M; ++k) phi[n][k] = lda[n][k] + dis_get(n, k);
   } else if constexpr (SC == SC_B && (PREC == 2 || PREC == 5)) {
     // external nodal coefficient; below 1e-6 at every node the cell keeps
     // its LDA residuals
@@ -733,16 +771,16 @@ __device__ __forceinline__ void cell_residual(const double *__restrict__ tab, co
 #pragma unroll
     for (int n = 0; n < ND; ++n)
 #pragma unroll
-      for (int k = 0; k < M; ++k) phi[n][k] = skip ? lda[n][k] : lda[n][k] + th[n] * dis[n][k];
+      for (int k = 0; k < M; ++k) phi[n][k] = skip ? lda[n][k] : lda[n][k] + th[n] * dis_get(n, k);
   } else if constexpr (SC == SC_B) {
     double an = 0.0;
 #pragma unroll
-    for (int n = 0; n < ND; ++n) an += fabs(lda[n][0] + dis[n][0]);
+    for (int n = 0; n < ND; ++n) an += fabs(lda[n][0] + dis_get(n, 0));
     const double theta = fabs(fint[0]) / (an + 1.e-6);
 #pragma unroll
     for (int n = 0; n < ND; ++n)
 #pragma unroll
-      for (int k = 0; k < M; ++k) phi[n][k] = lda[n][k] + theta * dis[n][k];
+      for (int k = 0; k < M; ++k) phi[n][k] = lda[n][k] + theta * dis_get(n, k);
   } else {
     // PSI: the x+ limiter of the N residuals, per equation; the PSI / LDA
     // blend then takes (1 - theta) LDA + theta PSI with PGB's cell theta
@@ -750,7 +788,7 @@ __device__ __forceinline__ void cell_residual(const double *__restrict__ tab, co
     if constexpr (SC == SC_BPSI) {
       double an = 0.0;
 #pragma unroll
-      for (int n = 0; n < ND; ++n) an += fabs(lda[n][0] + dis[n][0]);
+      for (int n = 0; n < ND; ++n) an += fabs(lda[n][0] + dis_get(n, 0));
       theta = fabs(fint[0]) / (an + 1.e-6);
     }
 #pragma unroll
@@ -762,12 +800,12 @@ __device__ __forceinline__ void cell_residual(const double *__restrict__ tab, co
       double sx = 0.0;
 #pragma unroll
       for (int n = 0; n < ND; ++n) {
-        xs[n] = fmax((lda[n][k] + dis[n][k]) * itot, 0.0) + 1.e-12;
+        xs[n] = fmax((lda[n][k] + dis_get(n, k)) * itot, 0.0) + 1.e-12;
         sx += xs[n];
       }
 #pragma unroll
       for (int n = 0; n < ND; ++n) {
-        const double psi = lim ? xs[n] / sx * tot : lda[n][k] + dis[n][k];
+        const double psi = lim ? xs[n] / sx * tot : lda[n][k] + dis_get(n, k);
         phi[n][k] = (SC == SC_BPSI) ? (1.0 - theta) * lda[n][k] + theta * psi : psi;
       }
     }
@@ -1197,30 +1235,30 @@ __device__ __forceinline__ void patch_body(const PatchArgs &a, uint32_t pl, doub
       for (int k = 0; k < M; ++k) U[n][k] = ul[row[n] * M + k];
     }
     if constexpr (PREC == 5 && M == 1)
-      cell_residual<D, NG, ND, NF, NQ, AFFINE, PH, SC, PREC>(
+      cell_residual<D, NG, ND, NF, NQ, AFFINE, PH, SC, PREC, true>(
           a.tab, X, U, URegs<ND, M>{U}, a.pp, phi, wave, bad, work + tid, PATCH,
           SourceThetaLds<ND, M>{nodal_l, nodal_l + static_cast<size_t>(a.maxl) * M, row});
     else if constexpr (PREC == 5)
-      cell_residual<D, NG, ND, NF, NQ, AFFINE, PH, SC, PREC>(
+      cell_residual<D, NG, ND, NF, NQ, AFFINE, PH, SC, PREC, true>(
           a.tab, X, U, ULds<ND, M>{ul, row}, a.pp, phi, wave, bad, work + tid, PATCH,
           SourceThetaLds<ND, M>{nodal_l, nodal_l + static_cast<size_t>(a.maxl) * M, row});
     else if constexpr (PREC == 3 && M == 1)
-      cell_residual<D, NG, ND, NF, NQ, AFFINE, PH, SC, PREC>(a.tab, X, U, URegs<ND, M>{U}, a.pp, phi, wave, bad, work + tid,
+      cell_residual<D, NG, ND, NF, NQ, AFFINE, PH, SC, PREC, true>(a.tab, X, U, URegs<ND, M>{U}, a.pp, phi, wave, bad, work + tid,
                                                        PATCH, SourceLds<ND, M>{nodal_l, row});
     else if constexpr (PREC == 3)
-      cell_residual<D, NG, ND, NF, NQ, AFFINE, PH, SC, PREC>(a.tab, X, U, ULds<ND, M>{ul, row}, a.pp, phi, wave, bad,
+      cell_residual<D, NG, ND, NF, NQ, AFFINE, PH, SC, PREC, true>(a.tab, X, U, ULds<ND, M>{ul, row}, a.pp, phi, wave, bad,
                                                        work + tid, PATCH, SourceLds<ND, M>{nodal_l, row});
     else if constexpr (PREC == 2 && M == 1)
-      cell_residual<D, NG, ND, NF, NQ, AFFINE, PH, SC, PREC>(a.tab, X, U, URegs<ND, M>{U}, a.pp, phi, wave, bad, work + tid,
+      cell_residual<D, NG, ND, NF, NQ, AFFINE, PH, SC, PREC, true>(a.tab, X, U, URegs<ND, M>{U}, a.pp, phi, wave, bad, work + tid,
                                                        PATCH, NodalLds<ND>{nodal_l, row});
     else if constexpr (PREC == 2)
-      cell_residual<D, NG, ND, NF, NQ, AFFINE, PH, SC, PREC>(a.tab, X, U, ULds<ND, M>{ul, row}, a.pp, phi, wave, bad,
+      cell_residual<D, NG, ND, NF, NQ, AFFINE, PH, SC, PREC, true>(a.tab, X, U, ULds<ND, M>{ul, row}, a.pp, phi, wave, bad,
                                                        work + tid, PATCH, NodalLds<ND>{nodal_l, row});
     else if constexpr (M == 1)
-      cell_residual<D, NG, ND, NF, NQ, AFFINE, PH, SC, PREC>(a.tab, X, U, URegs<ND, M>{U}, a.pp, phi, wave, bad, work + tid,
+      cell_residual<D, NG, ND, NF, NQ, AFFINE, PH, SC, PREC, true>(a.tab, X, U, URegs<ND, M>{U}, a.pp, phi, wave, bad, work + tid,
                                                        PATCH);
     else
-      cell_residual<D, NG, ND, NF, NQ, AFFINE, PH, SC, PREC>(a.tab, X, U, ULds<ND, M>{ul, row}, a.pp, phi, wave, bad,
+      cell_residual<D, NG, ND, NF, NQ, AFFINE, PH, SC, PREC, true>(a.tab, X, U, ULds<ND, M>{ul, row}, a.pp, phi, wave, bad,
                                                        work + tid, PATCH);
   }
   __syncthreads();  // the work region changes from divergence slices to pair sums
