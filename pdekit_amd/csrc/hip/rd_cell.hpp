@@ -235,6 +235,139 @@ struct NodalGlobal
   __device__ __forceinline__ double operator()(int n) const { return v[nid[n]]; }
 };
 
+// The K+ parameters of one node at one quadrature point of an Euler cell.
+// With the acoustic pairs written as
+//   r+- = (rho / 2a) (r0 +- a e_n),  l+- = (l0v +- a f_n) / (rho a),
+//   r0 = (1, V, H), e_n = (0, n, V.n), l0v = (gm1 |V|^2 / 2, -gm1 V, gm1),
+//   f_n = (-V.n, n, 0),
+// K+_n = l0_n I + dp_n r+ l+^T + dm_n r- l-^T becomes
+//   K+_n x = l0_n x + alpha r0 + beta e_n,
+//   alpha = (sg_n P + a dl_n Q) / 2a^2, beta = sg_n Q / 2 + dl_n P / 2a,
+//   P = l0v . x, Q = f_n . x, sg = dp + dm, dl = dp - dm,
+// and S = sum_n K+_n needs only sum l0, sum sg, w = sum dl n and
+// T = sum sg n n^T. nh = g / |g| for the physical gradient g of the node's
+// basis function, um = V . nh, emax the largest |eigenvalue| |g| (um, um +- a).
+template <int D>
+struct EulerNodeK
+{
+  double l0, sg, dl, um, nh[D], emax;
+};
+
+// Jq: the inverse Jacobian at the point, dvk: the reference gradients of the
+// ND basis functions there ([D][ND]).
+template <int D, int ND>
+__device__ __forceinline__ EulerNodeK<D> euler_node_k(const double (&Jq)[D][D], const double *dvk, int n,
+                                                      const double (&V)[D], double a)
+{
+  EulerNodeK<D> k;
+  double g[D];
+#pragma unroll
+  for (int x = 0; x < D; ++x) {
+    double t = 0.0;
+#pragma unroll
+    for (int xi = 0; xi < D; ++xi) t += Jq[x][xi] * dvk[xi * ND + n];
+    g[x] = t;
+  }
+  double nn = 0.0;
+#pragma unroll
+  for (int x = 0; x < D; ++x) nn += g[x] * g[x];
+  double norm, inorm;
+  sqrt_rsqrt(nn, norm, inorm);
+  norm = (nn > 0.0) ? norm : 0.0;  // sqrt_rsqrt(0) is NaN
+  const bool tiny = (D == 2) ? (norm < 1.e-9) : (norm == 0.0);
+  inorm = tiny ? 0.0 : inorm;
+  double u_n = 0.0;
+#pragma unroll
+  for (int x = 0; x < D; ++x) {
+    k.nh[x] = g[x] * inorm;
+    u_n += V[x] * k.nh[x];
+  }
+  k.um = u_n;
+  const double e0 = norm * u_n, ep = norm * (u_n + a), em = norm * (u_n - a);
+  k.emax = fmax(fabs(e0), fmax(fabs(ep), fabs(em)));
+  const double p0 = fmax(0.0, e0);
+  const double dpn = tiny ? 0.0 : fmax(0.0, ep) - p0;
+  const double dmn = tiny ? 0.0 : fmax(0.0, em) - p0;
+  k.l0 = tiny ? 0.0 : p0;
+  k.sg = dpn + dmn;
+  k.dl = dpn - dmn;
+  return k;
+}
+
+// out = K+_n v at a point with velocity V, total enthalpy H, sound speed a,
+// i2a = 1 / 2a, i2a2 = 1 / 2a^2 and l0v as above.
+template <int D>
+__device__ __forceinline__ void euler_kplus(const EulerNodeK<D> &k, const double (&l0v)[D + 2], const double (&V)[D],
+                                            double H, double a, double i2a, double i2a2, const double (&v)[D + 2],
+                                            double (&out)[D + 2])
+{
+  constexpr int M = D + 2;
+  double P = 0.0;
+#pragma unroll
+  for (int c = 0; c < M; ++c) P += l0v[c] * v[c];
+  double Q = -k.um * v[0];
+#pragma unroll
+  for (int x = 0; x < D; ++x) Q += k.nh[x] * v[1 + x];
+  const double al = (k.sg * P + a * k.dl * Q) * i2a2;
+  const double be = 0.5 * k.sg * Q + k.dl * P * i2a;
+  out[0] = k.l0 * v[0] + al;
+#pragma unroll
+  for (int x = 0; x < D; ++x) out[1 + x] = k.l0 * v[1 + x] + al * V[x] + be * k.nh[x];
+  out[M - 1] = k.l0 * v[M - 1] + al * H + be * k.um;
+}
+
+// S = sum_n K+_n from the node sums (sl0 = sum l0, ssg = sum sg, w, T) as
+// S = sl0 I + ssg/2a^2 r0 l0v^T + 1/2 sum sg e f^T + 1/2a (r0 (F w)^T + (E w) l0v^T).
+template <int D>
+__device__ __forceinline__ void euler_kplus_sum(double sl0, double ssg, const double (&w)[D], const double (&T)[D][D],
+                                                const double (&l0v)[D + 2], const double (&V)[D], double H,
+                                                double i2a, double i2a2, double (&A)[D + 2][D + 2])
+{
+  constexpr int M = D + 2;
+  double r0[M], ew[M], fw[M];
+  r0[0] = 1.0;
+#pragma unroll
+  for (int x = 0; x < D; ++x) r0[1 + x] = V[x];
+  r0[M - 1] = H;
+  double vw = 0.0;
+#pragma unroll
+  for (int x = 0; x < D; ++x) vw += V[x] * w[x];
+  ew[0] = 0.0;
+  fw[0] = -vw;
+#pragma unroll
+  for (int x = 0; x < D; ++x) {
+    ew[1 + x] = w[x];
+    fw[1 + x] = w[x];
+  }
+  ew[M - 1] = vw;
+  fw[M - 1] = 0.0;
+  double TV[D], VTV = 0.0;
+#pragma unroll
+  for (int x = 0; x < D; ++x) {
+    double t = 0.0;
+#pragma unroll
+    for (int y = 0; y < D; ++y) t += T[x][y] * V[y];
+    TV[x] = t;
+    VTV += V[x] * t;
+  }
+  const double c1 = ssg * i2a2;
+#pragma unroll
+  for (int i = 0; i < M; ++i)
+#pragma unroll
+    for (int j = 0; j < M; ++j) {
+      // 1/2 sum_n sg_n e_n f_n^T
+      double et = 0.0;
+      if (i >= 1 && i <= D) {
+        if (j == 0) et = -TV[i - 1];
+        else if (j <= D) et = T[i - 1][j - 1];
+      } else if (i == M - 1) {
+        if (j == 0) et = -VTV;
+        else if (j <= D) et = TV[j - 1];
+      }
+      A[i][j] = ((i == j) ? sl0 : 0.0) + c1 * r0[i] * l0v[j] + 0.5 * et + i2a * (r0[i] * fw[j] + ew[i] * l0v[j]);
+    }
+}
+
 // Residuals phi[ND][M] and wave speed of one element from its vertex
 // coordinates X and nodal state U. lds_div: this thread's slice of NQ*M
 // doubles of LDS (stride `stride` between entries) that carries the flux
@@ -249,8 +382,9 @@ struct NodalGlobal
 // quadrature point is div F - s_q, s_q = V . source (PGLDA.hpp:553-741,
 // PGN.hpp:470-660, PGB.hpp:825-1050).
 // DLDS: the caller's LDS column at lds_div extends past the NQ M divergence
-// slices to ND (M + 1) entries (the patch kernel's work region), so the 3D P2
-// cells may keep part of their dissipation accumulators there.
+// slices to ND (M + 1) entries (the patch kernel's work region). The
+// dissipative schemes of the 3D P2 cells then run the quadrature in two
+// passes that carry the per-point results in that column (see TWO below).
 template <int D, int NG, int ND, int NF, int NQ, bool AFFINE, int PH, int SC, int PREC, bool DLDS = false, class UA,
           class NA = NodalNone>
 __device__ __forceinline__ void cell_residual(const double *__restrict__ tab, const double (&X)[NG][D],
@@ -377,26 +511,9 @@ __device__ __forceinline__ void cell_residual(const double *__restrict__ tab, co
   // ---- element scheme ------------------------------------------------------
   constexpr bool NDIS = (SC != SC_LDA);
   double lda[ND][M];
-  // dissipation accumulators: of the 3D P2 cells in the patch kernel, the
-  // first DKL components in the free part of the thread's LDS column (past the
-  // divergence slices; read back before the work region is reused), the
-  // others in registers -- registers the N / B / PSI cells would otherwise
-  // spill to scratch memory
-  constexpr int DKL_ = (ND * (M + 1) - NQ * M) / ND;
-  constexpr int DKL = (DLDS && NDIS && D == 3 && ND == 10 && DKL_ > 0) ? (DKL_ < M ? DKL_ : M) : 0;
-  constexpr int DKR = (M - DKL > 0) ? M - DKL : 1;
-  double dis[NDIS ? ND : 1][DKR];
-  double *const disl = lds_div + static_cast<size_t>(NQ) * M * stride;
-  auto dis_get = [&](const int n, const int k) -> double {
-    if (k < DKL) return disl[(n * DKL + k) * stride];
-    return dis[n][k >= DKL ? k - DKL : 0];
-  };
-  auto dis_add = [&](const int n, const int k, const double v) {
-    if (k < DKL)
-      disl[(n * DKL + k) * stride] += v;
-    else
-      dis[n][k >= DKL ? k - DKL : 0] += v;
-  };
+  double dis[NDIS ? ND : 1][M];
+  auto dis_get = [&](const int n, const int k) -> double { return dis[n][k]; };
+  auto dis_add = [&](const int n, const int k, const double v) { dis[n][k] += v; };
   // max over nodes and quadrature points of |eigenvalue|; the floor 1e-6 of
   // every (node, point) term is applied once at the end (max(1e-6, max x) =
   // max over x of max(1e-6, x))
@@ -410,15 +527,23 @@ __device__ __forceinline__ void cell_residual(const double *__restrict__ tab, co
 #pragma unroll
   for (int n = 0; n < (NDIS ? ND : 1); ++n)
 #pragma unroll
-    for (int k = 0; k < M; ++k) {
-      if (k < DKL)
-        disl[(n * DKL + k) * stride] = 0.0;
-      else
-        dis[n][k >= DKL ? k - DKL : 0] = 0.0;
-    }
+    for (int k = 0; k < M; ++k) dis[n][k] = 0.0;
 #pragma unroll
   for (int k = 0; k < M; ++k) fint[k] = 0.0;
   double vol = 0.0;
+
+  // TWO: the dissipative schemes of the 3D P2 cells (affine, NQ 3 M <= ND (M
+  // + 1)) in two passes over the quadrature points, so that neither holds the
+  // K+ parameters of all nodes beside the 2 ND M residual sums:
+  //   1. per point q: S_q = sum_n K+_n (one evaluation of each node's
+  //      parameters, which also gives sum_n K+_n U_n), the solve for the LDA
+  //      weights x_q = w_q S_q^-1 r_q and ub_q = S_q^-1 sum_n K+_n U_n; the
+  //      point's (V, a, H) replace its divergence slice in the LDS column and
+  //      x_q, w_q ub_q follow the slices;
+  //   2. per point q again, every node: K+_n x_q and K+_n (w_q U_n - w_q ub_q)
+  //      added to the node's LDA and dissipation sums (a second evaluation of
+  //      the parameters).
+  constexpr bool TWO = (D == 3) && NDIS && AFFINE && DLDS && (M == D + 2) && (NQ * 3 * M <= ND * (M + 1));
 
   auto quad_point = [&](const int q, const double (&r)[M], const double (&Jq)[D][D], const double detq) {
     const double wj = tab[O::w + q] * detq;
@@ -525,57 +650,11 @@ __device__ __forceinline__ void cell_residual(const double *__restrict__ tab, co
       bad = bad || st.bad;
       const double gm1 = pp.p0 - 1.0;
       const double *dvk = dvq;
-      // Compact K+ of every node. With the acoustic pairs written as
-      //   r+- = (rho / 2a) (r0 +- a e_n),  l+- = (l0v +- a f_n) / (rho a),
-      //   r0 = (1, V, H), e_n = (0, n, V.n), l0v = (gm1 |V|^2 / 2, -gm1 V, gm1),
-      //   f_n = (-V.n, n, 0),
-      // K+_n = l0_n I + dp_n r+ l+^T + dm_n r- l-^T becomes
-      //   K+_n x = l0_n x + alpha r0 + beta e_n,
-      //   alpha = (sg_n P + a dl_n Q) / 2a^2, beta = sg_n Q / 2 + dl_n P / 2a,
-      //   P = l0v . x, Q = f_n . x, sg = dp + dm, dl = dp - dm,
-      // and S = sum_n K+_n needs only sum l0, sum sg, w = sum dl n and
-      // T = sum sg n n^T.
-      struct NodeK
-      {
-        double l0, sg, dl, um, nh[D], emax;
-      };
-      // the K+ parameters of node n from the physical gradient of its basis
-      // function (recomputed where keeping them for all nodes costs spills)
-      auto node_k = [&](const int n) {
-        NodeK k;
-        double g[D];
-#pragma unroll
-        for (int x = 0; x < D; ++x) {
-          double t = double(0);
-#pragma unroll
-          for (int xi = 0; xi < D; ++xi) t += double(Jq[x][xi]) * dvk[xi * ND + n];
-          g[x] = t;
-        }
-        double nn = double(0);
-#pragma unroll
-        for (int x = 0; x < D; ++x) nn += g[x] * g[x];
-        double norm, inorm;
-        sqrt_rsqrt(nn, norm, inorm);
-        norm = (nn > double(0)) ? norm : double(0);  // sqrt_rsqrt(0) is NaN
-        const bool tiny = (D == 2) ? (norm < double(1.e-9)) : (norm == double(0));
-        inorm = tiny ? double(0) : inorm;
-        double u_n = double(0);
-#pragma unroll
-        for (int x = 0; x < D; ++x) {
-          k.nh[x] = g[x] * inorm;
-          u_n += st.V[x] * k.nh[x];
-        }
-        k.um = u_n;
-        const double e0 = norm * u_n, ep = norm * (u_n + st.a), em = norm * (u_n - st.a);
-        k.emax = fmax(fabs(e0), fmax(fabs(ep), fabs(em)));
-        const double p0 = fmax(double(0), e0);
-        const double dpn = tiny ? double(0) : fmax(double(0), ep) - p0;
-        const double dmn = tiny ? double(0) : fmax(double(0), em) - p0;
-        k.l0 = tiny ? double(0) : p0;
-        k.sg = dpn + dmn;
-        k.dl = dpn - dmn;
-        return k;
-      };
+      // compact K+ of every node (EulerNodeK, euler_kplus)
+      using NodeK = EulerNodeK<D>;
+      // the K+ parameters of node n (recomputed where keeping them for all
+      // nodes costs spills)
+      auto node_k = [&](const int n) { return euler_node_k<D, ND>(Jq, dvk, n, st.V, st.a); };
       // 3D: recompute in the application instead of keeping ND sets
       constexpr bool KEEP = (D == 2);
       NodeK nk[KEEP ? ND : 1];
@@ -608,18 +687,7 @@ __device__ __forceinline__ void cell_residual(const double *__restrict__ tab, co
       for (int x = 0; x < D; ++x) l0v[1 + x] = -gm1 * st.V[x];
       l0v[M - 1] = gm1;
       auto kplus = [&](const NodeK &k, const double (&v)[M], double (&out)[M]) {
-        double P = 0.0;
-#pragma unroll
-        for (int c = 0; c < M; ++c) P += l0v[c] * v[c];
-        double Q = -k.um * v[0];
-#pragma unroll
-        for (int x = 0; x < D; ++x) Q += k.nh[x] * v[1 + x];
-        const double al = (k.sg * P + a * k.dl * Q) * i2a2;
-        const double be = 0.5 * k.sg * Q + k.dl * P * i2a;
-        out[0] = k.l0 * v[0] + al;
-#pragma unroll
-        for (int x = 0; x < D; ++x) out[1 + x] = k.l0 * v[1 + x] + al * st.V[x] + be * k.nh[x];
-        out[M - 1] = k.l0 * v[M - 1] + al * st.H + be * k.um;
+        euler_kplus<D>(k, l0v, st.V, st.H, a, i2a, i2a2, v, out);
       };
       auto node_at = [&](const int n) -> NodeK {
         if constexpr (KEEP) {
@@ -638,6 +706,8 @@ __device__ __forceinline__ void cell_residual(const double *__restrict__ tab, co
           return node_k(n);
         }
       };
+      // S as in euler_kplus_sum, written out here: the call changes the
+      // register allocation of the 2D quadrilateral cells (a 16-byte spill)
       // S = sl0 I + ssg/2a^2 r0 l0v^T + 1/2 sum sg e f^T + 1/2a (r0 (F w)^T + (E w) l0v^T)
       double r0[M], ew[M], fw[M];
       r0[0] = 1.0;
@@ -722,6 +792,122 @@ __device__ __forceinline__ void cell_residual(const double *__restrict__ tab, co
     }
   };
 
+  if constexpr (TWO) {
+    const double gm1 = pp.p0 - 1.0;
+    const double(&Jq)[D][D] = Jinv[0];
+    // LDS column: [q M, q M + M) the divergence of q, then (V, a, H) of q;
+    // [NQ M + 2 M q, + M) x_q, [NQ M + 2 M q + M, + M) w_q ub_q
+    double *const qx = lds_div + static_cast<size_t>(NQ) * M * stride;
+#pragma unroll 1
+    for (int q = 0; q < NQ; ++q) {
+      double r[M];
+#pragma unroll
+      for (int k = 0; k < M; ++k) r[k] = lds_div[(q * M + k) * stride];
+      const double wj = tab[O::w + q] * detJ[0];
+      vol += wj;
+#pragma unroll
+      for (int k = 0; k < M; ++k) fint[k] += wj * r[k];
+      const double *dvk = tab + O::dVT + q * D * ND;
+      double uq[M];
+#pragma unroll
+      for (int k = 0; k < M; ++k) {
+        double t = 0.0;
+#pragma unroll
+        for (int n = 0; n < ND; ++n) t += tab[O::V + q * ND + n] * Uq(n, k);
+        uq[k] = t;
+      }
+      const EulerState<double, D> st = euler_state<double, D>(uq, double(pp.p0));
+      bad = bad || st.bad;
+      const double i2a = 0.5 * st.inv_a, i2a2 = i2a * st.inv_a;
+      double l0v[M];
+      l0v[0] = 0.5 * gm1 * st.v2;
+#pragma unroll
+      for (int x = 0; x < D; ++x) l0v[1 + x] = -gm1 * st.V[x];
+      l0v[M - 1] = gm1;
+      double sl0 = 0.0, ssg = 0.0, w[D], T[D][D], B[M][2];
+#pragma unroll
+      for (int x = 0; x < D; ++x) {
+        w[x] = 0.0;
+#pragma unroll
+        for (int y = 0; y < D; ++y) T[x][y] = 0.0;
+      }
+#pragma unroll
+      for (int k = 0; k < M; ++k) {
+        B[k][0] = r[k];
+        B[k][1] = 0.0;
+      }
+#pragma unroll
+      for (int n = 0; n < ND; ++n) {
+        const EulerNodeK<D> k = euler_node_k<D, ND>(Jq, dvk, n, st.V, st.a);
+        lmax = fmax(lmax, k.emax);
+        sl0 += k.l0;
+        ssg += k.sg;
+#pragma unroll
+        for (int x = 0; x < D; ++x) {
+          w[x] += k.dl * k.nh[x];
+#pragma unroll
+          for (int y = 0; y < D; ++y) T[x][y] += k.sg * k.nh[x] * k.nh[y];
+        }
+        double un[M], z[M];
+#pragma unroll
+        for (int c = 0; c < M; ++c) un[c] = Uq(n, c);
+        euler_kplus<D>(k, l0v, st.V, st.H, st.a, i2a, i2a2, un, z);
+#pragma unroll
+        for (int c = 0; c < M; ++c) B[c][1] += z[c];
+      }
+      double A[M][M];
+      euler_kplus_sum<D>(sl0, ssg, w, T, l0v, st.V, st.H, i2a, i2a2, A);
+      solve_pp<M, 2>(A, B);
+#pragma unroll
+      for (int x = 0; x < D; ++x) lds_div[(q * M + x) * stride] = st.V[x];
+      lds_div[(q * M + D) * stride] = st.a;
+      lds_div[(q * M + D + 1) * stride] = st.H;
+#pragma unroll
+      for (int k = 0; k < M; ++k) {
+        qx[(2 * M * q + k) * stride] = B[k][0] * wj;
+        qx[(2 * M * q + M + k) * stride] = B[k][1] * wj;
+      }
+    }
+#pragma unroll 1
+    for (int q = 0; q < NQ; ++q) {
+      double V[D], x[M], wub[M];
+#pragma unroll
+      for (int d = 0; d < D; ++d) V[d] = lds_div[(q * M + d) * stride];
+      const double a = lds_div[(q * M + D) * stride];
+      const double H = lds_div[(q * M + D + 1) * stride];
+#pragma unroll
+      for (int k = 0; k < M; ++k) {
+        x[k] = qx[(2 * M * q + k) * stride];
+        wub[k] = qx[(2 * M * q + M + k) * stride];
+      }
+      const double wj = tab[O::w + q] * detJ[0];
+      const double ia = recip(a);
+      const double i2a = 0.5 * ia, i2a2 = i2a * ia;
+      double v2 = 0.0;
+#pragma unroll
+      for (int d = 0; d < D; ++d) v2 += V[d] * V[d];
+      double l0v[M];
+      l0v[0] = 0.5 * gm1 * v2;
+#pragma unroll
+      for (int d = 0; d < D; ++d) l0v[1 + d] = -gm1 * V[d];
+      l0v[M - 1] = gm1;
+      const double *dvk = tab + O::dVT + q * D * ND;
+#pragma unroll
+      for (int n = 0; n < ND; ++n) {
+        const EulerNodeK<D> k = euler_node_k<D, ND>(Jq, dvk, n, V, a);
+        double t[M], du[M];
+        euler_kplus<D>(k, l0v, V, H, a, i2a, i2a2, x, t);
+#pragma unroll
+        for (int c = 0; c < M; ++c) {
+          lda[n][c] += t[c];
+          du[c] = wj * Uq(n, c) - wub[c];
+        }
+        euler_kplus<D>(k, l0v, V, H, a, i2a, i2a2, du, t);
+#pragma unroll
+        for (int c = 0; c < M; ++c) dis[n][c] += t[c];
+      }
+    }
+  } else {
 #pragma unroll 1
   for (int q = 0; q < NQ; ++q) {
     double r[M];
@@ -744,6 +930,7 @@ __device__ __forceinline__ void cell_residual(const double *__restrict__ tab, co
       const double dq = det_inv<D>(J, Ji);
       quad_point(q, r, Ji, dq);
     }
+  }
   }
 
   // ---- blend / limit -------------------------------------------------------
