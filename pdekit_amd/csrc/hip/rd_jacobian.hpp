@@ -563,10 +563,10 @@ __global__ __launch_bounds__(64) void rd_jacobian_kernel(JacArgs a)
 }
 
 // ---------------------------------------------------------------------------
-// Euler cells, one lane per (cell, quadrature point): LPC lanes per cell (NQ
-// rounded up to 4, 8 or 16; the extra lanes take the last point with weight
-// zero), so the per-point work -- the K+- parameters of the nodes, S and its
-// inverse -- is done once. Lane q forms its point's share of the residuals
+// Euler cells, one lane per (cell, quadrature point): LPC = NQ lanes per
+// cell, 64 / NQ cells per wave (the 64 mod NQ lanes left over idle), so the
+// per-point work -- the K+- parameters of the nodes, S and its inverse -- is
+// done once. Lane q forms its point's share of the residuals
 // and of every block J_mn = w B_m X_n (B_m = K+_m S^-1; X_n = K_n for LDA,
 // -K-_n plus K+_m on the diagonal for N, K_n - theta_m K+_n plus theta_m K+_m
 // on the diagonal for B), all K applied in the compact form
@@ -593,7 +593,8 @@ struct NodeQ
 template <int NQ>
 struct QpLanes
 {
-  static constexpr int value = NQ <= 4 ? 4 : (NQ <= 8 ? 8 : 16);
+  static_assert(NQ <= 64, "one cell's points must fit in a wave");
+  static constexpr int value = NQ;
 };
 
 // Stride of the exchange slab rows (doubles): 64 lanes plus one, so that the
@@ -601,82 +602,139 @@ struct QpLanes
 // banks.
 constexpr int QP_XS = 65;
 
-template <int PH>
-struct QpSlab
+// Waves per SIMD of the per-point kernel and rows of a wave's exchange slab.
+// The 2D Euler P1 / Q1 cells run at 2 waves (a few spilled registers; measured
+// 1.2x faster than 1 wave without spills, the exchanges' LDS waits of one
+// wave hidden behind the other's arithmetic), with 32 slab rows so that 8
+// waves fit in a CU's LDS; the others at 1 wave with 64 rows. The Jacobian
+// blocks of a row block go through the slab several at a time (as many as
+// fit), so that the wave waits for the LDS once per batch rather than once
+// per block.
+template <int M, int ND>
+struct QpWaves
 {
-  static constexpr int M = PhysTraits<PH>::M;
-  static constexpr int value = (M * M + 3) / 4 * 4 > 16 ? 32 : 16;  // rows per wave: the largest exchange
+  static constexpr int value = (M == 4 && ND <= 4) ? 2 : 1;
 };
 
-// out[r] = sum over the lanes l of the group of v_l[q R + r] (q: the lane's
-// index in its group, gl: the group's first lane in the wave).
-template <int LPC, int R>
-__device__ __forceinline__ void group_scatter(double *xw, int lane, int gl, int q, const double (&v)[LPC * R],
-                                              double (&out)[R])
+template <int M, int ND>
+struct QpRows
 {
-  // the slab's previous reads (by other lanes) complete before it is
-  // overwritten, and the writes before it is read; the memory clobber keeps
-  // the compiler from moving LDS accesses across the waits
-  asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+  static constexpr int value = QpWaves<M, ND>::value > 1 ? 32 : 64;
+};
+
+// The two halves of an exchange. Value j of a lane goes to row base + j; the
+// lane of index q in its group (gl: the group's first lane in the wave) owns
+// the values r LPC + q, r < R. With that order the 64 lanes of a read fall
+// on distinct banks (row stride QP_XS = 1 mod 16 doubles).
+template <int LPC, int R>
+__device__ __forceinline__ void slab_put(double *xw, int lane, int base, const double (&v)[LPC * R])
+{
 #pragma unroll
-  for (int j = 0; j < LPC * R; ++j) xw[j * QP_XS + lane] = v[j];
-  asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+  for (int j = 0; j < LPC * R; ++j) xw[(base + j) * QP_XS + lane] = v[j];
+}
+
+// out[r] = sum over the lanes l of the group of v_l[r LPC + q].
+template <int LPC, int R>
+__device__ __forceinline__ void slab_sum(const double *xw, int gl, int q, int base, double (&out)[R])
+{
 #pragma unroll
   for (int r = 0; r < R; ++r) {
     double s = 0.0;
 #pragma unroll
-    for (int l = 0; l < LPC; ++l) s += xw[(q * R + r) * QP_XS + gl + l];
+    for (int l = 0; l < LPC; ++l) s += xw[(base + r * LPC + q) * QP_XS + gl + l];
     out[r] = s;
   }
 }
 
+// Waits for the wave's outstanding LDS accesses; the memory clobber keeps the
+// compiler from moving LDS accesses across the wait.
+__device__ __forceinline__ void lds_wait() { asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory"); }
+
+// One exchange: the slab's previous reads (by other lanes) complete before it
+// is overwritten, and the writes before it is read.
+template <int LPC, int R>
+__device__ __forceinline__ void group_scatter(double *xw, int lane, int gl, int q, const double (&v)[LPC * R],
+                                              double (&out)[R])
+{
+  lds_wait();
+  slab_put<LPC, R>(xw, lane, 0, v);
+  lds_wait();
+  slab_sum<LPC, R>(xw, gl, q, 0, out);
+}
+
 template <int D, int NG, int ND, int NF, int NQ, bool AFFINE, int PH, int SC>
-__global__ __launch_bounds__(128) void rd_jacobian_qp_kernel(JacArgs a)
+__global__ __launch_bounds__(64, (QpWaves<PhysTraits<PH>::M, ND>::value)) void rd_jacobian_qp_kernel(JacArgs a)
 {
   constexpr int M = PhysTraits<PH>::M;
+  constexpr int QPR = QpRows<M, ND>::value;
   static_assert(M == D + 2, "Euler cells only");
   constexpr int NDM = ND * M;
   constexpr int LPC = QpLanes<NQ>::value;
-  constexpr int CPB = 128 / LPC;
+  constexpr int CPB = 64 / LPC;
   constexpr int RJ = (M * M + LPC - 1) / LPC;
   constexpr int NT = ND + 2;  // theta inputs: t0 of every node, fint0, vol
   constexpr int RT = (NT + LPC - 1) / LPC;
   constexpr int RP = (M + LPC - 1) / LPC;
-  static_assert(LPC * RJ <= QpSlab<PH>::value && LPC * RT <= QpSlab<PH>::value && LPC * RP <= QpSlab<PH>::value &&
-                    LPC <= QpSlab<PH>::value,
-                "exchange slab too small");
+  constexpr int JB = QPR / (LPC * RJ);  // Jacobian blocks per exchange
+  static_assert(JB >= 1 && LPC * RT <= QPR && LPC * RP <= QPR && LPC <= QPR, "exchange slab too small");
   using O = TabOffsets<D, NG, ND, NF, NQ>;
   extern __shared__ double xl[];
   const int lane = threadIdx.x & 63;
-  double *xw = xl + (threadIdx.x >> 6) * QP_XS * QpSlab<PH>::value;
-  const int q = threadIdx.x % LPC;
-  const int gl = lane - q;
-  const uint32_t e0 = blockIdx.x * CPB + threadIdx.x / LPC;
-  // lanes past the end work on the last cell and store nothing
+  double *xw = xl;  // one wave per workgroup
+  // the wave's last 64 mod LPC lanes (spare) and the lanes of cells past the
+  // end work on the last cell and store nothing; a spare lane reads the
+  // exchanges of the first group (in bounds, results unused)
+  const bool spare = lane >= CPB * LPC;
+  const int q = spare ? 0 : lane % LPC;
+  const int gl = spare ? 0 : lane - q;
+  const uint32_t e0 = spare ? a.ne : blockIdx.x * CPB + lane / LPC;
   const bool active = e0 < a.ne;
   const uint32_t e = active ? e0 : a.ne - 1;
   const bool qa = q < NQ;
   const int qq = qa ? q : NQ - 1;
-  const double *__restrict__ tab = a.tab;
   bool bad = false;
-  double X[NG][D];
+  // the cell's indices first, then the values they address (two rounds of
+  // memory latency; loads interleaved with their uses, or behind a branch per
+  // node, were issued one after another)
+  uint32_t gv[NG], sv[ND];
 #pragma unroll
-  for (int g = 0; g < NG; ++g) {
-    const size_t v = a.gconn[static_cast<size_t>(g) * a.ne + e];
+  for (int g = 0; g < NG; ++g) gv[g] = a.gconn[static_cast<size_t>(g) * a.ne + e];
 #pragma unroll
-    for (int d = 0; d < D; ++d) X[g][d] = a.coords[v * D + d];
-  }
-  double U[ND][M], nodal[ND];
-#pragma unroll
-  for (int n = 0; n < ND; ++n) {
-    const size_t v = a.sconn[static_cast<size_t>(n) * a.ne + e];
-#pragma unroll
-    for (int k = 0; k < M; ++k) U[n][k] = a.u[v * M + k];
-    nodal[n] = a.nodal ? a.nodal[v] : 0.0;
-  }
+  for (int n = 0; n < ND; ++n) sv[n] = a.sconn[static_cast<size_t>(n) * a.ne + e];
   const uint32_t c = a.perm[e];
   const bool lin = a.jlin != nullptr;
   const uint32_t pos = lin ? a.lin_pos[e] : 0u;
+  // the table values of the lane's point (vector loads, one round with the
+  // cell's data rather than one per phase)
+  const double *__restrict__ tab = a.tab;
+  double tV[ND], tdV[D][ND], tdVg[AFFINE ? 1 : D][NG];
+  const double tw = tab[O::w + qq];
+#pragma unroll
+  for (int n = 0; n < ND; ++n) {
+    tV[n] = tab[O::V + qq * ND + n];
+#pragma unroll
+    for (int xi = 0; xi < D; ++xi) tdV[xi][n] = tab[O::dVT + (qq * D + xi) * ND + n];
+  }
+  if constexpr (!AFFINE)
+#pragma unroll
+    for (int xi = 0; xi < D; ++xi)
+#pragma unroll
+      for (int g = 0; g < NG; ++g) tdVg[xi][g] = tab[O::dVgT + (qq * D + xi) * NG + g];
+  double X[NG][D];
+#pragma unroll
+  for (int g = 0; g < NG; ++g)
+#pragma unroll
+    for (int d = 0; d < D; ++d) X[g][d] = a.coords[static_cast<size_t>(gv[g]) * D + d];
+  double U[ND][M], nodal[ND];
+#pragma unroll
+  for (int n = 0; n < ND; ++n)
+#pragma unroll
+    for (int k = 0; k < M; ++k) U[n][k] = a.u[static_cast<size_t>(sv[n]) * M + k];
+#pragma unroll
+  for (int n = 0; n < ND; ++n) nodal[n] = 0.0;
+  if (a.nodal)
+#pragma unroll
+    for (int n = 0; n < ND; ++n) nodal[n] = a.nodal[sv[n]];
   const JacBlock jac{lin ? nullptr : a.jac + static_cast<size_t>(c) * NDM * NDM, lin ? a.jlin + pos : nullptr, a.ne};
   const bool ext_theta = (SC == SC_B) && a.nodal;
   const bool visc = (SC == SC_LDA) && a.nodal;
@@ -698,70 +756,94 @@ __global__ __launch_bounds__(128) void rd_jacobian_qp_kernel(JacArgs a)
     return det_inv<D>(J, Ji);
   };
 
-  // flux divergence: lane q evaluates the flux nodes q, q + LPC, ... and adds
-  // their share at every point; then lane q sums the point q of the group
-  double r[M];
-  {
-    double divr[LPC][M];
+  // the point's geometry
+  double Jinv[D][D];
+  double detq;
+  if constexpr (AFFINE) {
+    detq = geometry(0, Jinv);  // the same Jacobian at every point
+  } else {
+    double J[D][D];
 #pragma unroll
-    for (int t = 0; t < LPC; ++t)
+    for (int xi = 0; xi < D; ++xi)
 #pragma unroll
-      for (int k = 0; k < M; ++k) divr[t][k] = 0.0;
-    double Ja[D][D];
-    if constexpr (AFFINE) geometry(0, Ja);
-#pragma unroll 1
-    for (int f = q; f < NF; f += LPC) {
-      double s[M];
-#pragma unroll
-      for (int k = 0; k < M; ++k) {
+      for (int x = 0; x < D; ++x) {
         double t = 0.0;
 #pragma unroll
-        for (int n = 0; n < ND; ++n) t += tab[O::Vsf + f * ND + n] * U[n][k];
-        s[k] = t;
+        for (int g = 0; g < NG; ++g) t += tdVg[xi][g] * X[g][x];
+        J[xi][x] = t;
       }
-      double F[D][M];
-      point_flux<PH, double>(s, a.pp, F, bad);
+    detq = det_inv<D>(J, Jinv);
+  }
+
+  // flux divergence at the lane's point: the group's lanes evaluate the flux
+  // nodes f = q + i LPC and exchange the fluxes F_f (D M values each, value
+  // (d, k) of the lane's i-th node at slab row i D M + d M + k, in batches
+  // of FB nodes); lane q then forms div F(q) = sum_f sum_d b_fd F_f[d] with
+  // b_f = Jinv_q grad_ref psi_f(q) (the Jacobian of its own point only)
+  double r[M];
+  {
+    constexpr int NFL = (NF + LPC - 1) / LPC;  // flux nodes per lane
+    constexpr int FV = D * M;
+    constexpr int FB = QPR / FV;           // flux-node slots per exchange
+    static_assert(FB >= 1, "exchange slab too small");
 #pragma unroll
-      for (int t = 0; t < NQ; ++t) {
-        double Jq[D][D];
-        if constexpr (AFFINE) {
+    for (int k = 0; k < M; ++k) r[k] = 0.0;
 #pragma unroll
-          for (int x = 0; x < D; ++x)
+    for (int i0 = 0; i0 < NFL; i0 += FB) {
+      lds_wait();
 #pragma unroll
-            for (int y = 0; y < D; ++y) Jq[x][y] = Ja[x][y];
-        } else {
-          geometry(t, Jq);
+      for (int i = i0; i < i0 + FB && i < NFL; ++i) {
+        const int f = q + i * LPC;
+        if (f < NF) {
+          double sf[M];
+#pragma unroll
+          for (int k = 0; k < M; ++k) {
+            double t = 0.0;
+#pragma unroll
+            for (int n = 0; n < ND; ++n) t += tab[O::Vsf + f * ND + n] * U[n][k];
+            sf[k] = t;
+          }
+          double F[D][M];
+          point_flux<PH, double>(sf, a.pp, F, bad);
+#pragma unroll
+          for (int d = 0; d < D; ++d)
+#pragma unroll
+            for (int k = 0; k < M; ++k) xw[((i - i0) * FV + d * M + k) * QP_XS + lane] = F[d][k];
         }
+      }
+      lds_wait();
+      // rolled: unrolled, every read of the slab and the table was hoisted
+      // ahead (registers beyond the file)
+#pragma unroll 1
+      for (int f = i0 * LPC; f < NF && f < (i0 + FB) * LPC; ++f) {
+        const int i = f / LPC, src = gl + f % LPC;
+        double b[D];
 #pragma unroll
         for (int d = 0; d < D; ++d) {
-          double b = 0.0;
+          double t = 0.0;
 #pragma unroll
-          for (int xi = 0; xi < D; ++xi) b += Jq[d][xi] * tab[O::dVfT + (f * D + xi) * NQ + t];
+          for (int xi = 0; xi < D; ++xi) t += Jinv[d][xi] * tab[O::dVfT + (f * D + xi) * NQ + qq];
+          b[d] = t;
+        }
 #pragma unroll
-          for (int k = 0; k < M; ++k) divr[t][k] += b * F[d][k];
+        for (int k = 0; k < M; ++k) {
+          double t = r[k];
+#pragma unroll
+          for (int d = 0; d < D; ++d) t += b[d] * xw[((i - i0) * FV + d * M + k) * QP_XS + src];
+          r[k] = t;
         }
       }
-    }
-#pragma unroll
-    for (int k = 0; k < M; ++k) {
-      double v[LPC], o[1];
-#pragma unroll
-      for (int t = 0; t < LPC; ++t) v[t] = divr[t][k];
-      group_scatter<LPC, 1>(xw, lane, gl, q, v, o);
-      r[k] = o[0];
     }
   }
 
   // the point's state, node parameters, S and its inverse
-  double Jinv[D][D];
-  const double detq = geometry(qq, Jinv);
-  const double wj = qa ? tab[O::w + qq] * detq : 0.0;
+  const double wj = qa ? tw * detq : 0.0;
   double uq[M];
 #pragma unroll
   for (int k = 0; k < M; ++k) {
     double t = 0.0;
 #pragma unroll
-    for (int n = 0; n < ND; ++n) t += tab[O::V + qq * ND + n] * U[n][k];
+    for (int n = 0; n < ND; ++n) t += tV[n] * U[n][k];
     uq[k] = t;
   }
   const EulerState<double, D> st = euler_state<double, D>(uq, a.pp.p0);
@@ -789,7 +871,7 @@ __global__ __launch_bounds__(128) void rd_jacobian_qp_kernel(JacArgs a)
     for (int x = 0; x < D; ++x) {
       double t = 0.0;
 #pragma unroll
-      for (int xi = 0; xi < D; ++xi) t += Jinv[x][xi] * tab[O::dVT + (qq * D + xi) * ND + n];
+      for (int xi = 0; xi < D; ++xi) t += Jinv[x][xi] * tdV[xi][n];
       G[n][x] = t;
       nn += t * t;
     }
@@ -910,7 +992,7 @@ __global__ __launch_bounds__(128) void rd_jacobian_qp_kernel(JacArgs a)
     if (visc) {
       double nuq = 0.0;
 #pragma unroll
-      for (int n = 0; n < ND; ++n) nuq += tab[O::V + qq * ND + n] * nodal[n];
+      for (int n = 0; n < ND; ++n) nuq += tV[n] * nodal[n];
       double gu[D][M];
 #pragma unroll
       for (int x = 0; x < D; ++x)
@@ -974,7 +1056,7 @@ __global__ __launch_bounds__(128) void rd_jacobian_qp_kernel(JacArgs a)
     v[ND + 1] = wj;
     group_scatter<LPC, RT>(xw, lane, gl, q, v, o);
 #pragma unroll
-    for (int rr = 0; rr < RT; ++rr) xw[(q * RT + rr) * QP_XS + gl] = o[rr];
+    for (int rr = 0; rr < RT; ++rr) xw[(rr * LPC + q) * QP_XS + gl] = o[rr];
     asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
 #pragma unroll
     for (int t = 0; t < NT; ++t) tot[t] = xw[t * QP_XS + gl];
@@ -1006,24 +1088,35 @@ __global__ __launch_bounds__(128) void rd_jacobian_qp_kernel(JacArgs a)
   // residuals: lane q stores the entries q RP ... of every node
   if (a.phi) {
     double *phi = a.phi + static_cast<size_t>(c) * NDM;
+    constexpr int PB = QPR / (LPC * RP);  // nodes per exchange
 #pragma unroll
-    for (int n = 0; n < ND; ++n) {
-      double v[LPC * RP], o[RP];
+    for (int n0 = 0; n0 < ND; n0 += PB) {
+      lds_wait();
 #pragma unroll
-      for (int t = 0; t < LPC * RP; ++t) {
-        double val = 0.0;
-        if (t < M) {
-          if constexpr (SC == SC_LDA) val = lda[n][t];
-          else if constexpr (SC == SC_N) val = dis[n][t];
-          else val = lda[n][t] + th[n] * dis[n][t];
+      for (int n = n0; n < n0 + PB && n < ND; ++n) {
+        double v[LPC * RP];
+#pragma unroll
+        for (int t = 0; t < LPC * RP; ++t) {
+          double val = 0.0;
+          if (t < M) {
+            if constexpr (SC == SC_LDA) val = lda[n][t];
+            else if constexpr (SC == SC_N) val = dis[n][t];
+            else val = lda[n][t] + th[n] * dis[n][t];
+          }
+          v[t] = val;
         }
-        v[t] = val;
+        slab_put<LPC, RP>(xw, lane, (n - n0) * LPC * RP, v);
       }
-      group_scatter<LPC, RP>(xw, lane, gl, q, v, o);
+      lds_wait();
 #pragma unroll
-      for (int rr = 0; rr < RP; ++rr) {
-        const int k = q * RP + rr;
-        if (active && k < M) phi[n * M + k] = o[rr];
+      for (int n = n0; n < n0 + PB && n < ND; ++n) {
+        double o[RP];
+        slab_sum<LPC, RP>(xw, gl, q, (n - n0) * LPC * RP, o);
+#pragma unroll
+        for (int rr = 0; rr < RP; ++rr) {
+          const int k = rr * LPC + q;
+          if (active && k < M) phi[n * M + k] = o[rr];
+        }
       }
     }
   }
@@ -1058,10 +1151,11 @@ __global__ __launch_bounds__(128) void rd_jacobian_qp_kernel(JacArgs a)
         pa[j] = cfm[1] * sl + cfm[2] * sf;
         pb[j] = cfm[3] * sf + cfm[4] * sl;
       }
+      // with the point's weight: w B_m
 #pragma unroll
       for (int i = 0; i < M; ++i)
 #pragma unroll
-        for (int j = 0; j < M; ++j) Bm[i][j] = cfm[0] * Si[i][j] + r0[i] * pa[j] + evm[i] * pb[j];
+        for (int j = 0; j < M; ++j) Bm[i][j] = wj * (cfm[0] * Si[i][j] + r0[i] * pa[j] + evm[i] * pb[j]);
     }
     double cB[M];
 #pragma unroll
@@ -1071,8 +1165,13 @@ __global__ __launch_bounds__(128) void rd_jacobian_qp_kernel(JacArgs a)
       for (int k = 0; k < M; ++k) t += Bm[i][k] * r0[k];
       cB[i] = t;
     }
+    // JB blocks per exchange: the previous batch's reads complete before the
+    // slab is overwritten, then the batch's writes before its reads
 #pragma unroll
-    for (int n = 0; n < ND; ++n) {
+    for (int n0 = 0; n0 < ND; n0 += JB) {
+    lds_wait();
+#pragma unroll
+    for (int n = n0; n < n0 + JB && n < ND; ++n) {
       double ev[M], fv[M], cf[5];
       evec(nq[n], ev, fv);
       if constexpr (SC == SC_LDA) {
@@ -1098,7 +1197,7 @@ __global__ __launch_bounds__(128) void rd_jacobian_qp_kernel(JacArgs a)
       }
       // the diagonal block's K+_m term: N weight 1, B weight theta_m
       const double dw = (n == m) ? ((SC == SC_N) ? 1.0 : ((SC == SC_B) ? thm : 0.0)) : 0.0;
-      double v[LPC * RJ], o[RJ];
+      double v[LPC * RJ];
 #pragma unroll
       for (int t = M * M; t < LPC * RJ; ++t) v[t] = 0.0;
 #pragma unroll
@@ -1111,17 +1210,24 @@ __global__ __launch_bounds__(128) void rd_jacobian_qp_kernel(JacArgs a)
           if constexpr (SC != SC_LDA) {
             const double al = cfm[1] * l0v[j] + cfm[2] * fvm[j];
             const double be = cfm[3] * fvm[j] + cfm[4] * l0v[j];
-            val += dw * (((i == j) ? cfm[0] : 0.0) + r0[i] * al + evm[i] * be);
+            val += wj * dw * (((i == j) ? cfm[0] : 0.0) + r0[i] * al + evm[i] * be);
           }
-          v[i * M + j] = wj * val;
+          v[i * M + j] = val;
         }
       }
-      group_scatter<LPC, RJ>(xw, lane, gl, q, v, o);
+      slab_put<LPC, RJ>(xw, lane, (n - n0) * LPC * RJ, v);
+    }
+    lds_wait();
+#pragma unroll
+    for (int n = n0; n < n0 + JB && n < ND; ++n) {
+      double o[RJ];
+      slab_sum<LPC, RJ>(xw, gl, q, (n - n0) * LPC * RJ, o);
 #pragma unroll
       for (int rr = 0; rr < RJ; ++rr) {
-        const int t = q * RJ + rr;
+        const int t = rr * LPC + q;
         if (active && t < M * M) jac.set((m * M + t / M) * NDM + n * M + t % M, o[rr]);
       }
+    }
     }
   }
   if (active && bad) atomicOr(a.status, 1);

@@ -94,12 +94,14 @@ template <int D, int NG, int ND, int NF, int NQ, bool AFF, int PH, int SC>
 hipError_t launch_jacobian(const JacArgs &a, unsigned grid, hipStream_t s)
 {
   if constexpr (PhysTraits<PH>::M > 1) {
-    // the Euler cells: LPC lanes per cell, 128 threads per workgroup (grid
-    // from the cell count: `grid` counts workgroups of 64 cells)
-    constexpr int CPB = 128 / QpLanes<NQ>::value;
+    // the Euler cells: LPC lanes per cell, one wave per workgroup, a slab of
+    // QpRows rows (grid from the cell count: `grid` counts workgroups of 64
+    // cells)
+    constexpr int CPB = 64 / QpLanes<NQ>::value;
+    constexpr size_t lds = static_cast<size_t>(QP_XS) * QpRows<PhysTraits<PH>::M, ND>::value * sizeof(double);
     (void)grid;
-    hipLaunchKernelGGL((rd_jacobian_qp_kernel<D, NG, ND, NF, NQ, AFF, PH, SC>), dim3((a.ne + CPB - 1) / CPB), dim3(128),
-                       static_cast<size_t>(2) * QP_XS * QpSlab<PH>::value * sizeof(double), s, a);
+    hipLaunchKernelGGL((rd_jacobian_qp_kernel<D, NG, ND, NF, NQ, AFF, PH, SC>), dim3((a.ne + CPB - 1) / CPB), dim3(64),
+                       lds, s, a);
   }
   else
     hipLaunchKernelGGL((rd_jacobian_kernel<D, NG, ND, NF, NQ, AFF, PH, SC>), dim3(grid), dim3(64), 0, s, a);
