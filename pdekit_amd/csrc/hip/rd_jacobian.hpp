@@ -605,7 +605,7 @@ constexpr int QP_XS = 65;
 // Waves per SIMD of the per-point kernel and rows of a wave's exchange slab.
 // The 2D Euler P1 / Q1 cells run at 2 waves (a few spilled registers; measured
 // 1.2x faster than 1 wave without spills, the exchanges' LDS waits of one
-// wave hidden behind the other's arithmetic), with 32 slab rows so that 8
+// wave hidden behind the other's arithmetic), with 36 slab rows so that 8
 // waves fit in a CU's LDS; the others at 1 wave with 64 rows. The Jacobian
 // blocks of a row block go through the slab several at a time (as many as
 // fit), so that the wave waits for the LDS once per batch rather than once
@@ -619,7 +619,7 @@ struct QpWaves
 template <int M, int ND>
 struct QpRows
 {
-  static constexpr int value = QpWaves<M, ND>::value > 1 ? 32 : 64;
+  static constexpr int value = QpWaves<M, ND>::value > 1 ? 36 : 64;
 };
 
 // The two halves of an exchange. Value j of a lane goes to row base + j; the
@@ -1123,7 +1123,10 @@ __global__ __launch_bounds__(64, (QpWaves<PhysTraits<PH>::M, ND>::value)) void r
   if (active && q == 0 && a.wave) a.wave[c] = tot[ND + 1] * fmax(lmax, 1.e-6);
 
   // Jacobian blocks, row block by row block
-#pragma unroll 1
+  // unrolled for the cells of up to 4 nodes (their parameters then stay in
+  // place instead of being selected for every row block; fewer spills, except
+  // for B, whose row blocks also carry theta_m K+_m)
+#pragma unroll ((ND <= 4 && SC != SC_B) ? ND : 1)
   for (int m = 0; m < ND; ++m) {
     NodeQ<D> km = nq[0];
     double thm = th[0];
