@@ -1240,14 +1240,13 @@ __device__ __forceinline__ void sum_rows(const double *__restrict__ rows, uint32
 // sweep (lin_res set) the sums are those of J delta and r = lin_res + sum / V,
 // dt = lin_dt: the residual linearised about u_ref with the time step there.
 // Both the in-kernel finish and the deferred one use it, so they round alike.
+// v = vol[g] and di = dir_index[g] come from the caller (loaded ahead).
 template <int M>
 __device__ __forceinline__ void node_step(const PatchArgs &a, uint32_t g, const double *acc, const double *u_old,
-                                          double (&r)[M], double (&un)[M], double &d, int &di)
+                                          double v, int di, double (&r)[M], double (&un)[M], double &d)
 {
-  const double v = a.vol[g];
   const double iv = 1.0 / v;
   d = a.lin_res ? a.lin_dt[g] : a.cfl * v / acc[M];
-  di = a.dir_index[g];
 #pragma unroll
   for (int k = 0; k < M; ++k) {
     const size_t j = static_cast<size_t>(g) * M + k;
@@ -1259,12 +1258,11 @@ __device__ __forceinline__ void node_step(const PatchArgs &a, uint32_t g, const 
 
 template <int M>
 __device__ __forceinline__ void finish_node(const PatchArgs &a, uint32_t g, const double *acc, const double *u_old,
-                                            double (&sq)[M + 1])
+                                            double v, int di, double (&sq)[M + 1])
 {
   if (a.update) {
     double r[M], un[M], d;
-    int di;
-    node_step<M>(a, g, acc, u_old, r, un, d, di);
+    node_step<M>(a, g, acc, u_old, v, di, r, un, d);
     const bool count = a.outputs && !(a.excl && di >= 0);
 #pragma unroll
     for (int k = 0; k < M; ++k) {
@@ -1279,7 +1277,7 @@ __device__ __forceinline__ void finish_node(const PatchArgs &a, uint32_t g, cons
       a.ws[g] = acc[M];
     }
   } else {
-    const double iv = 1.0 / a.vol[g];
+    const double iv = 1.0 / v;
 #pragma unroll
     for (int k = 0; k < M; ++k) a.res[static_cast<size_t>(g) * M + k] = acc[k] * iv;
     a.ws[g] = acc[M];
@@ -1347,6 +1345,34 @@ __device__ __forceinline__ void patch_body(const PatchArgs &a, uint32_t pl, doub
   float *delta_l = reinterpret_cast<float *>(work + patch_lin_work_doubles<NQ, ND, M>(PATCH));  // PREC 4 only
   // (no global store before the cell work: the table loads stay scalar)
   const int ninner = static_cast<int>(a.pn_inner[p]);
+  // The kernels at 1 wave per SIMD load what the node finish needs of the
+  // thread's first FP nodes (node id, slot, volume, Dirichlet index) here, so
+  // that it arrives during the cell work: at the finish, those loads behind
+  // the node id were two rounds of memory latency per node that no other
+  // wave hid (a sixth of the wave's life on the P2 triangles).
+  // (the Euler cells only, not the P1 tetrahedra's LDA, which the compiler
+  // fits in 2 waves)
+  constexpr int FP = (M > 1 && PatchMinWaves<M, ND, AFFINE, SC, PREC>::value == 1 && PREC != 4 &&
+                      (ND > 4 || SC != SC_LDA))
+                         ? 4
+                         : 0;
+  constexpr int FPS = FP > 0 ? FP : 1;
+  uint32_t pg[FPS], psp[FPS];
+  double pv[FPS];
+  int pdi[FPS];
+  if constexpr (FP > 0) {
+#pragma unroll
+    for (int j = 0; j < FP; ++j) {
+      const int l = min(tid + j * PATCH, nl - 1);
+      pg[j] = a.pnodes[s0 + l];
+      psp[j] = a.slot_pos[s0 + l];
+    }
+#pragma unroll
+    for (int j = 0; j < FP; ++j) {
+      pv[j] = a.vol[pg[j]];
+      pdi[j] = a.dir_index[pg[j]];
+    }
+  }
   for (int l = tid; l < nl; l += PATCH) {
     const uint32_t g = a.pnodes[s0 + l];
     double un[M];
@@ -1355,11 +1381,10 @@ __device__ __forceinline__ void patch_body(const PatchArgs &a, uint32_t pl, doub
       const uint32_t i = a.pn_sid[s0 + l];
       const uint2 rr = a.pn_rows[s0 + l];
       double acc[M + 1], uo[M], r[M], d;
-      int di;
 #pragma unroll
       for (int k = 0; k < M; ++k) uo[k] = a.base_in[static_cast<size_t>(i) * M + k];
       sum_rows<M>(a.prev_acc, rr.x, rr.y, acc);
-      node_step<M>(a, g, acc, uo, r, un, d, di);
+      node_step<M>(a, g, acc, uo, a.vol[g], a.dir_index[g], r, un, d);
     } else {
 #pragma unroll
       for (int k = 0; k < M; ++k) un[k] = a.u[static_cast<size_t>(g) * M + k];
@@ -1464,7 +1489,10 @@ __device__ __forceinline__ void patch_body(const PatchArgs &a, uint32_t pl, doub
   double sq[M + 1];
 #pragma unroll
   for (int k = 0; k <= M; ++k) sq[k] = 0.0;
-  for (int l = tid; l < nl; l += PATCH) {
+  // node l of the patch; pre: its node id, slot, volume and Dirichlet index
+  // were loaded ahead (pgv, pspv, pvv, pdiv)
+  auto finish_l = [&](const int l, const bool pre, const uint32_t pgv, const uint32_t pspv, const double pvv,
+                      const int pdiv) {
     const int b = coff[l];
     const int e = (l + 1 < nl) ? static_cast<int>(coff[l + 1]) : npair;
     double acc[M + 1];
@@ -1474,9 +1502,10 @@ __device__ __forceinline__ void patch_body(const PatchArgs &a, uint32_t pl, doub
 #pragma unroll
       for (int k = 0; k <= M; ++k) acc[k] += work[k * NPAIR + i];
     if (l < ninner) {
-      finish_node<M>(a, a.pnodes[s0 + l], acc, ul + l * M, sq);
+      const uint32_t g = pre ? pgv : a.pnodes[s0 + l];
+      finish_node<M>(a, g, acc, ul + l * M, pre ? pvv : a.vol[g], pre ? pdiv : a.dir_index[g], sq);
     } else {
-      const uint32_t sp = a.slot_pos[s0 + l];
+      const uint32_t sp = pre ? pspv : a.slot_pos[s0 + l];
       double *slot = a.slot_acc + static_cast<size_t>(sp) * (M + 1);
 #pragma unroll
       for (int k = 0; k <= M; ++k) slot[k] = acc[k];
@@ -1493,6 +1522,20 @@ __device__ __forceinline__ void patch_body(const PatchArgs &a, uint32_t pl, doub
         }
       }
     }
+  };
+  if constexpr (FP > 0) {
+#pragma unroll
+    for (int j = 0; j < FP; ++j) {
+      const int l = tid + j * PATCH;
+      if (l < nl) finish_l(l, true, pg[j], psp[j], pv[j], pdi[j]);
+    }
+    for (int l = tid + FP * PATCH; l < nl; l += PATCH) finish_l(l, false, 0u, 0u, 0.0, 0);
+  } else {
+    (void)pg;
+    (void)psp;
+    (void)pv;
+    (void)pdi;
+    for (int l = tid; l < nl; l += PATCH) finish_l(l, false, 0u, 0u, 0.0, 0);
   }
   if (bad) atomicOr(a.status, 1);
   if (a.update && a.outputs) block_sum_store<M>(sq, work, a.norm_partial + static_cast<size_t>(p) * (M + 1));
@@ -1550,7 +1593,7 @@ __device__ __forceinline__ void fixup_body(const FixupArgs &f, uint32_t lb, uint
     double u_old[M];
 #pragma unroll
     for (int k = 0; k < M; ++k) u_old[k] = f.pa.u[static_cast<size_t>(g) * M + k];
-    finish_node<M>(f.pa, g, acc, u_old, sq);
+    finish_node<M>(f.pa, g, acc, u_old, f.pa.vol[g], f.pa.dir_index[g], sq);
   }
   if (f.pa.update && f.pa.outputs) block_sum_store<M>(sq, red, f.norm_partial + static_cast<size_t>(lb) * (M + 1));
 }
