@@ -1346,10 +1346,11 @@ __device__ __forceinline__ void patch_body(const PatchArgs &a, uint32_t pl, doub
   // (no global store before the cell work: the table loads stay scalar)
   const int ninner = static_cast<int>(a.pn_inner[p]);
   // The kernels at 1 wave per SIMD load what the node finish needs of the
-  // thread's first FP nodes (node id, slot, volume, Dirichlet index) here, so
-  // that it arrives during the cell work: at the finish, those loads behind
-  // the node id were two rounds of memory latency per node that no other
-  // wave hid (a sixth of the wave's life on the P2 triangles).
+  // thread's first FP nodes (node id and slot here, volume and Dirichlet
+  // index after the gather's barrier) so that it arrives during the cell
+  // work: at the finish, those loads behind the node id were two rounds of
+  // memory latency per node that no other wave hid (a sixth of the wave's
+  // life on the P2 triangles).
   // (the Euler cells only, not the P1 tetrahedra's LDA, which the compiler
   // fits in 2 waves)
   constexpr int FP = (M > 1 && PatchMinWaves<M, ND, AFFINE, SC, PREC>::value == 1 && PREC != 4 &&
@@ -1366,11 +1367,6 @@ __device__ __forceinline__ void patch_body(const PatchArgs &a, uint32_t pl, doub
       const int l = min(tid + j * PATCH, nl - 1);
       pg[j] = a.pnodes[s0 + l];
       psp[j] = a.slot_pos[s0 + l];
-    }
-#pragma unroll
-    for (int j = 0; j < FP; ++j) {
-      pv[j] = a.vol[pg[j]];
-      pdi[j] = a.dir_index[pg[j]];
     }
   }
   for (int l = tid; l < nl; l += PATCH) {
@@ -1403,6 +1399,15 @@ __device__ __forceinline__ void patch_body(const PatchArgs &a, uint32_t pl, doub
     if constexpr (PREC == 5) nodal_l[static_cast<size_t>(a.maxl) * M + l] = a.theta[g];
   }
   __syncthreads();
+  // issued after the barrier (which waits for every outstanding load), so
+  // that they complete during the cell work
+  if constexpr (FP > 0) {
+#pragma unroll
+    for (int j = 0; j < FP; ++j) {
+      pv[j] = a.vol[pg[j]];
+      pdi[j] = a.dir_index[pg[j]];
+    }
+  }
 
   bool bad = false;
   double phi[ND][M];
