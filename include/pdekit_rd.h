@@ -267,9 +267,10 @@ int pdrd_set_artificial_viscosity(pdrd_method_t h, const double *nu);
  * PGRDMExplicitCellWorker.hpp:535-544): the residual at a quadrature point
  * is div F - s_q with s_q the source interpolated like the solution
  * (PGLDA.hpp:553-741, PGN.hpp:470-660, PGB.hpp:825-1050). Runs with the
- * patch assembly outside batches; a blending coefficient or artificial
- * viscosity alongside it is refused (the reference's reaction LDA would drop
- * the viscosity). pdrd_element_jacobians ignores it, as
+ * patch assembly, in batches too; with the B scheme an external blending
+ * coefficient may accompany it (PGB.hpp:825-1050 takes both), an artificial
+ * viscosity set beside it is not applied (the reference's reaction LDA has
+ * no shock-capturing term). pdrd_element_jacobians ignores it, as
  * compute_adv_res_and_jacobian does. */
 int pdrd_set_sources(pdrd_method_t h, const double *sources);
 /* Providers, on the device from the current solution, each storing its
