@@ -95,7 +95,7 @@ def test_manufactured_steady_state_on_the_gpu():
     m.destroy()
 
 
-@pytest.mark.parametrize("name", ["burgers_p2", "euler2d_mixed", "adv_p2", "euler3d_p1"])
+@pytest.mark.parametrize("name", ["burgers_p2", "euler2d_mixed", "adv_p2", "euler3d_p1", "euler3d_p2"])
 def test_source_with_blending_coeff_matches_oracle(name):
     """The B scheme's reaction residuals with an external blending coefficient
     (PGB.hpp:825-1050, skip below 1e-6 at :878): one sweep against the oracle,

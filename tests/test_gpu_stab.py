@@ -41,6 +41,9 @@ CASES = {
                    lambda m: euler_state(m, pert=0.05)),
     "euler3d_p1": (lambda: box_tetra(4, 4, 3, extent=(1.0, 1.0, 0.5), jitter=0.1, seed=4), L.EULER3D,
                    (1.4, 287.05), lambda m: euler_state(m, dim=3, pert=0.05)),
+    # the B scheme's two-pass quadrature of the P2 tetrahedra with its nodal input
+    "euler3d_p2": (lambda: box_tetra(3, 3, 2, extent=(1.0, 1.0, 0.5), order=2, jitter=0.1, seed=4), L.EULER3D,
+                   (1.4, 287.05), lambda m: euler_state(m, dim=3, pert=0.05)),
 }
 
 
