@@ -253,13 +253,21 @@ struct EulerNodeK
   double l0, sg, dl, um, nh[D], emax;
 };
 
+// The geometric part of a node's K+ parameters: nh = g / |g| and |g| for the
+// physical gradient g of the node's basis function (tiny: |g| negligible).
+template <int D>
+struct EulerNodeG
+{
+  double nh[D], norm;
+  bool tiny;
+};
+
 // Jq: the inverse Jacobian at the point, dvk: the reference gradients of the
 // ND basis functions there ([D][ND]).
 template <int D, int ND>
-__device__ __forceinline__ EulerNodeK<D> euler_node_k(const double (&Jq)[D][D], const double *dvk, int n,
-                                                      const double (&V)[D], double a)
+__device__ __forceinline__ EulerNodeG<D> euler_node_g(const double (&Jq)[D][D], const double *dvk, int n)
 {
-  EulerNodeK<D> k;
+  EulerNodeG<D> ge;
   double g[D];
 #pragma unroll
   for (int x = 0; x < D; ++x) {
@@ -274,15 +282,29 @@ __device__ __forceinline__ EulerNodeK<D> euler_node_k(const double (&Jq)[D][D], 
   double norm, inorm;
   sqrt_rsqrt(nn, norm, inorm);
   norm = (nn > 0.0) ? norm : 0.0;  // sqrt_rsqrt(0) is NaN
-  const bool tiny = (D == 2) ? (norm < 1.e-9) : (norm == 0.0);
-  inorm = tiny ? 0.0 : inorm;
+  ge.tiny = (D == 2) ? (norm < 1.e-9) : (norm == 0.0);
+  inorm = ge.tiny ? 0.0 : inorm;
+#pragma unroll
+  for (int x = 0; x < D; ++x) ge.nh[x] = g[x] * inorm;
+  ge.norm = norm;
+  return ge;
+}
+
+// The K+ parameters from the geometric part and the point's velocity V and
+// sound speed a.
+template <int D>
+__device__ __forceinline__ EulerNodeK<D> euler_node_k(const EulerNodeG<D> &ge, const double (&V)[D], double a)
+{
+  EulerNodeK<D> k;
   double u_n = 0.0;
 #pragma unroll
   for (int x = 0; x < D; ++x) {
-    k.nh[x] = g[x] * inorm;
+    k.nh[x] = ge.nh[x];
     u_n += V[x] * k.nh[x];
   }
   k.um = u_n;
+  const double norm = ge.norm;
+  const bool tiny = ge.tiny;
   const double e0 = norm * u_n, ep = norm * (u_n + a), em = norm * (u_n - a);
   k.emax = fmax(fabs(e0), fmax(fabs(ep), fabs(em)));
   const double p0 = fmax(0.0, e0);
@@ -292,6 +314,13 @@ __device__ __forceinline__ EulerNodeK<D> euler_node_k(const double (&Jq)[D][D], 
   k.sg = dpn + dmn;
   k.dl = dpn - dmn;
   return k;
+}
+
+template <int D, int ND>
+__device__ __forceinline__ EulerNodeK<D> euler_node_k(const double (&Jq)[D][D], const double *dvk, int n,
+                                                      const double (&V)[D], double a)
+{
+  return euler_node_k<D>(euler_node_g<D, ND>(Jq, dvk, n), V, a);
 }
 
 // out = K+_n v at a point with velocity V, total enthalpy H, sound speed a,
@@ -548,6 +577,18 @@ __device__ __forceinline__ void cell_residual(const double *__restrict__ tab, co
   //      the parameters).
   constexpr bool TWO = (D == 3) && NDIS && AFFINE && DLDS && (M == D + 2) && (NQ * 3 * M <= ND * (M + 1));
 
+  // P1 simplices of the Euler physics (affine, D + 1 nodes): the basis
+  // gradients, and with them the geometric part of every node's K+
+  // parameters, are the same at every quadrature point; formed once per cell
+  // (not for the P1 tetrahedra's LDA, which the compiler fits in 2 waves
+  // without the registers they take)
+  constexpr bool CG = (M > 1) && AFFINE && (ND == D + 1) && !(D == 3 && SC == SC_LDA);
+  EulerNodeG<D> cg[CG ? ND : 1];
+  if constexpr (CG) {
+#pragma unroll
+    for (int n = 0; n < ND; ++n) cg[n] = euler_node_g<D, ND>(Jinv[0], tab + O::dVT, n);
+  }
+
   auto quad_point = [&](const int q, const double (&r)[M], const double (&Jq)[D][D], const double detq) {
     const double wj = tab[O::w + q] * detq;
     vol += wj;
@@ -657,7 +698,12 @@ __device__ __forceinline__ void cell_residual(const double *__restrict__ tab, co
       using NodeK = EulerNodeK<D>;
       // the K+ parameters of node n (recomputed where keeping them for all
       // nodes costs spills)
-      auto node_k = [&](const int n) { return euler_node_k<D, ND>(Jq, dvk, n, st.V, st.a); };
+      auto node_k = [&](const int n) {
+        if constexpr (CG)
+          return euler_node_k<D>(cg[n], st.V, st.a);
+        else
+          return euler_node_k<D, ND>(Jq, dvk, n, st.V, st.a);
+      };
       // 3D: recompute in the application instead of keeping ND sets
       constexpr bool KEEP = (D == 2);
       NodeK nk[KEEP ? ND : 1];
