@@ -544,8 +544,6 @@ __device__ __forceinline__ void cell_residual(const double *__restrict__ tab, co
   constexpr bool NDIS = (SC != SC_LDA);
   double lda[ND][M];
   double dis[NDIS ? ND : 1][M];
-  auto dis_get = [&](const int n, const int k) -> double { return dis[n][k]; };
-  auto dis_add = [&](const int n, const int k, const double v) { dis[n][k] += v; };
   // max over nodes and quadrature points of |eigenvalue|; the floor 1e-6 of
   // every (node, point) term is applied once at the end (max(1e-6, max x) =
   // max over x of max(1e-6, x))
@@ -679,7 +677,7 @@ __device__ __forceinline__ void cell_residual(const double *__restrict__ tab, co
 #pragma unroll
       for (int n = 0; n < ND; ++n) {
         lda[n][0] += kp[n] * wr;
-        if constexpr (NDIS) dis_add(n, 0, wj * (kp[n] * (Uq(n, 0) - ub)));
+        if constexpr (NDIS) dis[n][0] += wj * (kp[n] * (Uq(n, 0) - ub));
       }
     } else {
       double uq[M];
@@ -835,7 +833,7 @@ __device__ __forceinline__ void cell_residual(const double *__restrict__ tab, co
           for (int k = 0; k < M; ++k) du[k] = wj * (Uq(n, k) - B[k][R - 1]);
           kplus(kn, du, t);
 #pragma unroll
-          for (int k = 0; k < M; ++k) dis_add(n, k, t[k]);
+          for (int k = 0; k < M; ++k) dis[n][k] += t[k];
         }
       }
     }
@@ -992,7 +990,7 @@ __device__ __forceinline__ void cell_residual(const double *__restrict__ tab, co
 #pragma unroll
     for (int n = 0; n < ND; ++n)
 #pragma unroll
-      for (int k = 0; k < M; ++k) phi[n][k] = lda[n][k] + dis_get(n, k);
+      for (int k = 0; k < M; ++k) phi[n][k] = lda[n][k] + dis[n][k];
   } else if constexpr (SC == SC_B && (PREC == 2 || PREC == 5)) {
     // external nodal coefficient; below 1e-6 at every node the cell keeps
     // its LDA residuals
@@ -1007,16 +1005,16 @@ __device__ __forceinline__ void cell_residual(const double *__restrict__ tab, co
 #pragma unroll
     for (int n = 0; n < ND; ++n)
 #pragma unroll
-      for (int k = 0; k < M; ++k) phi[n][k] = skip ? lda[n][k] : lda[n][k] + th[n] * dis_get(n, k);
+      for (int k = 0; k < M; ++k) phi[n][k] = skip ? lda[n][k] : lda[n][k] + th[n] * dis[n][k];
   } else if constexpr (SC == SC_B) {
     double an = 0.0;
 #pragma unroll
-    for (int n = 0; n < ND; ++n) an += fabs(lda[n][0] + dis_get(n, 0));
+    for (int n = 0; n < ND; ++n) an += fabs(lda[n][0] + dis[n][0]);
     const double theta = fabs(fint[0]) / (an + 1.e-6);
 #pragma unroll
     for (int n = 0; n < ND; ++n)
 #pragma unroll
-      for (int k = 0; k < M; ++k) phi[n][k] = lda[n][k] + theta * dis_get(n, k);
+      for (int k = 0; k < M; ++k) phi[n][k] = lda[n][k] + theta * dis[n][k];
   } else {
     // PSI: the x+ limiter of the N residuals, per equation; the PSI / LDA
     // blend then takes (1 - theta) LDA + theta PSI with PGB's cell theta
@@ -1024,7 +1022,7 @@ __device__ __forceinline__ void cell_residual(const double *__restrict__ tab, co
     if constexpr (SC == SC_BPSI) {
       double an = 0.0;
 #pragma unroll
-      for (int n = 0; n < ND; ++n) an += fabs(lda[n][0] + dis_get(n, 0));
+      for (int n = 0; n < ND; ++n) an += fabs(lda[n][0] + dis[n][0]);
       theta = fabs(fint[0]) / (an + 1.e-6);
     }
 #pragma unroll
@@ -1036,12 +1034,12 @@ __device__ __forceinline__ void cell_residual(const double *__restrict__ tab, co
       double sx = 0.0;
 #pragma unroll
       for (int n = 0; n < ND; ++n) {
-        xs[n] = fmax((lda[n][k] + dis_get(n, k)) * itot, 0.0) + 1.e-12;
+        xs[n] = fmax((lda[n][k] + dis[n][k]) * itot, 0.0) + 1.e-12;
         sx += xs[n];
       }
 #pragma unroll
       for (int n = 0; n < ND; ++n) {
-        const double psi = lim ? xs[n] / sx * tot : lda[n][k] + dis_get(n, k);
+        const double psi = lim ? xs[n] / sx * tot : lda[n][k] + dis[n][k];
         phi[n][k] = (SC == SC_BPSI) ? (1.0 - theta) * lda[n][k] + theta * psi : psi;
       }
     }
