@@ -1,13 +1,14 @@
 """Batched iterations (pdrd_batch_iterate): several independent methods of
 different element types, orders, physics and schemes advanced in shared
 launches (direct, or replayed from a captured HIP graph) give bit for bit
-what each method's own pdrd_iterate gives."""
+what each method's own pdrd_iterate gives, and what the oracle's iterations
+give to 1e-12."""
 import numpy as np
 import pytest
 
 from pdekit_amd import _lib as L
 from pdekit_amd.rdm import Batch, PGRDMethodExplicit
-from tests.helpers import CASES, quad_orders
+from tests.helpers import CASES, oracle_problem, quad_orders, rel_err
 
 pytestmark = pytest.mark.gpu
 
@@ -109,3 +110,31 @@ def test_batch_with_nodal_inputs_and_sources(n_iter):
             a, c = ma.vec_function(f), mb.vec_function(f)
             assert np.array_equal(a, c), (k, f, np.abs(a - c).max())
         assert np.array_equal(na, nb), k
+
+
+@pytest.mark.parametrize("graphs", [False, True])
+def test_batch_matches_oracle(graphs):
+    """A batch of methods of four element types and physics, iterated together,
+    against the oracle's explicit iterations (PGRDMethodExplicit.hpp:427-463,
+    pseudo-time steps of every method from its own CFL): solutions <= 1e-12 and
+    residuals <= 1e-11 normwise relative, the Dirichlet inflow nodes of the
+    scalar methods kept at their values."""
+    chosen = [("adv_p2", L.LDA), ("burgers_p2", L.B), ("euler2d_mixed", L.LDA), ("euler3d_p1", L.N)]
+    n_iter = 5
+    print(f"batch against the oracle, {n_iter} iterations, graphs {graphs}", flush=True)
+    made = [_make(n, s) for n, s in chosen]
+    Batch(graphs=graphs).iterate([m for m, _ in made], [c for _, c in made], n_iter)
+    for (m, cfl), (name, scheme) in zip(made, chosen):
+        build, physics, params, state = CASES[name]
+        mesh = build()
+        u0 = state(mesh)
+        P = oracle_problem(mesh, physics, scheme, params, quad_orders(mesh))
+        bn = mesh.boundaries.get("xmin", mesh.boundaries.get("Edge3"))
+        if bn is not None and physics in (L.ADVECTION2D, L.BURGERS2D):
+            uo, res, dt, norms, rc = P.iterate(u0, cfl, n_iter, np.asarray(bn, np.uint32), u0[bn])
+        else:
+            uo, res, dt, norms, rc = P.iterate(u0, cfl, n_iter)
+        assert rc == 0, name
+        assert rel_err(m.vec_function("solution"), uo) <= 1e-12, name
+        assert rel_err(m.vec_function("residuals"), res) <= 1e-11, name
+        m.destroy()
