@@ -114,12 +114,14 @@ def test_batch_with_nodal_inputs_and_sources(n_iter):
 
 @pytest.mark.parametrize("graphs", [False, True])
 def test_batch_matches_oracle(graphs):
-    """A batch of methods of four element types and physics, iterated together,
+    """A batch of methods of every element type and physics (the P2 / Q2 Euler
+    cells with LDA, PSI and the PSI / LDA blend among them), iterated together,
     against the oracle's explicit iterations (PGRDMethodExplicit.hpp:427-463,
     pseudo-time steps of every method from its own CFL): solutions <= 1e-12 and
     residuals <= 1e-11 normwise relative, the Dirichlet inflow nodes of the
     scalar methods kept at their values."""
-    chosen = [("adv_p2", L.LDA), ("burgers_p2", L.B), ("euler2d_mixed", L.LDA), ("euler3d_p1", L.N)]
+    chosen = [("adv_p2", L.LDA), ("burgers_p2", L.B), ("euler2d_mixed", L.LDA), ("euler3d_p1", L.N),
+              ("euler2d_q2_mixed", L.LDA), ("euler2d_p2", L.PSI), ("euler3d_p2", L.BPSI)]
     n_iter = 5
     print(f"batch against the oracle, {n_iter} iterations, graphs {graphs}", flush=True)
     made = [_make(n, s) for n, s in chosen]
